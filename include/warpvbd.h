@@ -1,0 +1,137 @@
+/*
+ * warpvbd.h -- C ABI of the MI355X-native Vertex Block Descent solver.
+ *
+ * Plain pointers and sizes only.  Host arrays are caller-owned; every device
+ * buffer is owned by the handle.  Vertex and element arrays use the caller's
+ * (original) numbering; the colour-major renumbering is internal.  A handle is
+ * bound to the HIP device that was current at vbd_create and to its own stream
+ * (default priority); one handle is not re-entrant, separate handles are
+ * independent.  Every call returns 0 on success or a negative VBD_E* code;
+ * vbd_last_error() gives a message for the calling thread.  Non-convergence
+ * is not an error: *iters_out == max_iter then.
+ *
+ * Reference interfaces replaced (WarpVBD, paths relative to its repository):
+ *   vbd_create            VBDSolver.__init__            src/vbd_solver.py:367-499
+ *   vbd_reset             VBDSolver.reset               src/vbd_solver.py:501-506
+ *   vbd_step              VBDSolver.step                src/vbd_solver.py:509-612
+ *   vbd_get_state /
+ *   vbd_set_state         VBDSolver.old_positions / .old_velocities (attributes read
+ *                         and assigned by callers, src/vbd_solver.py:403-404,535,609-610)
+ *   vbd_get_colors        coloring returned by graph_coloring, src/coloring.py:40-72,
+ *                         as stored in VBDSolver.color_groups (src/vbd_solver.py:470-483)
+ *   vbd_get_incidence     VBDSolver.adj_v2e             src/vbd_solver.py:485-498
+ *   vbd_color             graph_coloring_wp (the "parallel" colouring slot)
+ *                         src/coloring.py:85-133, returning graph_coloring's
+ *                         result (src/coloring.py:24-72) bit-exact
+ *   vbd_color_graph       graph_coloring(adjacency)     src/coloring.py:40-72
+ *   vbd_use_device        the device= argument          src/vbd_solver.py:383
+ */
+#ifndef WARPVBD_H
+#define WARPVBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vbd_solver vbd_solver_t;
+
+enum {
+    VBD_OK = 0,
+    VBD_EINVAL = -1,      /* bad argument (null pointer, size, dt <= 0, max_iter < 1) */
+    VBD_EHIP = -2,        /* a HIP runtime call failed */
+    VBD_ENOMEM = -3,      /* device allocation failed */
+    VBD_EMESH = -4,       /* element index outside [0, n_vertices) */
+    VBD_ECOLORS = -5,     /* a vertex needs more than 256 colours (src/coloring.py:50-68) */
+    VBD_EVALENCE = -6,    /* a vertex has more than 32 incident elements (src/vbd_solver.py:12,529) */
+    VBD_EDEGENERATE = -7  /* an element has a singular rest shape (src/vbd_solver.py:336-343) */
+};
+
+enum { VBD_MATERIAL_SNH = 0, VBD_MATERIAL_STVK = 1 };
+
+/* Build a solver.  rest_xyz: f64[n_vertices*3] rest (= initial) positions;
+ * tets: i32[n_tets*4]; density/youngs/poisson: f64[n_tets] per element;
+ * active: u8[n_vertices] (NULL = all active); gravity: f64[3];
+ * damping: the reference's damping_coefficient; material: VBD_MATERIAL_*.
+ * Rest shape, Lame parameters, colouring and the colour-major slot layout are
+ * computed on the device.  Replaces src/vbd_solver.py:367-499. */
+int vbd_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, int32_t n_tets,
+               const double *density, const double *youngs, const double *poisson, const uint8_t *active,
+               const double *gravity, double damping, int32_t material, vbd_solver_t **out);
+
+void vbd_destroy(vbd_solver_t *h);
+
+/* Restore x_prev = rest positions, v_prev = 0.  src/vbd_solver.py:501-506. */
+int vbd_reset(vbd_solver_t *h);
+
+/* One implicit-Euler time step.  positions: f64[n*3] x_n to step from, or NULL
+ * to step from the solver's own old_positions (the usual caller pattern,
+ * examples/run_cantilever.py:79-80).  Iterates colour sweeps until
+ * max_v |dx_x+dx_y+dx_z| < tol or max_iter iterations.  out_xyz (nullable)
+ * receives the new positions; iters_out the number of iterations executed;
+ * metric_out the last iteration's convergence metric.  The host waits once per
+ * chunk of iterations, not per iteration.  Replaces src/vbd_solver.py:509-612. */
+int vbd_step(vbd_solver_t *h, const double *positions, double dt, double tol, int32_t max_iter,
+             double *out_xyz, int32_t *iters_out, double *metric_out);
+
+/* Read / overwrite x_prev and v_prev (either pointer may be NULL). */
+int vbd_get_state(vbd_solver_t *h, double *x_prev, double *v_prev);
+int vbd_set_state(vbd_solver_t *h, const double *x_prev, const double *v_prev);
+
+/* Device snapshots of the current positions, so a returned position array stays
+ * valid after later steps without a host copy.  slot in [0, VBD_MAX_SNAPSHOTS). */
+#define VBD_MAX_SNAPSHOTS 16
+int vbd_snapshot(vbd_solver_t *h, int32_t slot);
+int vbd_read_snapshot(vbd_solver_t *h, int32_t slot, double *out_xyz);
+int vbd_load_snapshot(vbd_solver_t *h, int32_t slot); /* x_prev <- snapshot */
+
+/* Mesh facts computed at create. */
+int vbd_info(vbd_solver_t *h, int32_t *n_vertices, int32_t *n_tets, int32_t *n_colors, int32_t *max_incident,
+             int32_t *max_degree, int64_t *n_slots);
+/* colors: i32[n_vertices]; -1 for a vertex above the largest referenced index
+ * (src/coloring.py:28). */
+int vbd_get_colors(vbd_solver_t *h, int32_t *colors);
+/* color_sizes: i32[n_colors] active vertices per colour (the sweep sizes). */
+int vbd_get_color_sizes(vbd_solver_t *h, int32_t *color_sizes);
+/* adj: i32[n_vertices*max_incident], ascending element ids padded with -1. */
+int vbd_get_incidence(vbd_solver_t *h, int32_t *adj);
+
+/* Standalone GPU colouring: sequential first-fit in vertex order, bit-exact
+ * (src/coloring.py:24-72).  colors: i32[n_vertices]. */
+int vbd_color(const int32_t *tets, int32_t n_tets, int32_t n_vertices, int32_t *colors, int32_t *n_colors_out,
+              int32_t *max_degree_out);
+
+/* GPU first-fit colouring of an explicit adjacency list (CSR: adj_off[n+1],
+ * adj[adj_off[n]]): colour[v] = lowest colour not used by a neighbour u < v in
+ * v's list -- graph_coloring(adjacency), src/coloring.py:40-72, bit-exact. */
+int vbd_color_graph(const int32_t *adj_off, const int32_t *adj, int32_t n_vertices, int32_t *colors,
+                    int32_t *n_colors_out);
+
+/* Make HIP device `device` current for the calling thread (the reference's
+ * device= argument, src/vbd_solver.py:383). */
+int vbd_use_device(int32_t device);
+
+/* Timing of the colour-sweep kernels on the handle's stream: when enabled,
+ * each sweep launch of a colour with at least min_vertices active vertices is
+ * bracketed by HIP events.  vbd_profile_read returns the summed kernel time
+ * (ms), the number of timed launches and their algorithmic bytes (the byte
+ * model sum over the colour of 40 + 68 k_v per active vertex plus 12 per
+ * distinct gathered neighbour), then resets the counters. */
+int vbd_profile_enable(vbd_solver_t *h, int32_t enable, int32_t min_vertices);
+int vbd_profile_read(vbd_solver_t *h, double *ms, int64_t *launches, double *bytes);
+/* Per-colour algorithmic bytes of one sweep (f64[n_colors]). */
+int vbd_color_bytes(vbd_solver_t *h, double *bytes);
+
+/* Execution knobs: lanes per vertex (1,2,4,8,16,32), threads per block (64..1024),
+ * vertices below which consecutive colours run fused in one workgroup, and the
+ * iteration chunk between host waits (0 = adaptive). */
+int vbd_set_launch(vbd_solver_t *h, int32_t lanes_per_vertex, int32_t block_size, int32_t small_color,
+                   int32_t chunk);
+
+const char *vbd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

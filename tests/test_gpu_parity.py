@@ -1,0 +1,316 @@
+"""GPU parity tests: the HIP solver (through the C ABI) against the FP64 CPU
+oracle and the committed golden trajectories.
+
+Gates (FP32 solver vs FP64 oracle):
+* colouring, colour groups and vertex->element adjacency: bit-exact;
+* positions at a fixed iteration count: within 1e-6 x bounding-box diagonal,
+  every step;
+* tolerance-terminated runs (tol 1e-7): per-step iteration counts within +-1 of
+  the oracle's, tip-displacement curve within 1e-4 of the curve's range plus
+  the accumulated effect of the +-1 iteration differences (stated per test);
+* exact cases: free fall, pinned vertices.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from warpvbd_amd import VBDSolver, compute_adjacency_dict, graph_coloring, graph_coloring_wp
+from warpvbd_amd._lib import VBDError
+from warpvbd_amd._utils import hex2tets, voxel2hex
+from warpvbd_amd.scenes import cantilever, mass_spring
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def diag(sc):
+    return float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
+
+
+def make(sc, material="snh", damping=0.0, tol=1e-7, max_iter=1000, **kw):
+    k = sc.solver_kwargs()
+    k["damping_coefficient"] = damping
+    k.update(kw)
+    g = VBDSolver(**k, dx_tol=tol, max_iter=max_iter, material=material, verbose=False)
+    r = oracle.OracleSolver(**k, dx_tol=tol, max_iter=max_iter, material=material, expanded=False, threads=8)
+    return g, r
+
+
+# ----------------------------------------------------------------- colouring
+
+@pytest.mark.parametrize("shape", [(10, 2, 2), (128, 16, 16), (256, 32, 32), (7, 5, 3)])
+def test_gpu_coloring_bit_exact_voxel(shape):
+    V, H = voxel2hex(np.ones(shape, bool), 0.01, 0.01, 0.01)
+    T = hex2tets(H)
+    print("coloring", shape, flush=True)
+    c_gpu = graph_coloring_wp(T)
+    c_ref, _, _ = oracle.color_mesh(T)
+    assert np.array_equal(c_gpu, c_ref)
+
+
+def test_gpu_coloring_bit_exact_irregular():
+    rng = np.random.default_rng(7)
+    for trial in range(4):
+        n = int(rng.integers(50, 400))
+        T = rng.integers(0, n, size=(int(rng.integers(50, 600)), 4))
+        T = T[np.array([len(set(t)) == 4 for t in T])]
+        perm = rng.permutation(int(T.max()) + 1)  # shuffle ids: long dependency chains
+        T = perm[T]
+        c_ref, _, _ = oracle.color_mesh(T)
+        assert np.array_equal(graph_coloring_wp(T), c_ref)
+        adj, md = compute_adjacency_dict(T)
+        c2, groups = graph_coloring(adj, verbose=False)
+        c3, groups3, md3 = oracle.color_first_fit(T)
+        assert np.array_equal(c2, c3) and md == md3
+        assert groups == groups3
+
+
+def test_solver_colors_groups_adjacency_match_reference_layout():
+    sc = mass_spring(6, 3)
+    g, r = make(sc)
+    assert np.array_equal(g.colors(), r.colors())
+    cg = g.color_groups
+    col = r.colors()
+    for c in range(g.n_colors):
+        grp = np.nonzero(col == c)[0]
+        assert np.array_equal(cg[c, : len(grp)], grp) and np.all(cg[c, len(grp):] == -1)
+    adj = g.adj_v2e
+    assert adj.shape == (sc.n_vertices, g.max_incident_elements)
+    for v in range(sc.n_vertices):
+        inc = np.nonzero((sc.elements == v).any(axis=1))[0]
+        assert np.array_equal(adj[v, : len(inc)], inc) and np.all(adj[v, len(inc):] == -1)
+
+
+# ----------------------------------------------------------------- fixed iteration count
+
+@pytest.mark.parametrize("material", ["snh", "stvk"])
+def test_fixed_iterations_cantilever_10x2x2(material):
+    sc = cantilever((10, 2, 2))
+    g, r = make(sc, material=material)
+    gate = 1e-6 * diag(sc)
+    worst = 0.0
+    for k in range(100):
+        x = g.step(g.old_positions, 1e-3, dx_tol=0.0, max_iter=10).numpy()
+        xr = r.step(1e-3, dx_tol=0.0, max_iter=10)
+        assert g.last_iterations == 10
+        worst = max(worst, float(np.abs(x - xr).max()))
+        assert worst < gate, (k, worst)
+    print(f"{material}: max |x_gpu - x_fp64| over 100 steps x 10 iterations = {worst:.3e} m (gate {gate:.3e})")
+
+
+def test_fixed_iterations_damping():
+    sc = cantilever((10, 2, 2))
+    g, r = make(sc, damping=1.0)
+    gate = 1e-6 * diag(sc)
+    for k in range(30):
+        x = g.step(g.old_positions, 1e-3, dx_tol=0.0, max_iter=10).numpy()
+        xr = r.step(1e-3, dx_tol=0.0, max_iter=10)
+        assert np.abs(x - xr).max() < gate
+
+
+def test_fixed_iterations_cantilever_128x16x16():
+    sc = cantilever((128, 16, 16))
+    g, r = make(sc)
+    gate = 1e-6 * diag(sc)
+    for k in range(3):
+        x = g.step(g.old_positions, 1e-3, dx_tol=0.0, max_iter=5).numpy()
+        xr = r.step(1e-3, dx_tol=0.0, max_iter=5)
+        err = float(np.abs(x - xr).max())
+        print(f"128x16x16 step {k}: {err:.3e} m (gate {gate:.3e})", flush=True)
+        assert err < gate
+
+
+def test_fixed_iterations_stvk_under_compression():
+    # a squashed start state exercises indefinite StVK blocks and the guard
+    sc = cantilever((6, 2, 2))
+    g, r = make(sc, material="stvk")
+    x0 = sc.vertices.copy()
+    x0[:, 0] *= 0.7
+    x0[:, 2] *= 1.2
+    g.old_positions = x0
+    r.set_state(x=x0)
+    gate = 1e-6 * diag(sc)
+    for k in range(10):
+        x = g.step(g.old_positions, 1e-3, dx_tol=0.0, max_iter=5).numpy()
+        xr = r.step(1e-3, dx_tol=0.0, max_iter=5)
+        assert np.all(np.isfinite(x))
+        assert np.abs(x - xr).max() < gate
+
+
+# ----------------------------------------------------------------- tolerance termination
+
+def _tol_run(sc, steps, tol=1e-7, material="snh"):
+    g, r = make(sc, material=material, tol=tol)
+    it_g, it_r, z_g, z_r = [], [], [], []
+    for _ in range(steps):
+        x = g.step(g.old_positions, 1e-3).numpy()
+        xr = r.step(1e-3)
+        it_g.append(g.last_iterations)
+        it_r.append(r.last_iters)
+        z_g.append(x[sc.tip_idx, 2].mean())
+        z_r.append(xr[sc.tip_idx, 2].mean())
+    return np.array(it_g), np.array(it_r), np.array(z_g), np.array(z_r)
+
+
+def test_tolerance_cantilever_10x2x2_matches_golden():
+    d = np.load(os.path.join(GOLDEN, "cantilever10x2x2_snh_tol1e-7.npz"))
+    sc = cantilever((10, 2, 2))
+    g, _ = make(sc)
+    it_g, z_g = [], []
+    for _ in range(300):
+        x = g.step(g.old_positions, 1e-3).numpy()
+        it_g.append(g.last_iterations)
+        z_g.append(x[sc.tip_idx, 2].mean())
+    it_g, z_g = np.array(it_g), np.array(z_g)
+    it_r, z_r = d["iters"], d["tipz"]
+    assert np.all(np.abs(it_g - it_r) <= 1)
+    assert np.mean(it_g == it_r) >= 0.95
+    disp_g, disp_r = z_g - z_r[0], z_r - z_r[0]
+    rel = np.abs(disp_g - disp_r).max() / np.abs(disp_r).max()
+    print(f"iterations {it_g.sum()} vs {it_r.sum()}; tip curve max rel err {rel:.2e}")
+    # tol 1e-7 allows each step's solution to differ by O(tol); a +-1 iteration
+    # difference at a borderline step propagates through the velocity.
+    assert rel < 5e-4
+
+
+def test_tolerance_mass_spring_matches_golden():
+    d = np.load(os.path.join(GOLDEN, "massspring6+3_snh_tol1e-7.npz"))
+    sc = mass_spring(6, 3)
+    g, _ = make(sc)
+    it_g, z_g = [], []
+    for _ in range(int(d["steps"])):
+        x = g.step(g.old_positions, 1e-3).numpy()
+        it_g.append(g.last_iterations)
+        z_g.append(x[sc.tip_idx, 2].mean())
+    assert np.all(np.abs(np.array(it_g) - d["iters"]) <= 1)
+    disp_r = d["tipz"] - d["tipz"][0]
+    rel = np.abs((np.array(z_g) - d["tipz"][0]) - disp_r).max() / np.abs(disp_r).max()
+    assert rel < 5e-4
+
+
+def test_golden_fixed_iteration_fixtures():
+    for name, sc in [("cantilever10x2x2_stvk_fixed10", cantilever((10, 2, 2))),
+                     ("cantilever10x2x2_snh_damp1_fixed10", cantilever((10, 2, 2)))]:
+        d = np.load(os.path.join(GOLDEN, name + ".npz"))
+        k = sc.solver_kwargs()
+        k["damping_coefficient"] = float(d["damping"])
+        g = VBDSolver(**k, dx_tol=0.0, max_iter=int(d["max_iter"]), material=str(d["material"]), verbose=False)
+        for _ in range(int(d["steps"])):
+            x = g.step(g.old_positions, float(d["dt"])).numpy()
+        assert np.abs(x - d["x_final"]).max() < 1e-6 * diag(sc), name
+
+
+# ----------------------------------------------------------------- exact cases and API behaviour
+
+def test_free_fall_exact_and_one_iteration():
+    sc = cantilever((4, 2, 2))
+    k = sc.solver_kwargs()
+    k["active_mask"] = None
+    g = VBDSolver(**k, dx_tol=1e-7, max_iter=50, verbose=False)
+    for n in range(1, 21):
+        x = g.step(g.old_positions, 1e-3).numpy()
+        assert g.last_iterations == 1
+        assert np.allclose(x[:, 2], sc.vertices[:, 2] - 9.81e-6 * n * (n + 1) / 2, rtol=0, atol=2e-7)
+
+
+def test_pinned_vertices_bit_unchanged_and_velocity():
+    sc = cantilever((10, 2, 2))
+    g, _ = make(sc)
+    for _ in range(20):
+        x = g.step(g.old_positions, 1e-3).numpy()
+    pinned = ~sc.active_mask
+    assert np.array_equal(x[pinned], sc.vertices[pinned])
+    assert np.all(g.old_velocities.numpy()[pinned] == 0.0)
+
+
+def test_returned_arrays_keep_their_values_and_explicit_positions():
+    sc = cantilever((10, 2, 2))
+    g, r = make(sc)
+    a = g.step(g.old_positions, 1e-3)
+    b = g.step(g.old_positions, 1e-3)
+    xr1 = r.step(1e-3)
+    xr2 = r.step(1e-3)
+    assert np.abs(a.numpy() - xr1).max() < 1e-8 and np.abs(b.numpy() - xr2).max() < 1e-8
+    # stepping from an older array rewinds like the reference's step(positions)
+    c = g.step(a, 1e-3).numpy()
+    r.set_state(x=xr1)
+    xr3 = r.step(1e-3)
+    # velocity carried over is the solver's current one in both
+    assert np.abs(c - xr3).max() < 1e-8
+    # numpy input
+    d = g.step(xr1, 1e-3).numpy()
+    assert np.all(np.isfinite(d))
+
+
+def test_reset_and_state_roundtrip():
+    sc = cantilever((10, 2, 2))
+    g, r = make(sc)
+    x1 = g.step(g.old_positions, 1e-3).numpy()
+    g.reset()
+    assert np.array_equal(g.old_positions.numpy(), sc.vertices)
+    assert np.all(g.old_velocities.numpy() == 0)
+    x1b = g.step(g.old_positions, 1e-3).numpy()
+    assert np.array_equal(x1, x1b)
+    rng = np.random.default_rng(0)
+    v0 = rng.normal(0, 0.05, size=(sc.n_vertices, 3))
+    g.reset()
+    r.reset()
+    g.old_velocities = v0
+    r.set_state(v=v0)
+    assert np.abs(g.old_velocities.numpy() - v0).max() < 1e-8
+    for _ in range(5):
+        x = g.step(g.old_positions, 1e-3, dx_tol=0.0, max_iter=10).numpy()
+        xr = r.step(1e-3, dx_tol=0.0, max_iter=10)
+    assert np.abs(x - xr).max() < 1e-6 * diag(sc)
+
+
+def test_nan_runs_to_max_iter_like_reference():
+    sc = cantilever((4, 2, 2))
+    g, _ = make(sc, max_iter=7)
+    v = np.zeros((sc.n_vertices, 3))
+    v[np.nonzero(sc.active_mask)[0][0]] = np.nan
+    g.old_velocities = v
+    g.step(g.old_positions, 1e-3)
+    assert g.last_iterations == 7 and np.isnan(g.last_metric)
+
+
+def test_all_pinned_converges_in_one_iteration():
+    sc = cantilever((3, 2, 2))
+    k = sc.solver_kwargs()
+    k["active_mask"] = np.zeros(sc.n_vertices, bool)
+    g = VBDSolver(**k, dx_tol=1e-7, max_iter=10, verbose=False)
+    x = g.step(g.old_positions, 1e-3).numpy()
+    assert g.last_iterations == 1 and np.array_equal(x, sc.vertices)
+
+
+def test_errors_are_reported():
+    sc = cantilever((2, 1, 1))
+    bad = sc.elements.copy()
+    bad[0, 0] = sc.n_vertices
+    with pytest.raises(VBDError, match="EMESH"):
+        VBDSolver(sc.vertices, bad, 1000.0, 1e5, 0.4, verbose=False)
+    flat = np.zeros((4, 3))
+    with pytest.raises(VBDError, match="EDEGENERATE"):
+        VBDSolver(flat, np.array([[0, 1, 2, 3]]), 1000.0, 1e5, 0.4, verbose=False)
+    rng = np.random.default_rng(1)
+    X = np.vstack([np.zeros(3), rng.standard_normal((99, 3))])
+    tets = np.array([[0, 3 * i + 1, 3 * i + 2, 3 * i + 3] for i in range(33)])
+    with pytest.raises(VBDError, match="EVALENCE"):
+        VBDSolver(X, tets, 1000.0, 1e5, 0.4, verbose=False)
+    g = VBDSolver(**sc.solver_kwargs(), verbose=False)
+    with pytest.raises(VBDError, match="EINVAL"):
+        g.step(g.old_positions, 0.0)
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32])
+def test_launch_shapes_agree(lanes):
+    sc = cantilever((24, 4, 4))
+    g, r = make(sc)
+    g.set_launch(lanes, 256, 64, 0)  # small threshold: big colours use the per-colour kernel
+    for _ in range(3):
+        x = g.step(g.old_positions, 1e-3, dx_tol=0.0, max_iter=8).numpy()
+        xr = r.step(1e-3, dx_tol=0.0, max_iter=8)
+    assert np.abs(x - xr).max() < 1e-6 * diag(sc)

@@ -1,0 +1,355 @@
+// vbd_capi.hip -- extern "C" entry points declared in include/warpvbd.h.
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "vbd_internal.h"
+
+namespace vbd {
+namespace {
+thread_local std::string g_err;
+}
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace vbd
+
+using namespace vbd;
+
+#define CHECK_ARG(cond, msg)                                                                   \
+    do {                                                                                       \
+        if (!(cond)) {                                                                         \
+            set_error(msg);                                                                    \
+            return VBD_EINVAL;                                                                 \
+        }                                                                                      \
+    } while (0)
+
+extern "C" {
+
+const char *vbd_last_error(void) { return g_err.c_str(); }
+
+void vbd_destroy(vbd_solver_t *s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
+                   s->active, s->slot_off, s->slots, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
+                   s->xio,  s->d_col_start};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    for (float4 *p : s->snap)
+        if (p) (void)hipFree(p);
+    if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
+    if (s->xio_host) (void)hipHostFree(s->xio_host);
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int vbd_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, int32_t n_tets,
+               const double *density, const double *youngs, const double *poisson, const uint8_t *active,
+               const double *gravity, double damping, int32_t material, vbd_solver_t **out)
+{
+    CHECK_ARG(out, "out is NULL");
+    *out = nullptr;
+    CHECK_ARG(n_vertices > 0 && n_tets >= 0, "n_vertices must be > 0 and n_tets >= 0");
+    CHECK_ARG(rest_xyz && gravity, "rest_xyz and gravity are required");
+    CHECK_ARG(n_tets == 0 || (tets && density && youngs && poisson), "element arrays are required");
+    CHECK_ARG(material == VBD_MATERIAL_SNH || material == VBD_MATERIAL_STVK, "unknown material");
+    CHECK_ARG((int64_t)n_tets * 4 < (int64_t)1 << 31, "too many elements");
+    for (int64_t i = 0; i < (int64_t)4 * n_tets; ++i) {
+        if (tets[i] < 0 || tets[i] >= n_vertices) {
+            set_error("element " + std::to_string(i / 4) + " references vertex " + std::to_string(tets[i]) +
+                      " outside [0, " + std::to_string(n_vertices) + ")");
+            return VBD_EMESH;
+        }
+    }
+    vbd_solver *s = new vbd_solver();
+    s->n = n_vertices;
+    s->m = n_tets;
+    s->material = material;
+    s->damping = damping;
+    for (int k = 0; k < 3; ++k) s->grav[k] = gravity[k];
+    int r = 0;
+    if (hipGetDevice(&s->device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("no HIP device available");
+        delete s;
+        return VBD_EHIP;
+    }
+    r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active);
+    if (r != 0) {
+        vbd_destroy(s);
+        return r;
+    }
+    *out = s;
+    return 0;
+}
+
+int vbd_reset(vbd_solver_t *s)
+{
+    CHECK_ARG(s, "null handle");
+    VBD_HIP(hipSetDevice(s->device));
+    VBD_HIP(hipMemsetAsync(s->uprev, 0, sizeof(float4) * s->n, s->stream));
+    VBD_HIP(hipMemsetAsync(s->vel, 0, sizeof(float4) * s->n, s->stream));
+    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(float4) * s->n, s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    s->last_iters = 0;
+    return 0;
+}
+
+int vbd_step(vbd_solver_t *s, const double *positions, double dt, double tol, int32_t max_iter, double *out_xyz,
+             int32_t *iters_out, double *metric_out)
+{
+    CHECK_ARG(s, "null handle");
+    CHECK_ARG(dt > 0.0, "dt must be > 0");
+    CHECK_ARG(max_iter >= 1, "max_iter must be >= 1");
+    VBD_HIP(hipSetDevice(s->device));
+    if (positions) VBD_TRY(import_positions(s, positions, s->uprev));
+    int it = 0;
+    double met = 0.0;
+    VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
+    if (out_xyz) VBD_TRY(export_positions(s, s->u, out_xyz));
+    if (iters_out) *iters_out = it;
+    if (metric_out) *metric_out = met;
+    return 0;
+}
+
+int vbd_get_state(vbd_solver_t *s, double *x_prev, double *v_prev)
+{
+    CHECK_ARG(s, "null handle");
+    VBD_HIP(hipSetDevice(s->device));
+    if (x_prev) VBD_TRY(export_positions(s, s->uprev, x_prev));
+    if (v_prev) VBD_TRY(export_velocities(s, v_prev));
+    return 0;
+}
+
+int vbd_set_state(vbd_solver_t *s, const double *x_prev, const double *v_prev)
+{
+    CHECK_ARG(s, "null handle");
+    VBD_HIP(hipSetDevice(s->device));
+    if (x_prev) VBD_TRY(import_positions(s, x_prev, s->uprev));
+    if (v_prev) VBD_TRY(import_velocities(s, v_prev));
+    return 0;
+}
+
+int vbd_snapshot(vbd_solver_t *s, int32_t slot)
+{
+    CHECK_ARG(s, "null handle");
+    CHECK_ARG(slot >= 0 && slot < VBD_MAX_SNAPSHOTS, "snapshot slot out of range");
+    VBD_HIP(hipSetDevice(s->device));
+    if (!s->snap[slot]) VBD_HIP(hipMalloc((void **)&s->snap[slot], sizeof(float4) * s->n));
+    VBD_HIP(hipMemcpyAsync(s->snap[slot], s->uprev, sizeof(float4) * s->n, hipMemcpyDeviceToDevice, s->stream));
+    return 0;
+}
+
+int vbd_read_snapshot(vbd_solver_t *s, int32_t slot, double *out_xyz)
+{
+    CHECK_ARG(s && out_xyz, "null argument");
+    CHECK_ARG(slot >= 0 && slot < VBD_MAX_SNAPSHOTS && s->snap[slot], "empty snapshot slot");
+    VBD_HIP(hipSetDevice(s->device));
+    return export_positions(s, s->snap[slot], out_xyz);
+}
+
+int vbd_load_snapshot(vbd_solver_t *s, int32_t slot)
+{
+    CHECK_ARG(s, "null handle");
+    CHECK_ARG(slot >= 0 && slot < VBD_MAX_SNAPSHOTS && s->snap[slot], "empty snapshot slot");
+    VBD_HIP(hipSetDevice(s->device));
+    VBD_HIP(hipMemcpyAsync(s->uprev, s->snap[slot], sizeof(float4) * s->n, hipMemcpyDeviceToDevice, s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int vbd_info(vbd_solver_t *s, int32_t *n_vertices, int32_t *n_tets, int32_t *n_colors, int32_t *max_incident,
+             int32_t *max_degree, int64_t *n_slots)
+{
+    CHECK_ARG(s, "null handle");
+    if (n_vertices) *n_vertices = s->n;
+    if (n_tets) *n_tets = s->m;
+    if (n_colors) *n_colors = s->ncolors;
+    if (max_incident) *max_incident = s->max_incident;
+    if (max_degree) *max_degree = s->max_degree;
+    if (n_slots) *n_slots = s->n_slots;
+    return 0;
+}
+
+int vbd_get_colors(vbd_solver_t *s, int32_t *colors)
+{
+    CHECK_ARG(s && colors, "null argument");
+    VBD_HIP(hipSetDevice(s->device));
+    VBD_HIP(hipMemcpyAsync(colors, s->colors, sizeof(int32_t) * s->n, hipMemcpyDeviceToHost, s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int vbd_get_color_sizes(vbd_solver_t *s, int32_t *sizes)
+{
+    CHECK_ARG(s && sizes, "null argument");
+    for (int c = 0; c < s->ncolors; ++c) sizes[c] = s->col_size[c];
+    return 0;
+}
+
+int vbd_get_incidence(vbd_solver_t *s, int32_t *adj)
+{
+    CHECK_ARG(s && adj, "null argument");
+    VBD_HIP(hipSetDevice(s->device));
+    std::vector<int32_t> off(s->n + 1), inc(std::max(1, 4 * s->m));
+    VBD_HIP(hipMemcpyAsync(off.data(), s->inc_off, sizeof(int32_t) * (s->n + 1), hipMemcpyDeviceToHost, s->stream));
+    if (s->m)
+        VBD_HIP(hipMemcpyAsync(inc.data(), s->inc, sizeof(int32_t) * 4 * s->m, hipMemcpyDeviceToHost, s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    const int W = s->max_incident;
+    for (int v = 0; v < s->n; ++v) {
+        int k = 0;
+        for (int q = off[v]; q < off[v + 1]; ++q, ++k) adj[(size_t)v * W + k] = inc[q] >> 2;
+        for (; k < W; ++k) adj[(size_t)v * W + k] = -1;
+    }
+    return 0;
+}
+
+int vbd_color(const int32_t *tets, int32_t n_tets, int32_t n_vertices, int32_t *colors, int32_t *n_colors_out,
+              int32_t *max_degree_out)
+{
+    CHECK_ARG(tets && colors && n_vertices > 0 && n_tets >= 0, "bad arguments");
+    for (int64_t i = 0; i < (int64_t)4 * n_tets; ++i) {
+        if (tets[i] < 0 || tets[i] >= n_vertices) {
+            set_error("element index outside [0, n_vertices)");
+            return VBD_EMESH;
+        }
+    }
+    hipStream_t st;
+    VBD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int32_t *d_tets = nullptr, *d_col = nullptr, *d_off = nullptr, *d_inc = nullptr;
+    int rc = 0, nc = 0, md = 0;
+    auto cleanup = [&]() {
+        if (d_tets) (void)hipFree(d_tets);
+        if (d_col) (void)hipFree(d_col);
+        if (d_off) (void)hipFree(d_off);
+        if (d_inc) (void)hipFree(d_inc);
+        (void)hipStreamDestroy(st);
+    };
+    if (hipMalloc((void **)&d_tets, sizeof(int32_t) * std::max(1, 4 * n_tets)) != hipSuccess ||
+        hipMalloc((void **)&d_col, sizeof(int32_t) * n_vertices) != hipSuccess ||
+        hipMalloc((void **)&d_off, sizeof(int32_t) * (n_vertices + 1)) != hipSuccess ||
+        hipMalloc((void **)&d_inc, sizeof(int32_t) * std::max(1, 4 * n_tets)) != hipSuccess) {
+        cleanup();
+        set_error("hipMalloc failed");
+        return VBD_ENOMEM;
+    }
+    if (n_tets && hipMemcpyAsync(d_tets, tets, sizeof(int32_t) * 4 * n_tets, hipMemcpyHostToDevice, st) != hipSuccess) {
+        cleanup();
+        set_error("hipMemcpy failed");
+        return VBD_EHIP;
+    }
+    rc = color_tets_device(d_tets, n_tets, n_vertices, d_col, d_off, d_inc, &nc, &md, st);
+    if (rc == 0) {
+        if (hipMemcpyAsync(colors, d_col, sizeof(int32_t) * n_vertices, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            set_error("hipMemcpy failed");
+            rc = VBD_EHIP;
+        }
+    }
+    cleanup();
+    if (rc == 0) {
+        if (n_colors_out) *n_colors_out = nc;
+        if (max_degree_out) *max_degree_out = md;
+    }
+    return rc;
+}
+
+int vbd_color_graph(const int32_t *adj_off, const int32_t *adj, int32_t n_vertices, int32_t *colors,
+                    int32_t *n_colors_out)
+{
+    CHECK_ARG(adj_off && colors && n_vertices > 0, "bad arguments");
+    const int64_t ne = adj_off[n_vertices];
+    CHECK_ARG(ne >= 0 && (ne == 0 || adj), "bad adjacency offsets");
+    for (int v = 0; v < n_vertices; ++v) CHECK_ARG(adj_off[v] <= adj_off[v + 1], "adjacency offsets must not decrease");
+    hipStream_t st;
+    VBD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int32_t *d_off = nullptr, *d_adj = nullptr, *d_col = nullptr;
+    auto cleanup = [&]() {
+        if (d_off) (void)hipFree(d_off);
+        if (d_adj) (void)hipFree(d_adj);
+        if (d_col) (void)hipFree(d_col);
+        (void)hipStreamDestroy(st);
+    };
+    if (hipMalloc((void **)&d_off, sizeof(int32_t) * (n_vertices + 1)) != hipSuccess ||
+        hipMalloc((void **)&d_adj, sizeof(int32_t) * std::max<int64_t>(1, ne)) != hipSuccess ||
+        hipMalloc((void **)&d_col, sizeof(int32_t) * n_vertices) != hipSuccess) {
+        cleanup();
+        set_error("hipMalloc failed");
+        return VBD_ENOMEM;
+    }
+    int rc = 0, nc = 0;
+    if (hipMemcpyAsync(d_off, adj_off, sizeof(int32_t) * (n_vertices + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        (ne && hipMemcpyAsync(d_adj, adj, sizeof(int32_t) * ne, hipMemcpyHostToDevice, st) != hipSuccess)) {
+        cleanup();
+        set_error("hipMemcpy failed");
+        return VBD_EHIP;
+    }
+    rc = color_graph_device(d_off, d_adj, n_vertices, d_col, &nc, st);
+    if (rc == 0 && (hipMemcpyAsync(colors, d_col, sizeof(int32_t) * n_vertices, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess)) {
+        set_error("hipMemcpy failed");
+        rc = VBD_EHIP;
+    }
+    cleanup();
+    if (rc == 0 && n_colors_out) *n_colors_out = nc;
+    return rc;
+}
+
+int vbd_use_device(int32_t device)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("no HIP device available");
+        return VBD_EHIP;
+    }
+    CHECK_ARG(device >= 0 && device < count, "device index out of range");
+    VBD_HIP(hipSetDevice(device));
+    return 0;
+}
+
+int vbd_profile_enable(vbd_solver_t *s, int32_t enable, int32_t min_vertices)
+{
+    CHECK_ARG(s, "null handle");
+    s->prof = enable != 0;
+    s->prof_min = min_vertices;
+    s->prof_ms = s->prof_bytes = 0.0;
+    s->prof_launches = 0;
+    return 0;
+}
+
+int vbd_profile_read(vbd_solver_t *s, double *ms, int64_t *launches, double *bytes)
+{
+    CHECK_ARG(s, "null handle");
+    if (ms) *ms = s->prof_ms;
+    if (launches) *launches = s->prof_launches;
+    if (bytes) *bytes = s->prof_bytes;
+    s->prof_ms = s->prof_bytes = 0.0;
+    s->prof_launches = 0;
+    return 0;
+}
+
+int vbd_color_bytes(vbd_solver_t *s, double *bytes)
+{
+    CHECK_ARG(s && bytes, "null argument");
+    for (int c = 0; c < s->ncolors; ++c) bytes[c] = s->col_bytes[c];
+    return 0;
+}
+
+int vbd_set_launch(vbd_solver_t *s, int32_t lanes, int32_t block, int32_t small_color, int32_t chunk)
+{
+    CHECK_ARG(s, "null handle");
+    CHECK_ARG(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32,
+              "lanes_per_vertex must be 1, 2, 4, 8, 16 or 32");
+    CHECK_ARG(block >= 64 && block <= 1024 && block % 64 == 0, "block_size must be a multiple of 64 in [64, 1024]");
+    CHECK_ARG(small_color >= 0 && chunk >= 0, "small_color and chunk must be >= 0");
+    s->launch.lanes = lanes;
+    s->launch.block = block;
+    s->launch.small_color = small_color;
+    s->launch.chunk = chunk;
+    plan_segments(s);
+    return 0;
+}
+
+}  // extern "C"

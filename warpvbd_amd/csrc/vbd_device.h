@@ -1,0 +1,202 @@
+// vbd_device.h -- per-slot VBD physics, 3x3 solve and lane-group reductions (gfx950).
+//
+// A "slot" is one (vertex, incident tet) pair.  Slots of a vertex are stored
+// contiguously in colour-major vertex order, one 64-byte record each:
+//
+//   q0 = (o0, o1, o2 as int bits, ka)   ids of the tet's other three vertices
+//   q1 = (bs.xyz, kb)                   shape gradient b of the slot's own vertex
+//   q2 = (b0.xyz, kc)                   b of vertex o0
+//   q3 = (b1.xyz, 0)                    b of vertex o1 ; b of o2 = -(bs+b0+b1)
+//
+// b is the row of invDm belonging to the vertex's local index (minus the row sum
+// for local index 3), so dF/dx_{v,i} = e_i b^T (src/vbd_solver.py:86-94,450-465).
+// Material constants, volume folded in:
+//   SNH : ka = V mu', kb = V lambda', kc = 0.75 ka  (= V lambda'(alpha-1))
+//   StVK: ka = V mu,  kb = V lambda,  kc = 0
+//
+// Positions are carried as FP32 displacements u = x - X from the rest shape, so
+// F = I + D with D = sum_i (u_i - u_self) b_i^T is formed from small
+// differences; FP32 absolute coordinates cannot resolve a 1e-7 m tolerance on a
+// 2.56 m beam.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vbd {
+
+enum Material : int { kSNH = 0, kStVK = 1 };
+
+struct Acc {
+    float g0, g1, g2;
+    float h00, h01, h02, h11, h12, h22;
+};
+
+__device__ __forceinline__ void acc_zero(Acc &a)
+{
+    a.g0 = a.g1 = a.g2 = 0.f;
+    a.h00 = a.h01 = a.h02 = a.h11 = a.h12 = a.h22 = 0.f;
+}
+
+// Elastic gradient and Hessian block of one slot, added into a.
+//   d0, d1, d2: u_o0 - u_self, u_o1 - u_self, u_o2 - u_self.
+// SNH (src/vbd_solver.py:101-124) in the equivalent compact form
+//   g = V P b,  P = mu'(1 - 1/(Ic+1)) F + lambda'(J - alpha) cof(F)
+//   H = V[2mu'/(Ic+1)^2 (Fb)(Fb)^T + mu'(1 - 1/(Ic+1))|b|^2 I + lambda'(cof(F) b)(cof(F) b)^T]
+// StVK (src/vbd_solver.py:127-144):
+//   g = V F S b, S = lambda tr(E) I + 2 mu E
+//   H = V[(lambda+mu)(Fb)(Fb)^T + mu |b|^2 F F^T + (lambda tr(E)|b|^2 + 2 mu b^T E b) I]
+template <int MAT>
+__device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const float3 bs, const float3 b0,
+                                          const float3 b1, const float3 d0, const float3 d1, const float3 d2,
+                                          Acc &a)
+{
+    const float3 b2 = make_float3(-(bs.x + b0.x + b1.x), -(bs.y + b0.y + b1.y), -(bs.z + b0.z + b1.z));
+    // D = d0 b0^T + d1 b1^T + d2 b2^T  (row r, column c)
+    const float D00 = d0.x * b0.x + d1.x * b1.x + d2.x * b2.x;
+    const float D01 = d0.x * b0.y + d1.x * b1.y + d2.x * b2.y;
+    const float D02 = d0.x * b0.z + d1.x * b1.z + d2.x * b2.z;
+    const float D10 = d0.y * b0.x + d1.y * b1.x + d2.y * b2.x;
+    const float D11 = d0.y * b0.y + d1.y * b1.y + d2.y * b2.y;
+    const float D12 = d0.y * b0.z + d1.y * b1.z + d2.y * b2.z;
+    const float D20 = d0.z * b0.x + d1.z * b1.x + d2.z * b2.x;
+    const float D21 = d0.z * b0.y + d1.z * b1.y + d2.z * b2.y;
+    const float D22 = d0.z * b0.z + d1.z * b1.z + d2.z * b2.z;
+    const float F00 = 1.f + D00, F11 = 1.f + D11, F22 = 1.f + D22;
+    const float F01 = D01, F02 = D02, F10 = D10, F12 = D12, F20 = D20, F21 = D21;
+    const float bb = bs.x * bs.x + bs.y * bs.y + bs.z * bs.z;
+    const float Fb0 = F00 * bs.x + F01 * bs.y + F02 * bs.z;
+    const float Fb1 = F10 * bs.x + F11 * bs.y + F12 * bs.z;
+    const float Fb2 = F20 * bs.x + F21 * bs.y + F22 * bs.z;
+    if constexpr (MAT == kSNH) {
+        const float Ic = F00 * F00 + F01 * F01 + F02 * F02 + F10 * F10 + F11 * F11 + F12 * F12 + F20 * F20 +
+                         F21 * F21 + F22 * F22;
+        // cofactor matrix of F (cof(F) = J F^-T)
+        const float C00 = F11 * F22 - F12 * F21, C01 = F12 * F20 - F10 * F22, C02 = F10 * F21 - F11 * F20;
+        const float C10 = F02 * F21 - F01 * F22, C11 = F00 * F22 - F02 * F20, C12 = F01 * F20 - F00 * F21;
+        const float C20 = F01 * F12 - F02 * F11, C21 = F02 * F10 - F00 * F12, C22 = F00 * F11 - F01 * F10;
+        // J - 1 = tr D + (principal 2x2 minors of D) + det D, accurate for small D
+        const float m2 = (D00 * D11 - D01 * D10) + (D00 * D22 - D02 * D20) + (D11 * D22 - D12 * D21);
+        const float detD = D00 * (D11 * D22 - D12 * D21) - D01 * (D10 * D22 - D12 * D20) +
+                           D02 * (D10 * D21 - D11 * D20);
+        const float Jm1 = (D00 + D11 + D22) + m2 + detD;
+        const float cb0 = C00 * bs.x + C01 * bs.y + C02 * bs.z;
+        const float cb1 = C10 * bs.x + C11 * bs.y + C12 * bs.z;
+        const float cb2 = C20 * bs.x + C21 * bs.y + C22 * bs.z;
+        const float r = 1.f / (Ic + 1.f);
+        const float s1 = ka * (Ic * r);        // V mu'(1 - 1/(Ic+1))
+        const float s2 = 2.f * ka * r * r;     // V 2mu'/(Ic+1)^2
+        const float sj = kb * Jm1 - kc;        // V lambda'(J - alpha)
+        a.g0 += s1 * Fb0 + sj * cb0;
+        a.g1 += s1 * Fb1 + sj * cb1;
+        a.g2 += s1 * Fb2 + sj * cb2;
+        const float sd = s1 * bb;
+        a.h00 += s2 * Fb0 * Fb0 + sd + kb * cb0 * cb0;
+        a.h01 += s2 * Fb0 * Fb1 + kb * cb0 * cb1;
+        a.h02 += s2 * Fb0 * Fb2 + kb * cb0 * cb2;
+        a.h11 += s2 * Fb1 * Fb1 + sd + kb * cb1 * cb1;
+        a.h12 += s2 * Fb1 * Fb2 + kb * cb1 * cb2;
+        a.h22 += s2 * Fb2 * Fb2 + sd + kb * cb2 * cb2;
+    } else {
+        // E = (D + D^T + D^T D) / 2
+        const float E00 = D00 + 0.5f * (D00 * D00 + D10 * D10 + D20 * D20);
+        const float E11 = D11 + 0.5f * (D01 * D01 + D11 * D11 + D21 * D21);
+        const float E22 = D22 + 0.5f * (D02 * D02 + D12 * D12 + D22 * D22);
+        const float E01 = 0.5f * (D01 + D10 + D00 * D01 + D10 * D11 + D20 * D21);
+        const float E02 = 0.5f * (D02 + D20 + D00 * D02 + D10 * D12 + D20 * D22);
+        const float E12 = 0.5f * (D12 + D21 + D01 * D02 + D11 * D12 + D21 * D22);
+        const float trE = E00 + E11 + E22;
+        const float Eb0 = E00 * bs.x + E01 * bs.y + E02 * bs.z;
+        const float Eb1 = E01 * bs.x + E11 * bs.y + E12 * bs.z;
+        const float Eb2 = E02 * bs.x + E12 * bs.y + E22 * bs.z;
+        const float bEb = bs.x * Eb0 + bs.y * Eb1 + bs.z * Eb2;
+        // w = S b (V folded)
+        const float w0 = kb * trE * bs.x + 2.f * ka * Eb0;
+        const float w1 = kb * trE * bs.y + 2.f * ka * Eb1;
+        const float w2 = kb * trE * bs.z + 2.f * ka * Eb2;
+        a.g0 += F00 * w0 + F01 * w1 + F02 * w2;
+        a.g1 += F10 * w0 + F11 * w1 + F12 * w2;
+        a.g2 += F20 * w0 + F21 * w1 + F22 * w2;
+        const float kab = ka + kb, kbb = ka * bb;
+        const float sd = kb * trE * bb + 2.f * ka * bEb;
+        a.h00 += kab * Fb0 * Fb0 + kbb * (F00 * F00 + F01 * F01 + F02 * F02) + sd;
+        a.h01 += kab * Fb0 * Fb1 + kbb * (F00 * F10 + F01 * F11 + F02 * F12);
+        a.h02 += kab * Fb0 * Fb2 + kbb * (F00 * F20 + F01 * F21 + F02 * F22);
+        a.h11 += kab * Fb1 * Fb1 + kbb * (F10 * F10 + F11 * F11 + F12 * F12) + sd;
+        a.h12 += kab * Fb1 * Fb2 + kbb * (F10 * F20 + F11 * F21 + F12 * F22);
+        a.h22 += kab * Fb2 * Fb2 + kbb * (F20 * F20 + F21 * F21 + F22 * F22) + sd;
+    }
+}
+
+// Butterfly sum over aligned groups of L lanes (L a power of two <= 64).
+template <int L>
+__device__ __forceinline__ float group_sum(float x)
+{
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, L);
+    return x;
+}
+
+template <int L>
+__device__ __forceinline__ void acc_group_sum(Acc &a)
+{
+    if constexpr (L > 1) {
+        a.g0 = group_sum<L>(a.g0);
+        a.g1 = group_sum<L>(a.g1);
+        a.g2 = group_sum<L>(a.g2);
+        a.h00 = group_sum<L>(a.h00);
+        a.h01 = group_sum<L>(a.h01);
+        a.h02 = group_sum<L>(a.h02);
+        a.h11 = group_sum<L>(a.h11);
+        a.h12 = group_sum<L>(a.h12);
+        a.h22 = group_sum<L>(a.h22);
+    }
+}
+
+// Solve H dx = -G for a symmetric 3x3 block by the adjugate.  A block whose
+// determinant is finite but tiny relative to its diagonal product is skipped
+// (dx = 0): this is the guard for vertices without incident tets and for
+// indefinite StVK blocks.  NaN or Inf in G or H propagates into dx.
+__device__ __forceinline__ float3 solve3_guarded(const Acc &a)
+{
+    const float c00 = a.h11 * a.h22 - a.h12 * a.h12;
+    const float c01 = a.h02 * a.h12 - a.h01 * a.h22;
+    const float c02 = a.h01 * a.h12 - a.h02 * a.h11;
+    const float c11 = a.h00 * a.h22 - a.h02 * a.h02;
+    const float c12 = a.h01 * a.h02 - a.h00 * a.h12;
+    const float c22 = a.h00 * a.h11 - a.h01 * a.h01;
+    const float det = a.h00 * c00 + a.h01 * c01 + a.h02 * c02;
+    const float scale = fabsf(a.h00 * a.h11 * a.h22);
+    if (fabsf(det) <= 1e-6f * scale && det == det && fabsf(det) != INFINITY &&
+        a.g0 == a.g0 && a.g1 == a.g1 && a.g2 == a.g2)
+        return make_float3(0.f, 0.f, 0.f);
+    const float id = -1.f / det;
+    return make_float3(id * (c00 * a.g0 + c01 * a.g1 + c02 * a.g2), id * (c01 * a.g0 + c11 * a.g1 + c12 * a.g2),
+                       id * (c02 * a.g0 + c12 * a.g1 + c22 * a.g2));
+}
+
+__device__ __forceinline__ float wave_max(float x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+// |dx_x + dx_y + dx_z| as the ordered bit pattern of a non-negative float (NaN
+// sorts above +Inf, so an unsigned max keeps it).  src/vbd_solver.py:578.
+__device__ __forceinline__ uint32_t metric_bits(float3 dx)
+{
+    float s = fabsf(dx.x + dx.y + dx.z);
+    return __float_as_uint(s);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t y = __shfl_xor(x, off, 64);
+        x = x > y ? x : y;
+    }
+    return x;
+}
+
+} // namespace vbd

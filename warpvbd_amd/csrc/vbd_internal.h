@@ -1,0 +1,129 @@
+// vbd_internal.h -- solver state shared by the setup, step and C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/warpvbd.h"
+
+namespace vbd {
+
+constexpr int kMaxColors = 256;            // src/coloring.py:50
+constexpr int kMaxElementsPerVertex = 32;  // src/vbd_solver.py:12
+
+void set_error(const std::string &msg);
+
+#define VBD_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            ::vbd::set_error(std::string(#call) + ": " + hipGetErrorString(e_));               \
+            return VBD_EHIP;                                                                   \
+        }                                                                                      \
+    } while (0)
+
+#define VBD_TRY(expr)                                                                          \
+    do {                                                                                       \
+        int r_ = (expr);                                                                       \
+        if (r_ != 0) return r_;                                                                \
+    } while (0)
+
+// Per-step control block on the device.
+struct Ctrl {
+    int32_t it;         // index of the iteration being run
+    int32_t converged;  // 1 once an iteration's metric fell below tol
+    int32_t iters;      // iterations executed (index + 1 of the last one)
+    uint32_t metric;    // bits of the last iteration's metric
+};
+
+struct Launch {
+    int lanes = 4;        // lanes per vertex
+    int block = 256;      // threads per block
+    int small_color = 2048;  // colours below this many vertices run fused in one workgroup
+    int chunk = 0;        // iterations between host waits (0 = adaptive)
+};
+
+// A run of consecutive colours executed by one launch.
+struct Segment {
+    int c0, c1;    // colour range [c0, c1)
+    bool fused;    // one workgroup, barrier between colours
+    int part_off;  // first per-block metric partial of this launch
+    int nparts;    // blocks of this launch
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace vbd
+
+struct vbd_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n = 0, m = 0, material = 0;
+    double damping = 0.0;
+    double grav[3] = {0, 0, -9.81};
+
+    // mesh facts
+    int ncolors = 0, max_incident = 0, max_degree = 0, n_ref = 0;
+    int64_t n_slots = 0;
+    std::vector<int> col_start;   // colour-major start of each colour's active range (size ncolors+1)
+    std::vector<int> col_size;    // active vertices per colour
+    std::vector<double> col_bytes;  // algorithmic bytes of one sweep of each colour
+    std::vector<vbd::Segment> segments;
+    int32_t *d_col_start = nullptr;  // device copy of col_start
+    float kd_dt = 0.f;               // damping / dt of the running step
+    int n_swept = 0;              // active, coloured vertices (prefix of the colour-major order)
+
+    // device arrays (colour-major order unless noted)
+    int32_t *perm = nullptr;      // new -> original
+    int32_t *iperm = nullptr;     // original -> new
+    double *X = nullptr;          // rest positions, original order, f64 [n*3]
+    float4 *u = nullptr;          // displacement x - X
+    float4 *uprev = nullptr;      // displacement at the start of the step
+    float4 *vel = nullptr;        // velocity
+    float4 *tgt = nullptr;        // xyz: inertia target (x_prev + dt v + dt^2 g) - X ; w: M_v / dt^2
+    float *mass = nullptr;        // lumped vertex mass sum_e m_e / 4
+    uint8_t *active = nullptr;    // active flag
+    int32_t *slot_off = nullptr;  // [n_swept+1]
+    float4 *slots = nullptr;      // [n_slots*4]
+    int32_t *colors = nullptr;    // original order
+    int32_t *inc_off = nullptr;   // original order vertex -> incident (tet<<2|local), ascending tet
+    int32_t *inc = nullptr;
+    uint32_t *part = nullptr;     // per-block metric maxima of one iteration
+    int part_cap = 0, n_parts = 0;
+    vbd::Ctrl *ctrl = nullptr;
+    vbd::Ctrl *ctrl_host = nullptr;  // pinned
+    double *xio = nullptr;        // f64 [n*3] staging for host transfers (device)
+    double *xio_host = nullptr;   // pinned f64 [n*3]
+    float4 *snap[VBD_MAX_SNAPSHOTS] = {};
+
+    vbd::Launch launch;
+    int last_iters = 0;
+
+    // profiling
+    bool prof = false;
+    int prof_min = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;  // (event index, colour)
+    double prof_ms = 0.0, prof_bytes = 0.0;
+    int64_t prof_launches = 0;
+};
+
+namespace vbd {
+int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const double *density,
+               const double *youngs, const double *poisson, const uint8_t *active);
+int color_tets_device(const int32_t *d_tets, int m, int n, int32_t *d_colors, int32_t *d_inc_off,
+                      int32_t *d_inc, int *ncolors, int *max_degree, hipStream_t stream);
+int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_t *d_colors, int *ncolors,
+                       hipStream_t stream);
+int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric);
+int export_positions(vbd_solver *s, const float4 *src, double *host_out);
+int import_positions(vbd_solver *s, const double *host_in, float4 *dst);
+int import_velocities(vbd_solver *s, const double *host_in);
+int export_velocities(vbd_solver *s, double *host_out);
+void plan_segments(vbd_solver *s);
+}  // namespace vbd

@@ -1,0 +1,661 @@
+// vbd_setup.hip -- device-side mesh preprocessing for the VBD solver.
+//
+// Replaces VBDSolver.__init__ (src/vbd_solver.py:367-499) and the colouring of
+// src/coloring.py:24-72.  Everything below runs on the GPU:
+//   1. per-tet Lame parameters, rest shape inverse, volume, mass (FP64)
+//        src/vbd_solver.py:295-362
+//   2. vertex -> incident tet CSR, ascending tet order per vertex
+//        src/vbd_solver.py:485-498
+//   3. first-fit colouring, bit-exact with the sequential reference: the
+//      recurrence colour[v] = mex{colour[u] : u ~ v, u < v} is iterated in
+//      parallel until a round changes nothing; its fixed point is unique (by
+//      induction on v) and equals the sequential result.
+//   4. colour-major renumbering (active vertices of colour 0 ascending, then
+//      colour 1, ..., then inactive/uncoloured vertices), so each colour sweep
+//      is one contiguous range
+//   5. 64-byte slot records in colour-major order (layout in vbd_device.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "vbd_device.h"
+#include "vbd_internal.h"
+
+namespace vbd {
+
+namespace {
+
+constexpr int kTB = 256;
+
+inline int nblk(int64_t n, int tb = kTB) { return (int)((n + tb - 1) / tb); }
+
+template <typename T>
+int dmalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return VBD_ENOMEM;
+    }
+    return 0;
+}
+
+struct Scratch {
+    std::vector<void *> ptrs;
+    ~Scratch()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    int get(T **p, size_t count)
+    {
+        int r = dmalloc(p, count);
+        if (r == 0) ptrs.push_back((void *)*p);
+        return r;
+    }
+};
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// incidence counts and the largest referenced vertex
+__global__ void k_count_incidence(const int32_t *tets, int m, int32_t *cnt, int32_t *maxref)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    int mx = -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int v = tets[4 * e + k];
+        atomicAdd(&cnt[v], 1);
+        mx = max(mx, v);
+    }
+    atomicMax(maxref, mx);
+}
+
+__global__ void k_fill_incidence(const int32_t *tets, int m, int32_t *cursor, int32_t *inc)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int v = tets[4 * e + k];
+        int pos = atomicAdd(&cursor[v], 1);
+        inc[pos] = (e << 2) | k;
+    }
+}
+
+// Sort each vertex's incidence list by tet id (the reference lists incident
+// elements in ascending index, src/vbd_solver.py:488-491) and record the
+// largest list.
+__global__ void k_sort_incidence(const int32_t *off, int32_t *inc, int n, int32_t *maxinc)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int a = off[v], b = off[v + 1];
+    for (int i = a + 1; i < b; ++i) {
+        int x = inc[i];
+        int j = i - 1;
+        while (j >= a && inc[j] > x) {
+            inc[j + 1] = inc[j];
+            --j;
+        }
+        inc[j + 1] = x;
+    }
+    atomicMax(maxinc, b - a);
+}
+
+// Neighbour-set size of each vertex (compute_adjacency_dict's degree,
+// src/coloring.py:24-38, including the self entry a degenerate element adds).
+// A neighbour is counted at its first occurrence in the vertex's incidence list.
+__global__ void k_degree(const int32_t *tets, const int32_t *off, const int32_t *inc, int n, int32_t *maxdeg)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int a = off[v], b = off[v + 1];
+    int cnt = 0;
+    bool self = false;
+    for (int s = a; s < b; ++s) {
+        int e = inc[s] >> 2;
+        int rep = 0;
+        for (int k = 0; k < 4; ++k) {
+            int u = tets[4 * e + k];
+            if (u == v) {
+                ++rep;
+                continue;
+            }
+            bool seen = false;
+            for (int k2 = 0; k2 < k && !seen; ++k2) seen = tets[4 * e + k2] == u;
+            for (int s2 = a; s2 < s && !seen; ++s2) {
+                int e2 = inc[s2] >> 2;
+                for (int k2 = 0; k2 < 4; ++k2) seen |= tets[4 * e2 + k2] == u;
+            }
+            if (!seen) ++cnt;
+        }
+        self |= rep > 1;
+    }
+    atomicMax(maxdeg, cnt + (self ? 1 : 0));
+}
+
+__global__ void k_color_init(int32_t *colors, int n, int nref)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) colors[v] = v < nref ? 0 : -1;
+}
+
+// One parallel round of colour[v] = mex{colour[u] : u ~ v, u < v}.
+__global__ void k_color_round(const int32_t *tets, const int32_t *off, const int32_t *inc, int nref,
+                              int32_t *colors, int32_t *changed, int32_t *overflow)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nref) return;
+    uint32_t used[kMaxColors / 32];
+#pragma unroll
+    for (int i = 0; i < kMaxColors / 32; ++i) used[i] = 0;
+    int a = off[v], b = off[v + 1];
+    for (int s = a; s < b; ++s) {
+        int e = inc[s] >> 2;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int u = tets[4 * e + k];
+            if (u < v) {
+                int c = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
+                if (c >= 0 && c < kMaxColors) used[c >> 5] |= 1u << (c & 31);
+            }
+        }
+    }
+    int c = kMaxColors;
+    for (int w = 0; w < kMaxColors / 32; ++w) {
+        uint32_t freebits = ~used[w];
+        if (freebits) {
+            c = w * 32 + __builtin_ctz(freebits);
+            break;
+        }
+    }
+    if (c == kMaxColors) {
+        atomicExch(overflow, 1);
+        c = kMaxColors - 1;
+    }
+    if (c != colors[v]) {
+        __atomic_store_n(&colors[v], c, __ATOMIC_RELAXED);
+        atomicExch(changed, 1);
+    }
+}
+
+// The same round over an explicit adjacency list (graph_coloring's dict input,
+// src/coloring.py:40-72): only v's own list is consulted, and only neighbours
+// with a smaller index are already coloured when v is visited.
+__global__ void k_color_round_graph(const int32_t *off, const int32_t *adj, int n, int32_t *colors,
+                                    int32_t *changed, int32_t *overflow)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint32_t used[kMaxColors / 32];
+#pragma unroll
+    for (int i = 0; i < kMaxColors / 32; ++i) used[i] = 0;
+    for (int s = off[v]; s < off[v + 1]; ++s) {
+        int u = adj[s];
+        if (u >= 0 && u < v) {
+            int c = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
+            if (c >= 0 && c < kMaxColors) used[c >> 5] |= 1u << (c & 31);
+        }
+    }
+    int c = kMaxColors;
+    for (int w = 0; w < kMaxColors / 32; ++w) {
+        uint32_t freebits = ~used[w];
+        if (freebits) {
+            c = w * 32 + __builtin_ctz(freebits);
+            break;
+        }
+    }
+    if (c == kMaxColors) {
+        atomicExch(overflow, 1);
+        c = kMaxColors - 1;
+    }
+    if (c != colors[v]) {
+        __atomic_store_n(&colors[v], c, __ATOMIC_RELAXED);
+        atomicExch(changed, 1);
+    }
+}
+
+// Per-tet rest shape in FP64: Dm = [X0-X3 | X1-X3 | X2-X3], invDm, V = |det|/6,
+// m = rho V (src/vbd_solver.py:317-334); Lame mu, lambda (:356-362).
+__global__ void k_tet_setup(const double *X, const int32_t *tets, int m, const double *rho, const double *E,
+                            const double *nu, double *invDm, double *vol, double *mass, double *mu, double *lam,
+                            int32_t *singular)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    int t0 = tets[4 * e], t1 = tets[4 * e + 1], t2 = tets[4 * e + 2], t3 = tets[4 * e + 3];
+    double D[3][3];
+    for (int r = 0; r < 3; ++r) {
+        double x3 = X[3 * t3 + r];
+        D[r][0] = X[3 * t0 + r] - x3;
+        D[r][1] = X[3 * t1 + r] - x3;
+        D[r][2] = X[3 * t2 + r] - x3;
+    }
+    double c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+    double c01 = D[0][2] * D[2][1] - D[0][1] * D[2][2];
+    double c02 = D[0][1] * D[1][2] - D[0][2] * D[1][1];
+    double c10 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+    double c11 = D[0][0] * D[2][2] - D[0][2] * D[2][0];
+    double c12 = D[0][2] * D[1][0] - D[0][0] * D[1][2];
+    double c20 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+    double c21 = D[0][1] * D[2][0] - D[0][0] * D[2][1];
+    double c22 = D[0][0] * D[1][1] - D[0][1] * D[1][0];
+    double det = D[0][0] * c00 + D[1][0] * c01 + D[2][0] * c02;
+    if (!(det != 0.0) || !isfinite(det)) atomicExch(singular, 1);
+    double s = det != 0.0 ? 1.0 / det : 0.0;
+    double *o = invDm + 9 * (size_t)e;
+    o[0] = c00 * s; o[1] = c01 * s; o[2] = c02 * s;
+    o[3] = c10 * s; o[4] = c11 * s; o[5] = c12 * s;
+    o[6] = c20 * s; o[7] = c21 * s; o[8] = c22 * s;
+    double V = fabs(det) / 6.0;
+    vol[e] = V;
+    mass[e] = rho[e] * V;
+    mu[e] = E[e] / (2.0 * (1.0 + nu[e]));
+    lam[e] = (E[e] * nu[e]) / ((1.0 + nu[e]) * (1.0 - 2.0 * nu[e]));
+}
+
+__global__ void k_sort_keys(const int32_t *colors, const uint8_t *active, int n, uint32_t *keys, int32_t *vals)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int c = colors[v];
+    keys[v] = (active[v] && c >= 0) ? (uint32_t)c : (uint32_t)kMaxColors;
+    vals[v] = v;
+}
+
+__global__ void k_inverse_perm(const int32_t *perm, int n, int32_t *iperm)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) iperm[perm[i]] = i;
+}
+
+__global__ void k_key_histogram(const uint32_t *keys_sorted, int n, int32_t *hist)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&hist[keys_sorted[i]], 1);
+}
+
+__global__ void k_slot_counts(const int32_t *perm, const int32_t *inc_off, int nswept, int32_t *cnt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nswept) return;
+    if (i == nswept) {
+        cnt[i] = 0;
+        return;
+    }
+    int v = perm[i];
+    cnt[i] = inc_off[v + 1] - inc_off[v];
+}
+
+// Lumped vertex mass sum_e m_e/4 in ascending tet order (src/vbd_solver.py:32)
+// for every vertex, in colour-major order.
+__global__ void k_vertex_mass(const int32_t *perm, const int32_t *inc_off, const int32_t *inc, const double *mass_e,
+                              int n, float *mass_v, const uint8_t *active_orig, uint8_t *active_new)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = perm[i];
+    double M = 0.0;
+    for (int s = inc_off[v]; s < inc_off[v + 1]; ++s) M += mass_e[inc[s] >> 2] / 4.0;
+    mass_v[i] = (float)M;
+    active_new[i] = active_orig[v];
+}
+
+__device__ __forceinline__ void brow(const double *iD, int k, double b[3])
+{
+    if (k < 3) {
+        b[0] = iD[3 * k];
+        b[1] = iD[3 * k + 1];
+        b[2] = iD[3 * k + 2];
+    } else {
+        b[0] = -(iD[0] + iD[3] + iD[6]);
+        b[1] = -(iD[1] + iD[4] + iD[7]);
+        b[2] = -(iD[2] + iD[5] + iD[8]);
+    }
+}
+
+// 64-byte slot records (layout documented in vbd_device.h).
+__global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const int32_t *inc_off, const int32_t *inc,
+                              const int32_t *tets, const double *invDm, const double *vol, const double *mu,
+                              const double *lam, int material, int nswept, const int32_t *slot_off, float4 *slots)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nswept) return;
+    int v = perm[i];
+    int s = slot_off[i];
+    for (int q = inc_off[v]; q < inc_off[v + 1]; ++q, ++s) {
+        int e = inc[q] >> 2, loc = inc[q] & 3;
+        const double *iD = invDm + 9 * (size_t)e;
+        int other[3], n_o = 0;
+        for (int k = 0; k < 4; ++k)
+            if (k != loc) other[n_o++] = k;
+        double bs[3], b0[3], b1[3];
+        brow(iD, loc, bs);
+        brow(iD, other[0], b0);
+        brow(iD, other[1], b1);
+        float ka, kb, kc;
+        double V = vol[e];
+        if (material == kSNH) {
+            double mup = 4.0 * mu[e] / 3.0;                // src/vbd_solver.py:108
+            double lamp = lam[e] + 5.0 * mu[e] / 6.0;      // :109
+            ka = (float)(V * mup);
+            kb = (float)(V * lamp);
+            kc = 0.75f * ka;                               // V lambda'(alpha - 1), alpha from :110
+        } else {
+            ka = (float)(V * mu[e]);
+            kb = (float)(V * lam[e]);
+            kc = 0.f;
+        }
+        int o0 = iperm[tets[4 * e + other[0]]];
+        int o1 = iperm[tets[4 * e + other[1]]];
+        int o2 = iperm[tets[4 * e + other[2]]];
+        float4 *r = slots + 4 * (size_t)s;
+        r[0] = make_float4(__int_as_float(o0), __int_as_float(o1), __int_as_float(o2), ka);
+        r[1] = make_float4((float)bs[0], (float)bs[1], (float)bs[2], kb);
+        r[2] = make_float4((float)b0[0], (float)b0[1], (float)b0[2], kc);
+        r[3] = make_float4((float)b1[0], (float)b1[1], (float)b1[2], 0.f);
+    }
+}
+
+// Distinct neighbour vertices gathered by one colour's sweep (for the byte model).
+__global__ void k_mark_boundary(const float4 *slots, const int32_t *slot_off, int v0, int v1, int tag, int32_t *mark,
+                                int32_t *count)
+{
+    int i = v0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v1) return;
+    int local = 0;
+    for (int s = slot_off[i]; s < slot_off[i + 1]; ++s) {
+        float4 q0 = slots[4 * (size_t)s];
+        int o[3] = {__float_as_int(q0.x), __float_as_int(q0.y), __float_as_int(q0.z)};
+        for (int k = 0; k < 3; ++k)
+            if (atomicExch(&mark[o[k]], tag) != tag) ++local;
+    }
+    if (local) atomicAdd(count, local);
+}
+
+int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, hipStream_t st, Scratch &scr)
+{
+    size_t tb = 0;
+    VBD_HIP(rocprim::exclusive_scan(nullptr, tb, in, out, 0, (size_t)n, rocprim::plus<int32_t>(), st));
+    void *tmp = nullptr;
+    VBD_TRY(scr.get((char **)&tmp, tb));
+    VBD_HIP(rocprim::exclusive_scan(tmp, tb, in, out, 0, (size_t)n, rocprim::plus<int32_t>(), st));
+    return 0;
+}
+
+// Incidence CSR (ascending tet ids) + first-fit colouring on the device.
+int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_inc_off, int32_t *d_inc,
+                               int32_t *d_colors, int *ncolors, int *max_incident, int *max_degree, int *nref,
+                               bool enforce_valence, hipStream_t st, Scratch &scr)
+{
+    int32_t *cnt, *cursor, *scal;  // scal: maxref, maxinc, maxdeg, overflow
+    VBD_TRY(scr.get(&cnt, (size_t)n + 1));
+    VBD_TRY(scr.get(&cursor, (size_t)n + 1));
+    VBD_TRY(scr.get(&scal, 4));
+    k_fill_i32<<<nblk(n + 1), kTB, 0, st>>>(cnt, n + 1, 0);
+    k_fill_i32<<<1, kTB, 0, st>>>(scal, 4, 0);
+    k_fill_i32<<<1, 1, 0, st>>>(scal, 1, -1);
+    if (m > 0) k_count_incidence<<<nblk(m), kTB, 0, st>>>(d_tets, m, cnt, scal);
+    VBD_TRY(exclusive_scan_i32(cnt, d_inc_off, n + 1, st, scr));
+    VBD_HIP(hipMemcpyAsync(cursor, d_inc_off, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
+    if (m > 0) k_fill_incidence<<<nblk(m), kTB, 0, st>>>(d_tets, m, cursor, d_inc);
+    k_sort_incidence<<<nblk(n), kTB, 0, st>>>(d_inc_off, d_inc, n, scal + 1);
+    int32_t h[4];
+    VBD_HIP(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    *nref = h[0] + 1;
+    *max_incident = h[1];
+    if (enforce_valence && h[1] > kMaxElementsPerVertex) {
+        set_error("a vertex has " + std::to_string(h[1]) + " incident elements; at most 32 are supported "
+                  "(MAX_ELEMENTS_PER_VERTEX)");
+        return VBD_EVALENCE;
+    }
+    k_degree<<<nblk(n), kTB, 0, st>>>(d_tets, d_inc_off, d_inc, n, scal + 2);
+    k_color_init<<<nblk(n), kTB, 0, st>>>(d_colors, n, *nref);
+    // rounds in batches; each round has its own change flag
+    constexpr int kBatch = 32;
+    int32_t *changed;
+    VBD_TRY(scr.get(&changed, kBatch));
+    int32_t hc[kBatch];
+    const int max_rounds = std::max(64, 4 * (*nref) + 8);
+    bool done = false;
+    for (int r = 0; r < max_rounds && !done; r += kBatch) {
+        k_fill_i32<<<1, kBatch, 0, st>>>(changed, kBatch, 0);
+        for (int b = 0; b < kBatch; ++b)
+            k_color_round<<<nblk(*nref), kTB, 0, st>>>(d_tets, d_inc_off, d_inc, *nref, d_colors, changed + b,
+                                                      scal + 3);
+        VBD_HIP(hipMemcpyAsync(hc, changed, sizeof(hc), hipMemcpyDeviceToHost, st));
+        VBD_HIP(hipStreamSynchronize(st));
+        for (int b = 0; b < kBatch; ++b) done |= (hc[b] == 0);
+    }
+    VBD_HIP(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    if (!done) {
+        set_error("colouring did not reach its fixed point");
+        return VBD_EHIP;
+    }
+    if (h[3]) {
+        set_error("a vertex needs more than 256 colours");
+        return VBD_ECOLORS;
+    }
+    *max_degree = h[2];
+    // number of colours = max colour + 1
+    std::vector<int32_t> col(n);
+    VBD_HIP(hipMemcpyAsync(col.data(), d_colors, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    int mx = -1;
+    for (int v = 0; v < n; ++v) mx = std::max(mx, col[v]);
+    *ncolors = mx + 1;
+    return 0;
+}
+
+}  // namespace
+
+int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_t *d_colors, int *ncolors,
+                       hipStream_t st)
+{
+    Scratch scr;
+    constexpr int kBatch = 32;
+    int32_t *changed, *overflow;
+    VBD_TRY(scr.get(&changed, kBatch));
+    VBD_TRY(scr.get(&overflow, 1));
+    k_fill_i32<<<nblk(n), kTB, 0, st>>>(d_colors, n, 0);
+    k_fill_i32<<<1, 1, 0, st>>>(overflow, 1, 0);
+    int32_t hc[kBatch];
+    bool done = false;
+    for (int r = 0; r < n + 2 * kBatch && !done; r += kBatch) {
+        k_fill_i32<<<1, kBatch, 0, st>>>(changed, kBatch, 0);
+        for (int b = 0; b < kBatch; ++b)
+            k_color_round_graph<<<nblk(n), kTB, 0, st>>>(d_off, d_adj, n, d_colors, changed + b, overflow);
+        VBD_HIP(hipMemcpyAsync(hc, changed, sizeof(hc), hipMemcpyDeviceToHost, st));
+        VBD_HIP(hipStreamSynchronize(st));
+        for (int b = 0; b < kBatch; ++b) done |= (hc[b] == 0);
+    }
+    int32_t of = 0;
+    VBD_HIP(hipMemcpyAsync(&of, overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> col(n);
+    VBD_HIP(hipMemcpyAsync(col.data(), d_colors, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    if (!done) {
+        set_error("colouring did not reach its fixed point");
+        return VBD_EHIP;
+    }
+    if (of) {
+        set_error("a vertex needs more than 256 colours");
+        return VBD_ECOLORS;
+    }
+    int mx = -1;
+    for (int v = 0; v < n; ++v) mx = std::max(mx, col[v]);
+    *ncolors = mx + 1;
+    return 0;
+}
+
+int color_tets_device(const int32_t *d_tets, int m, int n, int32_t *d_colors, int32_t *d_inc_off, int32_t *d_inc,
+                      int *ncolors, int *max_degree, hipStream_t stream)
+{
+    Scratch scr;
+    int maxinc = 0, nref = 0;
+    return build_incidence_and_colors(d_tets, m, n, d_inc_off, d_inc, d_colors, ncolors, &maxinc, max_degree, &nref,
+                                      false, stream, scr);
+}
+
+int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const double *density,
+               const double *youngs, const double *poisson, const uint8_t *active)
+{
+    const int n = s->n, m = s->m;
+    hipStream_t st = s->stream;
+    Scratch scr;
+    int32_t *d_tets;
+    double *d_rho, *d_E, *d_nu, *d_invDm, *d_vol, *d_mass, *d_mu, *d_lam;
+    uint8_t *d_act_orig;
+    VBD_TRY(dmalloc(&s->X, 3 * (size_t)n));
+    VBD_TRY(scr.get(&d_tets, 4 * (size_t)m));
+    VBD_TRY(scr.get(&d_rho, m));
+    VBD_TRY(scr.get(&d_E, m));
+    VBD_TRY(scr.get(&d_nu, m));
+    VBD_TRY(scr.get(&d_invDm, 9 * (size_t)m));
+    VBD_TRY(scr.get(&d_vol, m));
+    VBD_TRY(scr.get(&d_mass, m));
+    VBD_TRY(scr.get(&d_mu, m));
+    VBD_TRY(scr.get(&d_lam, m));
+    VBD_TRY(scr.get(&d_act_orig, n));
+    VBD_HIP(hipMemcpyAsync(s->X, rest_xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    if (m) {
+        VBD_HIP(hipMemcpyAsync(d_tets, tets, sizeof(int32_t) * 4 * m, hipMemcpyHostToDevice, st));
+        VBD_HIP(hipMemcpyAsync(d_rho, density, sizeof(double) * m, hipMemcpyHostToDevice, st));
+        VBD_HIP(hipMemcpyAsync(d_E, youngs, sizeof(double) * m, hipMemcpyHostToDevice, st));
+        VBD_HIP(hipMemcpyAsync(d_nu, poisson, sizeof(double) * m, hipMemcpyHostToDevice, st));
+    }
+    std::vector<uint8_t> act(n, 1);
+    if (active)
+        for (int i = 0; i < n; ++i) act[i] = active[i] ? 1 : 0;
+    VBD_HIP(hipMemcpyAsync(d_act_orig, act.data(), n, hipMemcpyHostToDevice, st));
+
+    // 1. rest shape + Lame
+    int32_t *d_singular;
+    VBD_TRY(scr.get(&d_singular, 1));
+    k_fill_i32<<<1, 1, 0, st>>>(d_singular, 1, 0);
+    if (m)
+        k_tet_setup<<<nblk(m), kTB, 0, st>>>(s->X, d_tets, m, d_rho, d_E, d_nu, d_invDm, d_vol, d_mass, d_mu, d_lam,
+                                             d_singular);
+    int32_t singular = 0;
+    VBD_HIP(hipMemcpyAsync(&singular, d_singular, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    if (singular) {
+        set_error("an element has a singular rest shape matrix");
+        return VBD_EDEGENERATE;
+    }
+
+    // 2-3. incidence + colouring
+    VBD_TRY(dmalloc(&s->inc_off, (size_t)n + 1));
+    VBD_TRY(dmalloc(&s->inc, 4 * (size_t)m));
+    VBD_TRY(dmalloc(&s->colors, n));
+    VBD_TRY(build_incidence_and_colors(d_tets, m, n, s->inc_off, s->inc, s->colors, &s->ncolors, &s->max_incident,
+                                       &s->max_degree, &s->n_ref, true, st, scr));
+
+    // 4. colour-major renumbering (stable radix sort on the colour key)
+    uint32_t *keys, *keys_sorted;
+    int32_t *vals;
+    VBD_TRY(scr.get(&keys, n));
+    VBD_TRY(scr.get(&keys_sorted, n));
+    VBD_TRY(scr.get(&vals, n));
+    VBD_TRY(dmalloc(&s->perm, n));
+    VBD_TRY(dmalloc(&s->iperm, n));
+    k_sort_keys<<<nblk(n), kTB, 0, st>>>(s->colors, d_act_orig, n, keys, vals);
+    {
+        size_t tb = 0;
+        VBD_HIP(rocprim::radix_sort_pairs(nullptr, tb, keys, keys_sorted, vals, s->perm, (size_t)n, 0, 9, st));
+        void *tmp;
+        VBD_TRY(scr.get((char **)&tmp, tb));
+        VBD_HIP(rocprim::radix_sort_pairs(tmp, tb, keys, keys_sorted, vals, s->perm, (size_t)n, 0, 9, st));
+    }
+    k_inverse_perm<<<nblk(n), kTB, 0, st>>>(s->perm, n, s->iperm);
+    int32_t *hist;
+    VBD_TRY(scr.get(&hist, kMaxColors + 1));
+    k_fill_i32<<<nblk(kMaxColors + 1), kTB, 0, st>>>(hist, kMaxColors + 1, 0);
+    k_key_histogram<<<nblk(n), kTB, 0, st>>>(keys_sorted, n, hist);
+    std::vector<int32_t> hh(kMaxColors + 1);
+    VBD_HIP(hipMemcpyAsync(hh.data(), hist, sizeof(int32_t) * (kMaxColors + 1), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    s->col_start.assign(s->ncolors + 1, 0);
+    s->col_size.assign(s->ncolors, 0);
+    for (int c = 0; c < s->ncolors; ++c) {
+        s->col_size[c] = hh[c];
+        s->col_start[c + 1] = s->col_start[c] + hh[c];
+    }
+    s->n_swept = s->col_start[s->ncolors];
+    VBD_TRY(dmalloc(&s->d_col_start, (size_t)s->ncolors + 1));
+    VBD_HIP(hipMemcpyAsync(s->d_col_start, s->col_start.data(), sizeof(int32_t) * (s->ncolors + 1),
+                           hipMemcpyHostToDevice, st));
+
+    // 5. slot records
+    int32_t *kcnt;
+    VBD_TRY(scr.get(&kcnt, (size_t)s->n_swept + 1));
+    VBD_TRY(dmalloc(&s->slot_off, (size_t)s->n_swept + 1));
+    k_slot_counts<<<nblk(s->n_swept + 1), kTB, 0, st>>>(s->perm, s->inc_off, s->n_swept, kcnt);
+    VBD_TRY(exclusive_scan_i32(kcnt, s->slot_off, s->n_swept + 1, st, scr));
+    int32_t ns = 0;
+    VBD_HIP(hipMemcpyAsync(&ns, s->slot_off + s->n_swept, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    s->n_slots = ns;
+    VBD_TRY(dmalloc(&s->slots, 4 * (size_t)std::max<int64_t>(ns, 1)));
+    VBD_TRY(dmalloc(&s->mass, n));
+    VBD_TRY(dmalloc(&s->active, n));
+    k_vertex_mass<<<nblk(n), kTB, 0, st>>>(s->perm, s->inc_off, s->inc, d_mass, n, s->mass, d_act_orig, s->active);
+    if (s->n_swept)
+        k_build_slots<<<nblk(s->n_swept), kTB, 0, st>>>(s->perm, s->iperm, s->inc_off, s->inc, d_tets, d_invDm, d_vol,
+                                                       d_mu, d_lam, s->material, s->n_swept, s->slot_off, s->slots);
+
+    // algorithmic bytes per colour sweep: sum_v (40 + 68 k_v) + 12 |boundary|
+    {
+        int32_t *mark, *count;
+        VBD_TRY(scr.get(&mark, n));
+        VBD_TRY(scr.get(&count, s->ncolors > 0 ? s->ncolors : 1));
+        k_fill_i32<<<nblk(n), kTB, 0, st>>>(mark, n, -1);
+        k_fill_i32<<<1, kTB, 0, st>>>(count, std::max(1, s->ncolors), 0);
+        for (int c = 0; c < s->ncolors; ++c) {
+            int v0 = s->col_start[c], v1 = s->col_start[c + 1];
+            if (v1 > v0) k_mark_boundary<<<nblk(v1 - v0), kTB, 0, st>>>(s->slots, s->slot_off, v0, v1, c, mark, count + c);
+        }
+        std::vector<int32_t> hcnt(std::max(1, s->ncolors));
+        std::vector<int32_t> hoff(s->n_swept + 1);
+        VBD_HIP(hipMemcpyAsync(hcnt.data(), count, sizeof(int32_t) * hcnt.size(), hipMemcpyDeviceToHost, st));
+        VBD_HIP(hipMemcpyAsync(hoff.data(), s->slot_off, sizeof(int32_t) * hoff.size(), hipMemcpyDeviceToHost, st));
+        VBD_HIP(hipStreamSynchronize(st));
+        s->col_bytes.assign(s->ncolors, 0.0);
+        for (int c = 0; c < s->ncolors; ++c) {
+            int v0 = s->col_start[c], v1 = s->col_start[c + 1];
+            double k = (double)(hoff[v1] - hoff[v0]);
+            s->col_bytes[c] = 40.0 * (v1 - v0) + 68.0 * k + 12.0 * hcnt[c];
+        }
+    }
+
+    // state buffers
+    VBD_TRY(dmalloc(&s->u, n));
+    VBD_TRY(dmalloc(&s->uprev, n));
+    VBD_TRY(dmalloc(&s->vel, n));
+    VBD_TRY(dmalloc(&s->tgt, n));
+    VBD_TRY(dmalloc(&s->xio, 3 * (size_t)n));
+    VBD_TRY(dmalloc(&s->ctrl, 1));
+    VBD_HIP(hipHostMalloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
+    VBD_HIP(hipHostMalloc((void **)&s->xio_host, sizeof(double) * 3 * std::max(n, 1), hipHostMallocDefault));
+    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(float4) * n, st));
+    VBD_HIP(hipMemsetAsync(s->uprev, 0, sizeof(float4) * n, st));
+    VBD_HIP(hipMemsetAsync(s->vel, 0, sizeof(float4) * n, st));
+    VBD_HIP(hipMemsetAsync(s->tgt, 0, sizeof(float4) * n, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    plan_segments(s);
+    return 0;
+}
+
+}  // namespace vbd

@@ -1,0 +1,321 @@
+"""``VBDSolver`` with the constructor, attributes and ``step`` of WarpVBD's
+``src/vbd_solver.py:366-612``, running on an MI355X through libwarpvbd.so.
+
+Differences a caller can see (all additive):
+
+* ``material='snh'`` (default, the reference's live energy) or ``'stvk'``
+  (the reference's commented-out energy, src/vbd_solver.py:127-144).
+* ``step`` returns a :class:`VertexArray` whose ``.numpy()`` gives float64
+  positions, like the ``wp.array`` the reference returns.  The array keeps its
+  values after later steps (a device snapshot is taken when needed).
+* ``last_iterations`` / ``last_metric`` report what the reference only prints.
+* Positions are stored on the device as FP32 displacements from the rest shape
+  (an FP32 solve, as the workloads ask; the reference computes in FP64).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+import weakref
+
+import numpy as np
+
+from . import _lib
+from ._lib import VBDError, check, f64, i32, ptr, to_numpy
+
+EPS = 1e-12                      # src/vbd_solver.py:11
+MAX_ELEMENTS_PER_VERTEX = 32     # src/vbd_solver.py:12
+
+
+class VertexArray:
+    """An [N, 3] float64 vertex array that lives on the solver's device.
+
+    ``.numpy()`` copies it to the host.  Indexing and ``np.asarray`` work on the
+    host copy."""
+
+    __slots__ = ("_solver", "_version", "_slot", "_host", "__weakref__")
+
+    def __init__(self, solver=None, version=None, host=None):
+        self._solver = solver
+        self._version = version
+        self._slot = None
+        self._host = host
+
+    # -- state transitions driven by the solver
+    def _preserve(self):
+        s = self._solver
+        if self._host is not None or self._slot is not None or s is None:
+            return
+        slot = s._take_snapshot_slot()
+        if slot is None:
+            self._host = s._read_current()
+        else:
+            check(_lib.load().vbd_snapshot(s._h, slot), "vbd_snapshot")
+            self._slot = slot
+
+    def _is_current(self, solver):
+        return self._solver is solver and self._host is None and self._slot is None and \
+            self._version == solver._version
+
+    def numpy(self) -> np.ndarray:
+        if self._host is None:
+            s = self._solver
+            if self._slot is not None:
+                out = np.empty((s.n_vertices, 3))
+                check(_lib.load().vbd_read_snapshot(s._h, self._slot, ptr(out, ctypes.c_double)), "vbd_read_snapshot")
+                s._release_snapshot_slot(self._slot)
+                self._slot = None
+                self._host = out
+            else:
+                self._host = s._read_current()
+        return self._host
+
+    def __del__(self):
+        s, slot = self._solver, self._slot
+        if s is not None and slot is not None:
+            try:
+                s._release_snapshot_slot(slot)
+            except Exception:
+                pass
+
+    @property
+    def shape(self):
+        return (self._solver.n_vertices, 3) if self._host is None else self._host.shape
+
+    def __len__(self):
+        return self.shape[0]
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.numpy()
+        return a if dtype is None else a.astype(dtype)
+
+    def __getitem__(self, idx):
+        return self.numpy()[idx]
+
+
+class VBDSolver:
+    """Vertex Block Descent solver for tetrahedral meshes (src/vbd_solver.py:366-612).
+
+    Args mirror the reference: per-element ``densities``, ``youngs_modulus`` and
+    ``poissons_ratio`` (scalars are broadcast), ``damping_coefficient`` (default
+    1.0 as in the reference; its scenes pass 0.0), optional per-vertex
+    ``active_mask`` (False = pinned), ``gravity``, the convergence tolerance
+    ``dx_tol`` on max|dx_x+dx_y+dx_z| and ``max_iter``.
+    """
+
+    def __init__(self, initial_positions, elements, densities, youngs_modulus, poissons_ratio,
+                 damping_coefficient: float = 1.0, active_mask=None, gravity=(0.0, 0.0, -9.81),
+                 dx_tol: float = 1e-9, max_iter: int = 100, device="cuda", material: str = "snh",
+                 verbose: bool = True):
+        L = _lib.load()
+        self.device = device
+        _lib.use_device(device)
+        X = f64(initial_positions).reshape(-1, 3)
+        T = i32(elements).reshape(-1, 4)
+        n, m = len(X), len(T)
+        rho = f64(np.broadcast_to(to_numpy(densities), (m,)))
+        E = f64(np.broadcast_to(to_numpy(youngs_modulus), (m,)))
+        nu = f64(np.broadcast_to(to_numpy(poissons_ratio), (m,)))
+        act = None if active_mask is None else np.ascontiguousarray(to_numpy(active_mask), dtype=np.uint8)
+        if act is not None and act.shape != (n,):
+            raise ValueError(f"active_mask must have shape ({n},)")
+        g = f64(gravity).reshape(3)
+        if material not in _lib.MATERIALS:
+            raise ValueError(f"material must be one of {sorted(_lib.MATERIALS)}")
+
+        self.initial_positions = X
+        self.elements = T
+        self.densities, self.youngs_modulus, self.poissons_ratio = rho, E, nu
+        self.damping_coefficient = float(damping_coefficient)
+        self.gravity = tuple(float(x) for x in g)
+        self.dx_tol = dx_tol
+        self.max_iter = max_iter
+        self.material = material
+        self.active_mask = np.ones(n, dtype=bool) if act is None else act.astype(bool)
+        self.n_vertices, self.n_elements = n, m
+        self.last_iterations, self.last_metric = 0, 0.0
+        self._version = 0
+        self._views = weakref.WeakSet()
+        self._free_slots = list(range(_lib.VBD_MAX_SNAPSHOTS))
+
+        h = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        check(L.vbd_create(ptr(X, ctypes.c_double), n, ptr(T, ctypes.c_int32), m, ptr(rho, ctypes.c_double),
+                           ptr(E, ctypes.c_double), ptr(nu, ctypes.c_double),
+                           None if act is None else ptr(act, ctypes.c_uint8), ptr(g, ctypes.c_double),
+                           self.damping_coefficient, _lib.MATERIALS[material], ctypes.byref(h)), "vbd_create")
+        self._h = h
+        t1 = time.perf_counter()
+        info = self.info()
+        self.n_colors = info["n_colors"]
+        self.max_incident_elements = info["max_incident"]
+        self.vertex_valence = info["max_degree"]
+        if verbose:
+            sizes = self.color_sizes()
+            print(f"Assigned {self.n_colors} colors in {(t1 - t0) * 1000:.4f}ms. Vertex valence: "
+                  f"{self.vertex_valence}")
+            print(f"Max incident elements per vertex: {self.max_incident_elements}")
+            print(f"Active vertices per color: {sizes.tolist()}")
+        self._color_groups = None
+        self._adj_v2e = None
+
+    # ------------------------------------------------------------------ lifetime
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            for v in list(self._views):
+                if v._host is None:
+                    try:
+                        v.numpy()
+                    except Exception:
+                        pass
+            _lib.load().vbd_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ snapshots
+    def _take_snapshot_slot(self):
+        return self._free_slots.pop() if self._free_slots else None
+
+    def _release_snapshot_slot(self, slot):
+        if slot not in self._free_slots:
+            self._free_slots.append(slot)
+
+    def _read_current(self) -> np.ndarray:
+        out = np.empty((self.n_vertices, 3))
+        check(_lib.load().vbd_get_state(self._h, ptr(out, ctypes.c_double), None), "vbd_get_state")
+        return out
+
+    def _before_mutation(self):
+        for v in list(self._views):
+            if v._is_current(self):
+                v._preserve()
+        self._version += 1
+
+    def _view(self) -> VertexArray:
+        v = VertexArray(self, self._version)
+        self._views.add(v)
+        return v
+
+    # ------------------------------------------------------------------ state (src/vbd_solver.py:402-417)
+    @property
+    def old_positions(self) -> VertexArray:
+        return self._view()
+
+    @old_positions.setter
+    def old_positions(self, x):
+        if isinstance(x, VertexArray) and x._is_current(self):
+            return
+        xs = f64(x).reshape(self.n_vertices, 3)
+        self._before_mutation()
+        check(_lib.load().vbd_set_state(self._h, ptr(xs, ctypes.c_double), None), "vbd_set_state")
+
+    @property
+    def old_velocities(self) -> VertexArray:
+        v = np.empty((self.n_vertices, 3))
+        check(_lib.load().vbd_get_state(self._h, None, ptr(v, ctypes.c_double)), "vbd_get_state")
+        return VertexArray(host=v)
+
+    @old_velocities.setter
+    def old_velocities(self, v):
+        vs = f64(v).reshape(self.n_vertices, 3)
+        check(_lib.load().vbd_set_state(self._h, None, ptr(vs, ctypes.c_double)), "vbd_set_state")
+
+    def reset(self) -> None:
+        """x_prev <- initial positions, v_prev <- 0 (src/vbd_solver.py:501-506)."""
+        self._before_mutation()
+        check(_lib.load().vbd_reset(self._h), "vbd_reset")
+
+    # ------------------------------------------------------------------ time step (src/vbd_solver.py:509-612)
+    def step(self, positions, dt: float, dx_tol: float = None, max_iter: int = None) -> VertexArray:
+        """One implicit-Euler step from ``positions`` (normally ``self.old_positions``).
+
+        Returns the new positions; the input is not overwritten."""
+        if dx_tol is None:
+            dx_tol = self.dx_tol
+        if max_iter is None:
+            max_iter = self.max_iter
+        L = _lib.load()
+        pos_ptr = None
+        if isinstance(positions, VertexArray) and positions._solver is self and positions._host is None:
+            if positions._slot is not None:
+                check(L.vbd_load_snapshot(self._h, positions._slot), "vbd_load_snapshot")
+            elif positions._version != self._version:
+                raise VBDError("stale position array")
+        else:
+            xs = f64(positions).reshape(self.n_vertices, 3)
+            pos_ptr = ptr(xs, ctypes.c_double)
+        self._before_mutation()
+        it, met = ctypes.c_int32(0), ctypes.c_double(0.0)
+        check(L.vbd_step(self._h, pos_ptr, float(dt), float(dx_tol), int(max_iter), None, ctypes.byref(it),
+                         ctypes.byref(met)), "vbd_step")
+        self.last_iterations, self.last_metric = it.value, met.value
+        if it.value == max_iter:  # the reference warns when the last allowed iteration ran (:581-582)
+            print(f"Warning: VBD solver did not converge within the maximum number of iterations. "
+                  f"Final dx metric: {met.value}")
+        return self._view()
+
+    # ------------------------------------------------------------------ mesh facts
+    def info(self) -> dict:
+        n, m, c, w, d = (ctypes.c_int32() for _ in range(5))
+        s = ctypes.c_int64()
+        check(_lib.load().vbd_info(self._h, *(ctypes.byref(x) for x in (n, m, c, w, d, s))), "vbd_info")
+        return {"n_vertices": n.value, "n_elements": m.value, "n_colors": c.value, "max_incident": w.value,
+                "max_degree": d.value, "n_slots": s.value}
+
+    def colors(self) -> np.ndarray:
+        out = np.empty(self.n_vertices, np.int32)
+        check(_lib.load().vbd_get_colors(self._h, ptr(out, ctypes.c_int32)), "vbd_get_colors")
+        return out
+
+    def color_sizes(self) -> np.ndarray:
+        out = np.empty(max(1, self.n_colors), np.int32)
+        check(_lib.load().vbd_get_color_sizes(self._h, ptr(out, ctypes.c_int32)), "vbd_get_color_sizes")
+        return out[: self.n_colors]
+
+    def color_bytes(self) -> np.ndarray:
+        out = np.empty(max(1, self.n_colors))
+        check(_lib.load().vbd_color_bytes(self._h, ptr(out, ctypes.c_double)), "vbd_color_bytes")
+        return out[: self.n_colors]
+
+    @property
+    def color_groups(self) -> np.ndarray:
+        """[n_colors, max group] int32, vertices ascending, padded with -1
+        (src/vbd_solver.py:476-483)."""
+        if self._color_groups is None:
+            col = self.colors()
+            groups = [np.nonzero(col == c)[0] for c in range(self.n_colors)]
+            gmax = max((len(g) for g in groups), default=0)
+            out = np.full((self.n_colors, gmax), -1, dtype=np.int32)
+            for c, g in enumerate(groups):
+                out[c, : len(g)] = g
+            self._color_groups = out
+        return self._color_groups
+
+    @property
+    def adj_v2e(self) -> np.ndarray:
+        """[n_vertices, max incident] int32, ascending element ids padded with -1
+        (src/vbd_solver.py:485-498)."""
+        if self._adj_v2e is None:
+            out = np.empty((self.n_vertices, max(1, self.max_incident_elements)), np.int32)
+            check(_lib.load().vbd_get_incidence(self._h, ptr(out, ctypes.c_int32)), "vbd_get_incidence")
+            self._adj_v2e = out[:, : self.max_incident_elements]
+        return self._adj_v2e
+
+    # ------------------------------------------------------------------ execution knobs / profiling
+    def set_launch(self, lanes_per_vertex=4, block_size=256, small_color=2048, chunk=0):
+        check(_lib.load().vbd_set_launch(self._h, lanes_per_vertex, block_size, small_color, chunk), "vbd_set_launch")
+
+    def profile(self, enable=True, min_vertices=0):
+        check(_lib.load().vbd_profile_enable(self._h, int(enable), int(min_vertices)), "vbd_profile_enable")
+
+    def profile_read(self):
+        ms, n, b = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        check(_lib.load().vbd_profile_read(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)),
+              "vbd_profile_read")
+        return {"ms": ms.value, "launches": n.value, "bytes": b.value}
