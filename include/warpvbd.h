@@ -86,6 +86,37 @@ int vbd_snapshot(vbd_solver_t *h, int32_t slot);
 int vbd_read_snapshot(vbd_solver_t *h, int32_t slot, double *out_xyz);
 int vbd_load_snapshot(vbd_solver_t *h, int32_t slot); /* x_prev <- snapshot */
 
+/* ------------------------------------------------------------------ batches
+ * Many bodies in one handle (the reference's parallel environments,
+ * examples/run_mass_spring_parallel.py:35-119, concatenate bodies into one
+ * mesh with one global convergence test; here each body converges on its own
+ * and its result equals its single-body solve).  Body b owns vertices
+ * [body_vertex_offsets[b], body_vertex_offsets[b+1]); every element must lie in
+ * one body.  gravity: f64[n_bodies*3].  Per-body materials come through the
+ * per-element arrays.  vbd_step/get/set_state also work on a batched handle
+ * (vbd_step then reports the largest per-body iteration count). */
+int vbd_batch_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, int32_t n_tets,
+                     const int32_t *body_vertex_offsets, int32_t n_bodies, const double *density,
+                     const double *youngs, const double *poisson, const uint8_t *active, const double *gravity,
+                     double damping, int32_t material, vbd_solver_t **out);
+/* One step of every body; iters_out / metric_out: per body (nullable). */
+int vbd_batch_step(vbd_solver_t *h, double dt, double tol, int32_t max_iter, int32_t *iters_out, double *metric_out);
+/* Execution form: VBD_BATCH_LAUNCHES (one launch per colour across all bodies,
+ * iterations replayed from a captured hipGraph when use_graph) or
+ * VBD_BATCH_RESIDENT (one workgroup of block_size threads per body runs its
+ * whole step). */
+#define VBD_BATCH_LAUNCHES 1
+#define VBD_BATCH_RESIDENT 2
+int vbd_batch_set_mode(vbd_solver_t *h, int32_t mode, int32_t lanes_per_vertex, int32_t block_size,
+                       int32_t use_graph);
+int vbd_batch_get_mode(vbd_solver_t *h, int32_t *mode, int32_t *lanes_per_vertex, int32_t *block_size,
+                       int32_t *use_graph);
+/* Autotuner: times every (form, lanes per vertex, block size) candidate for
+ * trial_steps steps from the current state, restores the state, keeps the
+ * fastest and returns its ms per step. */
+int vbd_batch_autotune(vbd_solver_t *h, double dt, double tol, int32_t max_iter, int32_t trial_steps,
+                       double *best_ms_per_step);
+
 /* Mesh facts computed at create. */
 int vbd_info(vbd_solver_t *h, int32_t *n_vertices, int32_t *n_tets, int32_t *n_colors, int32_t *max_incident,
              int32_t *max_degree, int64_t *n_slots);

@@ -1,0 +1,113 @@
+"""GPU tests of the batched solver: every body of a batch against its own
+single-body FP64 oracle solve (colour-major offset table, per-body
+convergence, both execution forms, hipGraph replay, autotuner)."""
+import numpy as np
+import pytest
+
+import oracle
+from warpvbd_amd import BatchedVBDSolver
+from warpvbd_amd.scenes import cantilever, mass_spring
+
+pytestmark = pytest.mark.gpu
+
+
+def bodies():
+    out = [mass_spring(6, 3), mass_spring(7, 5, youngs_modulus=3e4, poissons_ratio=0.35),
+           cantilever((8, 2, 2), youngs_modulus=2e5), cantilever((12, 3, 2), poissons_ratio=0.3),
+           mass_spring(6, 4, youngs_modulus=5e5)]
+    out[3].gravity = (0.0, 1.5, -9.7)
+    return out
+
+
+def diag(sc):
+    return float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
+
+
+@pytest.mark.parametrize("mode,lanes,block,graph", [("launches", 4, 256, True), ("launches", 2, 128, False),
+                                                    ("resident", 4, 1024, False), ("resident", 8, 512, False)])
+def test_batch_fixed_iterations_match_single_body_oracle(mode, lanes, block, graph):
+    bs = bodies()
+    g = BatchedVBDSolver(bs, dx_tol=0.0, max_iter=8, verbose=False)
+    g.set_mode(mode, lanes, block, graph)
+    refs = [oracle.OracleSolver(**b.solver_kwargs(), dx_tol=0.0, max_iter=8) for b in bs]
+    for k in range(6):
+        x = g.step(dt=1e-3).numpy()
+        assert np.all(g.body_iterations == 8)
+        for i, (b, r) in enumerate(zip(bs, refs)):
+            xr = r.step(1e-3)
+            err = np.abs(x[g.body_slice(i)] - xr).max()
+            assert err < 1e-6 * diag(b), (mode, k, i, err)
+
+
+@pytest.mark.parametrize("mode", ["launches", "resident"])
+def test_batch_tolerance_per_body_convergence(mode):
+    bs = bodies()
+    g = BatchedVBDSolver(bs, dx_tol=1e-7, max_iter=1000, verbose=False)
+    g.set_mode(mode, 4, 256 if mode == "launches" else 1024, True)
+    refs = [oracle.OracleSolver(**b.solver_kwargs(), dx_tol=1e-7, max_iter=1000) for b in bs]
+    diffs = []
+    tip_g = [[] for _ in bs]
+    tip_r = [[] for _ in bs]
+    for k in range(30):
+        x = g.step(dt=1e-3).numpy()
+        for i, (b, r) in enumerate(zip(bs, refs)):
+            xr = r.step(1e-3)
+            diffs.append(abs(int(g.body_iterations[i]) - r.last_iters))
+            assert abs(int(g.body_iterations[i]) - r.last_iters) <= 1
+            # tolerance termination: each step may differ by O(tol) per iteration-count flip
+            assert np.abs(x[g.body_slice(i)] - xr).max() < 1e-5 * diag(b)
+            tip_g[i].append(x[g.body_slice(i)][b.tip_idx, 2].mean())
+            tip_r[i].append(xr[b.tip_idx, 2].mean())
+    assert np.mean(np.array(diffs) == 0) > 0.9
+    for i, b in enumerate(bs):
+        zr = np.array(tip_r[i]) - b.vertices[b.tip_idx, 2].mean()
+        zg = np.array(tip_g[i]) - b.vertices[b.tip_idx, 2].mean()
+        assert np.abs(zg - zr).max() <= 1e-3 * np.abs(zr).max()
+
+
+def test_batch_autotune_restores_state_and_keeps_parity(tmp_path, monkeypatch):
+    monkeypatch.setenv("WARPVBD_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    bs = bodies()
+    g = BatchedVBDSolver(bs, dx_tol=1e-7, max_iter=1000, verbose=False)
+    g.step(dt=1e-3)
+    x_before = g.old_positions.numpy().copy()
+    res = g.autotune(1e-3, trial_steps=2)
+    assert res["mode"] in ("launches", "resident") and not res["cached"]
+    assert np.array_equal(g.old_positions.numpy(), x_before)
+    res2 = g.autotune(1e-3, trial_steps=2)
+    assert res2["cached"] and res2["mode"] == res["mode"]
+    refs = [oracle.OracleSolver(**b.solver_kwargs(), dx_tol=0.0, max_iter=6) for b in bs]
+    g2 = BatchedVBDSolver(bs, dx_tol=0.0, max_iter=6, verbose=False)
+    g2.set_mode(res["mode"], res["lanes_per_vertex"], res["block_size"], res["use_graph"])
+    for k in range(3):
+        x = g2.step(dt=1e-3).numpy()
+        for i, (b, r) in enumerate(zip(bs, refs)):
+            assert np.abs(x[g2.body_slice(i)] - r.step(1e-3)).max() < 1e-6 * diag(b)
+
+
+def test_batch_rejects_element_across_bodies():
+    from warpvbd_amd._lib import VBDError
+    bs = [mass_spring(6, 3), mass_spring(6, 3)]
+    g = BatchedVBDSolver(bs, verbose=False)
+    assert g.n_bodies == 2
+    T = g.elements.copy()
+    T[0, 0] = g.body_vertex_offsets[1]  # points into the second body
+    bad = [dict(initial_positions=g.initial_positions, elements=T, densities=1000.0, youngs_modulus=1e5,
+                poissons_ratio=0.4, active_mask=None, gravity=(0, 0, -9.81))]
+    # a single "body" containing both is fine; two bodies joined by an element is not
+    BatchedVBDSolver(bad, verbose=False)
+    import ctypes
+    from warpvbd_amd import _lib
+    h = ctypes.c_void_p()
+    X = np.ascontiguousarray(g.initial_positions)
+    Ti = np.ascontiguousarray(T, dtype=np.int32)
+    off = np.ascontiguousarray(g.body_vertex_offsets, dtype=np.int32)
+    m = len(Ti)
+    one = np.ones(m)
+    nu = np.full(m, 0.3)
+    G = np.zeros(6)
+    rc = _lib.load().vbd_batch_create(_lib.ptr(X, ctypes.c_double), len(X), _lib.ptr(Ti, ctypes.c_int32), m,
+                                      _lib.ptr(off, ctypes.c_int32), 2, _lib.ptr(one, ctypes.c_double),
+                                      _lib.ptr(one, ctypes.c_double), _lib.ptr(nu, ctypes.c_double), None,
+                                      _lib.ptr(G, ctypes.c_double), 0.0, 0, ctypes.byref(h))
+    assert rc == -4

@@ -43,6 +43,12 @@ SIGNATURES = {
     "vbd_color_bytes": (ctypes.c_int, [_vp, _P(_d)]),
     "vbd_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
     "vbd_use_device": (ctypes.c_int, [_i32]),
+    "vbd_batch_create": (ctypes.c_int, [_P(_d), _i32, _P(_i32), _i32, _P(_i32), _i32, _P(_d), _P(_d), _P(_d), _P(_u8),
+                                        _P(_d), _d, _i32, _P(_vp)]),
+    "vbd_batch_step": (ctypes.c_int, [_vp, _d, _d, _i32, _P(_i32), _P(_d)]),
+    "vbd_batch_set_mode": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
+    "vbd_batch_get_mode": (ctypes.c_int, [_vp, _P(_i32), _P(_i32), _P(_i32), _P(_i32)]),
+    "vbd_batch_autotune": (ctypes.c_int, [_vp, _d, _d, _i32, _i32, _P(_d)]),
     "vbd_last_error": (ctypes.c_char_p, []),
 }
 

@@ -1,0 +1,355 @@
+// vbd_batch.hip -- many bodies of uneven size in one solver.
+//
+// The reference runs parallel environments by concatenating them into one
+// mesh with one global convergence test (examples/run_mass_spring_parallel.py:
+// 40-58,107-119).  Here the vertices are ordered colour-major, body-minor (a
+// colour-major offset table: colour c of body 0, colour c of body 1, ...), each
+// body keeps its own convergence state, and a body that has converged drops
+// out of the remaining sweeps of the step.  Two execution forms:
+//
+//   colour launches  one launch per colour across every body, then a check
+//                    launch (one workgroup per body); the iterations are
+//                    captured in a hipGraph and replayed in chunks, the host
+//                    waiting once per chunk on a device count of live bodies
+//   resident         one workgroup per body runs its whole step
+//                    (k_step_resident), one launch per step
+//
+// The per-body result equals that body's single-body solve: colouring a
+// disjoint union in body-major vertex order gives each body its own first-fit
+// colouring, and the body's iterations stop on its own max|dx_x+dx_y+dx_z|.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "vbd_device.h"
+#include "vbd_internal.h"
+#include "vbd_sweep.h"
+
+namespace vbd {
+
+namespace {
+
+template <int L, int B, int MAT>
+__global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_t *body_new, const BodyState *bs,
+                                                       uint32_t *dxm, int v0, int nv)
+{
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = b * blockDim.x + threadIdx.x;
+    const int vi = t / L, lane = t % L;
+    bool valid = vi < nv;
+    const int v = v0 + (valid ? vi : 0);
+    valid = valid && !bs[body_new[v]].done;
+    const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane);
+    if (valid && lane == 0) dxm[v] = m;
+}
+
+// One workgroup per body: fold the body's per-vertex metrics of this iteration
+// and retire the body when it converged or used its iterations.
+__global__ __launch_bounds__(256) void k_check_bodies(const int2 *ranges, int C, const uint32_t *dxm, BodyState *bs,
+                                                      int max_iter, float tol, int32_t *n_live)
+{
+    BodyState *st = bs + blockIdx.x;
+    if (st->done) return;
+    const int2 *r = ranges + (size_t)blockIdx.x * (C + 1);
+    uint32_t mx = 0;
+    for (int c = 0; c < C; ++c) {
+        const int2 rc = r[c];
+        for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) {
+            const uint32_t y = dxm[i];
+            mx = mx > y ? mx : y;
+        }
+    }
+    mx = block_max_u32(mx);
+    if (threadIdx.x == 0) {
+        const int it = st->iters + 1;
+        st->iters = it;
+        st->metric = mx;
+        if (__uint_as_float(mx) < tol || it >= max_iter) {
+            st->done = 1;
+            atomicSub(n_live, 1);
+        }
+    }
+}
+
+__global__ void k_init_bodies(int n, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt, const float *mass,
+                              const uint8_t *active, const int32_t *body_new, const float4 *grav, float dt, int nb,
+                              BodyState *bs, int32_t *n_live)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) bs[i] = BodyState{0, 0, 0u, 0};
+    if (i == 0) *n_live = nb;
+    if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body_new[i]]);
+}
+
+__global__ void k_finish_all(int n, const float4 *u, float4 *uprev, float4 *vel, float inv_dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = u[i], p = uprev[i];
+    vel[i] = make_float4((x.x - p.x) * inv_dt, (x.y - p.y) * inv_dt, (x.z - p.z) * inv_dt, 0.f);
+    uprev[i] = x;
+}
+
+template <int L, int MAT>
+void launch_sweep_bodies(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm, int v0,
+                         int nv, int block, hipStream_t st)
+{
+    k_sweep_bodies<L, Batch<L>::value, MAT><<<nblk((int64_t)nv * L, block), block, 0, st>>>(a, body_new, bs, dxm,
+                                                                                            v0, nv);
+}
+
+template <int MAT>
+void dispatch_sweep_bodies(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm,
+                           int v0, int nv, int block, hipStream_t st)
+{
+    switch (L) {
+    case 1: launch_sweep_bodies<1, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
+    case 2: launch_sweep_bodies<2, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
+    case 8: launch_sweep_bodies<8, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
+    case 16: launch_sweep_bodies<16, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
+    case 32: launch_sweep_bodies<32, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
+    default: launch_sweep_bodies<4, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
+    }
+}
+
+SweepArgs sweep_args(vbd_solver *s, float kd_dt)
+{
+    SweepArgs a;
+    a.u = s->u;
+    a.tgt = s->tgt;
+    a.uprev = s->uprev;
+    a.slot_off = s->slot_off;
+    a.slots = s->slots;
+    a.part = s->part;
+    a.ctrl = s->ctrl;
+    a.kd_dt = kd_dt;
+    return a;
+}
+
+int ensure_batch_buffers(vbd_solver *s)
+{
+    if (s->bstate) return 0;
+    VBD_HIP(hipMalloc((void **)&s->bstate, sizeof(BodyState) * s->n_bodies));
+    VBD_HIP(hipMalloc((void **)&s->dxm, sizeof(uint32_t) * std::max(1, s->n)));
+    VBD_HIP(hipMalloc((void **)&s->n_live, sizeof(int32_t)));
+    VBD_HIP(hipHostMalloc((void **)&s->n_live_host, sizeof(int32_t), hipHostMallocDefault));
+    VBD_HIP(hipHostMalloc((void **)&s->bstate_host, sizeof(BodyState) * s->n_bodies, hipHostMallocDefault));
+    return 0;
+}
+
+// C colour launches + one check launch = one iteration of every live body.
+int enqueue_body_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
+{
+    const SweepArgs a = sweep_args(s, kd_dt);
+    const int L = s->launch.lanes, block = s->launch.block;
+    for (int c = 0; c < s->ncolors; ++c) {
+        const int nv = s->col_size[c];
+        if (!nv) continue;
+        if (s->material == kSNH)
+            dispatch_sweep_bodies<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
+        else
+            dispatch_sweep_bodies<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
+    }
+    k_check_bodies<<<s->n_bodies, 256, 0, s->stream>>>(s->d_ranges, s->ncolors, s->dxm, s->bstate, max_iter, tol,
+                                                        s->n_live);
+    VBD_HIP(hipGetLastError());
+    return 0;
+}
+
+void drop_graphs(vbd_solver *s)
+{
+    for (auto &g : s->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    s->graphs.clear();
+}
+
+// A graph of k iterations for the current (lanes, block, tol, max_iter, kd).
+int get_graph(vbd_solver *s, int k, float kd_dt, float tol, int max_iter, hipGraphExec_t *out)
+{
+    for (auto &g : s->graphs)
+        if (g.k == k && g.tol == tol && g.max_iter == max_iter && g.kd_dt == kd_dt && g.lanes == s->launch.lanes &&
+            g.block == s->launch.block) {
+            *out = g.exec;
+            return 0;
+        }
+    if (s->graphs.size() > 16) drop_graphs(s);
+    CachedGraph g;
+    g.k = k;
+    g.tol = tol;
+    g.max_iter = max_iter;
+    g.kd_dt = kd_dt;
+    g.lanes = s->launch.lanes;
+    g.block = s->launch.block;
+    VBD_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int j = 0; j < k && rc == 0; ++j) rc = enqueue_body_iteration(s, kd_dt, tol, max_iter);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc != 0) return rc;
+    if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return VBD_EHIP;
+    }
+    g.graph = graph;
+    VBD_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    s->graphs.push_back(g);
+    *out = g.exec;
+    return 0;
+}
+
+int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
+{
+    VBD_TRY(ensure_batch_buffers(s));
+    const float kd_dt = (float)(s->damping / dt);
+    const float ftol = (float)tol;
+    k_init_bodies<<<nblk(std::max(s->n, s->n_bodies), 256), 256, 0, s->stream>>>(
+        s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt, s->n_bodies,
+        s->bstate, s->n_live);
+    VBD_HIP(hipGetLastError());
+    int launched = 0;
+    int chunk = std::max(1, s->last_iters);
+    while (launched < max_iter) {
+        const int k = std::min(chunk, max_iter - launched);
+        if (s->launch.graph) {
+            hipGraphExec_t ex;
+            VBD_TRY(get_graph(s, k, kd_dt, ftol, max_iter, &ex));
+            VBD_HIP(hipGraphLaunch(ex, s->stream));
+        } else {
+            for (int j = 0; j < k; ++j) VBD_TRY(enqueue_body_iteration(s, kd_dt, ftol, max_iter));
+        }
+        launched += k;
+        VBD_HIP(hipMemcpyAsync(s->n_live_host, s->n_live, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+        VBD_HIP(hipStreamSynchronize(s->stream));
+        if (*s->n_live_host == 0) break;
+        chunk = s->launch.chunk > 0 ? s->launch.chunk : 2;
+    }
+    k_finish_all<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->n, s->u, s->uprev, s->vel, (float)(1.0 / dt));
+    VBD_HIP(hipGetLastError());
+    VBD_HIP(hipMemcpyAsync(s->bstate_host, s->bstate, sizeof(BodyState) * s->n_bodies, hipMemcpyDeviceToHost,
+                           s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    int mx = 0;
+    for (int b = 0; b < s->n_bodies; ++b) {
+        s->body_out_host[b] = make_int2(s->bstate_host[b].iters, (int)s->bstate_host[b].metric);
+        mx = std::max(mx, s->bstate_host[b].iters);
+    }
+    s->last_iters = mx;
+    return 0;
+}
+
+int step_bodies_resident(vbd_solver *s, double dt, double tol, int max_iter)
+{
+    SweepArgs a = sweep_args(s, (float)(s->damping / dt));
+    StepParams p;
+    p.dt = (float)dt;
+    p.inv_dt = (float)(1.0 / dt);
+    p.tol = (float)tol;
+    p.kd_dt = a.kd_dt;
+    p.max_iter = max_iter;
+    const int L = s->launch.lanes, block = s->launch.resident_block;
+    if (s->material == kSNH)
+        dispatch_resident<kSNH>(L, a, s->d_ranges, s->ncolors, s->d_grav, s->mass, s->active, s->uprev, s->vel, p,
+                                s->d_body_out, s->n_bodies, block, s->stream);
+    else
+        dispatch_resident<kStVK>(L, a, s->d_ranges, s->ncolors, s->d_grav, s->mass, s->active, s->uprev, s->vel, p,
+                                 s->d_body_out, s->n_bodies, block, s->stream);
+    VBD_HIP(hipGetLastError());
+    VBD_HIP(hipMemcpyAsync(s->body_out_host, s->d_body_out, sizeof(int2) * s->n_bodies, hipMemcpyDeviceToHost,
+                           s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    int mx = 0;
+    for (int b = 0; b < s->n_bodies; ++b) mx = std::max(mx, s->body_out_host[b].x);
+    s->last_iters = mx;
+    return 0;
+}
+
+}  // namespace
+
+int step_bodies(vbd_solver *s, double dt, double tol, int max_iter)
+{
+    if (s->batch_mode == kBatchResident) return step_bodies_resident(s, dt, tol, max_iter);
+    return step_bodies_launches(s, dt, tol, max_iter);
+}
+
+void release_batch(vbd_solver *s)
+{
+    drop_graphs(s);
+    if (s->bstate) (void)hipFree(s->bstate);
+    if (s->dxm) (void)hipFree(s->dxm);
+    if (s->n_live) (void)hipFree(s->n_live);
+    if (s->n_live_host) (void)hipHostFree(s->n_live_host);
+    if (s->bstate_host) (void)hipHostFree(s->bstate_host);
+    s->bstate = nullptr;
+    s->dxm = nullptr;
+    s->n_live = nullptr;
+    s->n_live_host = nullptr;
+    s->bstate_host = nullptr;
+}
+
+// Time each candidate execution form on the solver's own state (saved and
+// restored around the trials) and keep the fastest.
+int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int trial_steps, double *best_ms)
+{
+    const int n = s->n;
+    float4 *save_u = nullptr, *save_v = nullptr;
+    VBD_HIP(hipMalloc((void **)&save_u, sizeof(float4) * n));
+    VBD_HIP(hipMalloc((void **)&save_v, sizeof(float4) * n));
+    VBD_HIP(hipMemcpyAsync(save_u, s->uprev, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
+    VBD_HIP(hipMemcpyAsync(save_v, s->vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
+    struct Cand {
+        int mode, lanes, block;
+    };
+    std::vector<Cand> cands;
+    for (int L : {1, 2, 4, 8})
+        for (int blk : {128, 256, 512}) cands.push_back({kBatchLaunches, L, blk});
+    for (int L : {1, 2, 4, 8, 16})
+        for (int blk : {256, 512, 1024}) cands.push_back({kBatchResident, L, blk});
+    double best = 1e300;
+    Cand bc = {s->batch_mode, s->launch.lanes, s->launch.block};
+    const Launch keep = s->launch;
+    const int keep_mode = s->batch_mode;
+    int rc = 0;
+    for (const Cand &c : cands) {
+        s->batch_mode = c.mode;
+        s->launch.lanes = c.lanes;
+        if (c.mode == kBatchResident) s->launch.resident_block = c.block;
+        else s->launch.block = c.block;
+        s->last_iters = 0;
+        // restore the start state, one untimed step, then the timed ones
+        for (int rep = 0; rep < 2 && rc == 0; ++rep) {
+            (void)hipMemcpyAsync(s->uprev, save_u, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+            (void)hipMemcpyAsync(s->vel, save_v, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+            const int steps = rep == 0 ? 1 : trial_steps;
+            auto t0 = std::chrono::steady_clock::now();
+            for (int k = 0; k < steps && rc == 0; ++k) rc = step_bodies(s, dt, tol, max_iter);
+            auto t1 = std::chrono::steady_clock::now();
+            const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+            if (rep == 1 && rc == 0 && ms < best) {
+                best = ms;
+                bc = c;
+            }
+        }
+        if (rc != 0) break;
+    }
+    s->launch = keep;
+    s->batch_mode = bc.mode;
+    s->launch.lanes = bc.lanes;
+    if (bc.mode == kBatchResident) s->launch.resident_block = bc.block;
+    else s->launch.block = bc.block;
+    if (best >= 1e300) {
+        s->batch_mode = keep_mode;
+        s->launch = keep;
+    }
+    (void)hipMemcpyAsync(s->uprev, save_u, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+    (void)hipMemcpyAsync(s->vel, save_v, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(save_u);
+    (void)hipFree(save_v);
+    s->last_iters = 0;
+    if (best_ms) *best_ms = best;
+    return rc;
+}
+
+}  // namespace vbd

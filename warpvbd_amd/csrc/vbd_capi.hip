@@ -31,15 +31,17 @@ void vbd_destroy(vbd_solver_t *s)
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    release_batch(s);
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slots, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
-                   s->xio,  s->d_col_start};
+                   s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (float4 *p : s->snap)
         if (p) (void)hipFree(p);
     if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
     if (s->xio_host) (void)hipHostFree(s->xio_host);
+    if (s->body_out_host) (void)hipHostFree(s->body_out_host);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -75,13 +77,117 @@ int vbd_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, 
         delete s;
         return VBD_EHIP;
     }
-    r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active);
+    r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active, nullptr, 1, nullptr);
     if (r != 0) {
         vbd_destroy(s);
         return r;
     }
     *out = s;
     return 0;
+}
+
+int vbd_batch_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, int32_t n_tets,
+                     const int32_t *body_vertex_offsets, int32_t n_bodies, const double *density,
+                     const double *youngs, const double *poisson, const uint8_t *active, const double *gravity,
+                     double damping, int32_t material, vbd_solver_t **out)
+{
+    CHECK_ARG(out, "out is NULL");
+    *out = nullptr;
+    CHECK_ARG(n_vertices > 0 && n_tets >= 0 && n_bodies >= 1, "bad sizes");
+    CHECK_ARG(rest_xyz && gravity && body_vertex_offsets, "rest_xyz, gravity and body offsets are required");
+    CHECK_ARG(n_tets == 0 || (tets && density && youngs && poisson), "element arrays are required");
+    CHECK_ARG(material == VBD_MATERIAL_SNH || material == VBD_MATERIAL_STVK, "unknown material");
+    CHECK_ARG((int64_t)(kMaxColors + 1) * n_bodies < ((int64_t)1 << 31), "too many bodies");
+    CHECK_ARG(body_vertex_offsets[0] == 0 && body_vertex_offsets[n_bodies] == n_vertices,
+              "body offsets must start at 0 and end at n_vertices");
+    std::vector<int32_t> body_of(n_vertices);
+    for (int b = 0; b < n_bodies; ++b) {
+        CHECK_ARG(body_vertex_offsets[b] <= body_vertex_offsets[b + 1], "body offsets must not decrease");
+        for (int v = body_vertex_offsets[b]; v < body_vertex_offsets[b + 1]; ++v) body_of[v] = b;
+    }
+    for (int64_t e = 0; e < n_tets; ++e) {
+        int b0 = -1;
+        for (int k = 0; k < 4; ++k) {
+            const int v = tets[4 * e + k];
+            if (v < 0 || v >= n_vertices) {
+                set_error("element " + std::to_string(e) + " references a vertex outside [0, n_vertices)");
+                return VBD_EMESH;
+            }
+            if (k == 0) b0 = body_of[v];
+            else if (body_of[v] != b0) {
+                set_error("element " + std::to_string(e) + " joins vertices of two bodies");
+                return VBD_EMESH;
+            }
+        }
+    }
+    vbd_solver *s = new vbd_solver();
+    s->n = n_vertices;
+    s->m = n_tets;
+    s->material = material;
+    s->damping = damping;
+    s->batched = true;
+    for (int k = 0; k < 3; ++k) s->grav[k] = gravity[k];
+    if (hipGetDevice(&s->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("no HIP device available");
+        delete s;
+        return VBD_EHIP;
+    }
+    int r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active, body_vertex_offsets, n_bodies, gravity);
+    if (r != 0) {
+        vbd_destroy(s);
+        return r;
+    }
+    *out = s;
+    return 0;
+}
+
+int vbd_batch_step(vbd_solver_t *s, double dt, double tol, int32_t max_iter, int32_t *iters_out, double *metric_out)
+{
+    CHECK_ARG(s && s->batched, "not a batched handle");
+    CHECK_ARG(dt > 0.0, "dt must be > 0");
+    CHECK_ARG(max_iter >= 1, "max_iter must be >= 1");
+    VBD_HIP(hipSetDevice(s->device));
+    VBD_TRY(step_bodies(s, dt, tol, max_iter));
+    for (int b = 0; b < s->n_bodies; ++b) {
+        if (iters_out) iters_out[b] = s->body_out_host[b].x;
+        if (metric_out) metric_out[b] = (double)__builtin_bit_cast(float, (uint32_t)s->body_out_host[b].y);
+    }
+    return 0;
+}
+
+int vbd_batch_set_mode(vbd_solver_t *s, int32_t mode, int32_t lanes, int32_t block, int32_t use_graph)
+{
+    CHECK_ARG(s && s->batched, "not a batched handle");
+    CHECK_ARG(mode == VBD_BATCH_LAUNCHES || mode == VBD_BATCH_RESIDENT, "unknown batch mode");
+    CHECK_ARG(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32,
+              "lanes_per_vertex must be 1, 2, 4, 8, 16 or 32");
+    CHECK_ARG(block >= 64 && block <= 1024 && block % 64 == 0, "block_size must be a multiple of 64 in [64, 1024]");
+    s->batch_mode = mode;
+    s->launch.lanes = lanes;
+    if (mode == VBD_BATCH_RESIDENT) s->launch.resident_block = block;
+    else s->launch.block = block;
+    s->launch.graph = use_graph != 0;
+    return 0;
+}
+
+int vbd_batch_get_mode(vbd_solver_t *s, int32_t *mode, int32_t *lanes, int32_t *block, int32_t *use_graph)
+{
+    CHECK_ARG(s && s->batched, "not a batched handle");
+    if (mode) *mode = s->batch_mode;
+    if (lanes) *lanes = s->launch.lanes;
+    if (block) *block = s->batch_mode == VBD_BATCH_RESIDENT ? s->launch.resident_block : s->launch.block;
+    if (use_graph) *use_graph = s->launch.graph ? 1 : 0;
+    return 0;
+}
+
+int vbd_batch_autotune(vbd_solver_t *s, double dt, double tol, int32_t max_iter, int32_t trial_steps,
+                       double *best_ms_per_step)
+{
+    CHECK_ARG(s && s->batched, "not a batched handle");
+    CHECK_ARG(dt > 0.0 && max_iter >= 1 && trial_steps >= 1, "bad arguments");
+    VBD_HIP(hipSetDevice(s->device));
+    return autotune_bodies(s, dt, tol, max_iter, trial_steps, best_ms_per_step);
 }
 
 int vbd_reset(vbd_solver_t *s)
@@ -106,7 +212,18 @@ int vbd_step(vbd_solver_t *s, const double *positions, double dt, double tol, in
     if (positions) VBD_TRY(import_positions(s, positions, s->uprev));
     int it = 0;
     double met = 0.0;
-    VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
+    if (s->batched) {
+        VBD_TRY(step_bodies(s, dt, tol, max_iter));
+        float mx = 0.f;
+        for (int b = 0; b < s->n_bodies; ++b) {
+            it = std::max(it, s->body_out_host[b].x);
+            const float m = __builtin_bit_cast(float, (uint32_t)s->body_out_host[b].y);
+            if (!(m <= mx)) mx = m;  // NaN wins
+        }
+        met = mx;
+    } else {
+        VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
+    }
     if (out_xyz) VBD_TRY(export_positions(s, s->u, out_xyz));
     if (iters_out) *iters_out = it;
     if (metric_out) *metric_out = met;

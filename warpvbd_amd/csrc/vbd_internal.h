@@ -36,13 +36,34 @@ struct Ctrl {
     int32_t converged;  // 1 once an iteration's metric fell below tol
     int32_t iters;      // iterations executed (index + 1 of the last one)
     uint32_t metric;    // bits of the last iteration's metric
+    uint32_t blocks_done;  // blocks of the iteration's last launch that finished
+};
+
+// Per-body convergence state of the running step (batched solver).
+struct BodyState {
+    int32_t done;     // converged or out of iterations
+    int32_t iters;    // iterations executed
+    uint32_t metric;  // bits of the last iteration's max|dx_x+dx_y+dx_z|
+    int32_t pad;
+};
+
+enum { kBatchLaunches = 1, kBatchResident = 2 };
+
+struct CachedGraph {
+    int k = 0, max_iter = 0, lanes = 0, block = 0;
+    float tol = 0.f, kd_dt = 0.f;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 struct Launch {
-    int lanes = 4;        // lanes per vertex
+    int lanes = 8;        // lanes per vertex
     int block = 256;      // threads per block
     int small_color = 2048;  // colours below this many vertices run fused in one workgroup
     int chunk = 0;        // iterations between host waits (0 = adaptive)
+    bool resident = true; // allow the one-workgroup whole-step kernel when every colour is small
+    bool graph = true;    // batched colour launches: replay captured iterations
+    int resident_block = 1024;  // threads per body in the one-workgroup-per-body form
 };
 
 // A run of consecutive colours executed by one launch.
@@ -51,6 +72,7 @@ struct Segment {
     bool fused;    // one workgroup, barrier between colours
     int part_off;  // first per-block metric partial of this launch
     int nparts;    // blocks of this launch
+    int lanes;     // lanes per vertex of this launch
 };
 
 struct DevBuf {
@@ -76,6 +98,21 @@ struct vbd_solver {
     std::vector<vbd::Segment> segments;
     int32_t *d_col_start = nullptr;  // device copy of col_start
     float kd_dt = 0.f;               // damping / dt of the running step
+    bool resident = false;           // whole step in one workgroup (all colours small)
+    int resident_lanes = 32;         // lanes per vertex of that form
+    int n_bodies = 1;
+    std::vector<int2> ranges_host;   // per body: (start, count) of each colour, then of its other vertices
+    int2 *d_ranges = nullptr;        // device copy of ranges_host
+    int32_t *body_new = nullptr;     // body of each vertex, colour-major order
+    bool batched = false;            // built by vbd_batch_create
+    int batch_mode = vbd::kBatchLaunches;
+    vbd::BodyState *bstate = nullptr, *bstate_host = nullptr;
+    uint32_t *dxm = nullptr;         // per-vertex metric bits of the current iteration
+    int32_t *n_live = nullptr, *n_live_host = nullptr;
+    std::vector<vbd::CachedGraph> graphs;
+    float4 *d_grav = nullptr;        // per body gravity
+    int2 *d_body_out = nullptr;      // per body (iterations, metric bits) of the last step
+    int2 *body_out_host = nullptr;   // pinned
     int n_swept = 0;              // active, coloured vertices (prefix of the colour-major order)
 
     // device arrays (colour-major order unless noted)
@@ -115,7 +152,8 @@ struct vbd_solver {
 
 namespace vbd {
 int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const double *density,
-               const double *youngs, const double *poisson, const uint8_t *active);
+               const double *youngs, const double *poisson, const uint8_t *active, const int32_t *body_voff,
+               int n_bodies, const double *body_grav);
 int color_tets_device(const int32_t *d_tets, int m, int n, int32_t *d_colors, int32_t *d_inc_off,
                       int32_t *d_inc, int *ncolors, int *max_degree, hipStream_t stream);
 int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_t *d_colors, int *ncolors,
@@ -126,4 +164,7 @@ int import_positions(vbd_solver *s, const double *host_in, float4 *dst);
 int import_velocities(vbd_solver *s, const double *host_in);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
+int step_bodies(vbd_solver *s, double dt, double tol, int max_iter);
+void release_batch(vbd_solver *s);
+int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int trial_steps, double *best_ms);
 }  // namespace vbd

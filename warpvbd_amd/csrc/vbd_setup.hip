@@ -144,54 +144,46 @@ __global__ void k_degree(const int32_t *tets, const int32_t *off, const int32_t 
     atomicMax(maxdeg, cnt + (self ? 1 : 0));
 }
 
+// Distinct lower-indexed neighbours of each vertex (the only ones first-fit
+// consults): counted in the first pass, written in the second.
+template <bool FILL>
+__global__ void k_lower_neighbours(const int32_t *tets, const int32_t *off, const int32_t *inc, int n, int32_t *cnt,
+                                   const int32_t *lo_off, int32_t *lo)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int a = off[v], b = off[v + 1];
+    int k = 0;
+    int w = FILL ? lo_off[v] : 0;
+    for (int s = a; s < b; ++s) {
+        int e = inc[s] >> 2;
+        for (int q = 0; q < 4; ++q) {
+            int u = tets[4 * e + q];
+            if (u >= v) continue;
+            bool seen = false;
+            for (int q2 = 0; q2 < q && !seen; ++q2) seen = tets[4 * e + q2] == u;
+            for (int s2 = a; s2 < s && !seen; ++s2) {
+                int e2 = inc[s2] >> 2;
+                for (int q2 = 0; q2 < 4; ++q2) seen |= tets[4 * e2 + q2] == u;
+            }
+            if (seen) continue;
+            if (FILL) lo[w + k] = u;
+            ++k;
+        }
+    }
+    if (!FILL) cnt[v] = k;
+}
+
 __global__ void k_color_init(int32_t *colors, int n, int nref)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n) colors[v] = v < nref ? 0 : -1;
 }
 
-// One parallel round of colour[v] = mex{colour[u] : u ~ v, u < v}.
-__global__ void k_color_round(const int32_t *tets, const int32_t *off, const int32_t *inc, int nref,
-                              int32_t *colors, int32_t *changed, int32_t *overflow)
-{
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nref) return;
-    uint32_t used[kMaxColors / 32];
-#pragma unroll
-    for (int i = 0; i < kMaxColors / 32; ++i) used[i] = 0;
-    int a = off[v], b = off[v + 1];
-    for (int s = a; s < b; ++s) {
-        int e = inc[s] >> 2;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int u = tets[4 * e + k];
-            if (u < v) {
-                int c = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
-                if (c >= 0 && c < kMaxColors) used[c >> 5] |= 1u << (c & 31);
-            }
-        }
-    }
-    int c = kMaxColors;
-    for (int w = 0; w < kMaxColors / 32; ++w) {
-        uint32_t freebits = ~used[w];
-        if (freebits) {
-            c = w * 32 + __builtin_ctz(freebits);
-            break;
-        }
-    }
-    if (c == kMaxColors) {
-        atomicExch(overflow, 1);
-        c = kMaxColors - 1;
-    }
-    if (c != colors[v]) {
-        __atomic_store_n(&colors[v], c, __ATOMIC_RELAXED);
-        atomicExch(changed, 1);
-    }
-}
-
-// The same round over an explicit adjacency list (graph_coloring's dict input,
-// src/coloring.py:40-72): only v's own list is consulted, and only neighbours
-// with a smaller index are already coloured when v is visited.
+// One parallel round of colour[v] = mex{colour[u] : u in adj(v), u < v} over an
+// adjacency list (the mesh's lower-neighbour lists, or graph_coloring's dict
+// input, src/coloring.py:40-72): only neighbours with a smaller index are
+// already coloured when sequential first-fit visits v.
 __global__ void k_color_round_graph(const int32_t *off, const int32_t *adj, int n, int32_t *colors,
                                     int32_t *changed, int32_t *overflow)
 {
@@ -264,13 +256,23 @@ __global__ void k_tet_setup(const double *X, const int32_t *tets, int m, const d
     lam[e] = (E[e] * nu[e]) / ((1.0 + nu[e]) * (1.0 - 2.0 * nu[e]));
 }
 
-__global__ void k_sort_keys(const int32_t *colors, const uint8_t *active, int n, uint32_t *keys, int32_t *vals)
+// Sort key of the colour-major order: colour-major, body-minor; vertices that
+// are not swept (pinned or uncoloured) go after every colour.
+__global__ void k_sort_keys(const int32_t *colors, const uint8_t *active, const int32_t *body_of, int nb, int n,
+                            uint32_t *keys, int32_t *vals)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     int c = colors[v];
-    keys[v] = (active[v] && c >= 0) ? (uint32_t)c : (uint32_t)kMaxColors;
+    uint32_t ck = (active[v] && c >= 0) ? (uint32_t)c : (uint32_t)kMaxColors;
+    keys[v] = ck * (uint32_t)nb + (uint32_t)body_of[v];
     vals[v] = v;
+}
+
+__global__ void k_gather_i32(const int32_t *src, const int32_t *idx, int n, int32_t *dst)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
 }
 
 __global__ void k_inverse_perm(const int32_t *perm, int n, int32_t *iperm)
@@ -279,10 +281,15 @@ __global__ void k_inverse_perm(const int32_t *perm, int n, int32_t *iperm)
     if (i < n) iperm[perm[i]] = i;
 }
 
-__global__ void k_key_histogram(const uint32_t *keys_sorted, int n, int32_t *hist)
+// Run lengths of the sorted keys: each run's first element records its start,
+// its last its end (no atomics; absent keys keep 0).
+__global__ void k_key_runs(const uint32_t *keys_sorted, int n, int32_t *first, int32_t *end)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&hist[keys_sorted[i]], 1);
+    if (i >= n) return;
+    const uint32_t k = keys_sorted[i];
+    if (i == 0 || keys_sorted[i - 1] != k) first[k] = i;
+    if (i == n - 1 || keys_sorted[i + 1] != k) end[k] = i + 1;
 }
 
 __global__ void k_slot_counts(const int32_t *perm, const int32_t *inc_off, int nswept, int32_t *cnt)
@@ -421,6 +428,18 @@ int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_i
         return VBD_EVALENCE;
     }
     k_degree<<<nblk(n), kTB, 0, st>>>(d_tets, d_inc_off, d_inc, n, scal + 2);
+    // lower-neighbour CSR, then parallel first-fit rounds over it
+    int32_t *lo_cnt, *lo_off, *lo;
+    VBD_TRY(scr.get(&lo_cnt, (size_t)n + 1));
+    VBD_TRY(scr.get(&lo_off, (size_t)n + 1));
+    k_fill_i32<<<nblk(n + 1), kTB, 0, st>>>(lo_cnt, n + 1, 0);
+    k_lower_neighbours<false><<<nblk(n), kTB, 0, st>>>(d_tets, d_inc_off, d_inc, n, lo_cnt, nullptr, nullptr);
+    VBD_TRY(exclusive_scan_i32(lo_cnt, lo_off, n + 1, st, scr));
+    int32_t n_lo = 0;
+    VBD_HIP(hipMemcpyAsync(&n_lo, lo_off + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    VBD_TRY(scr.get(&lo, (size_t)std::max(n_lo, 1)));
+    k_lower_neighbours<true><<<nblk(n), kTB, 0, st>>>(d_tets, d_inc_off, d_inc, n, nullptr, lo_off, lo);
     k_color_init<<<nblk(n), kTB, 0, st>>>(d_colors, n, *nref);
     // rounds in batches; each round has its own change flag
     constexpr int kBatch = 32;
@@ -432,8 +451,7 @@ int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_i
     for (int r = 0; r < max_rounds && !done; r += kBatch) {
         k_fill_i32<<<1, kBatch, 0, st>>>(changed, kBatch, 0);
         for (int b = 0; b < kBatch; ++b)
-            k_color_round<<<nblk(*nref), kTB, 0, st>>>(d_tets, d_inc_off, d_inc, *nref, d_colors, changed + b,
-                                                      scal + 3);
+            k_color_round_graph<<<nblk(*nref), kTB, 0, st>>>(lo_off, lo, *nref, d_colors, changed + b, scal + 3);
         VBD_HIP(hipMemcpyAsync(hc, changed, sizeof(hc), hipMemcpyDeviceToHost, st));
         VBD_HIP(hipStreamSynchronize(st));
         for (int b = 0; b < kBatch; ++b) done |= (hc[b] == 0);
@@ -510,9 +528,12 @@ int color_tets_device(const int32_t *d_tets, int m, int n, int32_t *d_colors, in
 }
 
 int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const double *density,
-               const double *youngs, const double *poisson, const uint8_t *active)
+               const double *youngs, const double *poisson, const uint8_t *active, const int32_t *body_voff,
+               int n_bodies, const double *body_grav)
 {
     const int n = s->n, m = s->m;
+    const int nb = std::max(1, n_bodies);
+    s->n_bodies = nb;
     hipStream_t st = s->stream;
     Scratch scr;
     int32_t *d_tets;
@@ -563,7 +584,17 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_TRY(build_incidence_and_colors(d_tets, m, n, s->inc_off, s->inc, s->colors, &s->ncolors, &s->max_incident,
                                        &s->max_degree, &s->n_ref, true, st, scr));
 
-    // 4. colour-major renumbering (stable radix sort on the colour key)
+    // 4. colour-major renumbering (stable radix sort on colour, then body)
+    std::vector<int32_t> body_of(n, 0);
+    if (body_voff)
+        for (int b = 0; b < nb; ++b)
+            for (int v = body_voff[b]; v < body_voff[b + 1]; ++v) body_of[v] = b;
+    int32_t *d_body_of;
+    VBD_TRY(scr.get(&d_body_of, n));
+    VBD_HIP(hipMemcpyAsync(d_body_of, body_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    const int64_t nkeys = (int64_t)(kMaxColors + 1) * nb;
+    unsigned end_bit = 1;
+    while (((int64_t)1 << end_bit) < nkeys) ++end_bit;
     uint32_t *keys, *keys_sorted;
     int32_t *vals;
     VBD_TRY(scr.get(&keys, n));
@@ -571,29 +602,50 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_TRY(scr.get(&vals, n));
     VBD_TRY(dmalloc(&s->perm, n));
     VBD_TRY(dmalloc(&s->iperm, n));
-    k_sort_keys<<<nblk(n), kTB, 0, st>>>(s->colors, d_act_orig, n, keys, vals);
+    VBD_TRY(dmalloc(&s->body_new, n));
+    k_sort_keys<<<nblk(n), kTB, 0, st>>>(s->colors, d_act_orig, d_body_of, nb, n, keys, vals);
     {
         size_t tb = 0;
-        VBD_HIP(rocprim::radix_sort_pairs(nullptr, tb, keys, keys_sorted, vals, s->perm, (size_t)n, 0, 9, st));
+        VBD_HIP(rocprim::radix_sort_pairs(nullptr, tb, keys, keys_sorted, vals, s->perm, (size_t)n, 0, end_bit, st));
         void *tmp;
         VBD_TRY(scr.get((char **)&tmp, tb));
-        VBD_HIP(rocprim::radix_sort_pairs(tmp, tb, keys, keys_sorted, vals, s->perm, (size_t)n, 0, 9, st));
+        VBD_HIP(rocprim::radix_sort_pairs(tmp, tb, keys, keys_sorted, vals, s->perm, (size_t)n, 0, end_bit, st));
     }
     k_inverse_perm<<<nblk(n), kTB, 0, st>>>(s->perm, n, s->iperm);
-    int32_t *hist;
-    VBD_TRY(scr.get(&hist, kMaxColors + 1));
-    k_fill_i32<<<nblk(kMaxColors + 1), kTB, 0, st>>>(hist, kMaxColors + 1, 0);
-    k_key_histogram<<<nblk(n), kTB, 0, st>>>(keys_sorted, n, hist);
-    std::vector<int32_t> hh(kMaxColors + 1);
-    VBD_HIP(hipMemcpyAsync(hh.data(), hist, sizeof(int32_t) * (kMaxColors + 1), hipMemcpyDeviceToHost, st));
+    k_gather_i32<<<nblk(n), kTB, 0, st>>>(d_body_of, s->perm, n, s->body_new);
+    int32_t *kfirst, *kend;
+    VBD_TRY(scr.get(&kfirst, nkeys));
+    VBD_TRY(scr.get(&kend, nkeys));
+    k_fill_i32<<<nblk(nkeys), kTB, 0, st>>>(kfirst, nkeys, 0);
+    k_fill_i32<<<nblk(nkeys), kTB, 0, st>>>(kend, nkeys, 0);
+    k_key_runs<<<nblk(n), kTB, 0, st>>>(keys_sorted, n, kfirst, kend);
+    std::vector<int32_t> hf(nkeys), he(nkeys), hh(nkeys);
+    VBD_HIP(hipMemcpyAsync(hf.data(), kfirst, sizeof(int32_t) * nkeys, hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipMemcpyAsync(he.data(), kend, sizeof(int32_t) * nkeys, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
+    for (int64_t k = 0; k < nkeys; ++k) hh[k] = he[k] - hf[k];
     s->col_start.assign(s->ncolors + 1, 0);
     s->col_size.assign(s->ncolors, 0);
     for (int c = 0; c < s->ncolors; ++c) {
-        s->col_size[c] = hh[c];
-        s->col_start[c + 1] = s->col_start[c] + hh[c];
+        int tot = 0;
+        for (int b = 0; b < nb; ++b) tot += hh[(size_t)c * nb + b];
+        s->col_size[c] = tot;
+        s->col_start[c + 1] = s->col_start[c] + tot;
     }
     s->n_swept = s->col_start[s->ncolors];
+    // per body: (start, count) of each colour, then of its other vertices
+    s->ranges_host.assign((size_t)nb * (s->ncolors + 1), make_int2(0, 0));
+    {
+        int64_t pos = 0;
+        for (int c = 0; c <= kMaxColors; ++c) {
+            const int cc = c < s->ncolors ? c : (c == kMaxColors ? s->ncolors : -1);
+            for (int b = 0; b < nb; ++b) {
+                const int cnt = hh[(size_t)c * nb + b];
+                if (cc >= 0) s->ranges_host[(size_t)b * (s->ncolors + 1) + cc] = make_int2((int)pos, cnt);
+                pos += cnt;
+            }
+        }
+    }
     VBD_TRY(dmalloc(&s->d_col_start, (size_t)s->ncolors + 1));
     VBD_HIP(hipMemcpyAsync(s->d_col_start, s->col_start.data(), sizeof(int32_t) * (s->ncolors + 1),
                            hipMemcpyHostToDevice, st));
@@ -653,6 +705,21 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_HIP(hipMemsetAsync(s->uprev, 0, sizeof(float4) * n, st));
     VBD_HIP(hipMemsetAsync(s->vel, 0, sizeof(float4) * n, st));
     VBD_HIP(hipMemsetAsync(s->tgt, 0, sizeof(float4) * n, st));
+    // per-body range table, gravity and step results
+    {
+        VBD_TRY(dmalloc(&s->d_ranges, s->ranges_host.size()));
+        VBD_HIP(hipMemcpyAsync(s->d_ranges, s->ranges_host.data(), sizeof(int2) * s->ranges_host.size(),
+                               hipMemcpyHostToDevice, st));
+        std::vector<float4> g(nb);
+        for (int b = 0; b < nb; ++b) {
+            const double *gb = body_grav ? body_grav + 3 * b : s->grav;
+            g[b] = make_float4((float)gb[0], (float)gb[1], (float)gb[2], 0.f);
+        }
+        VBD_TRY(dmalloc(&s->d_grav, nb));
+        VBD_HIP(hipMemcpyAsync(s->d_grav, g.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, st));
+        VBD_TRY(dmalloc(&s->d_body_out, nb));
+        VBD_HIP(hipHostMalloc((void **)&s->body_out_host, sizeof(int2) * nb, hipHostMallocDefault));
+    }
     VBD_HIP(hipStreamSynchronize(st));
     plan_segments(s);
     return 0;

@@ -1,0 +1,248 @@
+// vbd_sweep.h -- vertex update shared by the per-colour, fused, resident and
+// batched kernels (included by vbd_step.hip and vbd_batch.hip).
+#pragma once
+#include "vbd_device.h"
+#include "vbd_internal.h"
+
+namespace vbd {
+namespace {
+
+inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
+
+struct SweepArgs {
+    float4 *__restrict__ u;
+    const float4 *__restrict__ tgt;
+    const float4 *__restrict__ uprev;
+    const int32_t *__restrict__ slot_off;
+    const float4 *__restrict__ slots;
+    uint32_t *part;  // this launch's per-block metric maxima
+    Ctrl *ctrl;
+    float kd_dt;  // damping_coefficient / dt
+};
+
+// Max over the block of the metric bits; thread 0 gets the result.  Avoids a
+// device-scope atomic per wave: those serialise on one address across XCDs.
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t x)
+{
+    __shared__ uint32_t red[16];
+    x = wave_max_u32(x);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < nw; ++i) x = x > red[i] ? x : red[i];
+    return x;
+}
+
+// Max over np per-block partial maxima with the whole (<= 1024-thread) block;
+// thread 0 gets the result.
+__device__ __forceinline__ uint32_t reduce_partials(const uint32_t *part, int np, uint32_t x)
+{
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        const uint32_t y = part[i];
+        x = x > y ? x : y;
+    }
+    return block_max_u32(x);
+}
+
+__device__ __forceinline__ void close_iteration(Ctrl *c, int it, uint32_t bits, float tol)
+{
+    c->metric = bits;
+    c->iters = it + 1;
+    if (__uint_as_float(bits) < tol) c->converged = 1;
+    c->it = it + 1;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD one
+// contiguous share of the colour so its L2 sees a compact neighbourhood.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    const int per = nb >> 3, rem = nb & 7;
+    const int x = b & 7, i = b >> 3;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+// Evaluate and apply one vertex update for a lane group.  Returns the metric
+// bits (0 for lanes without a vertex).
+template <int L, int B, int MAT>
+__device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane)
+{
+    Acc acc;
+    acc_zero(acc);
+    const float4 uv4 = a.u[v];
+    const float3 us = make_float3(uv4.x, uv4.y, uv4.z);
+    if (valid) {
+        const int s0 = a.slot_off[v], s1 = a.slot_off[v + 1];
+        // B slots per lane per batch: all records, then all gathers, then the
+        // arithmetic, so each batch costs two memory latencies, not 2B.
+        for (int s = s0 + lane; s < s1; s += B * L) {
+            float4 q[B][4];
+            float4 x[B][3];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int sj = s + j * L;
+                if (sj < s1) {
+                    const float4 *r = a.slots + 4 * (size_t)sj;
+                    q[j][0] = r[0];
+                    q[j][1] = r[1];
+                    q[j][2] = r[2];
+                    q[j][3] = r[3];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    x[j][0] = a.u[__float_as_int(q[j][0].x)];
+                    x[j][1] = a.u[__float_as_int(q[j][0].y)];
+                    x[j][2] = a.u[__float_as_int(q[j][0].z)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    const float4 *qq = q[j];
+                    const float4 *xx = x[j];
+                    slot_eval<MAT>(qq[0].w, qq[1].w, qq[2].w, make_float3(qq[1].x, qq[1].y, qq[1].z),
+                                   make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[3].x, qq[3].y, qq[3].z),
+                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
+                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
+                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
+                }
+            }
+        }
+    }
+    acc_group_sum<L>(acc);
+    if (!valid) return 0u;
+    if (a.kd_dt != 0.f) {
+        // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
+        const float4 up = a.uprev[v];
+        const float e0 = us.x - up.x, e1 = us.y - up.y, e2 = us.z - up.z;
+        const float k = a.kd_dt;
+        acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
+        acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
+        acc.g2 += k * (acc.h02 * e0 + acc.h12 * e1 + acc.h22 * e2);
+        const float f = 1.f + k;
+        acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
+    }
+    // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
+    const float4 tg = a.tgt[v];
+    const float w = tg.w;
+    acc.g0 += w * (us.x - tg.x);
+    acc.g1 += w * (us.y - tg.y);
+    acc.g2 += w * (us.z - tg.z);
+    acc.h00 += w;
+    acc.h11 += w;
+    acc.h22 += w;
+    const float3 dx = solve3_guarded(acc);
+    if (lane == 0) a.u[v] = make_float4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.f);
+    return metric_bits(dx);
+}
+
+// slots per lane per load batch, by lanes per vertex
+template <int L>
+struct Batch {
+    static constexpr int value = L == 1 ? 4 : L == 2 ? 4 : L == 4 ? 3 : L == 8 ? 2 : 1;
+};
+
+struct StepParams {
+    float dt, inv_dt, tol, kd_dt;
+    int max_iter;
+};
+
+__device__ __forceinline__ void init_vertex(int i, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt,
+                                            const float *mass, const uint8_t *active, float dt, float4 g)
+{
+    const float4 p = uprev[i];
+    if (active[i]) {
+        const float4 v = vel[i];
+        const float dt2 = dt * dt;
+        const float4 t = make_float4(p.x + dt * v.x + dt2 * g.x, p.y + dt * v.y + dt2 * g.y,
+                                     p.z + dt * v.z + dt2 * g.z, mass[i] / dt2);
+        u[i] = make_float4(t.x, t.y, t.z, 0.f);
+        tgt[i] = t;
+    } else {
+        u[i] = p;
+        tgt[i] = make_float4(p.x, p.y, p.z, 0.f);
+    }
+}
+
+// A whole time step of one body in one workgroup: initial guess, colour
+// sweeps with a barrier between colours until the body's own
+// max|dx_x+dx_y+dx_z| < tol (or max_iter), then the velocity update.  Used
+// when every colour is small enough that a per-colour launch would be
+// launch-latency bound; blockIdx.x selects the body (ranges: per body, C
+// colour ranges then the range of its other vertices, as (start, count)).
+template <int L, int B, int MAT>
+__global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 *ranges, int C, const float4 *grav,
+                                                        const float *mass, const uint8_t *active, float4 *uprev,
+                                                        float4 *vel, StepParams p, int2 *out)
+{
+    __shared__ uint32_t s_bits;
+    const int2 *r = ranges + (size_t)blockIdx.x * (C + 1);
+    const float4 g = grav[blockIdx.x];
+    for (int c = 0; c <= C; ++c) {
+        const int2 rc = r[c];
+        for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x)
+            init_vertex(i, a.u, uprev, vel, const_cast<float4 *>(a.tgt), mass, active, p.dt, g);
+    }
+    __syncthreads();
+    const int groups = blockDim.x / L;
+    const int gi = threadIdx.x / L, lane = threadIdx.x % L;
+    int it = 0;
+    uint32_t bits = 0;
+    for (; it < p.max_iter; ++it) {
+        uint32_t mb = 0;
+        for (int c = 0; c < C; ++c) {
+            const int2 rc = r[c];
+            for (int base = rc.x; base < rc.x + rc.y; base += groups) {
+                const int v = base + gi;
+                const bool valid = v < rc.x + rc.y;
+                const uint32_t m = update_vertex<L, B, MAT>(a, valid ? v : rc.x, valid, lane);
+                mb = mb > m ? mb : m;
+            }
+            __syncthreads();
+        }
+        const uint32_t all = block_max_u32(mb);
+        if (threadIdx.x == 0) s_bits = all;
+        __syncthreads();
+        bits = s_bits;
+        if (__uint_as_float(bits) < p.tol) break;
+    }
+    if (it == p.max_iter) it = p.max_iter - 1;
+    for (int c = 0; c <= C; ++c) {
+        const int2 rc = r[c];
+        for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) {
+            const float4 x = a.u[i], q = uprev[i];
+            vel[i] = make_float4((x.x - q.x) * p.inv_dt, (x.y - q.y) * p.inv_dt, (x.z - q.z) * p.inv_dt, 0.f);
+            uprev[i] = x;
+        }
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = make_int2(it + 1, (int)bits);
+}
+
+template <int L, int MAT>
+void launch_resident(const SweepArgs &a, const int2 *ranges, int C, const float4 *grav, const float *mass,
+                     const uint8_t *active, float4 *uprev, float4 *vel, StepParams p, int2 *out, int n_bodies,
+                     int block, hipStream_t st)
+{
+    k_step_resident<L, Batch<L>::value, MAT><<<n_bodies, block, 0, st>>>(a, ranges, C, grav, mass, active, uprev, vel,
+                                                                         p, out);
+}
+
+template <int MAT>
+void dispatch_resident(int L, const SweepArgs &a, const int2 *ranges, int C, const float4 *grav, const float *mass,
+                       const uint8_t *active, float4 *uprev, float4 *vel, StepParams p, int2 *out, int n_bodies,
+                       int block, hipStream_t st)
+{
+    switch (L) {
+    case 1: launch_resident<1, MAT>(a, ranges, C, grav, mass, active, uprev, vel, p, out, n_bodies, block, st); break;
+    case 2: launch_resident<2, MAT>(a, ranges, C, grav, mass, active, uprev, vel, p, out, n_bodies, block, st); break;
+    case 8: launch_resident<8, MAT>(a, ranges, C, grav, mass, active, uprev, vel, p, out, n_bodies, block, st); break;
+    case 16: launch_resident<16, MAT>(a, ranges, C, grav, mass, active, uprev, vel, p, out, n_bodies, block, st); break;
+    case 32: launch_resident<32, MAT>(a, ranges, C, grav, mass, active, uprev, vel, p, out, n_bodies, block, st); break;
+    default: launch_resident<4, MAT>(a, ranges, C, grav, mass, active, uprev, vel, p, out, n_bodies, block, st); break;
+    }
+}
+
+}  // namespace
+}  // namespace vbd

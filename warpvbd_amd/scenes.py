@@ -72,3 +72,33 @@ def mass_spring(cube=6, column=3, h=0.01, youngs_modulus=1e5, poissons_ratio=0.4
     return Scene(f"massspring{cube}+{column}", V, T, active, tip, np.full(m, float(density)),
                  np.full(m, float(youngs_modulus)), np.full(m, float(poissons_ratio)), tuple(gravity),
                  {"cube": cube, "column": column, "h": h})
+
+
+def mass_spring_batch(n_bodies=512, seed=3, youngs_modulus=1e5, poissons_ratio=0.4, density=1000.0):
+    """Bodies of the batched mass-spring workload: cube edge uniform in 6..14
+    voxels, column length uniform in 3..12 voxels, drawn from `seed`."""
+    rng = np.random.default_rng(seed)
+    cubes = rng.integers(6, 15, size=n_bodies)
+    cols = rng.integers(3, 13, size=n_bodies)
+    return [mass_spring(int(c), int(s), youngs_modulus=youngs_modulus, poissons_ratio=poissons_ratio,
+                        density=density) for c, s in zip(cubes, cols)]
+
+
+def mixed_batch(n_bodies=1024, seed=5, density=1000.0):
+    """Bodies of the mixed workload: half cantilevers (length uniform in 16..64
+    voxels, square cross-section 4..8 voxels), half mass-springs as in
+    mass_spring_batch; E log-uniform in [1e4, 1e6] Pa, nu uniform in
+    [0.3, 0.45], drawn from `seed`."""
+    rng = np.random.default_rng(seed)
+    half = n_bodies // 2
+    bodies = []
+    for i in range(n_bodies):
+        E = float(10 ** rng.uniform(4.0, 6.0))
+        nu = float(rng.uniform(0.3, 0.45))
+        if i < half:
+            L, c = int(rng.integers(16, 65)), int(rng.integers(4, 9))
+            bodies.append(cantilever((L, c, c), youngs_modulus=E, poissons_ratio=nu, density=density))
+        else:
+            m, s = int(rng.integers(6, 15)), int(rng.integers(3, 13))
+            bodies.append(mass_spring(m, s, youngs_modulus=E, poissons_ratio=nu, density=density))
+    return bodies
