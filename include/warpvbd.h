@@ -75,6 +75,14 @@ int vbd_reset(vbd_solver_t *h);
 int vbd_step(vbd_solver_t *h, const double *positions, double dt, double tol, int32_t max_iter,
              double *out_xyz, int32_t *iters_out, double *metric_out);
 
+/* Change the material of a solver whose elements share one (E, nu): the
+ * slot records are rescaled in place (no rebuild).  Replaces constructing a
+ * new VBDSolver with other youngs_modulus / poissons_ratio arrays. */
+int vbd_set_uniform_material(vbd_solver_t *h, double youngs, double poisson);
+/* Change the gravity vector of a single-body solver (the gravity argument,
+ * src/vbd_solver.py:378). */
+int vbd_set_gravity(vbd_solver_t *h, const double *gravity);
+
 /* Read / overwrite x_prev and v_prev (either pointer may be NULL). */
 int vbd_get_state(vbd_solver_t *h, double *x_prev, double *v_prev);
 int vbd_set_state(vbd_solver_t *h, const double *x_prev, const double *v_prev);
@@ -116,6 +124,40 @@ int vbd_batch_get_mode(vbd_solver_t *h, int32_t *mode, int32_t *lanes_per_vertex
  * fastest and returns its ms per step. */
 int vbd_batch_autotune(vbd_solver_t *h, double dt, double tol, int32_t max_iter, int32_t trial_steps,
                        double *best_ms_per_step);
+
+/* ------------------------------------------------------------------ warm start
+ * A 4-layer message-passing network (width 64, BF16 weights on MFMA, FP32
+ * accumulation) on the vertex graph of a single-body solver predicts each
+ * step's converged correction from the state at the start of the step.  No
+ * counterpart in WarpVBD.  Training samples come from the solver's own plain
+ * steps (vbd_mpnn_collect) and live in device memory (sample_capacity of them,
+ * 140 bytes per vertex each). */
+typedef struct vbd_mpnn vbd_mpnn_t;
+int vbd_mpnn_create(vbd_solver_t *h, int32_t sample_capacity, vbd_mpnn_t **out);
+void vbd_mpnn_destroy(vbd_mpnn_t *m);
+int vbd_mpnn_num_params(void);
+/* flat FP32 parameters: W_in[64x32] b_in[64] (W_l[64x128] b_l[64]) x 4 W_out[3x64] b_out[3] */
+int vbd_mpnn_set_params(vbd_mpnn_t *m, const float *params);
+int vbd_mpnn_get_params(vbd_mpnn_t *m, float *params);
+/* material features of the body (uniform E, nu) */
+int vbd_mpnn_set_material(vbd_mpnn_t *m, double youngs, double poisson);
+/* one plain solver step that stores (features before, correction after) in `slot` */
+int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t max_iter, int32_t *iters_out);
+/* loss of a stored sample; grads (nullable, vbd_mpnn_num_params floats) receives its gradient */
+int vbd_mpnn_loss_grad(vbd_mpnn_t *m, int32_t slot, float *loss_out, float *grads);
+/* Adam over stored samples [0, n_samples) */
+int vbd_mpnn_train(vbd_mpnn_t *m, int32_t n_samples, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                   float *loss_per_epoch);
+/* prediction [n,3] (original vertex order, units of dt^2 |g|) for sample `slot`, or the current state if slot < 0 */
+int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out);
+/* a stored sample and the graph, in original vertex order (each pointer nullable) */
+int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *targets, int32_t *nb_off, int32_t *nb,
+                         int32_t *n_edges);
+/* vbd_step from x_prev with the network's initial guess when its first-sweep
+ * residual (Jacobi max|dx_x+dx_y+dx_z|) is below the plain guess's;
+ * *used_network_out says which guess the step started from. */
+int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t max_iter, int32_t *iters_out,
+                  double *metric_out, int32_t *used_network_out);
 
 /* Mesh facts computed at create. */
 int vbd_info(vbd_solver_t *h, int32_t *n_vertices, int32_t *n_tets, int32_t *n_colors, int32_t *max_incident,

@@ -49,6 +49,21 @@ SIGNATURES = {
     "vbd_batch_set_mode": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
     "vbd_batch_get_mode": (ctypes.c_int, [_vp, _P(_i32), _P(_i32), _P(_i32), _P(_i32)]),
     "vbd_batch_autotune": (ctypes.c_int, [_vp, _d, _d, _i32, _i32, _P(_d)]),
+    "vbd_set_uniform_material": (ctypes.c_int, [_vp, _d, _d]),
+    "vbd_set_gravity": (ctypes.c_int, [_vp, _P(_d)]),
+    "vbd_mpnn_create": (ctypes.c_int, [_vp, _i32, _P(_vp)]),
+    "vbd_mpnn_destroy": (None, [_vp]),
+    "vbd_mpnn_num_params": (ctypes.c_int, []),
+    "vbd_mpnn_set_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
+    "vbd_mpnn_get_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
+    "vbd_mpnn_set_material": (ctypes.c_int, [_vp, _d, _d]),
+    "vbd_mpnn_collect": (ctypes.c_int, [_vp, _i32, _d, _d, _i32, _P(_i32)]),
+    "vbd_mpnn_loss_grad": (ctypes.c_int, [_vp, _i32, _P(ctypes.c_float), _P(ctypes.c_float)]),
+    "vbd_mpnn_train": (ctypes.c_int, [_vp, _i32, _i32, _i32, ctypes.c_float, ctypes.c_uint64, _P(ctypes.c_float)]),
+    "vbd_mpnn_predict": (ctypes.c_int, [_vp, _i32, _d, _P(ctypes.c_float)]),
+    "vbd_mpnn_read_sample": (ctypes.c_int, [_vp, _i32, _P(ctypes.c_float), _P(ctypes.c_float), _P(_i32), _P(_i32),
+                                            _P(_i32)]),
+    "vbd_step_warm": (ctypes.c_int, [_vp, _vp, _d, _d, _i32, _P(_i32), _P(_d), _P(_i32)]),
     "vbd_last_error": (ctypes.c_char_p, []),
 }
 

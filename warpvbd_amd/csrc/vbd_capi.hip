@@ -34,7 +34,7 @@ void vbd_destroy(vbd_solver_t *s)
     release_batch(s);
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slots, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
-                   s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new};
+                   s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (float4 *p : s->snap)
@@ -77,6 +77,13 @@ int vbd_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, 
         delete s;
         return VBD_EHIP;
     }
+    s->uniform_E = n_tets ? youngs[0] : 0.0;
+    s->uniform_nu = n_tets ? poisson[0] : 0.0;
+    for (int e = 1; e < n_tets; ++e)
+        if (youngs[e] != youngs[0] || poisson[e] != poisson[0]) {
+            s->uniform_E = s->uniform_nu = NAN;
+            break;
+        }
     r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active, nullptr, 1, nullptr);
     if (r != 0) {
         vbd_destroy(s);
@@ -188,6 +195,30 @@ int vbd_batch_autotune(vbd_solver_t *s, double dt, double tol, int32_t max_iter,
     CHECK_ARG(dt > 0.0 && max_iter >= 1 && trial_steps >= 1, "bad arguments");
     VBD_HIP(hipSetDevice(s->device));
     return autotune_bodies(s, dt, tol, max_iter, trial_steps, best_ms_per_step);
+}
+
+int vbd_set_uniform_material(vbd_solver_t *s, double youngs, double poisson)
+{
+    CHECK_ARG(s, "null handle");
+    CHECK_ARG(youngs > 0.0 && poisson > -1.0 && poisson < 0.5, "need E > 0 and -1 < nu < 0.5");
+    CHECK_ARG(s->uniform_E == s->uniform_E, "the solver's elements do not share one material");
+    VBD_HIP(hipSetDevice(s->device));
+    VBD_TRY(rescale_material(s, s->uniform_E, s->uniform_nu, youngs, poisson));
+    s->uniform_E = youngs;
+    s->uniform_nu = poisson;
+    return 0;
+}
+
+int vbd_set_gravity(vbd_solver_t *s, const double *gravity)
+{
+    CHECK_ARG(s && gravity, "null argument");
+    CHECK_ARG(!s->batched, "batched handles take per-body gravity at create");
+    VBD_HIP(hipSetDevice(s->device));
+    for (int k = 0; k < 3; ++k) s->grav[k] = gravity[k];
+    const float4 g = make_float4((float)gravity[0], (float)gravity[1], (float)gravity[2], 0.f);
+    VBD_HIP(hipMemcpyAsync(s->d_grav, &g, sizeof(float4), hipMemcpyHostToDevice, s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    return 0;
 }
 
 int vbd_reset(vbd_solver_t *s)

@@ -8,6 +8,8 @@
 
 #include "../../include/warpvbd.h"
 
+struct vbd_mpnn;
+
 namespace vbd {
 
 constexpr int kMaxColors = 256;            // src/coloring.py:50
@@ -110,6 +112,10 @@ struct vbd_solver {
     uint32_t *dxm = nullptr;         // per-vertex metric bits of the current iteration
     int32_t *n_live = nullptr, *n_live_host = nullptr;
     std::vector<vbd::CachedGraph> graphs;
+    int32_t *tets_dev = nullptr;     // elements (original ids), kept for the warm-start graph
+    double edge_length = 0.01;       // cube root of 6 x mean tet volume
+    vbd_mpnn *warm = nullptr;        // warm-start network used by the running step
+    double uniform_E = 0.0, uniform_nu = 0.0;  // the material when every element shares it (else NaN)
     float4 *d_grav = nullptr;        // per body gravity
     int2 *d_body_out = nullptr;      // per body (iterations, metric bits) of the last step
     int2 *body_out_host = nullptr;   // pinned
@@ -164,6 +170,9 @@ int import_positions(vbd_solver *s, const double *host_in, float4 *dst);
 int import_velocities(vbd_solver *s, const double *host_in);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
+int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);
+hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,
+                                      hipStream_t st);
 int step_bodies(vbd_solver *s, double dt, double tol, int max_iter);
 void release_batch(vbd_solver *s);
 int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int trial_steps, double *best_ms);

@@ -195,7 +195,7 @@ __global__ void k_color_round_graph(const int32_t *off, const int32_t *adj, int 
     for (int s = off[v]; s < off[v + 1]; ++s) {
         int u = adj[s];
         if (u >= 0 && u < v) {
-            int c = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
+            int c = colors[u];
             if (c >= 0 && c < kMaxColors) used[c >> 5] |= 1u << (c & 31);
         }
     }
@@ -211,10 +211,11 @@ __global__ void k_color_round_graph(const int32_t *off, const int32_t *adj, int 
         atomicExch(overflow, 1);
         c = kMaxColors - 1;
     }
-    if (c != colors[v]) {
-        __atomic_store_n(&colors[v], c, __ATOMIC_RELAXED);
-        atomicExch(changed, 1);
-    }
+    const bool ch = c != colors[v];
+    if (ch) __atomic_store_n(&colors[v], c, __ATOMIC_RELAXED);
+    // one plain store per wave that changed something: a device atomic per
+    // vertex serialises on the flag's address
+    if (__any(ch) && (threadIdx.x & 63) == 0) *(volatile int32_t *)changed = 1;
 }
 
 // Per-tet rest shape in FP64: Dm = [X0-X3 | X1-X3 | X2-X3], invDm, V = |det|/6,
@@ -331,10 +332,12 @@ __device__ __forceinline__ void brow(const double *iD, int k, double b[3])
     }
 }
 
-// 64-byte slot records (layout documented in vbd_device.h).
+// 64-byte slot records (layout documented in vbd_device.h), stored as four
+// arrays of n_slots float4 (q0 | q1 | q2 | q3).
 __global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const int32_t *inc_off, const int32_t *inc,
                               const int32_t *tets, const double *invDm, const double *vol, const double *mu,
-                              const double *lam, int material, int nswept, const int32_t *slot_off, float4 *slots)
+                              const double *lam, int material, int nswept, const int32_t *slot_off, float4 *slots,
+                              int64_t ns)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nswept) return;
@@ -366,11 +369,10 @@ __global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const i
         int o0 = iperm[tets[4 * e + other[0]]];
         int o1 = iperm[tets[4 * e + other[1]]];
         int o2 = iperm[tets[4 * e + other[2]]];
-        float4 *r = slots + 4 * (size_t)s;
-        r[0] = make_float4(__int_as_float(o0), __int_as_float(o1), __int_as_float(o2), ka);
-        r[1] = make_float4((float)bs[0], (float)bs[1], (float)bs[2], kb);
-        r[2] = make_float4((float)b0[0], (float)b0[1], (float)b0[2], kc);
-        r[3] = make_float4((float)b1[0], (float)b1[1], (float)b1[2], 0.f);
+        slots[s] = make_float4(__int_as_float(o0), __int_as_float(o1), __int_as_float(o2), ka);
+        slots[ns + s] = make_float4((float)bs[0], (float)bs[1], (float)bs[2], kb);
+        slots[2 * ns + s] = make_float4((float)b0[0], (float)b0[1], (float)b0[2], kc);
+        slots[3 * ns + s] = make_float4((float)b1[0], (float)b1[1], (float)b1[2], 0.f);
     }
 }
 
@@ -382,7 +384,7 @@ __global__ void k_mark_boundary(const float4 *slots, const int32_t *slot_off, in
     if (i >= v1) return;
     int local = 0;
     for (int s = slot_off[i]; s < slot_off[i + 1]; ++s) {
-        float4 q0 = slots[4 * (size_t)s];
+        float4 q0 = slots[s];
         int o[3] = {__float_as_int(q0.x), __float_as_int(q0.y), __float_as_int(q0.z)};
         for (int k = 0; k < 3; ++k)
             if (atomicExch(&mark[o[k]], tag) != tag) ++local;
@@ -477,7 +479,52 @@ int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_i
     return 0;
 }
 
+// Scale every slot's material constants from one uniform material to another.
+__global__ void k_rescale_material(int64_t ns, float4 *slots, float fa, float fb, int material)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    float4 q0 = slots[s], q1 = slots[ns + s], q2 = slots[2 * ns + s];
+    q0.w *= fa;
+    q1.w *= fb;
+    q2.w = material == kSNH ? 0.75f * q0.w : 0.f;
+    slots[s] = q0;
+    slots[ns + s] = q1;
+    slots[2 * ns + s] = q2;
+}
+
 }  // namespace
+
+int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1)
+{
+    auto lame = [](double E, double nu, double &mu, double &lam) {
+        mu = E / (2.0 * (1.0 + nu));
+        lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    };
+    double mu0, lam0, mu1, lam1;
+    lame(E0, nu0, mu0, lam0);
+    lame(E1, nu1, mu1, lam1);
+    double fa, fb;
+    if (s->material == kSNH) {
+        fa = mu1 / mu0;  // mu' = 4mu/3
+        fb = (lam1 + 5.0 * mu1 / 6.0) / (lam0 + 5.0 * mu0 / 6.0);
+    } else {
+        fa = mu1 / mu0;
+        fb = lam1 / lam0;
+    }
+    if (s->n_slots > 0)
+        k_rescale_material<<<nblk(s->n_slots), kTB, 0, s->stream>>>(s->n_slots, s->slots, (float)fa, (float)fb,
+                                                                   s->material);
+    VBD_HIP(hipGetLastError());
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,
+                                      hipStream_t st)
+{
+    return rocprim::exclusive_scan(tmp, bytes, in, out, 0, (size_t)n, rocprim::plus<int32_t>(), st);
+}
 
 int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_t *d_colors, int *ncolors,
                        hipStream_t st)
@@ -540,7 +587,8 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     double *d_rho, *d_E, *d_nu, *d_invDm, *d_vol, *d_mass, *d_mu, *d_lam;
     uint8_t *d_act_orig;
     VBD_TRY(dmalloc(&s->X, 3 * (size_t)n));
-    VBD_TRY(scr.get(&d_tets, 4 * (size_t)m));
+    VBD_TRY(dmalloc(&s->tets_dev, 4 * (size_t)m));
+    d_tets = s->tets_dev;
     VBD_TRY(scr.get(&d_rho, m));
     VBD_TRY(scr.get(&d_E, m));
     VBD_TRY(scr.get(&d_nu, m));
@@ -572,6 +620,13 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     int32_t singular = 0;
     VBD_HIP(hipMemcpyAsync(&singular, d_singular, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
+    if (m) {
+        std::vector<double> vol(m);
+        VBD_HIP(hipMemcpy(vol.data(), d_vol, sizeof(double) * m, hipMemcpyDeviceToHost));
+        double sv = 0.0;
+        for (double x : vol) sv += x;
+        s->edge_length = std::cbrt(6.0 * sv / m);
+    }
     if (singular) {
         set_error("an element has a singular rest shape matrix");
         return VBD_EDEGENERATE;
@@ -666,7 +721,8 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     k_vertex_mass<<<nblk(n), kTB, 0, st>>>(s->perm, s->inc_off, s->inc, d_mass, n, s->mass, d_act_orig, s->active);
     if (s->n_swept)
         k_build_slots<<<nblk(s->n_swept), kTB, 0, st>>>(s->perm, s->iperm, s->inc_off, s->inc, d_tets, d_invDm, d_vol,
-                                                       d_mu, d_lam, s->material, s->n_swept, s->slot_off, s->slots);
+                                                       d_mu, d_lam, s->material, s->n_swept, s->slot_off, s->slots,
+                                                       (int64_t)std::max<int64_t>(ns, 1));
 
     // algorithmic bytes per colour sweep: sum_v (40 + 68 k_v) + 12 |boundary|
     {

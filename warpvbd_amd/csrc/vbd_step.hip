@@ -24,6 +24,7 @@
 
 #include "vbd_device.h"
 #include "vbd_internal.h"
+#include "vbd_mpnn.h"
 #include "vbd_sweep.h"
 
 namespace vbd {
@@ -230,6 +231,7 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
+    a.ns = std::max<int64_t>(s->n_slots, 1);
     a.ctrl = s->ctrl;
     a.kd_dt = s->kd_dt;
     const int block = s->launch.block;
@@ -258,6 +260,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
+    a.ns = std::max<int64_t>(s->n_slots, 1);
     a.part = s->part;
     a.ctrl = s->ctrl;
     a.kd_dt = (float)(s->damping / dt);
@@ -285,7 +288,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
 
 int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric)
 {
-    if (s->resident) return step_resident(s, dt, tol, max_iter, iters, metric);
+    if (s->resident && !s->warm) return step_resident(s, dt, tol, max_iter, iters, metric);
     VBD_TRY(ensure_partials(s));
     const int n = s->n;
     s->kd_dt = (float)(s->damping / dt);
@@ -293,6 +296,7 @@ int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, 
     k_step_init<<<nblk(std::max(n, 1), 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, s->tgt, s->mass,
                                                                    s->active, (float)dt, g, s->ctrl);
     VBD_HIP(hipGetLastError());
+    if (s->warm) VBD_TRY(mpnn_apply_guess(s->warm, dt));
     // A tolerance is compared in FP32 on the device; tol = 0 never converges.
     const float ftol = (float)tol;
     int launched = 0;

@@ -9,12 +9,22 @@ namespace {
 
 inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
 
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// 16-byte load that bypasses cache allocation (read-once streams)
+__device__ __forceinline__ float4 load_stream(const float4 *p)
+{
+    const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct SweepArgs {
     float4 *__restrict__ u;
     const float4 *__restrict__ tgt;
     const float4 *__restrict__ uprev;
     const int32_t *__restrict__ slot_off;
-    const float4 *__restrict__ slots;
+    const float4 *__restrict__ slots;  // four arrays of ns float4 records: q0 | q1 | q2 | q3
+    int64_t ns;
     uint32_t *part;  // this launch's per-block metric maxima
     Ctrl *ctrl;
     float kd_dt;  // damping_coefficient / dt
@@ -82,11 +92,13 @@ __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, boo
             for (int j = 0; j < B; ++j) {
                 const int sj = s + j * L;
                 if (sj < s1) {
-                    const float4 *r = a.slots + 4 * (size_t)sj;
-                    q[j][0] = r[0];
-                    q[j][1] = r[1];
-                    q[j][2] = r[2];
-                    q[j][3] = r[3];
+                    // records are read once per sweep: stream them past the
+                    // caches that hold the gathered positions
+                    const float4 *r = a.slots + sj;
+                    q[j][0] = load_stream(r);
+                    q[j][1] = load_stream(r + a.ns);
+                    q[j][2] = load_stream(r + 2 * a.ns);
+                    q[j][3] = load_stream(r + 3 * a.ns);
                 }
             }
 #pragma unroll
@@ -138,10 +150,87 @@ __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, boo
     return metric_bits(dx);
 }
 
+// As update_vertex, without writing the update: the metric bits of the dx a
+// Jacobi pass from the positions in a.u would take.
+template <int L, int B, int MAT>
+__device__ __forceinline__ uint32_t eval_vertex(const SweepArgs &a, int v, bool valid, int lane)
+{
+    Acc acc;
+    acc_zero(acc);
+    const float4 uv4 = a.u[v];
+    const float3 us = make_float3(uv4.x, uv4.y, uv4.z);
+    if (valid) {
+        const int s0 = a.slot_off[v], s1 = a.slot_off[v + 1];
+        // B slots per lane per batch: all records, then all gathers, then the
+        // arithmetic, so each batch costs two memory latencies, not 2B.
+        for (int s = s0 + lane; s < s1; s += B * L) {
+            float4 q[B][4];
+            float4 x[B][3];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int sj = s + j * L;
+                if (sj < s1) {
+                    // records are read once per sweep: stream them past the
+                    // caches that hold the gathered positions
+                    const float4 *r = a.slots + sj;
+                    q[j][0] = load_stream(r);
+                    q[j][1] = load_stream(r + a.ns);
+                    q[j][2] = load_stream(r + 2 * a.ns);
+                    q[j][3] = load_stream(r + 3 * a.ns);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    x[j][0] = a.u[__float_as_int(q[j][0].x)];
+                    x[j][1] = a.u[__float_as_int(q[j][0].y)];
+                    x[j][2] = a.u[__float_as_int(q[j][0].z)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    const float4 *qq = q[j];
+                    const float4 *xx = x[j];
+                    slot_eval<MAT>(qq[0].w, qq[1].w, qq[2].w, make_float3(qq[1].x, qq[1].y, qq[1].z),
+                                   make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[3].x, qq[3].y, qq[3].z),
+                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
+                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
+                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
+                }
+            }
+        }
+    }
+    acc_group_sum<L>(acc);
+    if (!valid) return 0u;
+    if (a.kd_dt != 0.f) {
+        // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
+        const float4 up = a.uprev[v];
+        const float e0 = us.x - up.x, e1 = us.y - up.y, e2 = us.z - up.z;
+        const float k = a.kd_dt;
+        acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
+        acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
+        acc.g2 += k * (acc.h02 * e0 + acc.h12 * e1 + acc.h22 * e2);
+        const float f = 1.f + k;
+        acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
+    }
+    // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
+    const float4 tg = a.tgt[v];
+    const float w = tg.w;
+    acc.g0 += w * (us.x - tg.x);
+    acc.g1 += w * (us.y - tg.y);
+    acc.g2 += w * (us.z - tg.z);
+    acc.h00 += w;
+    acc.h11 += w;
+    acc.h22 += w;
+    const float3 dx = solve3_guarded(acc);
+    return metric_bits(dx);
+}
+
 // slots per lane per load batch, by lanes per vertex
 template <int L>
 struct Batch {
-    static constexpr int value = L == 1 ? 4 : L == 2 ? 4 : L == 4 ? 3 : L == 8 ? 2 : 1;
+    static constexpr int value = L == 1 ? 4 : L == 2 ? 4 : L == 4 ? 5 : L == 8 ? 3 : L == 16 ? 2 : 1;
 };
 
 struct StepParams {

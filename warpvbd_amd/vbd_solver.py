@@ -134,6 +134,7 @@ class VBDSolver:
         self.active_mask = np.ones(n, dtype=bool) if act is None else act.astype(bool)
         self.n_vertices, self.n_elements = n, m
         self.last_iterations, self.last_metric = 0, 0.0
+        self.last_used_network = False
         self._version = 0
         self._views = weakref.WeakSet()
         self._free_slots = list(range(_lib.VBD_MAX_SNAPSHOTS))
@@ -232,10 +233,15 @@ class VBDSolver:
         check(_lib.load().vbd_reset(self._h), "vbd_reset")
 
     # ------------------------------------------------------------------ time step (src/vbd_solver.py:509-612)
-    def step(self, positions, dt: float, dx_tol: float = None, max_iter: int = None) -> VertexArray:
+    def step(self, positions, dt: float, dx_tol: float = None, max_iter: int = None,
+             warm_start=None) -> VertexArray:
         """One implicit-Euler step from ``positions`` (normally ``self.old_positions``).
 
-        Returns the new positions; the input is not overwritten."""
+        Returns the new positions; the input is not overwritten.  ``warm_start``
+        (a :class:`warpvbd_amd.warm_start.WarmStart` built on this solver)
+        starts the iterations from the network's guess when its first-sweep
+        residual beats the plain inertia guess; ``last_used_network`` records
+        which one the step used."""
         if dx_tol is None:
             dx_tol = self.dx_tol
         if max_iter is None:
@@ -252,13 +258,35 @@ class VBDSolver:
             pos_ptr = ptr(xs, ctypes.c_double)
         self._before_mutation()
         it, met = ctypes.c_int32(0), ctypes.c_double(0.0)
-        check(L.vbd_step(self._h, pos_ptr, float(dt), float(dx_tol), int(max_iter), None, ctypes.byref(it),
-                         ctypes.byref(met)), "vbd_step")
+        if warm_start is not None:
+            if pos_ptr is not None:
+                check(L.vbd_set_state(self._h, pos_ptr, None), "vbd_set_state")
+            used = ctypes.c_int32(0)
+            check(L.vbd_step_warm(self._h, warm_start._h, float(dt), float(dx_tol), int(max_iter), ctypes.byref(it),
+                                  ctypes.byref(met), ctypes.byref(used)), "vbd_step_warm")
+            self.last_used_network = bool(used.value)
+        else:
+            check(L.vbd_step(self._h, pos_ptr, float(dt), float(dx_tol), int(max_iter), None, ctypes.byref(it),
+                             ctypes.byref(met)), "vbd_step")
         self.last_iterations, self.last_metric = it.value, met.value
         if it.value == max_iter:  # the reference warns when the last allowed iteration ran (:581-582)
             print(f"Warning: VBD solver did not converge within the maximum number of iterations. "
                   f"Final dx metric: {met.value}")
         return self._view()
+
+    # ------------------------------------------------------------------ material / gravity
+    def set_material(self, youngs_modulus: float, poissons_ratio: float):
+        """Give every element a new (E, nu); the elements must share one
+        material already."""
+        check(_lib.load().vbd_set_uniform_material(self._h, float(youngs_modulus), float(poissons_ratio)),
+              "vbd_set_uniform_material")
+        self.youngs_modulus = np.full(self.n_elements, float(youngs_modulus))
+        self.poissons_ratio = np.full(self.n_elements, float(poissons_ratio))
+
+    def set_gravity(self, gravity):
+        g = f64(gravity).reshape(3)
+        check(_lib.load().vbd_set_gravity(self._h, ptr(g, ctypes.c_double)), "vbd_set_gravity")
+        self.gravity = tuple(float(x) for x in g)
 
     # ------------------------------------------------------------------ mesh facts
     def info(self) -> dict:
