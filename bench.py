@@ -1,21 +1,37 @@
-"""Benchmark of the VBD time step on one MI355X.
+"""Benchmark of the VBD time step on one MI355X.  Prints one JSON line.
 
-Default workload: the 256x32x32-voxel cantilever (279,873 vertices, 1,310,720
-tets, x = 0 face pinned), stable Neo-Hookean, E = 1e5 Pa, nu = 0.4,
+Headline (`value`): the 256x32x32-voxel cantilever (279,873 vertices,
+1,310,720 tets, x = 0 face pinned), stable Neo-Hookean, E = 1e5 Pa, nu = 0.4,
 rho = 1000 kg/m^3, gravity -z, dt = 1e-3 s, tolerance 1e-7 m, at most 1000
 iterations per step, FP32.  A "step" is one implicit-Euler time step (all
-colour-sweep iterations to the tolerance).  Prints one JSON line.
+colour-sweep iterations to the tolerance); value = simulated seconds per wall
+second over the K timed steps.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--material snh|stvk]
-                  [--workload cantilever256|cantilever128|cantilever10] [--no-cpu]
+The roofline entry is for the colour-sweep kernel k_sweep: the model's
+algorithmic bytes of each timed launch (per colour: sum over its vertices of
+40 + 68 x incident tets, plus 12 per distinct gathered neighbour) divided by
+the launch's duration, taken by HIP events that hipExtLaunchKernelGGL attaches
+to the dispatch on the solver's stream, summed over every sweep launch of a
+colour with >= 4096 vertices during `--roofline-steps` further steps of the
+same run right after the timed region (the events add packets between
+launches, so they are kept out of the steps that give `value`).  `traffic` is the HBM
+bytes per launch from the committed rocprofv3 counter summary (FETCH_SIZE and
+WRITE_SIZE, calibrated against a copy kernel).
 
-value = simulated seconds per wall second over the K timed steps.  The
-roofline entry is for the colour-sweep kernel: algorithmic bytes of the swept
-colours divided by their summed kernel time, from HIP events recorded on the
-solver's stream around every sweep launch of a colour with >= 4096 vertices
-inside the timed region.  cpu_baseline times the FP64 CPU oracle (a C
-restatement of the reference's algorithm) on a bounded sample of the same
-workload on this host.
+Extra entries (each bounded to seconds):
+  stvk        the same beam with the StVK material
+  batched     512 mass-spring bodies (seed 3): one BatchedVBDSolver (per-body
+              convergence, autotuned form, hipGraph) vs the reference's form,
+              one VBDSolver on the concatenated mesh
+  warm_start  128x16x16 cantilever: the network trained on solver-made
+              trajectories (reduced budget, stated), then held-out seeds with
+              and without the warm start
+  mixed       1024 bodies (512 cantilevers + 512 mass-springs, seed 5):
+              batched + warm start vs plain batched, per-step latency percentiles
+cpu_baseline: the FP64 CPU oracle (C restatement of the reference algorithm,
+1 thread) on a bounded sample of the headline workload.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
 """
 import argparse
 import json
@@ -30,11 +46,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sim-s/wall-s at tol 1e-7 (warm-start+batched vs plain VBD); HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-WORKLOADS = {
-    "cantilever256": (256, 32, 32),
-    "cantilever128": (128, 16, 16),
-    "cantilever10": (10, 2, 2),
-}
 
 
 def parse():
@@ -42,28 +53,69 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=300)
     p.add_argument("--warmup", type=int, default=10)
-    p.add_argument("--workload", default="cantilever256", choices=sorted(WORKLOADS))
     p.add_argument("--material", default="snh", choices=["snh", "stvk"])
     p.add_argument("--tol", type=float, default=1e-7)
     p.add_argument("--max-iter", type=int, default=1000)
     p.add_argument("--dt", type=float, default=1e-3)
+    p.add_argument("--quick", action="store_true", help="headline workload only")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
-    p.add_argument("--lanes", type=int, default=0, help="lanes per vertex (0 = library default)")
+    p.add_argument("--batch-steps", type=int, default=30)
+    p.add_argument("--warm-trajectories", type=int, default=16)
+    p.add_argument("--warm-steps", type=int, default=60)
+    p.add_argument("--warm-epochs", type=int, default=12)
+    p.add_argument("--mixed-samples", type=int, default=24)
+    p.add_argument("--roofline-steps", type=int, default=100)
     return p.parse_args()
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_steps(solver, steps, dt, **kw):
+    iters, ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        solver.step(solver.old_positions, dt, **kw)
+        ms.append((time.perf_counter() - a) * 1e3)
+        iters.append(solver.last_iterations)
+    return time.perf_counter() - t0, np.array(iters), np.array(ms)
+
+
+def headline(args, material):
+    from warpvbd_amd import VBDSolver
+    from warpvbd_amd.scenes import cantilever
+
+    sc = cantilever((256, 32, 32))
+    solver = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, material=material,
+                       verbose=False)
+    for _ in range(args.warmup):
+        solver.step(solver.old_positions, args.dt)
+    wall, iters, ms = time_steps(solver, args.steps, args.dt)
+    # roofline pass: the same workload continued, every sweep launch timed by
+    # its dispatch's own start/end events (kept out of the timed region above,
+    # where the extra event packets would add ~10% to each iteration)
+    solver.profile(True, 4096)
+    solver.profile_read()
+    time_steps(solver, min(args.steps, args.roofline_steps), args.dt)
+    prof = solver.profile_read()
+    solver.profile(False)
+    return sc, solver, wall, iters, ms, prof
+
+
 def cpu_baseline(scene, args, gpu_iters_per_step):
-    """Time the FP64 oracle (reference formula, 1 thread) on a bounded sample:
-    one time step of the same workload, capped at a number of iterations that
-    fits the time budget; scaled to sim-s/wall-s with the GPU run's mean
-    iterations per step (both solve the same system to the same tolerance)."""
+    """FP64 oracle (reference formula, 1 thread) on a bounded sample: one time
+    step of the same workload capped at the iterations that fit the budget;
+    scaled to sim-s/wall-s with the GPU run's mean iterations per step (both
+    solve the same system to the same tolerance)."""
     import oracle
 
     oracle.build()
-    kw = scene.solver_kwargs()
     t0 = time.perf_counter()
-    s = oracle.OracleSolver(**kw, dx_tol=0.0, max_iter=1, material=args.material, expanded=True, threads=1)
+    s = oracle.OracleSolver(**scene.solver_kwargs(), dx_tol=0.0, max_iter=1, material=args.material,
+                            expanded=True, threads=1)
     setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     s.step(args.dt, dx_tol=0.0, max_iter=1)
@@ -73,18 +125,182 @@ def cpu_baseline(scene, args, gpu_iters_per_step):
     t0 = time.perf_counter()
     s.step(args.dt, dx_tol=0.0, max_iter=n_it)
     t_it = (time.perf_counter() - t0) / n_it
-    sim_per_wall = args.dt / (t_it * gpu_iters_per_step)
     return {
-        "value": sim_per_wall,
+        "value": args.dt / (t_it * gpu_iters_per_step),
         "unit": "sim-s/wall-s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"FP64 oracle (reference formula, oracle/vbd_oracle.c, 1 thread) on the same mesh: "
-                   f"{n_it} colour-sweep iterations of one time step, {t_it * 1e3:.1f} ms/iteration, "
-                   f"scaled by the GPU's {gpu_iters_per_step:.2f} iterations/step; setup {setup:.1f} s excluded"),
+        "sample": (f"FP64 oracle (reference formula, oracle/vbd_oracle.c, 1 thread) on the same 256x32x32 mesh: "
+                   f"{n_it} colour-sweep iterations of one time step at {t_it * 1e3:.1f} ms/iteration, scaled by "
+                   f"the GPU's {gpu_iters_per_step:.2f} iterations/step; setup {setup:.1f} s excluded"),
         "ms_per_iteration": t_it * 1e3,
         "host_cpus": len(os.sched_getaffinity(0)),
     }
+
+
+def batched_section(args):
+    from warpvbd_amd import BatchedVBDSolver, VBDSolver
+    from warpvbd_amd.batched import concat_bodies
+    from warpvbd_amd.scenes import mass_spring_batch
+
+    bodies = mass_spring_batch(512, seed=3)
+    cb = concat_bodies(bodies)
+    plain = VBDSolver(cb["initial_positions"], cb["elements"], cb["densities"], cb["youngs_modulus"],
+                      cb["poissons_ratio"], damping_coefficient=0.0, active_mask=cb["active_mask"],
+                      gravity=(0.0, 0.0, -9.81), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    for _ in range(3):
+        plain.step(plain.old_positions, args.dt)
+    w_plain, it_plain, ms_plain = time_steps(plain, args.batch_steps, args.dt)
+    del plain
+    bat = BatchedVBDSolver(bodies, dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    tune = bat.autotune(args.dt, trial_steps=2)
+    for _ in range(3):
+        bat.step(dt=args.dt)
+    w_bat, it_bat, ms_bat = time_steps(bat, args.batch_steps, args.dt)
+    nb = len(bodies)
+    sim = args.batch_steps * args.dt
+    return {
+        "workload": f"{nb} mass-spring bodies (cube 6..14, column 3..12 voxels, seed 3): "
+                    f"{len(cb['initial_positions'])} vertices, {len(cb['elements'])} tets, SNH E=1e5 nu=0.4, "
+                    f"dt=1e-3, tol 1e-7, {args.batch_steps} timed steps",
+        "plain_concatenated": {"aggregate_sim_s_per_wall_s": nb * sim / w_plain, "ms_per_step": 1e3 * w_plain /
+                               args.batch_steps, "iterations_per_step": float(it_plain.mean())},
+        "batched": {"aggregate_sim_s_per_wall_s": nb * sim / w_bat, "ms_per_step": 1e3 * w_bat / args.batch_steps,
+                    "max_body_iterations_per_step": float(it_bat.mean()),
+                    "mean_body_iterations_per_step": float(np.mean(bat.body_iterations)),
+                    "autotuned": {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}},
+        "speedup": w_plain / w_bat,
+    }
+
+
+def warm_section(args):
+    from warpvbd_amd import VBDSolver
+    from warpvbd_amd.scenes import cantilever
+    from warpvbd_amd.warm_start import WarmStart, gravity_in_cone
+
+    sc = cantilever((128, 16, 16))
+    s = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    cap = args.warm_trajectories * args.warm_steps // 2
+    ws = WarmStart(s, capacity=cap, seed=0)
+    t0 = time.perf_counter()
+    ws.generate(trajectories=args.warm_trajectories, steps=args.warm_steps, dt=args.dt, tol=args.tol, seed=0,
+                sample_every=2)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = ws.train(epochs=args.warm_epochs, batch=8, lr=2e-3, seed=0)
+    t_train = time.perf_counter() - t0
+    ref = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    rng = np.random.default_rng(1000)
+    it_p, it_w, used, err = [], [], [], []
+    w_p = w_w = 0.0
+    n_eval, eval_steps = 3, 40
+    diag = float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
+    for _ in range(n_eval):
+        E = float(10 ** rng.uniform(4, 6))
+        nu = float(rng.uniform(0.3, 0.45))
+        g = gravity_in_cone(rng, 30.0)
+        v0 = rng.normal(0, 0.05, size=(sc.n_vertices, 3)) * sc.active_mask[:, None]
+        for sol in (s, ref):
+            sol.reset()
+            sol.set_material(E, nu)
+            sol.set_gravity(g)
+            sol.old_velocities = v0
+        ws.set_material(E, nu)
+        for _ in range(eval_steps):
+            a = time.perf_counter()
+            xw = s.step(s.old_positions, args.dt, warm_start=ws)
+            b = time.perf_counter()
+            xp = ref.step(ref.old_positions, args.dt)
+            c = time.perf_counter()
+            w_w += b - a
+            w_p += c - b
+            it_w.append(s.last_iterations)
+            it_p.append(ref.last_iterations)
+            used.append(s.last_used_network)
+        err.append(float(np.abs(xw.numpy() - xp.numpy()).max()))
+    sim = n_eval * eval_steps * args.dt
+    return {
+        "workload": f"cantilever 128x16x16 ({sc.n_vertices} vertices); training: {args.warm_trajectories} seeded "
+                    f"trajectories x {args.warm_steps} steps (every 2nd step stored, {ws.n_samples} samples; "
+                    f"reduced from 200 x 300 to fit the benchmark budget), E log-U[1e4,1e6], nu U[0.3,0.45], gravity "
+                    f"in a 30 deg cone, v0 ~ N(0, 0.05); {args.warm_epochs} epochs; evaluation: {n_eval} held-out "
+                    f"seeds x {eval_steps} steps",
+        "generate_s": t_gen, "train_s": t_train, "train_loss_first_last": [float(hist[0]), float(hist[-1])],
+        "plain": {"iterations_per_step": float(np.mean(it_p)), "sim_s_per_wall_s": sim / w_p},
+        "warm": {"iterations_per_step": float(np.mean(it_w)), "sim_s_per_wall_s": sim / w_w,
+                 "network_used_fraction": float(np.mean(used))},
+        "max_abs_diff_vs_plain_m": max(err), "bbox_diagonal_m": diag,
+        "iteration_reduction": 1.0 - float(np.mean(it_w)) / float(np.mean(it_p)),
+    }
+
+
+def mixed_section(args):
+    """1024 bodies (512 cantilevers + 512 mass-springs, seed 5): the reference's
+    form (one VBDSolver on the concatenated mesh, global convergence), the
+    batched solver, and the batched solver with the warm start (trained on
+    samples the batched solver itself produced; reduced budget, stated)."""
+    from warpvbd_amd import BatchedVBDSolver, VBDSolver
+    from warpvbd_amd.batched import concat_bodies
+    from warpvbd_amd.scenes import mixed_batch
+    from warpvbd_amd.warm_start import WarmStart
+
+    bodies = mixed_batch(1024, seed=5)
+    cb = concat_bodies(bodies)
+    steps = args.batch_steps
+    sim = steps * args.dt
+    nb = len(bodies)
+    out = {"workload": f"{nb} bodies (512 cantilevers L 16..64 x c 4..8 voxels, 512 mass-springs), seed 5: "
+                       f"{len(cb['initial_positions'])} vertices, {len(cb['elements'])} tets, E log-U[1e4,1e6], "
+                       f"nu U[0.3,0.45], SNH, dt=1e-3, tol 1e-7, {steps} timed steps"}
+    plain = VBDSolver(cb["initial_positions"], cb["elements"], cb["densities"], cb["youngs_modulus"],
+                      cb["poissons_ratio"], damping_coefficient=0.0, active_mask=cb["active_mask"],
+                      gravity=(0.0, 0.0, -9.81), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    for _ in range(3):
+        plain.step(plain.old_positions, args.dt)
+    w, it, ms = time_steps(plain, steps, args.dt)
+    out["plain_concatenated"] = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
+                                 "iterations_per_step": float(it.mean()),
+                                 "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)]}
+    del plain
+    bat = BatchedVBDSolver(bodies, dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    tune = bat.autotune(args.dt, trial_steps=2)
+    ws = WarmStart(bat, capacity=args.mixed_samples, seed=0)
+    t0 = time.perf_counter()
+    for _ in range(args.mixed_samples):
+        ws.collect(args.dt, args.tol, args.max_iter)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = ws.train(epochs=args.warm_epochs, batch=4, lr=2e-3, seed=0)
+    t_train = time.perf_counter() - t0
+    for _ in range(3):
+        bat.step(dt=args.dt)
+    w, it, ms = time_steps(bat, steps, args.dt)
+    out["batched"] = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
+                      "max_body_iterations_per_step": float(it.mean()),
+                      "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)],
+                      "autotuned": {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}}
+    used = []
+    ms_w, it_w = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        bat.step(dt=args.dt, warm_start=ws)
+        ms_w.append((time.perf_counter() - a) * 1e3)
+        it_w.append(bat.last_iterations)
+        used.append(bat.last_used_network / nb)
+    w = time.perf_counter() - t0
+    out["batched_warm"] = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
+                           "max_body_iterations_per_step": float(np.mean(it_w)),
+                           "bodies_using_network": float(np.mean(used)),
+                           "step_ms_p50_p90_p99": [float(np.percentile(ms_w, q)) for q in (50, 90, 99)],
+                           "training": f"{args.mixed_samples} samples from the batch's own steps, "
+                                       f"{args.warm_epochs} epochs, {t_gen:.1f} s + {t_train:.1f} s",
+                           "train_loss_first_last": [float(hist[0]), float(hist[-1])]}
+    out["speedup_batched_vs_plain"] = out["batched"]["aggregate_sim_s_per_wall_s"] / \
+        out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
+    out["speedup_batched_warm_vs_plain"] = out["batched_warm"]["aggregate_sim_s_per_wall_s"] / \
+        out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
+    return out
 
 
 def main():
@@ -93,40 +309,18 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not built)"}))
         sys.exit(2)
-    from warpvbd_amd import VBDSolver
-    from warpvbd_amd.scenes import cantilever
-
-    nx = WORKLOADS[args.workload]
-    sc = cantilever(nx)
-    solver = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, material=args.material,
-                       verbose=False)
-    if args.lanes:
-        solver.set_launch(args.lanes, 256, 2048, 0)
-    for _ in range(args.warmup):
-        solver.step(solver.old_positions, args.dt)
-    solver.profile(True, 4096)
-    solver.profile_read()
-    iters = []
-    step_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        a = time.perf_counter()
-        solver.step(solver.old_positions, args.dt)
-        step_ms.append((time.perf_counter() - a) * 1e3)
-        iters.append(solver.last_iterations)
-    wall = time.perf_counter() - t0
-    prof = solver.profile_read()
+    log("headline")
+    sc, solver, wall, iters, ms, prof = headline(args, args.material)
     n_active = int(sc.active_mask.sum())
     achieved = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"sweep_traffic_{args.workload}_{args.material}.json")
+    tf = os.path.join(ROOT, "profiles", f"k_sweep_cantilever256x32x32_{args.material}_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    sim_per_wall = args.steps * args.dt / wall
     out = {
         "metric": METRIC,
-        "value": sim_per_wall,
+        "value": args.steps * args.dt / wall,
         "unit": "sim-s/wall-s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -136,16 +330,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (voxel cantilever built in-process; no dataset)",
+        "data": "synthetic (voxel meshes built in-process; no dataset)",
         "config": {
-            "workload": f"plain VBD cantilever {nx[0]}x{nx[1]}x{nx[2]} voxels of 1 cm "
-                        f"({sc.n_vertices} vertices, {sc.n_elements} tets), {args.material.upper()}",
-            "E": 1e5, "nu": 0.4, "rho": 1000.0, "dt": args.dt, "tol": args.tol, "max_iter": args.max_iter,
+            "workload": f"plain VBD cantilever 256x32x32 voxels of 1 cm ({sc.n_vertices} vertices, "
+                        f"{sc.n_elements} tets), {args.material.upper()}, E=1e5 Pa, nu=0.4, rho=1000, dt=1e-3 s, "
+                        f"tol 1e-7 m, max_iter {args.max_iter}",
             "parallelism": "single GPU",
         },
-        "iterations_per_step": float(np.mean(iters)),
-        "vertex_updates_per_s": float(np.sum(iters)) * n_active / wall,
-        "step_ms_p50_p90_p99": [float(np.percentile(step_ms, q)) for q in (50, 90, 99)],
+        "iterations_per_step": float(iters.mean()),
+        "vertex_updates_per_s": float(iters.sum()) * n_active / wall,
+        "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)],
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -153,14 +347,29 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_sweep (one colour, >= 4096 vertices)",
+            "kernel": "k_sweep (one colour sweep, colours with >= 4096 vertices)",
             "launches": prof["launches"],
             "avg_launch_us": prof["ms"] * 1e3 / max(1, prof["launches"]),
             "algorithmic_bytes_per_launch": prof["bytes"] / max(1, prof["launches"]),
         },
     }
+    del solver
+    if not args.quick:
+        if args.material == "snh":
+            log("stvk")
+            sc2, s2, w2, it2, ms2, pr2 = headline(argparse.Namespace(**{**vars(args), "steps": 100}), "stvk")
+            out["stvk"] = {"sim_s_per_wall_s": 100 * args.dt / w2, "iterations_per_step": float(it2.mean()),
+                           "sweep_GBps": pr2["bytes"] / (pr2["ms"] * 1e-3) / 1e9 if pr2["ms"] else None}
+            del s2
+        log("batched")
+        out["batched"] = batched_section(args)
+        log("warm start")
+        out["warm_start"] = warm_section(args)
+        log("mixed")
+        out["mixed"] = mixed_section(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, args, float(np.mean(iters)))
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(sc, args, float(iters.mean()))
     print(json.dumps(out), flush=True)
 
 

@@ -75,6 +75,17 @@ int vbd_reset(vbd_solver_t *h);
 int vbd_step(vbd_solver_t *h, const double *positions, double dt, double tol, int32_t max_iter,
              double *out_xyz, int32_t *iters_out, double *metric_out);
 
+/* vbd_step on device memory, for callers that hold positions in GPU tensors
+ * (replaces the wp.array in/out of VBDSolver.step, src/vbd_solver.py:509-612).
+ * positions_dev (nullable: start from x_prev) and out_dev (nullable) are f64
+ * [N*3] device pointers in original vertex order, on the handle's device.
+ * stream: the caller's hipStream_t (NULL = the null stream); the step is
+ * ordered after work already queued there, and work queued there afterwards
+ * sees out_dev.  The call still returns only when the iterations are done
+ * (the iteration count is read back). */
+int vbd_step_dev(vbd_solver_t *h, const double *positions_dev, double dt, double tol, int32_t max_iter,
+                 double *out_dev, void *stream, int32_t *iters_out, double *metric_out);
+
 /* Change the material of a solver whose elements share one (E, nu): the
  * slot records are rescaled in place (no rebuild).  Replaces constructing a
  * new VBDSolver with other youngs_modulus / poissons_ratio arrays. */
@@ -109,6 +120,8 @@ int vbd_batch_create(const double *rest_xyz, int32_t n_vertices, const int32_t *
                      double damping, int32_t material, vbd_solver_t **out);
 /* One step of every body; iters_out / metric_out: per body (nullable). */
 int vbd_batch_step(vbd_solver_t *h, double dt, double tol, int32_t max_iter, int32_t *iters_out, double *metric_out);
+/* Per-body iterations and metrics of the last step (after vbd_step or vbd_step_warm). */
+int vbd_batch_results(vbd_solver_t *h, int32_t *iters_out, double *metric_out);
 /* Execution form: VBD_BATCH_LAUNCHES (one launch per colour across all bodies,
  * iterations replayed from a captured hipGraph when use_graph) or
  * VBD_BATCH_RESIDENT (one workgroup of block_size threads per body runs its
@@ -139,8 +152,9 @@ int vbd_mpnn_num_params(void);
 /* flat FP32 parameters: W_in[64x32] b_in[64] (W_l[64x128] b_l[64]) x 4 W_out[3x64] b_out[3] */
 int vbd_mpnn_set_params(vbd_mpnn_t *m, const float *params);
 int vbd_mpnn_get_params(vbd_mpnn_t *m, float *params);
-/* material features of the body (uniform E, nu) */
+/* material features: one (E, nu) for every body, or one per body (arrays of n_bodies) */
 int vbd_mpnn_set_material(vbd_mpnn_t *m, double youngs, double poisson);
+int vbd_mpnn_set_body_materials(vbd_mpnn_t *m, const double *youngs, const double *poisson);
 /* one plain solver step that stores (features before, correction after) in `slot` */
 int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t max_iter, int32_t *iters_out);
 /* loss of a stored sample; grads (nullable, vbd_mpnn_num_params floats) receives its gradient */
@@ -153,9 +167,17 @@ int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out);
 /* a stored sample and the graph, in original vertex order (each pointer nullable) */
 int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *targets, int32_t *nb_off, int32_t *nb,
                          int32_t *n_edges);
-/* vbd_step from x_prev with the network's initial guess when its first-sweep
- * residual (Jacobi max|dx_x+dx_y+dx_z|) is below the plain guess's;
- * *used_network_out says which guess the step started from. */
+/* Which initial guess vbd_step_warm takes: per body the one with the smaller
+ * first-sweep residual (default), always the network's, or always the plain
+ * inertia prediction (the network is then not evaluated). */
+#define VBD_WARM_CHOOSE 0
+#define VBD_WARM_NETWORK 1
+#define VBD_WARM_PLAIN 2
+int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy);
+/* vbd_step (or vbd_batch_step on a batched handle) from x_prev with, per body,
+ * the network's initial guess when its first-sweep residual (Jacobi
+ * max|dx_x+dx_y+dx_z| over the body) is below the plain guess's;
+ * *used_network_out: the number of bodies that started from the network. */
 int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t max_iter, int32_t *iters_out,
                   double *metric_out, int32_t *used_network_out);
 
@@ -201,6 +223,10 @@ int vbd_color_bytes(vbd_solver_t *h, double *bytes);
  * iteration chunk between host waits (0 = adaptive). */
 int vbd_set_launch(vbd_solver_t *h, int32_t lanes_per_vertex, int32_t block_size, int32_t small_color,
                    int32_t chunk);
+
+/* Replay the iterations of a step from captured hipGraphs (default on; launches
+ * are enqueued one by one while profiling is enabled). */
+int vbd_set_graph(vbd_solver_t *h, int32_t use_graph);
 
 const char *vbd_last_error(void);
 

@@ -265,6 +265,38 @@ def test_returned_arrays_keep_their_values_and_explicit_positions():
     assert np.all(np.isfinite(d))
 
 
+def test_device_tensor_step_matches_host_step():
+    """step() on a torch tensor in GPU memory (vbd_step_dev, on torch's current
+    stream) returns a GPU tensor equal to the host path's result, and work
+    queued on the stream after it sees the result."""
+    import torch
+    sc = cantilever((12, 3, 2))
+    g, _ = make(sc)
+    h, _ = make(sc)
+    dev = torch.device("cuda:0")
+    x = torch.as_tensor(sc.vertices, dtype=torch.float64, device=dev)
+    xh = sc.vertices.copy()
+    # both paths take explicit f64 positions, so both round x - X to FP32 alike
+    for k in range(6):
+        xn = g.step(x, 1e-3)
+        assert isinstance(xn, torch.Tensor) and xn.is_cuda and xn.dtype == torch.float64
+        tip = xn[:, 2].max()  # queued after the step on the same stream
+        xh = h.step(xh, 1e-3).numpy()
+        assert np.array_equal(xn.cpu().numpy(), xh)
+        assert float(tip) == xh[:, 2].max()
+        assert g.last_iterations == h.last_iterations
+        x = xn
+    # a side stream orders the step after its own pending work
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        y = x * 1.0
+        xn = g.step(y, 1e-3)
+        z = xn.clone()
+    torch.cuda.synchronize()
+    xh = h.step(xh, 1e-3).numpy()
+    assert np.array_equal(z.cpu().numpy(), xh)
+
+
 def test_reset_and_state_roundtrip():
     sc = cantilever((10, 2, 2))
     g, r = make(sc)

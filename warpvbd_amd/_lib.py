@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwarpvbd.so")
+# WARPVBD_LIB selects another build of the same library (e.g. one compiled
+# with different layout defines); the default is the in-tree build.
+LIB_PATH = os.environ.get("WARPVBD_LIB", os.path.join(_HERE, "libwarpvbd.so"))
 
 VBD_MAX_SNAPSHOTS = 16
 MATERIALS = {"snh": 0, "stvk": 1}
@@ -27,6 +29,7 @@ SIGNATURES = {
     "vbd_destroy": (None, [_vp]),
     "vbd_reset": (ctypes.c_int, [_vp]),
     "vbd_step": (ctypes.c_int, [_vp, _P(_d), _d, _d, _i32, _P(_d), _P(_i32), _P(_d)]),
+    "vbd_step_dev": (ctypes.c_int, [_vp, _vp, _d, _d, _i32, _vp, _vp, _P(_i32), _P(_d)]),
     "vbd_get_state": (ctypes.c_int, [_vp, _P(_d), _P(_d)]),
     "vbd_set_state": (ctypes.c_int, [_vp, _P(_d), _P(_d)]),
     "vbd_snapshot": (ctypes.c_int, [_vp, _i32]),
@@ -43,10 +46,12 @@ SIGNATURES = {
     "vbd_color_bytes": (ctypes.c_int, [_vp, _P(_d)]),
     "vbd_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
     "vbd_use_device": (ctypes.c_int, [_i32]),
+    "vbd_set_graph": (ctypes.c_int, [_vp, _i32]),
     "vbd_batch_create": (ctypes.c_int, [_P(_d), _i32, _P(_i32), _i32, _P(_i32), _i32, _P(_d), _P(_d), _P(_d), _P(_u8),
                                         _P(_d), _d, _i32, _P(_vp)]),
     "vbd_batch_step": (ctypes.c_int, [_vp, _d, _d, _i32, _P(_i32), _P(_d)]),
     "vbd_batch_set_mode": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
+    "vbd_batch_results": (ctypes.c_int, [_vp, _P(_i32), _P(_d)]),
     "vbd_batch_get_mode": (ctypes.c_int, [_vp, _P(_i32), _P(_i32), _P(_i32), _P(_i32)]),
     "vbd_batch_autotune": (ctypes.c_int, [_vp, _d, _d, _i32, _i32, _P(_d)]),
     "vbd_set_uniform_material": (ctypes.c_int, [_vp, _d, _d]),
@@ -54,9 +59,11 @@ SIGNATURES = {
     "vbd_mpnn_create": (ctypes.c_int, [_vp, _i32, _P(_vp)]),
     "vbd_mpnn_destroy": (None, [_vp]),
     "vbd_mpnn_num_params": (ctypes.c_int, []),
+    "vbd_mpnn_set_policy": (ctypes.c_int, [_vp, _i32]),
     "vbd_mpnn_set_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_get_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_set_material": (ctypes.c_int, [_vp, _d, _d]),
+    "vbd_mpnn_set_body_materials": (ctypes.c_int, [_vp, _P(_d), _P(_d)]),
     "vbd_mpnn_collect": (ctypes.c_int, [_vp, _i32, _d, _d, _i32, _P(_i32)]),
     "vbd_mpnn_loss_grad": (ctypes.c_int, [_vp, _i32, _P(ctypes.c_float), _P(ctypes.c_float)]),
     "vbd_mpnn_train": (ctypes.c_int, [_vp, _i32, _i32, _i32, ctypes.c_float, ctypes.c_uint64, _P(ctypes.c_float)]),
@@ -74,9 +81,25 @@ class VBDError(RuntimeError):
     pass
 
 
+def _share_torch_runtime():
+    """Load PyTorch first when it is installed.  PyTorch ships its own HIP
+    runtime with the same soname as the system one this library links; the
+    first one loaded serves the whole process, and PyTorch cannot initialise
+    its GPU on the system runtime.  With PyTorch loaded first, both use its
+    runtime and tensors can be passed to the solver.  WARPVBD_NO_TORCH=1 skips
+    this."""
+    if os.environ.get("WARPVBD_NO_TORCH", "0") not in ("", "0"):
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load() -> ctypes.CDLL:
     global _lib
     if _lib is None:
+        _share_torch_runtime()
         if not os.path.exists(LIB_PATH):
             raise VBDError(f"{LIB_PATH} is missing: build it with `python -m warpvbd_amd.build` "
                            "(there is no CPU fallback)")

@@ -30,6 +30,33 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "warpvbd_amd", "autotune.json"))
 
 
+def concat_bodies(bodies):
+    """Concatenate bodies into one mesh (the reference's parallel-environment
+    form, examples/run_mass_spring_parallel.py:35-119).  Returns a dict with
+    initial_positions, elements, densities, youngs_modulus, poissons_ratio,
+    active_mask, gravity (per body, [B, 3]) and body_vertex_offsets."""
+    Xs, Ts, rho, E, nu, act, grav, voff = [], [], [], [], [], [], [], [0]
+    for b in bodies:
+        d = b if isinstance(b, dict) else b.solver_kwargs()
+        X = f64(d["initial_positions"]).reshape(-1, 3)
+        T = np.asarray(to_numpy(d["elements"]), dtype=np.int64).reshape(-1, 4)
+        m = len(T)
+        Xs.append(X)
+        Ts.append(T + voff[-1])
+        rho.append(np.broadcast_to(f64(d["densities"]), (m,)))
+        E.append(np.broadcast_to(f64(d["youngs_modulus"]), (m,)))
+        nu.append(np.broadcast_to(f64(d["poissons_ratio"]), (m,)))
+        a = d.get("active_mask")
+        act.append(np.ones(len(X), bool) if a is None else np.asarray(to_numpy(a), bool).reshape(len(X)))
+        grav.append(f64(d.get("gravity", (0.0, 0.0, -9.81))).reshape(3))
+        voff.append(voff[-1] + len(X))
+    return dict(initial_positions=np.ascontiguousarray(np.concatenate(Xs)), elements=i32(np.concatenate(Ts)),
+                densities=np.ascontiguousarray(np.concatenate(rho)),
+                youngs_modulus=np.ascontiguousarray(np.concatenate(E)),
+                poissons_ratio=np.ascontiguousarray(np.concatenate(nu)), active_mask=np.concatenate(act),
+                gravity=np.ascontiguousarray(np.stack(grav)), body_vertex_offsets=np.array(voff, dtype=np.int32))
+
+
 class BatchedVBDSolver(VBDSolver):
     """Bodies are objects with ``vertices``/``initial_positions``, ``elements``,
     per-element ``density``/``densities``, ``youngs_modulus``, ``poissons_ratio``,
@@ -40,27 +67,13 @@ class BatchedVBDSolver(VBDSolver):
                  device="cuda", material: str = "snh", verbose: bool = True):
         L = _lib.load()
         self.device = device
-        _lib.use_device(device)
-        Xs, Ts, rho, E, nu, act, grav, voff = [], [], [], [], [], [], [], [0]
-        for b in bodies:
-            d = b if isinstance(b, dict) else b.solver_kwargs()
-            X = f64(d["initial_positions"]).reshape(-1, 3)
-            T = np.asarray(to_numpy(d["elements"]), dtype=np.int64).reshape(-1, 4)
-            m = len(T)
-            Xs.append(X)
-            Ts.append(T + voff[-1])
-            rho.append(np.broadcast_to(f64(d["densities"]), (m,)))
-            E.append(np.broadcast_to(f64(d["youngs_modulus"]), (m,)))
-            nu.append(np.broadcast_to(f64(d["poissons_ratio"]), (m,)))
-            a = d.get("active_mask")
-            act.append(np.ones(len(X), bool) if a is None else np.asarray(to_numpy(a), bool).reshape(len(X)))
-            grav.append(f64(d.get("gravity", (0.0, 0.0, -9.81))).reshape(3))
-            voff.append(voff[-1] + len(X))
-        X = np.ascontiguousarray(np.concatenate(Xs))
-        T = i32(np.concatenate(Ts))
-        rho, E, nu = (np.ascontiguousarray(np.concatenate(a)) for a in (rho, E, nu))
-        actu8 = np.ascontiguousarray(np.concatenate(act), dtype=np.uint8)
-        G = np.ascontiguousarray(np.stack(grav))
+        self.device_index = _lib.use_device(device)
+        cb = concat_bodies(bodies)
+        X, T = cb["initial_positions"], cb["elements"]
+        rho, E, nu = cb["densities"], cb["youngs_modulus"], cb["poissons_ratio"]
+        actu8 = np.ascontiguousarray(cb["active_mask"], dtype=np.uint8)
+        G = cb["gravity"]
+        voff = cb["body_vertex_offsets"]
         self.body_vertex_offsets = np.array(voff, dtype=np.int32)
         self.n_bodies = len(bodies)
         n, m = len(X), len(T)
@@ -95,9 +108,13 @@ class BatchedVBDSolver(VBDSolver):
             print(f"Batched solver: {self.n_bodies} bodies, {n} vertices, {m} elements, {self.n_colors} colors")
 
     # ------------------------------------------------------------------ stepping
-    def step(self, positions=None, dt: float = 1e-3, dx_tol: float = None, max_iter: int = None) -> VertexArray:
+    def step(self, positions=None, dt: float = 1e-3, dx_tol: float = None, max_iter: int = None,
+             warm_start=None) -> VertexArray:
         """Advance every body by dt.  ``positions`` (optional, all bodies
-        concatenated) overrides x_prev as in ``VBDSolver.step``."""
+        concatenated) overrides x_prev as in ``VBDSolver.step``.  With
+        ``warm_start`` each body starts from the network guess when that wins
+        the first-sweep comparison; ``last_used_network`` is the number of
+        bodies that did."""
         if positions is not None and not (isinstance(positions, VertexArray) and positions._is_current(self)):
             self.old_positions = positions
         tol = self.dx_tol if dx_tol is None else dx_tol
@@ -105,8 +122,15 @@ class BatchedVBDSolver(VBDSolver):
         self._before_mutation()
         its = np.empty(self.n_bodies, np.int32)
         mets = np.empty(self.n_bodies)
+        if warm_start is not None:
+            it, met, used = ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
+            check(_lib.load().vbd_step_warm(self._h, warm_start._h, float(dt), float(tol), int(mi), ctypes.byref(it),
+                                            ctypes.byref(met), ctypes.byref(used)), "vbd_step_warm")
+            self.last_used_network = int(used.value)
         check(_lib.load().vbd_batch_step(self._h, float(dt), float(tol), int(mi), ptr(its, ctypes.c_int32),
-                                         ptr(mets, ctypes.c_double)), "vbd_batch_step")
+                                         ptr(mets, ctypes.c_double)) if warm_start is None else
+              _lib.load().vbd_batch_results(self._h, ptr(its, ctypes.c_int32), ptr(mets, ctypes.c_double)),
+              "vbd_batch_step")
         self.body_iterations, self.body_metrics = its, mets
         self.last_iterations = int(its.max()) if len(its) else 0
         self.last_metric = float(np.nanmax(mets)) if len(mets) else 0.0

@@ -23,6 +23,7 @@
 
 #include "vbd_device.h"
 #include "vbd_internal.h"
+#include "vbd_mpnn.h"
 #include "vbd_sweep.h"
 
 namespace vbd {
@@ -209,6 +210,7 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
         s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt, s->n_bodies,
         s->bstate, s->n_live);
     VBD_HIP(hipGetLastError());
+    if (s->warm) VBD_TRY(mpnn_apply_guess(s->warm, dt));
     int launched = 0;
     int chunk = std::max(1, s->last_iters);
     while (launched < max_iter) {
@@ -249,6 +251,17 @@ int step_bodies_resident(vbd_solver *s, double dt, double tol, int max_iter)
     p.tol = (float)tol;
     p.kd_dt = a.kd_dt;
     p.max_iter = max_iter;
+    p.skip_init = 0;
+    if (s->warm) {
+        // initial guess outside the resident kernel: plain guess, then the network's where it wins
+        VBD_TRY(ensure_batch_buffers(s));
+        k_init_bodies<<<nblk(std::max(s->n, s->n_bodies), 256), 256, 0, s->stream>>>(
+            s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt,
+            s->n_bodies, s->bstate, s->n_live);
+        VBD_HIP(hipGetLastError());
+        VBD_TRY(mpnn_apply_guess(s->warm, dt));
+        p.skip_init = 1;
+    }
     const int L = s->launch.lanes, block = s->launch.resident_block;
     if (s->material == kSNH)
         dispatch_resident<kSNH>(L, a, s->d_ranges, s->ncolors, s->d_grav, s->mass, s->active, s->uprev, s->vel, p,

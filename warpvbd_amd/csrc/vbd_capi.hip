@@ -43,6 +43,8 @@ void vbd_destroy(vbd_solver_t *s)
     if (s->xio_host) (void)hipHostFree(s->xio_host);
     if (s->body_out_host) (void)hipHostFree(s->body_out_host);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    if (s->ev_in) (void)hipEventDestroy(s->ev_in);
+    if (s->ev_out) (void)hipEventDestroy(s->ev_out);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -163,6 +165,16 @@ int vbd_batch_step(vbd_solver_t *s, double dt, double tol, int32_t max_iter, int
     return 0;
 }
 
+int vbd_batch_results(vbd_solver_t *s, int32_t *iters_out, double *metric_out)
+{
+    CHECK_ARG(s && s->batched, "not a batched handle");
+    for (int b = 0; b < s->n_bodies; ++b) {
+        if (iters_out) iters_out[b] = s->body_out_host[b].x;
+        if (metric_out) metric_out[b] = (double)__builtin_bit_cast(float, (uint32_t)s->body_out_host[b].y);
+    }
+    return 0;
+}
+
 int vbd_batch_set_mode(vbd_solver_t *s, int32_t mode, int32_t lanes, int32_t block, int32_t use_graph)
 {
     CHECK_ARG(s && s->batched, "not a batched handle");
@@ -256,6 +268,43 @@ int vbd_step(vbd_solver_t *s, const double *positions, double dt, double tol, in
         VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
     }
     if (out_xyz) VBD_TRY(export_positions(s, s->u, out_xyz));
+    if (iters_out) *iters_out = it;
+    if (metric_out) *metric_out = met;
+    return 0;
+}
+
+int vbd_step_dev(vbd_solver_t *s, const double *positions_dev, double dt, double tol, int32_t max_iter,
+                 double *out_dev, void *stream, int32_t *iters_out, double *metric_out)
+{
+    CHECK_ARG(s, "null handle");
+    CHECK_ARG(dt > 0.0, "dt must be > 0");
+    CHECK_ARG(max_iter >= 1, "max_iter must be >= 1");
+    VBD_HIP(hipSetDevice(s->device));
+    hipStream_t cs = (hipStream_t)stream;
+    if (!s->ev_in) VBD_HIP(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
+    if (!s->ev_out) VBD_HIP(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+    // the step runs after what the caller queued on its stream ...
+    VBD_HIP(hipEventRecord(s->ev_in, cs));
+    VBD_HIP(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    if (positions_dev) VBD_TRY(import_positions_dev(s, positions_dev, s->uprev));
+    int it = 0;
+    double met = 0.0;
+    if (s->batched) {
+        VBD_TRY(step_bodies(s, dt, tol, max_iter));
+        float mx = 0.f;
+        for (int b = 0; b < s->n_bodies; ++b) {
+            it = std::max(it, s->body_out_host[b].x);
+            const float m = __builtin_bit_cast(float, (uint32_t)s->body_out_host[b].y);
+            if (!(m <= mx)) mx = m;
+        }
+        met = mx;
+    } else {
+        VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
+    }
+    if (out_dev) VBD_TRY(export_positions_dev(s, s->u, out_dev));
+    // ... and what the caller queues next runs after the result is written
+    VBD_HIP(hipEventRecord(s->ev_out, s->stream));
+    VBD_HIP(hipStreamWaitEvent(cs, s->ev_out, 0));
     if (iters_out) *iters_out = it;
     if (metric_out) *metric_out = met;
     return 0;
@@ -457,6 +506,13 @@ int vbd_use_device(int32_t device)
     return 0;
 }
 
+int vbd_set_graph(vbd_solver_t *s, int32_t use_graph)
+{
+    CHECK_ARG(s, "null handle");
+    s->launch.graph = use_graph != 0;
+    return 0;
+}
+
 int vbd_profile_enable(vbd_solver_t *s, int32_t enable, int32_t min_vertices)
 {
     CHECK_ARG(s, "null handle");
@@ -497,6 +553,11 @@ int vbd_set_launch(vbd_solver_t *s, int32_t lanes, int32_t block, int32_t small_
     s->launch.small_color = small_color;
     s->launch.chunk = chunk;
     plan_segments(s);
+    for (auto &g : s->graphs) {  // captured with the previous plan
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    s->graphs.clear();
     return 0;
 }
 

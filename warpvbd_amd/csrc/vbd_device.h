@@ -22,7 +22,24 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Slot record storage: 0 = four arrays of n_slots float4 (q0 | q1 | q2 | q3),
+// 1 = interleaved 64-byte records.
+#ifndef VBD_SLOT_AOS
+#define VBD_SLOT_AOS 1
+#endif
+
 namespace vbd {
+
+// index of float4 q (0..3) of slot s among ns slots
+__host__ __device__ __forceinline__ int64_t slot_q(int64_t s, int q, int64_t ns)
+{
+#if VBD_SLOT_AOS
+    (void)ns;
+    return 4 * s + q;
+#else
+    return (int64_t)q * ns + s;
+#endif
+}
 
 enum Material : int { kSNH = 0, kStVK = 1 };
 

@@ -103,6 +103,7 @@ struct vbd_solver {
     bool resident = false;           // whole step in one workgroup (all colours small)
     int resident_lanes = 32;         // lanes per vertex of that form
     int n_bodies = 1;
+    std::vector<int32_t> body_voff_host;  // body b owns original vertices [body_voff_host[b], body_voff_host[b+1])
     std::vector<int2> ranges_host;   // per body: (start, count) of each colour, then of its other vertices
     int2 *d_ranges = nullptr;        // device copy of ranges_host
     int32_t *body_new = nullptr;     // body of each vertex, colour-major order
@@ -151,6 +152,7 @@ struct vbd_solver {
     bool prof = false;
     int prof_min = 0;
     std::vector<hipEvent_t> ev_pool;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against a caller's stream (vbd_step_dev)
     std::vector<std::pair<int, int>> ev_pending;  // (event index, colour)
     double prof_ms = 0.0, prof_bytes = 0.0;
     int64_t prof_launches = 0;
@@ -168,6 +170,9 @@ int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, 
 int export_positions(vbd_solver *s, const float4 *src, double *host_out);
 int import_positions(vbd_solver *s, const double *host_in, float4 *dst);
 int import_velocities(vbd_solver *s, const double *host_in);
+// device f64 [n*3] buffers in original vertex order, enqueued on s->stream
+int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out);
+int import_positions_dev(vbd_solver *s, const double *dev_in, float4 *dst);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);

@@ -290,39 +290,50 @@ __global__ void k_refresh_bf16(int np, const float *P, uint16_t *Pb, uint16_t *P
 struct FeatParams {
     float dt, s;             // step, target scale dt^2 |g|
     float inv_h;             // 1 / edge length
-    float inv_L;             // 1 / bounding-box diagonal
-    float cx, cy, cz;        // bounding-box centre
-    float gx, gy, gz;        // gravity
-    float logE, nu;          // material
+};
+
+// Per body: geo = (centre xyz, 1 / bounding-box diagonal), mat = (log10 E, nu).
+struct BodyFeat {
+    const int32_t *body;     // body of each vertex (colour-major)
+    const float4 *geo;
+    const float2 *mat;
+    const float4 *grav;
 };
 
 __device__ __forceinline__ float3 inertia_disp(int i, const float4 *uprev, const float4 *vel, const uint8_t *active,
-                                               const FeatParams &p)
+                                               float dt, float4 g)
 {
     const float4 u = uprev[i];
     if (!active[i]) return make_float3(u.x, u.y, u.z);
     const float4 v = vel[i];
-    const float dt2 = p.dt * p.dt;
-    return make_float3(u.x + p.dt * v.x + dt2 * p.gx, u.y + p.dt * v.y + dt2 * p.gy, u.z + p.dt * v.z + dt2 * p.gz);
+    const float dt2 = dt * dt;
+    return make_float3(u.x + dt * v.x + dt2 * g.x, u.y + dt * v.y + dt2 * g.y, u.z + dt * v.z + dt2 * g.z);
 }
 
-// Per-vertex input features (colour-major order), see oracle/mpnn_ref.py for
-// the network; the feature definition is the solver state at the start of a
-// step: rest position, inertia-prediction displacement, its increment,
-// velocity, a local deformation estimate from the neighbours' inertia
-// predictions, pin flag, material, gravity direction.
+// Per-vertex input features (colour-major order); the network is defined in
+// oracle/mpnn_ref.py.  From the solver state at the start of a step: rest
+// position relative to its body, inertia-prediction displacement, its
+// increment, velocity, a local deformation estimate from the neighbours'
+// inertia predictions, pin flag, the body's material and gravity direction,
+// and the update a Jacobi pass from the inertia prediction would take (dxj,
+// swept vertices [0, nsw) only).
 __global__ void k_features(int n, const float4 *Xr, const float4 *uprev, const float4 *vel, const uint8_t *active,
-                           const int32_t *nb_off, const int32_t *nb, FeatParams p, float *f)
+                           const int32_t *nb_off, const int32_t *nb, FeatParams p, BodyFeat bf, const float4 *dxj,
+                           int nsw, float *f)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float *o = f + (size_t)i * kFin;
+    const int b = bf.body[i];
+    const float4 geo = bf.geo[b];
+    const float2 mat = bf.mat[b];
+    const float4 g = bf.grav[b];
     const float4 X = Xr[i];
-    const float3 uy = inertia_disp(i, uprev, vel, active, p);
+    const float3 uy = inertia_disp(i, uprev, vel, active, p.dt, g);
     const float4 up = uprev[i], v = vel[i];
-    o[0] = (X.x - p.cx) * p.inv_L;
-    o[1] = (X.y - p.cy) * p.inv_L;
-    o[2] = (X.z - p.cz) * p.inv_L;
+    o[0] = (X.x - geo.x) * geo.w;
+    o[1] = (X.y - geo.y) * geo.w;
+    o[2] = (X.z - geo.z) * geo.w;
     o[3] = uy.x * p.inv_h;
     o[4] = uy.y * p.inv_h;
     o[5] = uy.z * p.inv_h;
@@ -333,28 +344,42 @@ __global__ void k_features(int n, const float4 *Xr, const float4 *uprev, const f
     o[10] = v.y * p.dt * p.inv_h;
     o[11] = v.z * p.dt * p.inv_h;
     float D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int b = nb_off[i], e = nb_off[i + 1];
-    for (int q = b; q < e; ++q) {
+    const int q0 = nb_off[i], q1 = nb_off[i + 1];
+    for (int q = q0; q < q1; ++q) {
         const int j = nb[q];
         const float4 Xj = Xr[j];
-        const float3 uj = inertia_disp(j, uprev, vel, active, p);
+        const float3 uj = inertia_disp(j, uprev, vel, active, p.dt, g);
         const float dX[3] = {Xj.x - X.x, Xj.y - X.y, Xj.z - X.z};
         const float du[3] = {uj.x - uy.x, uj.y - uy.y, uj.z - uy.z};
         const float w = 1.f / (dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2]);
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) D[3 * r + c] += du[r] * dX[c] * w;
     }
-    const float sc = e > b ? 1.f / (float)(e - b) : 0.f;
+    const float sc = q1 > q0 ? 1.f / (float)(q1 - q0) : 0.f;
     for (int q = 0; q < 9; ++q) o[12 + q] = D[q] * sc * 100.f;
     o[21] = active[i] ? 0.f : 1.f;
-    o[22] = p.logE - 5.f;
-    o[23] = (p.nu - 0.375f) * 10.f;
+    o[22] = mat.x - 5.f;
+    o[23] = (mat.y - 0.375f) * 10.f;
     o[24] = 1.f;
-    const float gn = sqrtf(p.gx * p.gx + p.gy * p.gy + p.gz * p.gz) + 1e-30f;
-    o[25] = p.gx / gn;
-    o[26] = p.gy / gn;
-    o[27] = p.gz / gn;
-    o[28] = o[29] = o[30] = o[31] = 0.f;
+    const float gn = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z) + 1e-30f;
+    o[25] = g.x / gn;
+    o[26] = g.y / gn;
+    o[27] = g.z / gn;
+    const float4 d = i < nsw ? dxj[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    o[28] = d.x / p.s;
+    o[29] = d.y / p.s;
+    o[30] = d.z / p.s;
+    o[31] = 0.f;
+}
+
+// The plain initial guess (inertia prediction, per-body gravity) in u and the
+// inertia target in tgt, as the step's own init writes them.
+__global__ void k_plain_guess(int n, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt,
+                              const float *mass, const uint8_t *active, const int32_t *body, const float4 *grav,
+                              float dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body[i]]);
 }
 
 // training target of the step just taken: (x_{n+1} - x_inertia) / s
@@ -382,46 +407,55 @@ __global__ void k_guess(int n, const float4 *tgt, const uint8_t *active, const f
         dst[i] = make_float4(y.x, y.y, y.z, 0.f);
 }
 
-// First-sweep residual of a guess: the largest |dx_x+dx_y+dx_z| a Jacobi pass
-// would produce from it (no update).  Evaluated for the plain guess (a.u) and
-// the network guess (alt) in one pass; per-block maxima go to part[2*block..].
+// The update a Jacobi pass from the positions in a.u would take at each swept
+// vertex (no update is applied): dx (nullable) and the first-sweep residual
+// bits |dx_x+dx_y+dx_z| per vertex in m.
 template <int L, int B, int MAT>
-__global__ __launch_bounds__(256) void k_first_sweep_residual(SweepArgs a, const float4 *alt, int nv,
-                                                              uint32_t *part)
+__global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int nv, float4 *dx, uint32_t *m)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int vi = t / L, lane = t % L;
     const bool valid = vi < nv;
     const int v = valid ? vi : 0;
-    SweepArgs b = a;
-    b.u = const_cast<float4 *>(alt);
-    const uint32_t m0 = block_max_u32(eval_vertex<L, B, MAT>(a, v, valid, lane));
-    if (threadIdx.x == 0) part[2 * blockIdx.x] = m0;
-    __syncthreads();
-    const uint32_t m1 = block_max_u32(eval_vertex<L, B, MAT>(b, v, valid, lane));
-    if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = m1;
+    const float3 d = eval_vertex<L, B, MAT>(a, v, valid, lane);
+    if (valid && lane == 0) {
+        if (dx) dx[v] = make_float4(d.x, d.y, d.z, 0.f);
+        m[v] = metric_bits(d);
+    }
 }
 
-// keep the guess with the smaller first-sweep residual
-__global__ __launch_bounds__(1024) void k_choose_guess(int np, const uint32_t *part, int n, const float4 *alt,
-                                                       float4 *u, int32_t *use_alt)
+// One workgroup per body: keep the guess with the smaller first-sweep
+// residual (max over the body's vertices) and write it to u; force = 1 takes
+// the network guess without the comparison.
+__global__ __launch_bounds__(256) void k_choose_guess(const int2 *ranges, int C, const uint32_t *m0,
+                                                      const uint32_t *m1, const float4 *alt, float4 *u,
+                                                      int32_t *use_alt, int force)
 {
     __shared__ int s_use;
-    uint32_t m0 = 0, m1 = 0;
-    for (int q = threadIdx.x; q < np; q += blockDim.x) {
-        m0 = max(m0, part[2 * q]);
-        m1 = max(m1, part[2 * q + 1]);
+    const int2 *r = ranges + (size_t)blockIdx.x * (C + 1);
+    uint32_t a0 = 0, a1 = 0;
+    if (!force) {
+        for (int c = 0; c < C; ++c) {
+            const int2 rc = r[c];
+            for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) {
+                a0 = max(a0, m0[i]);
+                a1 = max(a1, m1[i]);
+            }
+        }
+        a0 = block_max_u32(a0);
+        __syncthreads();
+        a1 = block_max_u32(a1);
     }
-    m0 = block_max_u32(m0);
-    __syncthreads();
-    m1 = block_max_u32(m1);
     if (threadIdx.x == 0) {
-        s_use = __uint_as_float(m1) < __uint_as_float(m0) ? 1 : 0;
-        *use_alt = s_use;
+        s_use = force || __uint_as_float(a1) < __uint_as_float(a0) ? 1 : 0;
+        use_alt[blockIdx.x] = s_use;
     }
     __syncthreads();
     if (!s_use) return;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) u[i] = alt[i];
+    for (int c = 0; c < C; ++c) {
+        const int2 rc = r[c];
+        for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) u[i] = alt[i];
+    }
 }
 
 // full neighbour lists (distinct, without self), colour-major ids
@@ -463,9 +497,9 @@ __global__ void k_rest_new(int n, const int32_t *perm, const double *X, float4 *
 }
 
 template <int L, int MAT>
-void launch_residual(const SweepArgs &a, const float4 *alt, int nv, int nblocks, uint32_t *part, hipStream_t st)
+void launch_jacobi(const SweepArgs &a, int nv, float4 *dx, uint32_t *m, hipStream_t st)
 {
-    k_first_sweep_residual<L, Batch<L>::value, MAT><<<nblocks, 256, 0, st>>>(a, alt, nv, part);
+    k_jacobi<L, Batch<L>::value, MAT><<<nblk((int64_t)std::max(nv, 1) * L, 256), 256, 0, st>>>(a, nv, dx, m);
 }
 
 }  // namespace
@@ -487,12 +521,16 @@ struct vbd_mpnn {
     int cap = 0;
     float *F = nullptr, *T = nullptr;
     float4 *guess = nullptr;
-    uint32_t *rpart = nullptr;
-    int32_t *use_alt = nullptr, *use_alt_host = nullptr;
+    uint32_t *rm0 = nullptr, *rm1 = nullptr;  // per-vertex first-sweep residuals of the two guesses
+    float4 *dxj = nullptr;                     // Jacobi update from the plain guess (a feature)
+    int policy = VBD_WARM_CHOOSE;
+    int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body
     float *loss_host = nullptr;
     int adam_t = 0;
     FeatParams fp{};
-    float logE = 5.f, nu = 0.4f;
+    float4 *body_geo = nullptr;
+    float2 *body_mat = nullptr;
+    std::vector<float2> body_mat_host;
     int n_active = 0;
 };
 
@@ -501,7 +539,8 @@ namespace {
 int mpnn_free(vbd_mpnn *m)
 {
     void *ps[] = {m->nb_off, m->nb, m->Xr, m->P, m->G, m->Am, m->Av, m->Pb, m->PbT, m->f, m->z0, m->out, m->dh,
-                  m->dx, m->dz, m->partial, m->loss, m->F, m->T, m->guess, m->rpart, m->use_alt};
+                  m->dx, m->dz, m->partial, m->loss, m->F, m->T, m->guess, m->rm0, m->rm1, m->use_alt,
+                  m->body_geo, m->body_mat, m->dxj};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (auto p : m->h)
@@ -570,11 +609,56 @@ void set_feature_params(vbd_mpnn *m, double dt)
     m->fp.dt = (float)dt;
     const double gn = std::sqrt(s->grav[0] * s->grav[0] + s->grav[1] * s->grav[1] + s->grav[2] * s->grav[2]);
     m->fp.s = (float)(dt * dt * std::max(gn, 1e-3));
-    m->fp.gx = (float)s->grav[0];
-    m->fp.gy = (float)s->grav[1];
-    m->fp.gz = (float)s->grav[2];
-    m->fp.logE = m->logE;
-    m->fp.nu = m->nu;
+}
+
+BodyFeat body_feat(vbd_mpnn *m)
+{
+    return BodyFeat{m->s->body_new, m->body_geo, m->body_mat, m->s->d_grav};
+}
+
+SweepArgs sweep_args_at(vbd_solver *s, float4 *u, double dt)
+{
+    SweepArgs a;
+    a.u = u;
+    a.tgt = s->tgt;
+    a.uprev = s->uprev;
+    a.slot_off = s->slot_off;
+    a.slots = s->slots;
+    a.ns = std::max<int64_t>(s->n_slots, 1);
+    a.part = nullptr;
+    a.ctrl = s->ctrl;
+    a.kd_dt = (float)(s->damping / dt);
+    return a;
+}
+
+// Jacobi update from the guess in u (and its residual bits into res).
+void jacobi(vbd_mpnn *m, float4 *u, double dt, float4 *dx, uint32_t *res)
+{
+    vbd_solver *s = m->s;
+    constexpr int L = 8;
+    const SweepArgs a = sweep_args_at(s, u, dt);
+    if (s->material == kSNH) launch_jacobi<L, kSNH>(a, s->n_swept, dx, res, s->stream);
+    else launch_jacobi<L, kStVK>(a, s->n_swept, dx, res, s->stream);
+}
+
+// Features of the solver's state at the start of a step of size dt into f.
+// plain_in_u: s->u and s->tgt already hold the plain guess and the inertia
+// target (the step's init ran); otherwise the plain guess is formed in
+// m->guess and the solver's positions are left alone.
+void features(vbd_mpnn *m, double dt, bool plain_in_u, float *f)
+{
+    vbd_solver *s = m->s;
+    hipStream_t st = s->stream;
+    set_feature_params(m, dt);
+    float4 *u = s->u;
+    if (!plain_in_u) {
+        u = m->guess;
+        k_plain_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, u, s->uprev, s->vel, s->tgt, s->mass, s->active,
+                                                       s->body_new, s->d_grav, (float)dt);
+    }
+    jacobi(m, u, dt, m->dxj, m->rm0);
+    k_features<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, s->uprev, s->vel, s->active, m->nb_off, m->nb, m->fp,
+                                                 body_feat(m), m->dxj, s->n_swept, f);
 }
 
 }  // namespace
@@ -583,41 +667,35 @@ void set_feature_params(vbd_mpnn *m, double dt)
 namespace vbd {
 
 // Replace the plain initial guess in s->u by the network guess when its
-// first-sweep residual is smaller.  Called after k_step_init.
+// first-sweep residual is smaller (per body; the policy may force either
+// guess).  Called after the step's init.
 int mpnn_apply_guess(vbd_mpnn *m, double dt)
 {
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
-    set_feature_params(m, dt);
-    k_features<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, s->uprev, s->vel, s->active, m->nb_off, m->nb, m->fp,
-                                                 m->f);
+    if (m->policy == VBD_WARM_PLAIN) {
+        VBD_HIP(hipMemsetAsync(m->use_alt, 0, sizeof(int32_t) * s->n_bodies, st));
+        return 0;
+    }
+    features(m, dt, true, m->f);
     VBD_TRY(mpnn_forward_backward(m, nullptr, false, 0.f));
     k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->tgt, s->active, m->out, m->fp.s, m->guess);
-    SweepArgs a;
-    a.u = s->u;
-    a.tgt = s->tgt;
-    a.uprev = s->uprev;
-    a.slot_off = s->slot_off;
-    a.slots = s->slots;
-    a.ns = std::max<int64_t>(s->n_slots, 1);
-    a.part = nullptr;
-    a.ctrl = s->ctrl;
-    a.kd_dt = s->kd_dt;
-    constexpr int L = 8;
-    const int nv = std::max(1, s->n_swept);
-    const int nb = nblk((int64_t)nv * L, 256);
-    if (s->material == kSNH) launch_residual<L, kSNH>(a, m->guess, s->n_swept, nb, m->rpart, st);
-    else launch_residual<L, kStVK>(a, m->guess, s->n_swept, nb, m->rpart, st);
-    k_choose_guess<<<1, 1024, 0, st>>>(nb, m->rpart, s->n, m->guess, s->u, m->use_alt);
+    const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
+    if (!force) jacobi(m, m->guess, dt, nullptr, m->rm1);
+    k_choose_guess<<<s->n_bodies, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, m->rm1, m->guess, s->u, m->use_alt,
+                                                force);
     VBD_HIP(hipGetLastError());
     return 0;
 }
 
 int mpnn_used_guess(vbd_mpnn *m, int *used)
 {
-    VBD_HIP(hipMemcpyAsync(m->use_alt_host, m->use_alt, sizeof(int32_t), hipMemcpyDeviceToHost, m->s->stream));
+    const int nb = m->s->n_bodies;
+    VBD_HIP(hipMemcpyAsync(m->use_alt_host, m->use_alt, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, m->s->stream));
     VBD_HIP(hipStreamSynchronize(m->s->stream));
-    *used = *m->use_alt_host;
+    int c = 0;
+    for (int b = 0; b < nb; ++b) c += m->use_alt_host[b] ? 1 : 0;
+    *used = c;
     return 0;
 }
 
@@ -628,8 +706,8 @@ extern "C" {
 
 int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
 {
-    if (!s || !out || sample_capacity < 0 || s->batched) {
-        set_error("vbd_mpnn_create: needs a single-body solver handle and sample_capacity >= 0");
+    if (!s || !out || sample_capacity < 0) {
+        set_error("vbd_mpnn_create: needs a solver handle and sample_capacity >= 0");
         return VBD_EINVAL;
     }
     *out = nullptr;
@@ -682,7 +760,10 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
     const size_t nw = (size_t)n * kW;
     if ((rc = dalloc(&m->f, (size_t)n * kFin)) || (rc = dalloc(&m->z0, nw)) || (rc = dalloc(&m->out, (size_t)n * 3)) ||
         (rc = dalloc(&m->dh, nw)) || (rc = dalloc(&m->dx, 2 * nw)) || (rc = dalloc(&m->dz, nw)) ||
-        (rc = dalloc(&m->loss, 1)) || (rc = dalloc(&m->guess, n)) || (rc = dalloc(&m->use_alt, 1)))
+        (rc = dalloc(&m->loss, 1)) || (rc = dalloc(&m->guess, n)) || (rc = dalloc(&m->use_alt, s->n_bodies)) ||
+        (rc = dalloc(&m->rm0, std::max(1, s->n_swept))) || (rc = dalloc(&m->rm1, std::max(1, s->n_swept))) ||
+        (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) ||
+        (rc = dalloc(&m->body_geo, s->n_bodies)) || (rc = dalloc(&m->body_mat, s->n_bodies)))
         return fail(rc);
     for (int l = 0; l <= kLayers; ++l)
         if ((rc = dalloc(&m->h[l], nw))) return fail(rc);
@@ -690,8 +771,7 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         if ((rc = dalloc(&m->a[l], nw)) || (rc = dalloc(&m->z[l], nw))) return fail(rc);
     m->wg_blocks = std::max(1, std::min(512, nblk(n, 32)));
     if ((rc = dalloc(&m->partial, (size_t)m->wg_blocks * (kW * 2 * kW + kW)))) return fail(rc);
-    if ((rc = dalloc(&m->rpart, 2 * (size_t)nblk((int64_t)std::max(1, s->n_swept) * 8, 256)))) return fail(rc);
-    if (hipHostMalloc((void **)&m->use_alt_host, sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc((void **)&m->use_alt_host, sizeof(int32_t) * s->n_bodies, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&m->loss_host, sizeof(float), hipHostMallocDefault) != hipSuccess)
         return fail(VBD_ENOMEM);
     // samples
@@ -700,21 +780,30 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         if ((rc = dalloc(&m->F, (size_t)m->cap * n * kFin)) || (rc = dalloc(&m->T, (size_t)m->cap * n * 3)))
             return fail(rc);
     }
-    // feature normalisation from the rest shape
+    // per-body feature normalisation from the rest shape
     std::vector<double> X(3 * (size_t)n);
     if (hipMemcpy(X.data(), s->X, sizeof(double) * 3 * n, hipMemcpyDeviceToHost) != hipSuccess) return fail(VBD_EHIP);
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int v = 0; v < n; ++v)
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = std::min(lo[k], X[3 * v + k]);
-            hi[k] = std::max(hi[k], X[3 * v + k]);
+    {
+        const int nb = s->n_bodies;
+        std::vector<float4> geo(nb);
+        for (int b = 0; b < nb; ++b) {
+            const int v0 = s->body_voff_host[b], v1 = s->body_voff_host[b + 1];
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int v = v0; v < v1; ++v)
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = std::min(lo[k], X[3 * v + k]);
+                    hi[k] = std::max(hi[k], X[3 * v + k]);
+                }
+            if (v1 <= v0) lo[0] = lo[1] = lo[2] = hi[0] = hi[1] = hi[2] = 0.0;
+            const double L = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                       (hi[2] - lo[2]) * (hi[2] - lo[2]));
+            geo[b] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                                 (float)(0.5 * (lo[2] + hi[2])), (float)(1.0 / std::max(L, 1e-12)));
         }
-    const double L = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
-                               (hi[2] - lo[2]) * (hi[2] - lo[2]));
-    m->fp.inv_L = (float)(1.0 / std::max(L, 1e-12));
-    m->fp.cx = (float)(0.5 * (lo[0] + hi[0]));
-    m->fp.cy = (float)(0.5 * (lo[1] + hi[1]));
-    m->fp.cz = (float)(0.5 * (lo[2] + hi[2]));
+        m->body_mat_host.assign(nb, make_float2(5.f, 0.4f));
+        (void)hipMemcpy(m->body_geo, geo.data(), sizeof(float4) * nb, hipMemcpyHostToDevice);
+        (void)hipMemcpy(m->body_mat, m->body_mat_host.data(), sizeof(float2) * nb, hipMemcpyHostToDevice);
+    }
     // edge length: cube root of the mean tet volume x 6
     m->fp.inv_h = (float)(1.0 / std::max(s->edge_length, 1e-12));
     std::vector<uint8_t> act(n);
@@ -731,6 +820,16 @@ void vbd_mpnn_destroy(vbd_mpnn_t *m)
 }
 
 int vbd_mpnn_num_params(void) { return kNumParams; }
+
+int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy)
+{
+    if (!m || policy < VBD_WARM_CHOOSE || policy > VBD_WARM_PLAIN) {
+        set_error("vbd_mpnn_set_policy: policy must be VBD_WARM_CHOOSE, VBD_WARM_NETWORK or VBD_WARM_PLAIN");
+        return VBD_EINVAL;
+    }
+    m->policy = policy;
+    return 0;
+}
 
 int vbd_mpnn_set_params(vbd_mpnn_t *m, const float *params)
 {
@@ -766,8 +865,27 @@ int vbd_mpnn_set_material(vbd_mpnn_t *m, double youngs, double poisson)
         set_error("bad arguments");
         return VBD_EINVAL;
     }
-    m->logE = (float)std::log10(youngs);
-    m->nu = (float)poisson;
+    for (auto &x : m->body_mat_host) x = make_float2((float)std::log10(youngs), (float)poisson);
+    VBD_HIP(hipMemcpy(m->body_mat, m->body_mat_host.data(), sizeof(float2) * m->body_mat_host.size(),
+                      hipMemcpyHostToDevice));
+    return 0;
+}
+
+int vbd_mpnn_set_body_materials(vbd_mpnn_t *m, const double *youngs, const double *poisson)
+{
+    if (!m || !youngs || !poisson) {
+        set_error("null argument");
+        return VBD_EINVAL;
+    }
+    for (size_t b = 0; b < m->body_mat_host.size(); ++b) {
+        if (!(youngs[b] > 0.0)) {
+            set_error("Young's modulus must be > 0");
+            return VBD_EINVAL;
+        }
+        m->body_mat_host[b] = make_float2((float)std::log10(youngs[b]), (float)poisson[b]);
+    }
+    VBD_HIP(hipMemcpy(m->body_mat, m->body_mat_host.data(), sizeof(float2) * m->body_mat_host.size(),
+                      hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -782,12 +900,15 @@ int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     VBD_HIP(hipSetDevice(s->device));
-    set_feature_params(m, dt);
-    float *Fs = m->F + (size_t)slot * m->n * kFin;
-    k_features<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, s->uprev, s->vel, s->active, m->nb_off, m->nb, m->fp, Fs);
+    features(m, dt, false, m->F + (size_t)slot * m->n * kFin);
     int it = 0;
     double met = 0.0;
-    VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
+    if (s->batched) {
+        VBD_TRY(step_bodies(s, dt, tol, max_iter));
+        it = s->last_iters;
+    } else {
+        VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
+    }
     k_target<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->uprev, s->tgt, s->active, 1.f / m->fp.s,
                                               m->T + (size_t)slot * m->n * 3);
     VBD_HIP(hipGetLastError());
@@ -880,8 +1001,7 @@ int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out_host)
         VBD_HIP(hipMemcpyAsync(m->f, m->F + (size_t)slot * m->n * kFin, sizeof(float) * m->n * kFin,
                                hipMemcpyDeviceToDevice, st));
     else
-        k_features<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, s->uprev, s->vel, s->active, m->nb_off, m->nb, m->fp,
-                                                     m->f);
+        features(m, dt, false, m->f);
     VBD_TRY(mpnn_forward_backward(m, nullptr, false, 0.f));
     std::vector<float> tmp((size_t)m->n * 3);
     std::vector<int32_t> perm(m->n);
@@ -931,8 +1051,9 @@ int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *ta
     return 0;
 }
 
-// A time step that starts from the network's guess when it beats the plain
-// guess on the first-sweep residual.  used_network_out: 1 if it did.
+// A time step that starts from the network's guess where it beats the plain
+// guess on the first-sweep residual (per body).  used_network_out: the number
+// of bodies that started from the network guess.
 int vbd_step_warm(vbd_solver_t *s, vbd_mpnn_t *m, double dt, double tol, int32_t max_iter, int32_t *iters_out,
                   double *metric_out, int32_t *used_network_out)
 {
@@ -944,7 +1065,19 @@ int vbd_step_warm(vbd_solver_t *s, vbd_mpnn_t *m, double dt, double tol, int32_t
     s->warm = m;
     int it = 0;
     double met = 0.0;
-    const int rc = step_device(s, dt, tol, max_iter, &it, &met);
+    int rc = 0;
+    if (s->batched) {
+        rc = step_bodies(s, dt, tol, max_iter);
+        it = s->last_iters;
+        float mx = 0.f;
+        for (int b = 0; b < s->n_bodies && rc == 0; ++b) {
+            const float x = __builtin_bit_cast(float, (uint32_t)s->body_out_host[b].y);
+            if (!(x <= mx)) mx = x;
+        }
+        met = mx;
+    } else {
+        rc = step_device(s, dt, tol, max_iter, &it, &met);
+    }
     s->warm = nullptr;
     if (rc) return rc;
     int used = 0;

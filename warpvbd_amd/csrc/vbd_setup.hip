@@ -369,22 +369,22 @@ __global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const i
         int o0 = iperm[tets[4 * e + other[0]]];
         int o1 = iperm[tets[4 * e + other[1]]];
         int o2 = iperm[tets[4 * e + other[2]]];
-        slots[s] = make_float4(__int_as_float(o0), __int_as_float(o1), __int_as_float(o2), ka);
-        slots[ns + s] = make_float4((float)bs[0], (float)bs[1], (float)bs[2], kb);
-        slots[2 * ns + s] = make_float4((float)b0[0], (float)b0[1], (float)b0[2], kc);
-        slots[3 * ns + s] = make_float4((float)b1[0], (float)b1[1], (float)b1[2], 0.f);
+        slots[slot_q(s, 0, ns)] = make_float4(__int_as_float(o0), __int_as_float(o1), __int_as_float(o2), ka);
+        slots[slot_q(s, 1, ns)] = make_float4((float)bs[0], (float)bs[1], (float)bs[2], kb);
+        slots[slot_q(s, 2, ns)] = make_float4((float)b0[0], (float)b0[1], (float)b0[2], kc);
+        slots[slot_q(s, 3, ns)] = make_float4((float)b1[0], (float)b1[1], (float)b1[2], 0.f);
     }
 }
 
 // Distinct neighbour vertices gathered by one colour's sweep (for the byte model).
 __global__ void k_mark_boundary(const float4 *slots, const int32_t *slot_off, int v0, int v1, int tag, int32_t *mark,
-                                int32_t *count)
+                                int32_t *count, int64_t ns)
 {
     int i = v0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v1) return;
     int local = 0;
     for (int s = slot_off[i]; s < slot_off[i + 1]; ++s) {
-        float4 q0 = slots[s];
+        float4 q0 = slots[slot_q(s, 0, ns)];
         int o[3] = {__float_as_int(q0.x), __float_as_int(q0.y), __float_as_int(q0.z)};
         for (int k = 0; k < 3; ++k)
             if (atomicExch(&mark[o[k]], tag) != tag) ++local;
@@ -484,13 +484,13 @@ __global__ void k_rescale_material(int64_t ns, float4 *slots, float fa, float fb
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    float4 q0 = slots[s], q1 = slots[ns + s], q2 = slots[2 * ns + s];
+    float4 q0 = slots[slot_q(s, 0, ns)], q1 = slots[slot_q(s, 1, ns)], q2 = slots[slot_q(s, 2, ns)];
     q0.w *= fa;
     q1.w *= fb;
     q2.w = material == kSNH ? 0.75f * q0.w : 0.f;
-    slots[s] = q0;
-    slots[ns + s] = q1;
-    slots[2 * ns + s] = q2;
+    slots[slot_q(s, 0, ns)] = q0;
+    slots[slot_q(s, 1, ns)] = q1;
+    slots[slot_q(s, 2, ns)] = q2;
 }
 
 }  // namespace
@@ -581,6 +581,8 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     const int n = s->n, m = s->m;
     const int nb = std::max(1, n_bodies);
     s->n_bodies = nb;
+    s->body_voff_host.assign(nb + 1, 0);
+    for (int b = 0; b <= nb; ++b) s->body_voff_host[b] = body_voff ? body_voff[b] : (b == 0 ? 0 : n);
     hipStream_t st = s->stream;
     Scratch scr;
     int32_t *d_tets;
@@ -733,7 +735,9 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
         k_fill_i32<<<1, kTB, 0, st>>>(count, std::max(1, s->ncolors), 0);
         for (int c = 0; c < s->ncolors; ++c) {
             int v0 = s->col_start[c], v1 = s->col_start[c + 1];
-            if (v1 > v0) k_mark_boundary<<<nblk(v1 - v0), kTB, 0, st>>>(s->slots, s->slot_off, v0, v1, c, mark, count + c);
+            if (v1 > v0)
+                k_mark_boundary<<<nblk(v1 - v0), kTB, 0, st>>>(s->slots, s->slot_off, v0, v1, c, mark, count + c,
+                                                               std::max<int64_t>(s->n_slots, 1));
         }
         std::vector<int32_t> hcnt(std::max(1, s->ncolors));
         std::vector<int32_t> hoff(s->n_swept + 1);

@@ -252,6 +252,48 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
     return 0;
 }
 
+// A hipGraph of k iterations (all colour launches) for the current launch
+// plan, tolerance and damping; kernels take every per-step value from device
+// memory, so one graph serves every step.
+static int single_graph(vbd_solver *s, int k, float tol, hipGraphExec_t *out)
+{
+    for (auto &g : s->graphs)
+        if (g.k == k && g.tol == tol && g.kd_dt == s->kd_dt && g.lanes == s->launch.lanes &&
+            g.block == s->launch.block && g.max_iter == s->launch.small_color) {
+            *out = g.exec;
+            return 0;
+        }
+    if (s->graphs.size() > 24) {
+        for (auto &g : s->graphs) {
+            (void)hipGraphExecDestroy(g.exec);
+            (void)hipGraphDestroy(g.graph);
+        }
+        s->graphs.clear();
+    }
+    CachedGraph g;
+    g.k = k;
+    g.tol = tol;
+    g.kd_dt = s->kd_dt;
+    g.lanes = s->launch.lanes;
+    g.block = s->launch.block;
+    g.max_iter = s->launch.small_color;
+    VBD_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int j = 0; j < k && rc == 0; ++j) rc = enqueue_iteration(s, j, tol, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc != 0) return rc;
+    if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return VBD_EHIP;
+    }
+    g.graph = graph;
+    VBD_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    s->graphs.push_back(g);
+    *out = g.exec;
+    return 0;
+}
+
 static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric)
 {
     SweepArgs a;
@@ -270,6 +312,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     p.tol = (float)tol;
     p.kd_dt = a.kd_dt;
     p.max_iter = max_iter;
+    p.skip_init = 0;
     if (s->material == kSNH)
         dispatch_resident<kSNH>(s->resident_lanes, a, s->d_ranges, s->ncolors, s->d_grav, s->mass, s->active,
                                 s->uprev, s->vel, p, s->d_body_out, 1, 1024, s->stream);
@@ -304,7 +347,13 @@ int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, 
     bool conv = false;
     while (launched < max_iter && !conv) {
         const int k = std::min(chunk, max_iter - launched);
-        for (int j = 0; j < k; ++j) VBD_TRY(enqueue_iteration(s, launched + j, ftol, s->prof));
+        if (s->launch.graph && !s->prof) {
+            hipGraphExec_t ex;
+            VBD_TRY(single_graph(s, k, ftol, &ex));
+            VBD_HIP(hipGraphLaunch(ex, s->stream));
+        } else {
+            for (int j = 0; j < k; ++j) VBD_TRY(enqueue_iteration(s, launched + j, ftol, s->prof));
+        }
         launched += k;
         VBD_HIP(hipMemcpyAsync(s->ctrl_host, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
         VBD_HIP(hipStreamSynchronize(s->stream));
@@ -352,6 +401,22 @@ int import_positions(vbd_solver *s, const double *host_in, float4 *dst)
     k_import<<<nblk(n, 256), 256, 0, s->stream>>>(n, s->xio, s->perm, s->X, dst);
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out)
+{
+    const int n = s->n;
+    k_export<<<nblk(n, 256), 256, 0, s->stream>>>(n, src, s->iperm, s->X, dev_out);
+    VBD_HIP(hipGetLastError());
+    return 0;
+}
+
+int import_positions_dev(vbd_solver *s, const double *dev_in, float4 *dst)
+{
+    const int n = s->n;
+    k_import<<<nblk(n, 256), 256, 0, s->stream>>>(n, dev_in, s->perm, s->X, dst);
+    VBD_HIP(hipGetLastError());
     return 0;
 }
 

@@ -9,6 +9,13 @@ namespace {
 
 inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
 
+#ifndef VBD_STREAM_RECORDS
+#define VBD_STREAM_RECORDS 0
+#endif
+#ifndef VBD_BATCH_L8
+#define VBD_BATCH_L8 2
+#endif
+
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 // 16-byte load that bypasses cache allocation (read-once streams)
@@ -72,170 +79,128 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// Gradient and Hessian of one vertex's local energy (all incident tets plus
+// inertia and damping) accumulated by a group of L lanes, and the block solve:
+// the update dx (zero for lanes without a vertex).  us receives the vertex's
+// current displacement.
+template <int L, int B, int MAT>
+__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, float3 &us)
+{
+    Acc acc;
+    acc_zero(acc);
+    const float4 uv4 = a.u[v];
+    us = make_float3(uv4.x, uv4.y, uv4.z);
+    if (valid) {
+        const int s0 = a.slot_off[v], s1 = a.slot_off[v + 1];
+        // B slots per lane per batch: all records, then all gathers, then the
+        // arithmetic, so each batch costs two memory latencies, not 2B.
+        for (int s = s0 + lane; s < s1; s += B * L) {
+            float4 q[B][4];
+            float4 x[B][3];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int sj = s + j * L;
+                if (sj < s1) {
+#if VBD_SLOT_AOS
+                    const float4 *r = a.slots + 4 * (size_t)sj;
+                    const int64_t st = 1;
+#else
+                    const float4 *r = a.slots + sj;
+                    const int64_t st = a.ns;
+#endif
+#if VBD_STREAM_RECORDS
+                    // records are read once per sweep: stream them past the
+                    // caches that hold the gathered positions
+                    q[j][0] = load_stream(r);
+                    q[j][1] = load_stream(r + st);
+                    q[j][2] = load_stream(r + 2 * st);
+                    q[j][3] = load_stream(r + 3 * st);
+#else
+                    q[j][0] = r[0];
+                    q[j][1] = r[st];
+                    q[j][2] = r[2 * st];
+                    q[j][3] = r[3 * st];
+#endif
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    x[j][0] = a.u[__float_as_int(q[j][0].x)];
+                    x[j][1] = a.u[__float_as_int(q[j][0].y)];
+                    x[j][2] = a.u[__float_as_int(q[j][0].z)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    const float4 *qq = q[j];
+                    const float4 *xx = x[j];
+                    slot_eval<MAT>(qq[0].w, qq[1].w, qq[2].w, make_float3(qq[1].x, qq[1].y, qq[1].z),
+                                   make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[3].x, qq[3].y, qq[3].z),
+                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
+                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
+                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
+                }
+            }
+        }
+    }
+    acc_group_sum<L>(acc);
+    if (!valid) return make_float3(0.f, 0.f, 0.f);
+    if (a.kd_dt != 0.f) {
+        // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
+        const float4 up = a.uprev[v];
+        const float e0 = us.x - up.x, e1 = us.y - up.y, e2 = us.z - up.z;
+        const float k = a.kd_dt;
+        acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
+        acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
+        acc.g2 += k * (acc.h02 * e0 + acc.h12 * e1 + acc.h22 * e2);
+        const float f = 1.f + k;
+        acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
+    }
+    // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
+    const float4 tg = a.tgt[v];
+    const float w = tg.w;
+    acc.g0 += w * (us.x - tg.x);
+    acc.g1 += w * (us.y - tg.y);
+    acc.g2 += w * (us.z - tg.z);
+    acc.h00 += w;
+    acc.h11 += w;
+    acc.h22 += w;
+    return solve3_guarded(acc);
+}
+
+
 // Evaluate and apply one vertex update for a lane group.  Returns the metric
 // bits (0 for lanes without a vertex).
 template <int L, int B, int MAT>
 __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane)
 {
-    Acc acc;
-    acc_zero(acc);
-    const float4 uv4 = a.u[v];
-    const float3 us = make_float3(uv4.x, uv4.y, uv4.z);
-    if (valid) {
-        const int s0 = a.slot_off[v], s1 = a.slot_off[v + 1];
-        // B slots per lane per batch: all records, then all gathers, then the
-        // arithmetic, so each batch costs two memory latencies, not 2B.
-        for (int s = s0 + lane; s < s1; s += B * L) {
-            float4 q[B][4];
-            float4 x[B][3];
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                const int sj = s + j * L;
-                if (sj < s1) {
-                    // records are read once per sweep: stream them past the
-                    // caches that hold the gathered positions
-                    const float4 *r = a.slots + sj;
-                    q[j][0] = load_stream(r);
-                    q[j][1] = load_stream(r + a.ns);
-                    q[j][2] = load_stream(r + 2 * a.ns);
-                    q[j][3] = load_stream(r + 3 * a.ns);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                if (s + j * L < s1) {
-                    x[j][0] = a.u[__float_as_int(q[j][0].x)];
-                    x[j][1] = a.u[__float_as_int(q[j][0].y)];
-                    x[j][2] = a.u[__float_as_int(q[j][0].z)];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                if (s + j * L < s1) {
-                    const float4 *qq = q[j];
-                    const float4 *xx = x[j];
-                    slot_eval<MAT>(qq[0].w, qq[1].w, qq[2].w, make_float3(qq[1].x, qq[1].y, qq[1].z),
-                                   make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[3].x, qq[3].y, qq[3].z),
-                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
-                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
-                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
-                }
-            }
-        }
-    }
-    acc_group_sum<L>(acc);
+    float3 us;
+    const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us);
     if (!valid) return 0u;
-    if (a.kd_dt != 0.f) {
-        // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
-        const float4 up = a.uprev[v];
-        const float e0 = us.x - up.x, e1 = us.y - up.y, e2 = us.z - up.z;
-        const float k = a.kd_dt;
-        acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
-        acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
-        acc.g2 += k * (acc.h02 * e0 + acc.h12 * e1 + acc.h22 * e2);
-        const float f = 1.f + k;
-        acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
-    }
-    // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
-    const float4 tg = a.tgt[v];
-    const float w = tg.w;
-    acc.g0 += w * (us.x - tg.x);
-    acc.g1 += w * (us.y - tg.y);
-    acc.g2 += w * (us.z - tg.z);
-    acc.h00 += w;
-    acc.h11 += w;
-    acc.h22 += w;
-    const float3 dx = solve3_guarded(acc);
     if (lane == 0) a.u[v] = make_float4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.f);
     return metric_bits(dx);
 }
 
-// As update_vertex, without writing the update: the metric bits of the dx a
-// Jacobi pass from the positions in a.u would take.
+// The update a Jacobi pass from the positions in a.u would take (no write).
 template <int L, int B, int MAT>
-__device__ __forceinline__ uint32_t eval_vertex(const SweepArgs &a, int v, bool valid, int lane)
+__device__ __forceinline__ float3 eval_vertex(const SweepArgs &a, int v, bool valid, int lane)
 {
-    Acc acc;
-    acc_zero(acc);
-    const float4 uv4 = a.u[v];
-    const float3 us = make_float3(uv4.x, uv4.y, uv4.z);
-    if (valid) {
-        const int s0 = a.slot_off[v], s1 = a.slot_off[v + 1];
-        // B slots per lane per batch: all records, then all gathers, then the
-        // arithmetic, so each batch costs two memory latencies, not 2B.
-        for (int s = s0 + lane; s < s1; s += B * L) {
-            float4 q[B][4];
-            float4 x[B][3];
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                const int sj = s + j * L;
-                if (sj < s1) {
-                    // records are read once per sweep: stream them past the
-                    // caches that hold the gathered positions
-                    const float4 *r = a.slots + sj;
-                    q[j][0] = load_stream(r);
-                    q[j][1] = load_stream(r + a.ns);
-                    q[j][2] = load_stream(r + 2 * a.ns);
-                    q[j][3] = load_stream(r + 3 * a.ns);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                if (s + j * L < s1) {
-                    x[j][0] = a.u[__float_as_int(q[j][0].x)];
-                    x[j][1] = a.u[__float_as_int(q[j][0].y)];
-                    x[j][2] = a.u[__float_as_int(q[j][0].z)];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < B; ++j) {
-                if (s + j * L < s1) {
-                    const float4 *qq = q[j];
-                    const float4 *xx = x[j];
-                    slot_eval<MAT>(qq[0].w, qq[1].w, qq[2].w, make_float3(qq[1].x, qq[1].y, qq[1].z),
-                                   make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[3].x, qq[3].y, qq[3].z),
-                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
-                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
-                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
-                }
-            }
-        }
-    }
-    acc_group_sum<L>(acc);
-    if (!valid) return 0u;
-    if (a.kd_dt != 0.f) {
-        // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
-        const float4 up = a.uprev[v];
-        const float e0 = us.x - up.x, e1 = us.y - up.y, e2 = us.z - up.z;
-        const float k = a.kd_dt;
-        acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
-        acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
-        acc.g2 += k * (acc.h02 * e0 + acc.h12 * e1 + acc.h22 * e2);
-        const float f = 1.f + k;
-        acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
-    }
-    // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
-    const float4 tg = a.tgt[v];
-    const float w = tg.w;
-    acc.g0 += w * (us.x - tg.x);
-    acc.g1 += w * (us.y - tg.y);
-    acc.g2 += w * (us.z - tg.z);
-    acc.h00 += w;
-    acc.h11 += w;
-    acc.h22 += w;
-    const float3 dx = solve3_guarded(acc);
-    return metric_bits(dx);
+    float3 us;
+    return vertex_dx<L, B, MAT>(a, v, valid, lane, us);
 }
 
 // slots per lane per load batch, by lanes per vertex
 template <int L>
 struct Batch {
-    static constexpr int value = L == 1 ? 4 : L == 2 ? 4 : L == 4 ? 5 : L == 8 ? 3 : L == 16 ? 2 : 1;
+    static constexpr int value = L == 1 ? 4 : L == 2 ? 4 : L == 4 ? 5 : L == 8 ? VBD_BATCH_L8 : L == 16 ? 2 : 1;
 };
 
 struct StepParams {
     float dt, inv_dt, tol, kd_dt;
     int max_iter;
+    int skip_init;  // the initial guess is already in u (warm start)
 };
 
 __device__ __forceinline__ void init_vertex(int i, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt,
@@ -245,8 +210,9 @@ __device__ __forceinline__ void init_vertex(int i, float4 *u, const float4 *upre
     if (active[i]) {
         const float4 v = vel[i];
         const float dt2 = dt * dt;
-        const float4 t = make_float4(p.x + dt * v.x + dt2 * g.x, p.y + dt * v.y + dt2 * g.y,
-                                     p.z + dt * v.z + dt2 * g.z, mass[i] / dt2);
+        // explicit fused forms: every kernel that inlines this rounds alike
+        const float4 t = make_float4(fmaf(dt2, g.x, fmaf(dt, v.x, p.x)), fmaf(dt2, g.y, fmaf(dt, v.y, p.y)),
+                                     fmaf(dt2, g.z, fmaf(dt, v.z, p.z)), mass[i] / dt2);
         u[i] = make_float4(t.x, t.y, t.z, 0.f);
         tgt[i] = t;
     } else {
@@ -269,12 +235,14 @@ __global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 
     __shared__ uint32_t s_bits;
     const int2 *r = ranges + (size_t)blockIdx.x * (C + 1);
     const float4 g = grav[blockIdx.x];
-    for (int c = 0; c <= C; ++c) {
-        const int2 rc = r[c];
-        for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x)
-            init_vertex(i, a.u, uprev, vel, const_cast<float4 *>(a.tgt), mass, active, p.dt, g);
+    if (!p.skip_init) {
+        for (int c = 0; c <= C; ++c) {
+            const int2 rc = r[c];
+            for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x)
+                init_vertex(i, a.u, uprev, vel, const_cast<float4 *>(a.tgt), mass, active, p.dt, g);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     const int groups = blockDim.x / L;
     const int gi = threadIdx.x / L, lane = threadIdx.x % L;
     int it = 0;

@@ -27,6 +27,10 @@ EPS = 1e-12                      # src/vbd_solver.py:11
 MAX_ELEMENTS_PER_VERTEX = 32     # src/vbd_solver.py:12
 
 
+def _is_gpu_tensor(x) -> bool:
+    return type(x).__module__.startswith("torch") and bool(getattr(x, "is_cuda", False))
+
+
 class VertexArray:
     """An [N, 3] float64 vertex array that lives on the solver's device.
 
@@ -109,7 +113,7 @@ class VBDSolver:
                  verbose: bool = True):
         L = _lib.load()
         self.device = device
-        _lib.use_device(device)
+        self.device_index = _lib.use_device(device)
         X = f64(initial_positions).reshape(-1, 3)
         T = i32(elements).reshape(-1, 4)
         n, m = len(X), len(T)
@@ -246,6 +250,10 @@ class VBDSolver:
             dx_tol = self.dx_tol
         if max_iter is None:
             max_iter = self.max_iter
+        if _is_gpu_tensor(positions) and warm_start is None:
+            return self._step_tensor(positions, dt, dx_tol, max_iter)
+        if _is_gpu_tensor(positions):
+            positions = positions.detach().cpu().numpy()
         L = _lib.load()
         pos_ptr = None
         if isinstance(positions, VertexArray) and positions._solver is self and positions._host is None:
@@ -268,11 +276,34 @@ class VBDSolver:
         else:
             check(L.vbd_step(self._h, pos_ptr, float(dt), float(dx_tol), int(max_iter), None, ctypes.byref(it),
                              ctypes.byref(met)), "vbd_step")
+            self.last_used_network = False
         self.last_iterations, self.last_metric = it.value, met.value
         if it.value == max_iter:  # the reference warns when the last allowed iteration ran (:581-582)
             print(f"Warning: VBD solver did not converge within the maximum number of iterations. "
                   f"Final dx metric: {met.value}")
         return self._view()
+
+    def _step_tensor(self, positions, dt, dx_tol, max_iter):
+        """step() for an [N, 3] tensor on the solver's GPU: positions in and the
+        new positions out stay in device memory (a new float64 tensor), ordered
+        on torch's current stream."""
+        import torch
+        x = positions.detach().to(torch.float64).reshape(self.n_vertices, 3).contiguous()
+        if x.device.index != self.device_index:
+            raise VBDError(f"positions are on {x.device}, the solver on cuda:{self.device_index}")
+        out = torch.empty_like(x)
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        self._before_mutation()
+        it, met = ctypes.c_int32(0), ctypes.c_double(0.0)
+        check(_lib.load().vbd_step_dev(self._h, ctypes.c_void_p(x.data_ptr()), float(dt), float(dx_tol),
+                                       int(max_iter), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream),
+                                       ctypes.byref(it), ctypes.byref(met)), "vbd_step_dev")
+        self.last_iterations, self.last_metric = it.value, met.value
+        self.last_used_network = False
+        if it.value == max_iter:
+            print(f"Warning: VBD solver did not converge within the maximum number of iterations. "
+                  f"Final dx metric: {met.value}")
+        return out
 
     # ------------------------------------------------------------------ material / gravity
     def set_material(self, youngs_modulus: float, poissons_ratio: float):
@@ -338,6 +369,10 @@ class VBDSolver:
     # ------------------------------------------------------------------ execution knobs / profiling
     def set_launch(self, lanes_per_vertex=4, block_size=256, small_color=2048, chunk=0):
         check(_lib.load().vbd_set_launch(self._h, lanes_per_vertex, block_size, small_color, chunk), "vbd_set_launch")
+
+    def set_graph(self, use_graph=True):
+        """Replay each step's iterations from captured hipGraphs (default)."""
+        check(_lib.load().vbd_set_graph(self._h, int(bool(use_graph))), "vbd_set_graph")
 
     def profile(self, enable=True, min_vertices=0):
         check(_lib.load().vbd_profile_enable(self._h, int(enable), int(min_vertices)), "vbd_profile_enable")

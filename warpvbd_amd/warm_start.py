@@ -63,7 +63,9 @@ def init_params(seed: int = 0) -> dict:
 
 
 class WarmStart:
-    """The network bound to one single-body ``VBDSolver`` (its mesh and graph)."""
+    """The network bound to one ``VBDSolver`` or ``BatchedVBDSolver`` (its mesh
+    and graph).  Parameters can be moved between solvers with
+    get_params/set_params: the network itself is mesh independent."""
 
     def __init__(self, solver, capacity: int = 256, seed: int = 0):
         L = _lib.load()
@@ -75,9 +77,7 @@ class WarmStart:
         self.n_samples = 0
         self.n_params = L.vbd_mpnn_num_params()
         self.set_params(flatten(init_params(seed)))
-        E = np.asarray(solver.youngs_modulus).reshape(-1)
-        nu = np.asarray(solver.poissons_ratio).reshape(-1)
-        self.set_material(float(E[0]) if len(E) else 1e5, float(nu[0]) if len(nu) else 0.4)
+        self.refresh_materials()
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -104,6 +104,41 @@ class WarmStart:
     def set_material(self, youngs_modulus: float, poissons_ratio: float):
         check(_lib.load().vbd_mpnn_set_material(self._h, float(youngs_modulus), float(poissons_ratio)),
               "vbd_mpnn_set_material")
+
+    def refresh_materials(self):
+        """Material features from the solver's element arrays (first element of
+        each body)."""
+        s = self.solver
+        E = np.asarray(s.youngs_modulus, np.float64).reshape(-1)
+        nu = np.asarray(s.poissons_ratio, np.float64).reshape(-1)
+        if len(E) == 0:
+            return
+        offs = getattr(s, "body_vertex_offsets", None)
+        if offs is None:
+            self.set_material(float(E[0]), float(nu[0]))
+            return
+        T = np.asarray(s.elements)
+        offs = np.asarray(offs)
+        nb = len(offs) - 1
+        body = np.searchsorted(offs, T[:, 0], side="right") - 1
+        bodies, first = np.unique(body, return_index=True)
+        Eb = np.full(nb, E[0])
+        nub = np.full(nb, nu[0])
+        Eb[bodies] = E[first]
+        nub[bodies] = nu[first]
+        Eb = np.ascontiguousarray(Eb)
+        nub = np.ascontiguousarray(nub)
+        check(_lib.load().vbd_mpnn_set_body_materials(self._h, ptr(Eb, ctypes.c_double), ptr(nub, ctypes.c_double)),
+              "vbd_mpnn_set_body_materials")
+
+    # initial-guess policy of warm steps
+    CHOOSE, NETWORK, PLAIN = 0, 1, 2
+
+    def set_policy(self, policy: int):
+        """CHOOSE (default): per body the guess with the smaller first-sweep
+        residual; NETWORK: always the network's; PLAIN: always the inertia
+        prediction (the network is not evaluated)."""
+        check(_lib.load().vbd_mpnn_set_policy(self._h, int(policy)), "vbd_mpnn_set_policy")
 
     def save(self, path):
         np.save(path, self.get_params())
