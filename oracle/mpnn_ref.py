@@ -2,16 +2,17 @@
 INFRASTRUCTURE ONLY (the checker of warpvbd_amd/csrc/vbd_mpnn.hip).
 
 The network has no counterpart in WarpVBD; it is specified here and the GPU
-kernels must reproduce it.  The GPU multiplies BF16-rounded operands on MFMA
-with FP32 accumulation; this reference rounds the same operands to BF16
-(round to nearest even) and accumulates in FP64, so forward values and
-gradients agree to FP32 accumulation error.
+kernels must reproduce it.  The GPU stores features and activations in BF16
+and multiplies BF16 operands on MFMA with FP32 accumulation; this reference
+rounds the same values to BF16 (round to nearest even) and accumulates in
+FP64, so forward values and gradients agree to FP32 accumulation error (and
+the odd BF16 rounding that lands on the other side of a tie).
 
 Network (F = 32 input features, W = 64 width, 4 message-passing layers):
-    h0      = relu(bf(f) bf(W_in)^T + b_in)
-    a_l     = mean over graph neighbours j of h_l[j]          (0 without neighbours)
-    h_{l+1} = h_l + relu(bf([h_l, a_l]) bf(W_l)^T + b_l)      l = 0..3
-    out     = bf(h_4) bf(W_out)^T + b_out                     (3 outputs)
+    h0      = bf(relu(bf(f) bf(W_in)^T + b_in))
+    a_l     = bf(mean over graph neighbours j of h_l[j])      (0 without neighbours)
+    h_{l+1} = bf(h_l + relu([h_l, a_l] bf(W_l)^T + b_l))      l = 0..3
+    out     = h_4 bf(W_out)^T + b_out                         (3 outputs)
     loss    = sum_i m_i |out_i - t_i|^2 / (3 sum_i m_i)       (m: active-vertex mask)
 Backward as the GPU computes it: the data gradients pass through MFMA with
 BF16-rounded upstream gradients (bf(dz) bf(W)); weight gradients use the
@@ -77,16 +78,16 @@ def forward(p, f, nb_off, nb, rounding=True):
     bf = bf16_round if rounding else (lambda x: np.asarray(x, np.float64))
     cache = {"x": [], "z": []}
     z0 = bf(f) @ bf(p["W_in"]).T + p["b_in"]
-    h = np.maximum(z0, 0.0)
+    h = bf(np.maximum(z0, 0.0))
     cache["f"], cache["z0"] = f, z0
     hs = [h]
     for l in range(LAYERS):
-        a = neighbour_mean(h, nb_off, nb)
+        a = bf(neighbour_mean(h, nb_off, nb))
         x = np.concatenate([h, a], axis=1)
         z = bf(x) @ bf(p[f"W_{l}"]).T + p[f"b_{l}"]
         cache["x"].append(x)
         cache["z"].append(z)
-        h = h + np.maximum(z, 0.0)
+        h = bf(h + np.maximum(z, 0.0))
         hs.append(h)
     out = bf(h) @ bf(p["W_out"]).T + p["b_out"]
     cache["h"] = hs

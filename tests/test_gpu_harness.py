@@ -1,0 +1,89 @@
+"""GPU tests of the example drivers and tip-trace outputs (warpvbd_amd.harness):
+file formats of the reference's run loops, the traces against the FP64
+oracle, and the batched form of the parallel scene against the reference's
+concatenated form."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from warpvbd_amd import harness
+
+pytestmark = pytest.mark.gpu
+
+
+def test_run_cantilever_csv_matches_oracle(tmp_path):
+    """30 steps of the reference's cantilever run (E=150 kPa, nu=0.45,
+    rho=1070, tol 1e-9, max_iter 100, dt 1 ms): the CSV holds the tip's mean
+    position per step and matches the oracle's tip trace."""
+    tips = harness.run_cantilever(str(tmp_path), n_seconds=0.03)
+    path = os.path.join(tmp_path, "displacement_cantilever.csv")
+    csv = np.loadtxt(path, delimiter=",")
+    assert csv.shape == (30, 3) and np.array_equal(csv, tips)
+    sim = harness.CantileverSim(density=1070, youngs_modulus=150e3, poissons_ratio=0.45, dx_tol=1e-9,
+                                max_iter=100, verbose=False)
+    m = len(sim.elements)
+    ref = oracle.OracleSolver(sim.initial_positions, sim.elements, 1070 * np.ones(m), 150e3 * np.ones(m),
+                              0.45 * np.ones(m), damping_coefficient=0.0, active_mask=sim.active_mask,
+                              dx_tol=1e-9, max_iter=100, expanded=False, threads=8)
+    tr = np.array([ref.step(1e-3)[sim.tip_idx].mean(axis=0) for _ in range(30)])
+    diag = float(np.linalg.norm(sim.initial_positions.max(0) - sim.initial_positions.min(0)))
+    assert np.abs(tips - tr).max() < 1e-6 * diag
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_run_mass_spring_parallel_npz(tmp_path, batched):
+    """The parallel run writes one [steps, 3] trace per environment, keyed
+    "0".."n-1"; environments of different stiffness give different traces."""
+    tips = harness.run_mass_spring_parallel(str(tmp_path), nsim=3, nx=5, n_seconds=0.04, batched=batched)
+    data = np.load(os.path.join(tmp_path, "tip_positions_mass_spring_parallel.npz"))
+    assert sorted(data.files) == ["0", "1", "2"]
+    for k in data.files:
+        assert data[k].shape == (10, 3) and np.array_equal(data[k], tips[k])
+        assert np.all(np.isfinite(data[k]))
+    assert not np.allclose(data["0"][:, 2], data["1"][:, 2])
+
+
+def test_parallel_batched_matches_concatenated():
+    """The batched form against the reference's concatenated mesh: at a fixed
+    iteration count (tol 0) every environment takes the same sweeps in both,
+    so positions agree to FP32 rounding; at tol 1e-6 each environment stops on
+    its own test, and the slowest one when the single global test stops."""
+    kw = dict(nsim=4, nx=5, max_iter=30, verbose=False)
+    a = harness.MassSpringParallelSim(dx_tol=0.0, batched=False, **kw)
+    b = harness.MassSpringParallelSim(dx_tol=0.0, batched=True, **kw)
+    for _ in range(10):
+        xa = a.step(4e-3).numpy()
+        xb = b.step(4e-3).numpy()
+    diag = float(np.linalg.norm(a.initial_positions.max(0) - a.initial_positions.min(0)))
+    assert np.abs(xa - xb).max() < 1e-6 * diag
+    kw["max_iter"] = 1000
+    a = harness.MassSpringParallelSim(dx_tol=1e-6, batched=False, **kw)
+    b = harness.MassSpringParallelSim(dx_tol=1e-6, batched=True, **kw)
+    a.step(4e-3)
+    b.step(4e-3)
+    # same start: each environment's sweeps are identical until it converges
+    assert b.solver.body_iterations.max() == a.solver.last_iterations
+    assert b.solver.body_iterations.min() < a.solver.last_iterations
+    for _ in range(9):
+        a.step(4e-3)
+        b.step(4e-3)
+        assert abs(int(b.solver.body_iterations.max()) - a.solver.last_iterations) <= 2
+
+
+def test_timing_sweeps_quick(tmp_path):
+    """The README sweeps run end to end at reduced sizes and save their
+    metrics."""
+    m = harness.benchmark_tolerance(tolerances=(1e-6, 1e-7), num_samples=1, n_timesteps=20,
+                                    out=str(tmp_path / "tol.npz"))
+    assert len(m["time_per_sim_second"]) == 2 and m["tip_positions"][0].shape == (20, 3)
+    m = harness.benchmark_mesh_size(reduction_factors=(1, 2), n_timesteps=5, out=str(tmp_path / "mesh.npz"))
+    assert m["num_vertices"] == [176, 1029]
+    m = harness.benchmark_parallel(nsims=(1, 4), num_samples=1, n_timesteps=5, batched=True,
+                                   out=str(tmp_path / "par.npz"))
+    assert len(m["realtime_factor_per_env"]) == 2
+    m = harness.benchmark_coloring(num_voxels=(3, 4), num_samples=1, out=str(tmp_path / "col.npz"))
+    assert m["num_elements"] == [135, 320]
+    for f in ("tol.npz", "mesh.npz", "par.npz", "col.npz"):
+        assert (tmp_path / f).exists()

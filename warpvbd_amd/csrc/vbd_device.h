@@ -28,6 +28,21 @@
 #define VBD_SLOT_AOS 1
 #endif
 
+// VBD_TET_RECORDS = 1 replaces the per-slot records by one 64-byte record per
+// tet, shared by its four slots, and a 4-byte entry per slot (tet << 2 | local
+// index, the incidence CSR entry itself):
+//
+//   r0 = (id0, id1, id2, id3 as int bits)    colour-major vertex ids
+//   r1 = (b0.xyz, ka)   r2 = (b1.xyz, kb)   r3 = (b2.xyz, kc)
+//
+// with b0..b2 the rows of invDm and b3 = -(b0+b1+b2).  The tet records of a
+// mesh are a quarter of the per-slot records' size (84 MB for 1.31 M tets),
+// small enough to stay in the 256 MB Infinity Cache between the four colour
+// sweeps that read each tet in an iteration.
+#ifndef VBD_TET_RECORDS
+#define VBD_TET_RECORDS 0
+#endif
+
 namespace vbd {
 
 // index of float4 q (0..3) of slot s among ns slots
@@ -42,6 +57,28 @@ __host__ __device__ __forceinline__ int64_t slot_q(int64_t s, int q, int64_t ns)
 }
 
 enum Material : int { kSNH = 0, kStVK = 1 };
+
+// A slot's view of its tet record (VBD_TET_RECORDS layout): the other three
+// vertex ids (ascending local index) and the b vectors of the vertex itself
+// and of the first two others; the third other's b is -(bs + b0 + b1).
+__device__ __forceinline__ void tet_slot(const float4 r[4], int loc, int &oa, int &ob, int &oc, float3 &bs,
+                                         float3 &b0, float3 &b1, float &ka, float &kb, float &kc)
+{
+    const int i0 = __float_as_int(r[0].x), i1 = __float_as_int(r[0].y), i2 = __float_as_int(r[0].z),
+              i3 = __float_as_int(r[0].w);
+    oa = loc == 0 ? i1 : i0;
+    ob = loc <= 1 ? i2 : i1;
+    oc = loc <= 2 ? i3 : i2;
+    const float3 c0 = make_float3(r[1].x, r[1].y, r[1].z), c1 = make_float3(r[2].x, r[2].y, r[2].z),
+                 c2 = make_float3(r[3].x, r[3].y, r[3].z);
+    const float3 c3 = make_float3(-(c0.x + c1.x + c2.x), -(c0.y + c1.y + c2.y), -(c0.z + c1.z + c2.z));
+    bs = loc == 0 ? c0 : loc == 1 ? c1 : loc == 2 ? c2 : c3;
+    b0 = loc == 0 ? c1 : c0;
+    b1 = loc <= 1 ? c2 : c1;
+    ka = r[1].w;
+    kb = r[2].w;
+    kc = r[3].w;
+}
 
 struct Acc {
     float g0, g1, g2;

@@ -133,6 +133,7 @@ struct vbd_solver {
     float *mass = nullptr;        // lumped vertex mass sum_e m_e / 4
     uint8_t *active = nullptr;    // active flag
     int32_t *slot_off = nullptr;  // [n_swept+1]
+    int32_t *slot_ent = nullptr;  // [n_slots] tet << 2 | local index (tet-record layout)
     float4 *slots = nullptr;      // [n_slots*4]
     int32_t *colors = nullptr;    // original order
     int32_t *inc_off = nullptr;   // original order vertex -> incident (tet<<2|local), ascending tet

@@ -319,6 +319,24 @@ __global__ void k_vertex_mass(const int32_t *perm, const int32_t *inc_off, const
     active_new[i] = active_orig[v];
 }
 
+// Material constants of one tet: SNH ka = V mu', kb = V lambda', kc = V
+// lambda'(alpha - 1) = 0.75 ka; StVK ka = V mu, kb = V lambda, kc = 0.
+__device__ __forceinline__ void tet_material(int material, double V, double mu, double lam, float &ka, float &kb,
+                                             float &kc)
+{
+    if (material == kSNH) {
+        const double mup = 4.0 * mu / 3.0;       // src/vbd_solver.py:108
+        const double lamp = lam + 5.0 * mu / 6.0;  // :109
+        ka = (float)(V * mup);
+        kb = (float)(V * lamp);
+        kc = 0.75f * ka;  // alpha from :110
+    } else {
+        ka = (float)(V * mu);
+        kb = (float)(V * lam);
+        kc = 0.f;
+    }
+}
+
 __device__ __forceinline__ void brow(const double *iD, int k, double b[3])
 {
     if (k < 3) {
@@ -354,18 +372,7 @@ __global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const i
         brow(iD, other[0], b0);
         brow(iD, other[1], b1);
         float ka, kb, kc;
-        double V = vol[e];
-        if (material == kSNH) {
-            double mup = 4.0 * mu[e] / 3.0;                // src/vbd_solver.py:108
-            double lamp = lam[e] + 5.0 * mu[e] / 6.0;      // :109
-            ka = (float)(V * mup);
-            kb = (float)(V * lamp);
-            kc = 0.75f * ka;                               // V lambda'(alpha - 1), alpha from :110
-        } else {
-            ka = (float)(V * mu[e]);
-            kb = (float)(V * lam[e]);
-            kc = 0.f;
-        }
+        tet_material(material, vol[e], mu[e], lam[e], ka, kb, kc);
         int o0 = iperm[tets[4 * e + other[0]]];
         int o1 = iperm[tets[4 * e + other[1]]];
         int o2 = iperm[tets[4 * e + other[2]]];
@@ -376,16 +383,58 @@ __global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const i
     }
 }
 
+// 64-byte tet records (VBD_TET_RECORDS layout, documented in vbd_device.h).
+__global__ void k_build_tet_records(int m, const int32_t *tets, const int32_t *iperm, const double *invDm,
+                                    const double *vol, const double *mu, const double *lam, int material,
+                                    float4 *rec)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    const double *iD = invDm + 9 * (size_t)e;
+    float ka, kb, kc;
+    tet_material(material, vol[e], mu[e], lam[e], ka, kb, kc);
+    rec[4 * (size_t)e] = make_float4(__int_as_float(iperm[tets[4 * e]]), __int_as_float(iperm[tets[4 * e + 1]]),
+                                     __int_as_float(iperm[tets[4 * e + 2]]), __int_as_float(iperm[tets[4 * e + 3]]));
+    rec[4 * (size_t)e + 1] = make_float4((float)iD[0], (float)iD[1], (float)iD[2], ka);
+    rec[4 * (size_t)e + 2] = make_float4((float)iD[3], (float)iD[4], (float)iD[5], kb);
+    rec[4 * (size_t)e + 3] = make_float4((float)iD[6], (float)iD[7], (float)iD[8], kc);
+}
+
+// Per-slot entries in colour-major vertex order: the incidence CSR entries
+// (tet << 2 | local index, ascending tet) of each swept vertex.
+__global__ void k_build_slot_entries(const int32_t *perm, const int32_t *inc_off, const int32_t *inc, int nswept,
+                                     const int32_t *slot_off, int32_t *ent)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nswept) return;
+    const int v = perm[i];
+    int s = slot_off[i];
+    for (int q = inc_off[v]; q < inc_off[v + 1]; ++q, ++s) ent[s] = inc[q];
+}
+
 // Distinct neighbour vertices gathered by one colour's sweep (for the byte model).
-__global__ void k_mark_boundary(const float4 *slots, const int32_t *slot_off, int v0, int v1, int tag, int32_t *mark,
-                                int32_t *count, int64_t ns)
+__global__ void k_mark_boundary(const float4 *slots, const int32_t *ent, const int32_t *slot_off, int v0, int v1,
+                                int tag, int32_t *mark, int32_t *count, int64_t ns)
 {
     int i = v0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v1) return;
     int local = 0;
     for (int s = slot_off[i]; s < slot_off[i + 1]; ++s) {
+        int o[3];
+#if VBD_TET_RECORDS
+        (void)ns;
+        const float4 r0 = slots[4 * (size_t)(ent[s] >> 2)];
+        const int loc = ent[s] & 3;
+        const int ids[4] = {__float_as_int(r0.x), __float_as_int(r0.y), __float_as_int(r0.z), __float_as_int(r0.w)};
+        for (int k = 0, n_o = 0; k < 4; ++k)
+            if (k != loc) o[n_o++] = ids[k];
+#else
+        (void)ent;
         float4 q0 = slots[slot_q(s, 0, ns)];
-        int o[3] = {__float_as_int(q0.x), __float_as_int(q0.y), __float_as_int(q0.z)};
+        o[0] = __float_as_int(q0.x);
+        o[1] = __float_as_int(q0.y);
+        o[2] = __float_as_int(q0.z);
+#endif
         for (int k = 0; k < 3; ++k)
             if (atomicExch(&mark[o[k]], tag) != tag) ++local;
     }
@@ -479,11 +528,20 @@ int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_i
     return 0;
 }
 
-// Scale every slot's material constants from one uniform material to another.
+// Scale every record's material constants from one uniform material to
+// another (ns records: slots, or tets in the tet-record layout, whose
+// constants sit in the same positions).
 __global__ void k_rescale_material(int64_t ns, float4 *slots, float fa, float fb, int material)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
+#if VBD_TET_RECORDS
+    float4 *r = slots + 4 * s;
+    r[1].w *= fa;
+    r[2].w *= fb;
+    r[3].w = material == kSNH ? 0.75f * r[1].w : 0.f;
+    return;
+#endif
     float4 q0 = slots[slot_q(s, 0, ns)], q1 = slots[slot_q(s, 1, ns)], q2 = slots[slot_q(s, 2, ns)];
     q0.w *= fa;
     q1.w *= fb;
@@ -512,9 +570,13 @@ int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1
         fa = mu1 / mu0;
         fb = lam1 / lam0;
     }
-    if (s->n_slots > 0)
-        k_rescale_material<<<nblk(s->n_slots), kTB, 0, s->stream>>>(s->n_slots, s->slots, (float)fa, (float)fb,
-                                                                   s->material);
+#if VBD_TET_RECORDS
+    const int64_t nrec = s->m;
+#else
+    const int64_t nrec = s->n_slots;
+#endif
+    if (nrec > 0)
+        k_rescale_material<<<nblk(nrec), kTB, 0, s->stream>>>(nrec, s->slots, (float)fa, (float)fb, s->material);
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(s->stream));
     return 0;
@@ -717,14 +779,25 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_HIP(hipMemcpyAsync(&ns, s->slot_off + s->n_swept, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
     s->n_slots = ns;
-    VBD_TRY(dmalloc(&s->slots, 4 * (size_t)std::max<int64_t>(ns, 1)));
     VBD_TRY(dmalloc(&s->mass, n));
     VBD_TRY(dmalloc(&s->active, n));
     k_vertex_mass<<<nblk(n), kTB, 0, st>>>(s->perm, s->inc_off, s->inc, d_mass, n, s->mass, d_act_orig, s->active);
+#if VBD_TET_RECORDS
+    VBD_TRY(dmalloc(&s->slots, 4 * (size_t)std::max(m, 1)));
+    VBD_TRY(dmalloc(&s->slot_ent, (size_t)std::max<int64_t>(ns, 1)));
+    if (m)
+        k_build_tet_records<<<nblk(m), kTB, 0, st>>>(m, d_tets, s->iperm, d_invDm, d_vol, d_mu, d_lam, s->material,
+                                                     s->slots);
+    if (s->n_swept)
+        k_build_slot_entries<<<nblk(s->n_swept), kTB, 0, st>>>(s->perm, s->inc_off, s->inc, s->n_swept, s->slot_off,
+                                                              s->slot_ent);
+#else
+    VBD_TRY(dmalloc(&s->slots, 4 * (size_t)std::max<int64_t>(ns, 1)));
     if (s->n_swept)
         k_build_slots<<<nblk(s->n_swept), kTB, 0, st>>>(s->perm, s->iperm, s->inc_off, s->inc, d_tets, d_invDm, d_vol,
                                                        d_mu, d_lam, s->material, s->n_swept, s->slot_off, s->slots,
                                                        (int64_t)std::max<int64_t>(ns, 1));
+#endif
 
     // algorithmic bytes per colour sweep: sum_v (40 + 68 k_v) + 12 |boundary|
     {
@@ -736,7 +809,8 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
         for (int c = 0; c < s->ncolors; ++c) {
             int v0 = s->col_start[c], v1 = s->col_start[c + 1];
             if (v1 > v0)
-                k_mark_boundary<<<nblk(v1 - v0), kTB, 0, st>>>(s->slots, s->slot_off, v0, v1, c, mark, count + c,
+                k_mark_boundary<<<nblk(v1 - v0), kTB, 0, st>>>(s->slots, s->slot_ent, s->slot_off, v0, v1, c, mark,
+                                                               count + c,
                                                                std::max<int64_t>(s->n_slots, 1));
         }
         std::vector<int32_t> hcnt(std::max(1, s->ncolors));

@@ -13,7 +13,7 @@ inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
 #define VBD_STREAM_RECORDS 0
 #endif
 #ifndef VBD_BATCH_L8
-#define VBD_BATCH_L8 2
+#define VBD_BATCH_L8 1
 #endif
 
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -30,7 +30,8 @@ struct SweepArgs {
     const float4 *__restrict__ tgt;
     const float4 *__restrict__ uprev;
     const int32_t *__restrict__ slot_off;
-    const float4 *__restrict__ slots;  // four arrays of ns float4 records: q0 | q1 | q2 | q3
+    const float4 *__restrict__ slots;  // slot records, or tet records (VBD_TET_RECORDS)
+    const int32_t *__restrict__ ent;   // per-slot tet << 2 | local index (VBD_TET_RECORDS)
     int64_t ns;
     uint32_t *part;  // this launch's per-block metric maxima
     Ctrl *ctrl;
@@ -92,6 +93,49 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
     us = make_float3(uv4.x, uv4.y, uv4.z);
     if (valid) {
         const int s0 = a.slot_off[v], s1 = a.slot_off[v + 1];
+#if VBD_TET_RECORDS
+        // B slots per lane per batch: entries, tet records, gathers, arithmetic
+        for (int s = s0 + lane; s < s1; s += B * L) {
+            int e[B];
+            float4 q[B][4];
+            float4 x[B][3];
+#pragma unroll
+            for (int j = 0; j < B; ++j)
+                if (s + j * L < s1) e[j] = a.ent[s + j * L];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    const float4 *r = a.slots + 4 * (size_t)(e[j] >> 2);
+                    q[j][0] = r[0];
+                    q[j][1] = r[1];
+                    q[j][2] = r[2];
+                    q[j][3] = r[3];
+                }
+            }
+            int o[B][3];
+            float3 bs[B], b0[B], b1[B];
+            float ka[B], kb[B], kc[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    tet_slot(q[j], e[j] & 3, o[j][0], o[j][1], o[j][2], bs[j], b0[j], b1[j], ka[j], kb[j], kc[j]);
+                    x[j][0] = a.u[o[j][0]];
+                    x[j][1] = a.u[o[j][1]];
+                    x[j][2] = a.u[o[j][2]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (s + j * L < s1) {
+                    const float4 *xx = x[j];
+                    slot_eval<MAT>(ka[j], kb[j], kc[j], bs[j], b0[j], b1[j],
+                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
+                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
+                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
+                }
+            }
+        }
+#else
         // B slots per lane per batch: all records, then all gathers, then the
         // arithmetic, so each batch costs two memory latencies, not 2B.
         for (int s = s0 + lane; s < s1; s += B * L) {
@@ -144,6 +188,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                 }
             }
         }
+#endif
     }
     acc_group_sum<L>(acc);
     if (!valid) return make_float3(0.f, 0.f, 0.f);
