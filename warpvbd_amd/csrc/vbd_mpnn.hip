@@ -7,13 +7,19 @@
 // larger first-sweep residual (Jacobi evaluation of both guesses).
 //
 // Network (same definition as the FP64 reference in oracle/mpnn_ref.py):
-//   h0      = relu(bf(f) bf(W_in)^T + b_in)                      f: 32 features
-//   a_l     = mean_{j in N(i)} h_l[j]
-//   h_{l+1} = h_l + relu(bf([h_l, a_l]) bf(W_l)^T + b_l)         l = 0..3, width 64
-//   out     = bf(h_4) bf(W_out)^T + b_out                        3 outputs
-// The width-64 products run on MFMA (v_mfma_f32_16x16x16_bf16, BF16 operands,
-// FP32 accumulation); weights are kept in FP32 for Adam and copied to BF16
-// after every update.  Weight gradients accumulate in FP32.
+//   h0      = bf(relu(f bf(W_in)^T + b_in))                     f: 32 BF16 features
+//   a_l     = bf(mean_{j in N(i)} h_l[j])
+//   h_{l+1} = bf(h_l + relu([h_l, a_l] bf(W_l)^T + b_l))        l = 0..3, width 64
+//   out     = h_4 bf(W_out)^T + b_out                           3 outputs
+// Features and activations are stored in BF16 (128 bytes per vertex per
+// layer), so a layer is one pass over HBM: k_mp_layer gathers the neighbour
+// rows of a 64-vertex tile into LDS, averages them, and multiplies the tile on
+// MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation) in the same kernel.  The
+// network runs in the original vertex order, where mesh neighbours have nearby
+// indices and the gathers hit in L2; the solver's colour-major arrays are
+// mapped at the features (in) and at the guess (out).  Weights are kept in
+// FP32 for Adam and copied to BF16 after every update; weight gradients
+// accumulate in FP32.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -42,6 +48,7 @@ __host__ __device__ constexpr int off_b(int l) { return off_W(l) + kW * 2 * kW; 
 constexpr int kOffWout = off_W(kLayers);
 constexpr int kOffBout = kOffWout + kFout * kW;
 constexpr int kNumParams = kOffBout + kFout;
+constexpr int kOutPart = kFout * kW + kFout + 1;  // output-layer partial: gW_out, gb_out, loss
 
 __device__ __forceinline__ uint16_t f2bf(float x)
 {
@@ -50,76 +57,333 @@ __device__ __forceinline__ uint16_t f2bf(float x)
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ float bfr(float x) { return bf2f(f2bf(x)); }
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+__device__ __forceinline__ uint32_t bf_pack(float lo, float hi)
+{
+    return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
 
-// ----------------------------------------------------------------- MFMA product
-// Y[M, 64*gridDim.y] = X[M, K] W^T + bias with X = [X1 | X2] (row-major FP32,
-// converted to BF16 on load) and W row-major BF16 [Nout, K].  A block of 256
-// threads computes 64 rows x 64 columns: wave w owns rows 16w..16w+15 and four
-// 16x16 MFMA tiles.  Fragment layout of v_mfma_f32_16x16x16_bf16: lane l holds
-// A[row l%16][k 4(l/16)..+3], B[k 4(l/16)..+3][col l%16] and D[rows 4(l/16)..+3][col l%16].
-// MODE 0: z = Y, h = relu(Y)            (input layer; writes z and h)
-// MODE 1: z = Y, h = hin + relu(Y)      (message-passing layer)
-// MODE 2: z = Y                         (plain product, no bias)
-template <int K, int MODE>
-__global__ __launch_bounds__(256) void k_mfma_linear(int M, const float *__restrict__ X1, int K1,
-                                                     const float *__restrict__ X2, const uint16_t *__restrict__ Wb,
-                                                     const float *__restrict__ bias, float *__restrict__ z,
-                                                     const float *__restrict__ hin, float *__restrict__ h, int ldo)
+// s[0..7] += the eight BF16 values packed in v
+__device__ __forceinline__ void add_bf16x8(float s[8], uint4 v)
+{
+    s[0] += bf_lo(v.x);
+    s[1] += bf_hi(v.x);
+    s[2] += bf_lo(v.y);
+    s[3] += bf_hi(v.y);
+    s[4] += bf_lo(v.z);
+    s[5] += bf_hi(v.z);
+    s[6] += bf_lo(v.w);
+    s[7] += bf_hi(v.w);
+}
+
+// ----------------------------------------------------------------- forward
+// MFMA fragment layout of v_mfma_f32_16x16x16_bf16: lane l holds A[row l%16][k
+// 4(l/16)..+3], B[k 4(l/16)..+3][col l%16] and D[rows 4(l/16)..+3][col l%16].
+
+// Write a 64 x 64 BF16 tile staged in LDS (row stride S) to rows [r0, r0+64)
+// of a row-major [n, 64] array, 16 bytes per thread per store.
+template <int S>
+__device__ __forceinline__ void store_tile(const uint16_t (*t)[S], int r0, int n, uint16_t *out)
+{
+    for (int q = threadIdx.x; q < 64 * 8; q += blockDim.x) {
+        const int r = q >> 3, c = (q & 7) * 8;
+        if (r0 + r < n)
+            *reinterpret_cast<uint4 *>(out + (size_t)(r0 + r) * kW + c) = *reinterpret_cast<const uint4 *>(&t[r][c]);
+    }
+}
+
+// The forward kernels below use 256 threads on 64-row tiles with wave w owning
+// output columns 16w..16w+15 for all four 16-row tiles, so a wave's B
+// fragments (its 16 columns of W) are loaded once into registers and reused.
+
+// Input layer: h0 = bf(relu(f W_in^T + b_in)); TRAIN also keeps z0 (FP32).
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void k_in_layer(int n, const uint16_t *__restrict__ f,
+                                                  const uint16_t *__restrict__ Wb, const float *__restrict__ bias,
+                                                  float *__restrict__ z0, uint16_t *__restrict__ h0)
+{
+    __shared__ uint16_t s_f[64][kFin + 8];
+    __shared__ uint16_t s_o[64][kW + 8];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = blockIdx.x * 64;
+    {
+        const int r = threadIdx.x >> 2, c = (threadIdx.x & 3) * 8;  // 64 rows x 4 chunks of 16 bytes
+        *reinterpret_cast<uint4 *>(&s_f[r][c]) = r0 + r < n ? *reinterpret_cast<const uint4 *>(f + (size_t)(r0 + r) * kFin + c)
+                                                            : make_uint4(0u, 0u, 0u, 0u);
+    }
+    const int kq = 4 * (lane >> 4), col = 16 * wave + (lane & 15);
+    const bf16x4_t w0 = *reinterpret_cast<const bf16x4_t *>(Wb + (size_t)col * kFin + kq);
+    const bf16x4_t w1 = *reinterpret_cast<const bf16x4_t *>(Wb + (size_t)col * kFin + 16 + kq);
+    const float bo = bias[col];
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        f32x4v_t acc = f32x4v_t{0.f, 0.f, 0.f, 0.f};
+        const bf16x4_t a0 = *reinterpret_cast<const bf16x4_t *>(&s_f[16 * rt + (lane & 15)][kq]);
+        const bf16x4_t a1 = *reinterpret_cast<const bf16x4_t *>(&s_f[16 * rt + (lane & 15)][16 + kq]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, w0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, w1, acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int rl = 16 * rt + 4 * (lane >> 4) + i;
+            const float y = acc[i] + bo;
+            if (TRAIN && r0 + rl < n) z0[(size_t)(r0 + rl) * kW + col] = y;
+            s_o[rl][col] = f2bf(fmaxf(y, 0.f));
+        }
+    }
+    __syncthreads();
+    store_tile<kW + 8>(s_o, r0, n, h0);
+}
+
+// Message-passing layer.  A block takes a contiguous run of 64-vertex tiles
+// (one run per block, runs laid out so each XCD works through one contiguous
+// stretch of rows and the neighbour rows a tile gathers are mostly in that
+// XCD's L2).  Per tile: the tile's neighbour ids are staged in LDS, the
+// neighbour mean of each row (8 lanes per row, 8 features per lane) goes to
+// LDS next to the row's own h, then [h, a] (64 x 128 BF16) times W_l^T on
+// MFMA, then the residual.  TRAIN also writes a_l (BF16) and z_l (FP32).
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void k_mp_layer(int n, int ntiles, const int32_t *__restrict__ nb_off,
+                                                  const int32_t *__restrict__ nb, const uint16_t *__restrict__ h,
+                                                  const uint16_t *__restrict__ Wb, const float *__restrict__ bias,
+                                                  uint16_t *__restrict__ hout, uint16_t *__restrict__ aout,
+                                                  float *__restrict__ zout)
+{
+    constexpr int S = 2 * kW + 8;  // padded row: 272 bytes
+    constexpr int kMaxIds = 1280;  // neighbour ids of a tile staged in LDS (about 14 per row on voxel meshes)
+    __shared__ uint16_t s_x[64][S];
+    __shared__ int32_t s_ids[kMaxIds];
+    __shared__ int32_t s_off[65];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kq = 4 * (lane >> 4), col = 16 * wave + (lane & 15);
+    bf16x4_t wf[2 * kW / 16];
+#pragma unroll
+    for (int k = 0; k < 2 * kW / 16; ++k)
+        wf[k] = *reinterpret_cast<const bf16x4_t *>(Wb + (size_t)col * (2 * kW) + 16 * k + kq);
+    const float bo = bias[col];
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int t0 = (int)((int64_t)ntiles * blk / gridDim.x), t1 = (int)((int64_t)ntiles * (blk + 1) / gridDim.x);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int r0 = tile * 64;
+        const int rows = min(64, n - r0);
+        __syncthreads();  // the previous tile is stored: LDS is free
+        for (int q = threadIdx.x; q <= rows; q += blockDim.x) s_off[q] = nb_off[r0 + q];
+        __syncthreads();
+        const int e0 = s_off[0], ne = s_off[rows] - e0;
+        const bool staged = ne <= kMaxIds;  // uniform over the block
+        if (staged)
+            for (int q = threadIdx.x; q < ne; q += blockDim.x) s_ids[q] = nb[e0 + q];
+        __syncthreads();
+        {
+            // rows rr and rr + 32 together; neighbour rows fetched four at a
+            // time per row, so up to eight loads are in flight per lane
+            const int g = threadIdx.x >> 3, fl = (threadIdx.x & 7) * 8;
+            float sm[2][8] = {};
+            uint4 own[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+            int b[2], e[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int rr = g + 32 * k;
+                b[k] = e[k] = 0;
+                if (rr < rows) {
+                    own[k] = *reinterpret_cast<const uint4 *>(h + (size_t)(r0 + rr) * kW + fl);
+                    b[k] = s_off[rr] - e0;
+                    e[k] = s_off[rr + 1] - e0;
+                }
+            }
+            const int dmax = max(e[0] - b[0], e[1] - b[1]);
+            for (int q = 0; q < dmax; q += 4) {
+                uint4 v[2][4];
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int qq = b[k] + q + u;
+                        v[k][u] = make_uint4(0u, 0u, 0u, 0u);
+                        if (qq < e[k]) {
+                            const int j = staged ? s_ids[qq] : nb[e0 + qq];
+                            v[k][u] = *reinterpret_cast<const uint4 *>(h + (size_t)j * kW + fl);
+                        }
+                    }
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) add_bf16x8(sm[k], v[k][u]);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int rr = g + 32 * k;
+                const int deg = e[k] - b[k];
+                const float d = (float)deg;
+                uint4 am = make_uint4(0u, 0u, 0u, 0u);
+                if (deg > 0)
+                    am = make_uint4(bf_pack(sm[k][0] / d, sm[k][1] / d), bf_pack(sm[k][2] / d, sm[k][3] / d),
+                                    bf_pack(sm[k][4] / d, sm[k][5] / d), bf_pack(sm[k][6] / d, sm[k][7] / d));
+                *reinterpret_cast<uint4 *>(&s_x[rr][fl]) = own[k];
+                *reinterpret_cast<uint4 *>(&s_x[rr][kW + fl]) = am;
+                if (TRAIN && rr < rows) *reinterpret_cast<uint4 *>(aout + (size_t)(r0 + rr) * kW + fl) = am;
+            }
+        }
+        __syncthreads();
+        f32x4v_t acc[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) acc[rt] = f32x4v_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 2 * kW / 16; ++k)
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt) {
+                const bf16x4_t a = *reinterpret_cast<const bf16x4_t *>(&s_x[16 * rt + (lane & 15)][16 * k + kq]);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wf[k], acc[rt], 0, 0, 0);
+            }
+        float hv[4][4];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) hv[rt][i] = bf2f(s_x[16 * rt + 4 * (lane >> 4) + i][col]);
+        __syncthreads();  // every wave is done with s_x: reuse it to stage the output tile
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int rl = 16 * rt + 4 * (lane >> 4) + i;
+                const float y = acc[rt][i] + bo;
+                if (TRAIN && r0 + rl < n) zout[(size_t)(r0 + rl) * kW + col] = y;
+                s_x[rl][col] = f2bf(hv[rt][i] + fmaxf(y, 0.f));
+            }
+        __syncthreads();
+        store_tile<S>(s_x, r0, n, hout);
+    }
+}
+
+// Output layer, 16 lanes per vertex (4 features each), grid-stride over rows.
+//   out = h4 bf(W_out)^T + b_out
+// TRAIN: loss += sum m |out - t|^2 / denom;  dout = 2 m (out - t) / denom;
+//   dh4 = bf(dout) bf(W_out) (FP32 [n, 64]);  gW_out += dout h4^T, gb_out += dout,
+// folded per block into one partial row of kOutPart floats.
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void k_out_layer(int n, const uint16_t *__restrict__ h4,
+                                                   const uint16_t *__restrict__ Wb, const float *__restrict__ bias,
+                                                   const float *__restrict__ target, const uint8_t *__restrict__ mask,
+                                                   float inv_denom, float *__restrict__ out, float *__restrict__ dh4,
+                                                   float *__restrict__ partial)
+{
+    __shared__ float s_red[TRAIN ? 16 : 1][TRAIN ? kOutPart : 1];
+    const int g = threadIdx.x >> 4, sub = threadIdx.x & 15, fl = sub * 4;
+    float w[kFout][4];
+#pragma unroll
+    for (int c = 0; c < kFout; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[c][j] = bf2f(Wb[c * kW + fl + j]);
+    float gw[kFout][4] = {}, gb[kFout] = {}, gl = 0.f;
+    for (int i = blockIdx.x * 16 + g; i < n; i += gridDim.x * 16) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(h4 + (size_t)i * kW + fl);
+        const float x[4] = {bf_lo(v.x), bf_hi(v.x), bf_lo(v.y), bf_hi(v.y)};
+        float o[kFout];
+#pragma unroll
+        for (int c = 0; c < kFout; ++c) {
+            float s = x[0] * w[c][0] + x[1] * w[c][1] + x[2] * w[c][2] + x[3] * w[c][3];
+            s += __shfl_xor(s, 8, 16);
+            s += __shfl_xor(s, 4, 16);
+            s += __shfl_xor(s, 2, 16);
+            s += __shfl_xor(s, 1, 16);
+            o[c] = s + bias[c];
+        }
+        if (sub == 0)
+#pragma unroll
+            for (int c = 0; c < kFout; ++c) out[(size_t)i * kFout + c] = o[c];
+        if (TRAIN) {
+            const float m = mask[i] ? 1.f : 0.f;
+            float d[kFout];
+            float l = 0.f;
+#pragma unroll
+            for (int c = 0; c < kFout; ++c) {
+                const float df = m * (o[c] - target[(size_t)i * kFout + c]);
+                l += df * df;
+                d[c] = 2.f * df * inv_denom;
+            }
+            float dh[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float s = 0.f;
+#pragma unroll
+                for (int c = 0; c < kFout; ++c) s += bf2f(f2bf(d[c])) * w[c][j];
+                dh[j] = s;
+            }
+            *reinterpret_cast<float4 *>(dh4 + (size_t)i * kW + fl) = make_float4(dh[0], dh[1], dh[2], dh[3]);
+#pragma unroll
+            for (int c = 0; c < kFout; ++c) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) gw[c][j] += d[c] * x[j];
+                gb[c] += d[c];
+            }
+            gl += l * inv_denom;
+        }
+    }
+    if (!TRAIN) return;
+#pragma unroll
+    for (int c = 0; c < kFout; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s_red[g][c * kW + fl + j] = gw[c][j];
+    if (sub == 0) {
+#pragma unroll
+        for (int c = 0; c < kFout; ++c) s_red[g][kFout * kW + c] = gb[c];
+        s_red[g][kOutPart - 1] = gl;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kOutPart; e += blockDim.x) {
+        float s = 0.f;
+        for (int q = 0; q < 16; ++q) s += s_red[q][e];
+        partial[(size_t)blockIdx.x * kOutPart + e] = s;
+    }
+}
+
+// G[W_out, b_out] += sum of the output-layer partials; loss += their loss sum.
+__global__ void k_sum_out(int nparts, const float *partial, float *G, float *loss)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kOutPart) return;
+    float s = 0.f;
+    for (int b = 0; b < nparts; ++b) s += partial[(size_t)b * kOutPart + e];
+    if (e < kOutPart - 1) G[kOffWout + e] += s;  // W_out [3][64] then b_out [3], contiguous
+    else *loss += s;
+}
+
+// ----------------------------------------------------------------- backward
+// dx[n, 128] = bf(dz) bf(W_l) with W_l^T stored BF16 [128, 64] (Wt).
+__global__ __launch_bounds__(256) void k_dgrad(int n, const float *__restrict__ dz, const uint16_t *__restrict__ Wt,
+                                               float *__restrict__ dx)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row0 = blockIdx.x * 64 + wave * 16;
+    const int r0 = blockIdx.x * 64 + wave * 16;
     const int col0 = blockIdx.y * 64;
-    const int r = row0 + (lane & 15);
-    const int kq = 4 * (lane >> 4);
+    const int r = r0 + (lane & 15), kq = 4 * (lane >> 4);
     f32x4v_t acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f32x4v_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const int k = k0 + kq;
-        bf16x4_t a;
-        if (r < M) {
-            const float *src = k < K1 ? X1 + (size_t)r * K1 + k : X2 + (size_t)r * (K - K1) + (k - K1);
-            const float4 v = *reinterpret_cast<const float4 *>(src);
+    for (int k0 = 0; k0 < kW; k0 += 16) {
+        bf16x4_t a = bf16x4_t{0, 0, 0, 0};
+        if (r < n) {
+            const float4 v = *reinterpret_cast<const float4 *>(dz + (size_t)r * kW + k0 + kq);
             a = bf16x4_t{(short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w)};
-        } else {
-            a = bf16x4_t{0, 0, 0, 0};
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int o = col0 + 16 * t + (lane & 15);
-            const bf16x4_t b = *reinterpret_cast<const bf16x4_t *>(Wb + (size_t)o * K + k);
+            const bf16x4_t b = *reinterpret_cast<const bf16x4_t *>(Wt + (size_t)o * kW + k0 + kq);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
         }
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int o = col0 + 16 * t + (lane & 15);
-        const float bo = MODE == 2 ? 0.f : bias[o];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int rr = row0 + 4 * (lane >> 4) + i;
-            if (rr >= M) continue;
-            const float y = acc[t][i] + bo;
-            const size_t idx = (size_t)rr * ldo + o;
-            z[idx] = y;
-            if (MODE == 0) h[idx] = fmaxf(y, 0.f);
-            if (MODE == 1) h[idx] = hin[idx] + fmaxf(y, 0.f);
+            const int rr = r0 + 4 * (lane >> 4) + i;
+            if (rr < n) dx[(size_t)rr * 2 * kW + o] = acc[t][i];
         }
     }
-}
-
-// a[i] = mean of h over the graph neighbours of i (0 without neighbours)
-__global__ void k_neighbour_mean(int n, const int32_t *nb_off, const int32_t *nb, const float *h, float *a)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n * kW) return;
-    const int i = (int)(t / kW), f = (int)(t % kW);
-    const int b = nb_off[i], e = nb_off[i + 1];
-    float s = 0.f;
-    for (int q = b; q < e; ++q) s += h[(size_t)nb[q] * kW + f];
-    a[t] = e > b ? s / (float)(e - b) : 0.f;
 }
 
 // dh[i] += dx[i, :64] + sum_{j in N(i)} dx[j, 64:] / deg(j)   (adjoint of the mean)
@@ -142,63 +406,13 @@ __global__ void k_relu_grad(int64_t n, const float *dh, const float *z, float *d
     if (t < n) dz[t] = z[t] > 0.f ? dh[t] : 0.f;
 }
 
-// Output layer, loss and its gradient; one thread per vertex.
-//   out = bf(h4) bf(W_out)^T + b_out;  loss += sum m |out - t|^2 / denom
-//   dout = 2 m (out - t) / denom;  dh4 = bf(dout) bf(W_out);  gW_out += dout bf(h4)^T
-__global__ __launch_bounds__(256) void k_output(int n, const float *h4, const uint16_t *Wb, const float *P,
-                                                const float *target, const uint8_t *mask, float inv_denom,
-                                                int with_grad, float *out, float *dh4, float *G, float *loss)
-{
-    __shared__ float s_g[kFout * kW + kFout];
-    __shared__ float s_loss;
-    for (int q = threadIdx.x; q < kFout * kW + kFout; q += blockDim.x) s_g[q] = 0.f;
-    if (threadIdx.x == 0) s_loss = 0.f;
-    __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const float *hi = h4 + (size_t)i * kW;
-        float o[kFout];
-        for (int c = 0; c < kFout; ++c) {
-            float s = P[kOffBout + c];
-            for (int k = 0; k < kW; ++k) s += bfr(hi[k]) * bf2f(Wb[kOffWout + c * kW + k]);
-            o[c] = s;
-            out[(size_t)i * kFout + c] = s;
-        }
-        if (with_grad) {
-            const float m = mask[i] ? 1.f : 0.f;
-            float d[kFout], l = 0.f;
-            for (int c = 0; c < kFout; ++c) {
-                const float df = m * (o[c] - target[(size_t)i * kFout + c]);
-                l += df * df;
-                d[c] = 2.f * df * inv_denom;
-            }
-            atomicAdd(&s_loss, l * inv_denom);
-            for (int k = 0; k < kW; ++k) {
-                float s = 0.f;
-                for (int c = 0; c < kFout; ++c) s += bfr(d[c]) * bf2f(Wb[kOffWout + c * kW + k]);
-                dh4[(size_t)i * kW + k] = s;
-            }
-            for (int c = 0; c < kFout; ++c) {
-                if (d[c] == 0.f) continue;
-                for (int k = 0; k < kW; ++k) atomicAdd(&s_g[c * kW + k], d[c] * bfr(hi[k]));
-                atomicAdd(&s_g[kFout * kW + c], d[c]);
-            }
-        }
-    }
-    __syncthreads();
-    if (!with_grad) return;
-    for (int q = threadIdx.x; q < kFout * kW + kFout; q += blockDim.x)
-        if (s_g[q] != 0.f) atomicAdd(&G[kOffWout + q], s_g[q]);
-    if (threadIdx.x == 0) atomicAdd(loss, s_loss);
-}
-
-// Weight gradient of one linear layer: gW[o][k] += sum_i dz[i][o] bf(x[i][k]),
-// gb[o] += sum_i dz[i][o], x = [X1 | X2] with K columns.  Each block folds a
-// strided set of 32-row chunks (staged in LDS) into private sums and writes
-// one partial row; k_sum_partials adds the partials into G.
+// Weight gradient of one linear layer: gW[o][k] += sum_i dz[i][o] x[i][k],
+// gb[o] += sum_i dz[i][o], x = [X1 | X2] (BF16) with K columns.  Each block
+// folds a strided set of 32-row chunks (staged in LDS) into private sums and
+// writes one partial row; k_sum_partials adds the partials into G.
 template <int K>
-__global__ __launch_bounds__(256) void k_weight_grad(int n, const float *dz, const float *X1, int K1, const float *X2,
-                                                     float *partial)
+__global__ __launch_bounds__(256) void k_weight_grad(int n, const float *dz, const uint16_t *X1, int K1,
+                                                     const uint16_t *X2, float *partial)
 {
     constexpr int RB = 32;
     constexpr int PER = kW * K / 256;  // outputs per thread
@@ -214,8 +428,8 @@ __global__ __launch_bounds__(256) void k_weight_grad(int n, const float *dz, con
         for (int q = threadIdx.x; q < RB * K; q += 256) {
             const int rr = q / K, k = q % K, i = r0 + rr;
             float v = 0.f;
-            if (i < n) v = k < K1 ? X1[(size_t)i * K1 + k] : X2[(size_t)i * (K - K1) + (k - K1)];
-            s_x[rr][k] = bfr(v);
+            if (i < n) v = bf2f(k < K1 ? X1[(size_t)i * K1 + k] : X2[(size_t)i * (K - K1) + (k - K1)]);
+            s_x[rr][k] = v;
         }
         for (int q = threadIdx.x; q < RB * kW; q += 256) {
             const int rr = q / kW, oo = q % kW, i = r0 + rr;
@@ -245,8 +459,22 @@ __global__ void k_sum_partials(int nparts, int len, const float *partial, float 
     G[t < nw ? goff_w + t : goff_b + (t - nw)] += s;
 }
 
-// Adam on the FP32 master weights, then the BF16 copies (and the transposed
-// copies of the layer weights used by the data-gradient products); clears G.
+// BF16 copy of parameter t, plus the transposed copies of the layer weights
+// used by the data-gradient products.
+__device__ __forceinline__ void write_bf16(int t, float p, uint16_t *Pb, uint16_t *PbT)
+{
+    const uint16_t b = f2bf(p);
+    Pb[t] = b;
+    for (int l = 0; l < kLayers; ++l) {
+        const int w0 = off_W(l);
+        if (t >= w0 && t < w0 + kW * 2 * kW) {
+            const int o = (t - w0) / (2 * kW), k = (t - w0) % (2 * kW);
+            PbT[(size_t)l * 2 * kW * kW + (size_t)k * kW + o] = b;
+        }
+    }
+}
+
+// Adam on the FP32 master weights, then the BF16 copies; clears G.
 __global__ void k_adam(int np, float *P, float *G, float *m, float *v, float lr, float b1, float b2, float eps,
                        float c1, float c2, uint16_t *Pb, uint16_t *PbT)
 {
@@ -260,30 +488,13 @@ __global__ void k_adam(int np, float *P, float *G, float *m, float *v, float lr,
     const float p = P[t] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
     P[t] = p;
     G[t] = 0.f;
-    const uint16_t b = f2bf(p);
-    Pb[t] = b;
-    for (int l = 0; l < kLayers; ++l) {
-        const int w0 = off_W(l);
-        if (t >= w0 && t < w0 + kW * 2 * kW) {
-            const int o = (t - w0) / (2 * kW), k = (t - w0) % (2 * kW);
-            PbT[(size_t)l * 2 * kW * kW + (size_t)k * kW + o] = b;
-        }
-    }
+    write_bf16(t, p, Pb, PbT);
 }
 
 __global__ void k_refresh_bf16(int np, const float *P, uint16_t *Pb, uint16_t *PbT)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= np) return;
-    const uint16_t b = f2bf(P[t]);
-    Pb[t] = b;
-    for (int l = 0; l < kLayers; ++l) {
-        const int w0 = off_W(l);
-        if (t >= w0 && t < w0 + kW * 2 * kW) {
-            const int o = (t - w0) / (2 * kW), k = (t - w0) % (2 * kW);
-            PbT[(size_t)l * 2 * kW * kW + (size_t)k * kW + o] = b;
-        }
-    }
+    if (t < np) write_bf16(t, P[t], Pb, PbT);
 }
 
 // ----------------------------------------------------------------- features and targets
@@ -310,27 +521,31 @@ __device__ __forceinline__ float3 inertia_disp(int i, const float4 *uprev, const
     return make_float3(u.x + dt * v.x + dt2 * g.x, u.y + dt * v.y + dt2 * g.y, u.z + dt * v.z + dt2 * g.z);
 }
 
-// Per-vertex input features (colour-major order); the network is defined in
-// oracle/mpnn_ref.py.  From the solver state at the start of a step: rest
-// position relative to its body, inertia-prediction displacement, its
-// increment, velocity, a local deformation estimate from the neighbours'
-// inertia predictions, pin flag, the body's material and gravity direction,
+// Per-vertex input features, written in the network's (original) vertex
+// order; the network is defined in oracle/mpnn_ref.py.  From the solver
+// state at the start of a step (colour-major vertex i, original v = perm[i]):
+// rest position relative to its body, inertia-prediction displacement, its
+// increment, velocity, pin flag, the body's material and gravity direction,
 // and the update a Jacobi pass from the inertia prediction would take (dxj,
-// swept vertices [0, nsw) only).
-__global__ void k_features(int n, const float4 *Xr, const float4 *uprev, const float4 *vel, const uint8_t *active,
-                           const int32_t *nb_off, const int32_t *nb, FeatParams p, BodyFeat bf, const float4 *dxj,
-                           int nsw, float *f)
+// swept vertices [0, nsw) only).  Features 12..20, a local deformation
+// estimate from the neighbours' inertia predictions, follow in
+// k_features_neighbours from U (the inertia displacement in network order).
+__global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, const float4 *uprev,
+                                  const float4 *vel, const uint8_t *active, FeatParams p, BodyFeat bf,
+                                  const float4 *dxj, int nsw, float4 *U, uint16_t *f)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float *o = f + (size_t)i * kFin;
+    const int v = perm[i];
     const int b = bf.body[i];
     const float4 geo = bf.geo[b];
     const float2 mat = bf.mat[b];
     const float4 g = bf.grav[b];
-    const float4 X = Xr[i];
+    const float4 X = Xr[v];
     const float3 uy = inertia_disp(i, uprev, vel, active, p.dt, g);
-    const float4 up = uprev[i], v = vel[i];
+    const float4 up = uprev[i], vv = vel[i];
+    U[v] = make_float4(uy.x, uy.y, uy.z, 0.f);
+    float o[kFin];
     o[0] = (X.x - geo.x) * geo.w;
     o[1] = (X.y - geo.y) * geo.w;
     o[2] = (X.z - geo.z) * geo.w;
@@ -340,23 +555,11 @@ __global__ void k_features(int n, const float4 *Xr, const float4 *uprev, const f
     o[6] = (uy.x - up.x) / p.s;
     o[7] = (uy.y - up.y) / p.s;
     o[8] = (uy.z - up.z) / p.s;
-    o[9] = v.x * p.dt * p.inv_h;
-    o[10] = v.y * p.dt * p.inv_h;
-    o[11] = v.z * p.dt * p.inv_h;
-    float D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int q0 = nb_off[i], q1 = nb_off[i + 1];
-    for (int q = q0; q < q1; ++q) {
-        const int j = nb[q];
-        const float4 Xj = Xr[j];
-        const float3 uj = inertia_disp(j, uprev, vel, active, p.dt, g);
-        const float dX[3] = {Xj.x - X.x, Xj.y - X.y, Xj.z - X.z};
-        const float du[3] = {uj.x - uy.x, uj.y - uy.y, uj.z - uy.z};
-        const float w = 1.f / (dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2]);
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) D[3 * r + c] += du[r] * dX[c] * w;
-    }
-    const float sc = q1 > q0 ? 1.f / (float)(q1 - q0) : 0.f;
-    for (int q = 0; q < 9; ++q) o[12 + q] = D[q] * sc * 100.f;
+    o[9] = vv.x * p.dt * p.inv_h;
+    o[10] = vv.y * p.dt * p.inv_h;
+    o[11] = vv.z * p.dt * p.inv_h;
+#pragma unroll
+    for (int q = 12; q < 21; ++q) o[q] = 0.f;
     o[21] = active[i] ? 0.f : 1.f;
     o[22] = mat.x - 5.f;
     o[23] = (mat.y - 0.375f) * 10.f;
@@ -370,6 +573,35 @@ __global__ void k_features(int n, const float4 *Xr, const float4 *uprev, const f
     o[29] = d.y / p.s;
     o[30] = d.z / p.s;
     o[31] = 0.f;
+    uint4 *dst = reinterpret_cast<uint4 *>(f + (size_t)v * kFin);
+#pragma unroll
+    for (int q = 0; q < kFin / 8; ++q)
+        dst[q] = make_uint4(bf_pack(o[8 * q], o[8 * q + 1]), bf_pack(o[8 * q + 2], o[8 * q + 3]),
+                            bf_pack(o[8 * q + 4], o[8 * q + 5]), bf_pack(o[8 * q + 6], o[8 * q + 7]));
+}
+
+// Features 12..20 of vertex v (network order): 100 x the mean over neighbours
+// j of (U_j - U_v)(X_j - X_v)^T / |X_j - X_v|^2.
+__global__ void k_features_neighbours(int n, const float4 *Xr, const float4 *U, const int32_t *nb_off,
+                                      const int32_t *nb, uint16_t *f)
+{
+    const int v = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float4 X = Xr[v], uv = U[v];
+    float D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int q0 = nb_off[v], q1 = nb_off[v + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int j = nb[q];
+        const float4 Xj = Xr[j], uj = U[j];
+        const float dX[3] = {Xj.x - X.x, Xj.y - X.y, Xj.z - X.z};
+        const float du[3] = {uj.x - uv.x, uj.y - uv.y, uj.z - uv.z};
+        const float w = 1.f / (dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2]);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) D[3 * r + c] += du[r] * dX[c] * w;
+    }
+    const float sc = q1 > q0 ? 100.f / (float)(q1 - q0) : 0.f;
+    uint16_t *o = f + (size_t)v * kFin + 12;
+    for (int q = 0; q < 9; ++q) o[q] = f2bf(D[q] * sc);
 }
 
 // The plain initial guess (inertia prediction, per-body gravity) in u and the
@@ -382,27 +614,30 @@ __global__ void k_plain_guess(int n, float4 *u, const float4 *uprev, const float
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body[i]]);
 }
 
-// training target of the step just taken: (x_{n+1} - x_inertia) / s
-__global__ void k_target(int n, const float4 *u, const float4 *tgt, const uint8_t *active, float inv_s, float *t)
+// training target of the step just taken, in network order: (x_{n+1} - x_inertia) / s
+__global__ void k_target(int n, const int32_t *perm, const float4 *u, const float4 *tgt, const uint8_t *active,
+                         float inv_s, float *t)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 x = u[i], y = tgt[i];
     const bool a = active[i];
-    t[(size_t)i * 3] = a ? (x.x - y.x) * inv_s : 0.f;
-    t[(size_t)i * 3 + 1] = a ? (x.y - y.y) * inv_s : 0.f;
-    t[(size_t)i * 3 + 2] = a ? (x.z - y.z) * inv_s : 0.f;
+    float *o = t + (size_t)perm[i] * kFout;
+    o[0] = a ? (x.x - y.x) * inv_s : 0.f;
+    o[1] = a ? (x.y - y.y) * inv_s : 0.f;
+    o[2] = a ? (x.z - y.z) * inv_s : 0.f;
 }
 
-// network guess: x_inertia + s out (active vertices)
-__global__ void k_guess(int n, const float4 *tgt, const uint8_t *active, const float *out, float s, float4 *dst)
+// network guess (colour-major): x_inertia + s out[perm[i]] for active vertices
+__global__ void k_guess(int n, const int32_t *perm, const float4 *tgt, const uint8_t *active, const float *out,
+                        float s, float4 *dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 y = tgt[i];
+    const float *o = out + (size_t)perm[i] * kFout;
     if (active[i])
-        dst[i] = make_float4(y.x + s * out[(size_t)i * 3], y.y + s * out[(size_t)i * 3 + 1],
-                             y.z + s * out[(size_t)i * 3 + 2], 0.f);
+        dst[i] = make_float4(y.x + s * o[0], y.y + s * o[1], y.z + s * o[2], 0.f);
     else
         dst[i] = make_float4(y.x, y.y, y.z, 0.f);
 }
@@ -458,17 +693,18 @@ __global__ __launch_bounds__(256) void k_choose_guess(const int2 *ranges, int C,
     }
 }
 
-// full neighbour lists (distinct, without self), colour-major ids
+// full neighbour lists (distinct vertices sharing a tet, without self), in
+// original vertex ids, each list in order of first appearance over the
+// vertex's incident tets (ascending tet index)
 template <bool FILL>
-__global__ void k_graph(const int32_t *perm, const int32_t *iperm, const int32_t *tets, const int32_t *off,
-                        const int32_t *inc, int n, int32_t *cnt, const int32_t *g_off, int32_t *g)
+__global__ void k_graph(const int32_t *tets, const int32_t *off, const int32_t *inc, int n, int32_t *cnt,
+                        const int32_t *g_off, int32_t *g)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int v = perm[i];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
     const int a = off[v], b = off[v + 1];
     int k = 0;
-    const int w = FILL ? g_off[i] : 0;
+    const int w = FILL ? g_off[v] : 0;
     for (int s = a; s < b; ++s) {
         const int e = inc[s] >> 2;
         for (int q = 0; q < 4; ++q) {
@@ -481,19 +717,22 @@ __global__ void k_graph(const int32_t *perm, const int32_t *iperm, const int32_t
                 for (int q2 = 0; q2 < 4; ++q2) seen |= tets[4 * e2 + q2] == u;
             }
             if (seen) continue;
-            if (FILL) g[w + k] = iperm[u];
+            if (FILL) g[w + k] = u;
             ++k;
         }
     }
-    if (!FILL) cnt[i] = k;
+    if (!FILL) cnt[v] = k;
 }
 
-__global__ void k_rest_new(int n, const int32_t *perm, const double *X, float4 *Xr)
+// rest positions and the active mask in network (original) order
+__global__ void k_net_arrays(int n, const int32_t *perm, const double *X, const uint8_t *active, float4 *Xr,
+                             uint8_t *act)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int v = perm[i];
-    Xr[i] = make_float4((float)X[3 * v], (float)X[3 * v + 1], (float)X[3 * v + 2], 0.f);
+    Xr[v] = make_float4((float)X[3 * v], (float)X[3 * v + 1], (float)X[3 * v + 2], 0.f);
+    act[v] = active[i];
 }
 
 template <int L, int MAT>
@@ -510,16 +749,20 @@ using namespace vbd;
 struct vbd_mpnn {
     vbd_solver *s = nullptr;
     int n = 0, n_edges = 0;
-    int32_t *nb_off = nullptr, *nb = nullptr;
-    float4 *Xr = nullptr;
+    int32_t *nb_off = nullptr, *nb = nullptr;  // network graph (original ids)
+    float4 *Xr = nullptr, *U = nullptr;         // rest positions, inertia displacement (network order)
+    uint8_t *act = nullptr;                     // active mask (network order)
     float *P = nullptr, *G = nullptr, *Am = nullptr, *Av = nullptr;
     uint16_t *Pb = nullptr, *PbT = nullptr;
-    float *f = nullptr, *z0 = nullptr, *h[kLayers + 1] = {}, *a[kLayers] = {}, *z[kLayers] = {};
+    uint16_t *f = nullptr, *h[kLayers + 1] = {}, *a[kLayers] = {};
+    float *z0 = nullptr, *z[kLayers] = {};
     float *out = nullptr, *dh = nullptr, *dx = nullptr, *dz = nullptr, *partial = nullptr, *loss = nullptr;
-    int wg_blocks = 0;
-    // sample store
+    int wg_blocks = 0, out_blocks = 0;
+    // sample store (network order)
     int cap = 0;
-    float *F = nullptr, *T = nullptr;
+    uint16_t *F = nullptr;
+    float *T = nullptr;
+    // colour-major (solver order)
     float4 *guess = nullptr;
     uint32_t *rm0 = nullptr, *rm1 = nullptr;  // per-vertex first-sweep residuals of the two guesses
     float4 *dxj = nullptr;                     // Jacobi update from the plain guess (a feature)
@@ -538,9 +781,9 @@ namespace {
 
 int mpnn_free(vbd_mpnn *m)
 {
-    void *ps[] = {m->nb_off, m->nb, m->Xr, m->P, m->G, m->Am, m->Av, m->Pb, m->PbT, m->f, m->z0, m->out, m->dh,
-                  m->dx, m->dz, m->partial, m->loss, m->F, m->T, m->guess, m->rm0, m->rm1, m->use_alt,
-                  m->body_geo, m->body_mat, m->dxj};
+    void *ps[] = {m->nb_off, m->nb, m->Xr, m->U, m->act, m->P, m->G, m->Am, m->Av, m->Pb, m->PbT, m->f, m->z0,
+                  m->out, m->dh, m->dx, m->dz, m->partial, m->loss, m->F, m->T, m->guess, m->rm0, m->rm1,
+                  m->use_alt, m->body_geo, m->body_mat, m->dxj};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (auto p : m->h)
@@ -566,21 +809,31 @@ int dalloc(T **p, size_t count)
     return 0;
 }
 
-// forward of sample features f (m->f); with_grad: loss + backward into m->G
-int mpnn_forward_backward(vbd_mpnn *m, const float *target, bool with_grad, float inv_denom)
+// Forward of features f (network order, BF16 [n, 32]) into m->out; with_grad:
+// the loss against target goes to m->loss and the gradient into m->G.
+int mpnn_forward_backward(vbd_mpnn *m, const uint16_t *f, const float *target, bool with_grad, float inv_denom)
 {
     hipStream_t st = m->s->stream;
     const int n = m->n;
-    const dim3 g1(nblk(n, 64), 1), g2(nblk(n, 64), 2);
-    k_mfma_linear<kFin, 0><<<g1, 256, 0, st>>>(n, m->f, kFin, nullptr, m->Pb + kOffWin, m->P + kOffBin, m->z0,
-                                                nullptr, m->h[0], kW);
-    for (int l = 0; l < kLayers; ++l) {
-        k_neighbour_mean<<<nblk((int64_t)n * kW, 256), 256, 0, st>>>(n, m->nb_off, m->nb, m->h[l], m->a[l]);
-        k_mfma_linear<2 * kW, 1><<<g1, 256, 0, st>>>(n, m->h[l], kW, m->a[l], m->Pb + off_W(l), m->P + off_b(l),
-                                                      m->z[l], m->h[l], m->h[l + 1], kW);
+    const int tiles = nblk(n, 64);
+    const int mp_blocks = std::min(tiles, 1024);  // persistent blocks (4 per CU), a run of tiles each
+    if (with_grad) {
+        k_in_layer<true><<<tiles, 256, 0, st>>>(n, f, m->Pb + kOffWin, m->P + kOffBin, m->z0, m->h[0]);
+        for (int l = 0; l < kLayers; ++l)
+            k_mp_layer<true><<<mp_blocks, 256, 0, st>>>(n, tiles, m->nb_off, m->nb, m->h[l], m->Pb + off_W(l),
+                                                        m->P + off_b(l), m->h[l + 1], m->a[l], m->z[l]);
+        k_out_layer<true><<<m->out_blocks, 256, 0, st>>>(n, m->h[kLayers], m->Pb + kOffWout, m->P + kOffBout, target,
+                                                         m->act, inv_denom, m->out, m->dh, m->partial);
+        k_sum_out<<<1, 256, 0, st>>>(m->out_blocks, m->partial, m->G, m->loss);
+    } else {
+        // inference needs no a_l, z_l: ping-pong between two activation buffers
+        k_in_layer<false><<<tiles, 256, 0, st>>>(n, f, m->Pb + kOffWin, m->P + kOffBin, nullptr, m->h[0]);
+        for (int l = 0; l < kLayers; ++l)
+            k_mp_layer<false><<<mp_blocks, 256, 0, st>>>(n, tiles, m->nb_off, m->nb, m->h[l & 1], m->Pb + off_W(l),
+                                                         m->P + off_b(l), m->h[(l + 1) & 1], nullptr, nullptr);
+        k_out_layer<false><<<m->out_blocks, 256, 0, st>>>(n, m->h[kLayers & 1], m->Pb + kOffWout, m->P + kOffBout,
+                                                          nullptr, m->act, 0.f, m->out, nullptr, nullptr);
     }
-    k_output<<<nblk(n, 256), 256, 0, st>>>(n, m->h[kLayers], m->Pb, m->P, target, m->s->active, inv_denom,
-                                            with_grad ? 1 : 0, m->out, m->dh, m->G, m->loss);
     VBD_HIP(hipGetLastError());
     if (!with_grad) return 0;
     const int64_t nw = (int64_t)n * kW;
@@ -590,13 +843,11 @@ int mpnn_forward_backward(vbd_mpnn *m, const float *target, bool with_grad, floa
         const int len = kW * 2 * kW + kW;
         k_sum_partials<<<nblk(len, 256), 256, 0, st>>>(m->wg_blocks, len, m->partial, m->G, off_W(l), off_b(l),
                                                         kW * 2 * kW);
-        // dx = bf(dz) bf(W_l): [n, 128]; the transposed BF16 copy is [128, 64]
-        k_mfma_linear<kW, 2><<<g2, 256, 0, st>>>(n, m->dz, kW, nullptr, m->PbT + (size_t)l * 2 * kW * kW, nullptr,
-                                                  m->dx, nullptr, nullptr, 2 * kW);
+        k_dgrad<<<dim3(tiles, 2), 256, 0, st>>>(n, m->dz, m->PbT + (size_t)l * 2 * kW * kW, m->dx);
         k_backprop_mean<<<nblk(nw, 256), 256, 0, st>>>(n, m->nb_off, m->nb, m->dx, m->dh);
     }
     k_relu_grad<<<nblk(nw, 256), 256, 0, st>>>(nw, m->dh, m->z0, m->dz);
-    k_weight_grad<kFin><<<m->wg_blocks, 256, 0, st>>>(n, m->dz, m->f, kFin, nullptr, m->partial);
+    k_weight_grad<kFin><<<m->wg_blocks, 256, 0, st>>>(n, m->dz, f, kFin, nullptr, m->partial);
     const int len = kW * kFin + kW;
     k_sum_partials<<<nblk(len, 256), 256, 0, st>>>(m->wg_blocks, len, m->partial, m->G, kOffWin, kOffBin, kW * kFin);
     VBD_HIP(hipGetLastError());
@@ -642,11 +893,11 @@ void jacobi(vbd_mpnn *m, float4 *u, double dt, float4 *dx, uint32_t *res)
     else launch_jacobi<L, kStVK>(a, s->n_swept, dx, res, s->stream);
 }
 
-// Features of the solver's state at the start of a step of size dt into f.
-// plain_in_u: s->u and s->tgt already hold the plain guess and the inertia
-// target (the step's init ran); otherwise the plain guess is formed in
-// m->guess and the solver's positions are left alone.
-void features(vbd_mpnn *m, double dt, bool plain_in_u, float *f)
+// Features of the solver's state at the start of a step of size dt into f
+// (network order).  plain_in_u: s->u and s->tgt already hold the plain guess
+// and the inertia target (the step's init ran); otherwise the plain guess is
+// formed in m->guess and the solver's positions are left alone.
+void features(vbd_mpnn *m, double dt, bool plain_in_u, uint16_t *f)
 {
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
@@ -658,8 +909,9 @@ void features(vbd_mpnn *m, double dt, bool plain_in_u, float *f)
                                                        s->body_new, s->d_grav, (float)dt);
     }
     jacobi(m, u, dt, m->dxj, m->rm0);
-    k_features<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, s->uprev, s->vel, s->active, m->nb_off, m->nb, m->fp,
-                                                 body_feat(m), m->dxj, s->n_swept, f);
+    k_features_vertex<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, m->Xr, s->uprev, s->vel, s->active, m->fp,
+                                                       body_feat(m), m->dxj, s->n_swept, m->U, f);
+    k_features_neighbours<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, m->U, m->nb_off, m->nb, f);
 }
 
 }  // namespace
@@ -679,8 +931,8 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
         return 0;
     }
     features(m, dt, true, m->f);
-    VBD_TRY(mpnn_forward_backward(m, nullptr, false, 0.f));
-    k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->tgt, s->active, m->out, m->fp.s, m->guess);
+    VBD_TRY(mpnn_forward_backward(m, m->f, nullptr, false, 0.f));
+    k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, m->out, m->fp.s, m->guess);
     const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
     if (!force) jacobi(m, m->guess, dt, nullptr, m->rm1);
     k_choose_guess<<<s->n_bodies, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, m->rm1, m->guess, s->u, m->use_alt,
@@ -723,12 +975,11 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         mpnn_free(m);
         return r;
     };
-    // graph
+    // graph (original ids; the solver's incidence CSR is in original ids too)
     int32_t *cnt = nullptr;
     if ((rc = dalloc(&cnt, (size_t)n + 1)) || (rc = dalloc(&m->nb_off, (size_t)n + 1))) return fail(rc);
     (void)hipMemsetAsync(cnt, 0, sizeof(int32_t) * (n + 1), st);
-    k_graph<false><<<nblk(n, 256), 256, 0, st>>>(s->perm, s->iperm, s->tets_dev, s->inc_off, s->inc, n, cnt, nullptr,
-                                                 nullptr);
+    k_graph<false><<<nblk(n, 256), 256, 0, st>>>(s->tets_dev, s->inc_off, s->inc, n, cnt, nullptr, nullptr);
     size_t tb = 0;
     (void)rocprim_exclusive_scan_i32(nullptr, tb, cnt, m->nb_off, n + 1, st);
     void *tmp = nullptr;
@@ -744,10 +995,9 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
     (void)hipFree(cnt);
     m->n_edges = ne;
     if ((rc = dalloc(&m->nb, ne))) return fail(rc);
-    k_graph<true><<<nblk(n, 256), 256, 0, st>>>(s->perm, s->iperm, s->tets_dev, s->inc_off, s->inc, n, nullptr,
-                                                m->nb_off, m->nb);
-    if ((rc = dalloc(&m->Xr, n))) return fail(rc);
-    k_rest_new<<<nblk(n, 256), 256, 0, st>>>(n, s->perm, s->X, m->Xr);
+    k_graph<true><<<nblk(n, 256), 256, 0, st>>>(s->tets_dev, s->inc_off, s->inc, n, nullptr, m->nb_off, m->nb);
+    if ((rc = dalloc(&m->Xr, n)) || (rc = dalloc(&m->U, n)) || (rc = dalloc(&m->act, n))) return fail(rc);
+    k_net_arrays<<<nblk(n, 256), 256, 0, st>>>(n, s->perm, s->X, s->active, m->Xr, m->act);
     // parameters
     if ((rc = dalloc(&m->P, kNumParams)) || (rc = dalloc(&m->G, kNumParams)) || (rc = dalloc(&m->Am, kNumParams)) ||
         (rc = dalloc(&m->Av, kNumParams)) || (rc = dalloc(&m->Pb, kNumParams)) ||
@@ -763,15 +1013,17 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         (rc = dalloc(&m->dh, nw)) || (rc = dalloc(&m->dx, 2 * nw)) || (rc = dalloc(&m->dz, nw)) ||
         (rc = dalloc(&m->loss, 1)) || (rc = dalloc(&m->guess, n)) || (rc = dalloc(&m->use_alt, s->n_bodies)) ||
         (rc = dalloc(&m->rm0, std::max(1, s->n_swept))) || (rc = dalloc(&m->rm1, std::max(1, s->n_swept))) ||
-        (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) ||
-        (rc = dalloc(&m->body_geo, s->n_bodies)) || (rc = dalloc(&m->body_mat, s->n_bodies)))
+        (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) || (rc = dalloc(&m->body_geo, s->n_bodies)) ||
+        (rc = dalloc(&m->body_mat, s->n_bodies)))
         return fail(rc);
     for (int l = 0; l <= kLayers; ++l)
         if ((rc = dalloc(&m->h[l], nw))) return fail(rc);
     for (int l = 0; l < kLayers; ++l)
         if ((rc = dalloc(&m->a[l], nw)) || (rc = dalloc(&m->z[l], nw))) return fail(rc);
     m->wg_blocks = std::max(1, std::min(512, nblk(n, 32)));
-    if ((rc = dalloc(&m->partial, (size_t)m->wg_blocks * (kW * 2 * kW + kW)))) return fail(rc);
+    m->out_blocks = std::max(1, std::min(1024, nblk(n, 16)));
+    const size_t part = std::max((size_t)m->wg_blocks * (kW * 2 * kW + kW), (size_t)m->out_blocks * kOutPart);
+    if ((rc = dalloc(&m->partial, part))) return fail(rc);
     if (hipHostMalloc((void **)&m->use_alt_host, sizeof(int32_t) * s->n_bodies, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&m->loss_host, sizeof(float), hipHostMallocDefault) != hipSuccess)
         return fail(VBD_ENOMEM);
@@ -910,7 +1162,7 @@ int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t
     } else {
         VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
     }
-    k_target<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->uprev, s->tgt, s->active, 1.f / m->fp.s,
+    k_target<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->uprev, s->tgt, s->active, 1.f / m->fp.s,
                                               m->T + (size_t)slot * m->n * 3);
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(st));
@@ -928,11 +1180,10 @@ int vbd_mpnn_loss_grad(vbd_mpnn_t *m, int32_t slot, float *loss_out, float *grad
         return VBD_EINVAL;
     }
     hipStream_t st = m->s->stream;
-    VBD_HIP(hipMemcpyAsync(m->f, m->F + (size_t)slot * m->n * kFin, sizeof(float) * m->n * kFin,
-                           hipMemcpyDeviceToDevice, st));
     VBD_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), st));
     const float inv_denom = 1.f / (3.f * (float)std::max(1, m->n_active));
-    VBD_TRY(mpnn_forward_backward(m, m->T + (size_t)slot * m->n * 3, true, inv_denom));
+    VBD_TRY(mpnn_forward_backward(m, m->F + (size_t)slot * m->n * kFin, m->T + (size_t)slot * m->n * 3, true,
+                                  inv_denom));
     VBD_HIP(hipMemcpyAsync(m->loss_host, m->loss, sizeof(float), hipMemcpyDeviceToHost, st));
     if (grads) {
         VBD_HIP(hipMemcpyAsync(grads, m->G, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, st));
@@ -962,21 +1213,18 @@ int vbd_mpnn_train(vbd_mpnn_t *m, int32_t n_samples, int32_t epochs, int32_t bat
     for (int ep = 0; ep < epochs; ++ep) {
         std::shuffle(order.begin(), order.end(), rng);
         VBD_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), st));
-        int nb = 0;
         for (int b0 = 0; b0 < n_samples; b0 += batch) {
             const int b1 = std::min(n_samples, b0 + batch);
             for (int q = b0; q < b1; ++q) {
                 const int slot = order[q];
-                VBD_HIP(hipMemcpyAsync(m->f, m->F + (size_t)slot * m->n * kFin, sizeof(float) * m->n * kFin,
-                                       hipMemcpyDeviceToDevice, st));
-                VBD_TRY(mpnn_forward_backward(m, m->T + (size_t)slot * m->n * 3, true, inv_denom));
+                VBD_TRY(mpnn_forward_backward(m, m->F + (size_t)slot * m->n * kFin, m->T + (size_t)slot * m->n * 3,
+                                              true, inv_denom));
             }
             ++m->adam_t;
             const float b1c = 0.9f, b2c = 0.999f;
             const float c1 = 1.f - std::pow(b1c, (float)m->adam_t), c2 = 1.f - std::pow(b2c, (float)m->adam_t);
             k_adam<<<nblk(kNumParams, 256), 256, 0, st>>>(kNumParams, m->P, m->G, m->Am, m->Av, lr, b1c, b2c, 1e-8f,
                                                           c1, c2, m->Pb, m->PbT);
-            ++nb;
         }
         VBD_HIP(hipGetLastError());
         VBD_HIP(hipMemcpyAsync(m->loss_host, m->loss, sizeof(float), hipMemcpyDeviceToHost, st));
@@ -995,28 +1243,20 @@ int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out_host)
         set_error("vbd_mpnn_predict: bad arguments");
         return VBD_EINVAL;
     }
-    vbd_solver *s = m->s;
-    hipStream_t st = s->stream;
-    set_feature_params(m, dt);
-    if (slot >= 0)
-        VBD_HIP(hipMemcpyAsync(m->f, m->F + (size_t)slot * m->n * kFin, sizeof(float) * m->n * kFin,
-                               hipMemcpyDeviceToDevice, st));
-    else
-        features(m, dt, false, m->f);
-    VBD_TRY(mpnn_forward_backward(m, nullptr, false, 0.f));
-    std::vector<float> tmp((size_t)m->n * 3);
-    std::vector<int32_t> perm(m->n);
-    VBD_HIP(hipMemcpyAsync(tmp.data(), m->out, sizeof(float) * 3 * m->n, hipMemcpyDeviceToHost, st));
-    VBD_HIP(hipMemcpyAsync(perm.data(), s->perm, sizeof(int32_t) * m->n, hipMemcpyDeviceToHost, st));
+    hipStream_t st = m->s->stream;
+    const uint16_t *f = m->f;
+    if (slot >= 0) f = m->F + (size_t)slot * m->n * kFin;
+    else features(m, dt, false, m->f);
+    VBD_TRY(mpnn_forward_backward(m, f, nullptr, false, 0.f));
+    VBD_HIP(hipMemcpyAsync(out_host, m->out, sizeof(float) * 3 * m->n, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
-    for (int i = 0; i < m->n; ++i)
-        for (int k = 0; k < 3; ++k) out_host[(size_t)perm[i] * 3 + k] = tmp[(size_t)i * 3 + k];
     return 0;
 }
 
-// Copy stored sample `slot` to the host in the original vertex order:
-// features [n, 32], targets [n, 3] (either may be NULL); plus the graph in
-// original ids: nb_off [n+1], nb [n_edges] (nullable).
+// Copy stored sample `slot` to the host in the original vertex order (the
+// network's own): features [n, 32] as FP32 (the stored BF16 values), targets
+// [n, 3] (either may be NULL); plus the graph: nb_off [n+1], nb [n_edges]
+// (nullable).
 int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *targets, int32_t *nb_off,
                          int32_t *nb, int32_t *n_edges)
 {
@@ -1024,30 +1264,22 @@ int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *ta
         set_error("vbd_mpnn_read_sample: bad arguments");
         return VBD_EINVAL;
     }
-    vbd_solver *s = m->s;
     const int n = m->n;
-    std::vector<int32_t> perm(n), off(n + 1), g(std::max(1, m->n_edges));
-    std::vector<float> F((size_t)n * kFin), T((size_t)n * 3);
-    VBD_HIP(hipMemcpy(perm.data(), s->perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-    VBD_HIP(hipMemcpy(off.data(), m->nb_off, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
-    if (m->n_edges) VBD_HIP(hipMemcpy(g.data(), m->nb, sizeof(int32_t) * m->n_edges, hipMemcpyDeviceToHost));
-    VBD_HIP(hipMemcpy(F.data(), m->F + (size_t)slot * n * kFin, sizeof(float) * n * kFin, hipMemcpyDeviceToHost));
-    VBD_HIP(hipMemcpy(T.data(), m->T + (size_t)slot * n * 3, sizeof(float) * n * 3, hipMemcpyDeviceToHost));
     if (n_edges) *n_edges = m->n_edges;
-    if (features)
-        for (int i = 0; i < n; ++i) std::memcpy(features + (size_t)perm[i] * kFin, &F[(size_t)i * kFin], sizeof(float) * kFin);
-    if (targets)
-        for (int i = 0; i < n; ++i) std::memcpy(targets + (size_t)perm[i] * 3, &T[(size_t)i * 3], sizeof(float) * 3);
-    if (nb_off && nb) {
-        // original order: vertex perm[i] owns colour-major row i
-        std::vector<int32_t> deg(n);
-        for (int i = 0; i < n; ++i) deg[perm[i]] = off[i + 1] - off[i];
-        nb_off[0] = 0;
-        for (int v = 0; v < n; ++v) nb_off[v + 1] = nb_off[v] + deg[v];
-        for (int i = 0; i < n; ++i) {
-            const int v = perm[i];
-            for (int q = off[i]; q < off[i + 1]; ++q) nb[nb_off[v] + (q - off[i])] = perm[g[q]];
+    if (features) {
+        std::vector<uint16_t> F((size_t)n * kFin);
+        VBD_HIP(hipMemcpy(F.data(), m->F + (size_t)slot * n * kFin, sizeof(uint16_t) * n * kFin,
+                          hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < F.size(); ++q) {
+            const uint32_t u = (uint32_t)F[q] << 16;
+            std::memcpy(features + q, &u, 4);
         }
+    }
+    if (targets)
+        VBD_HIP(hipMemcpy(targets, m->T + (size_t)slot * n * 3, sizeof(float) * n * 3, hipMemcpyDeviceToHost));
+    if (nb_off && nb) {
+        VBD_HIP(hipMemcpy(nb_off, m->nb_off, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+        if (m->n_edges) VBD_HIP(hipMemcpy(nb, m->nb, sizeof(int32_t) * m->n_edges, hipMemcpyDeviceToHost));
     }
     return 0;
 }
