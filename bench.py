@@ -58,6 +58,8 @@ def parse():
     p.add_argument("--max-iter", type=int, default=1000)
     p.add_argument("--dt", type=float, default=1e-3)
     p.add_argument("--quick", action="store_true", help="headline workload only")
+    p.add_argument("--sections", default="stvk,batched,warm_start,mixed",
+                   help="comma list of the sections to run after the headline (ignored with --quick)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--batch-steps", type=int, default=30)
@@ -138,6 +140,27 @@ def cpu_baseline(scene, args, gpu_iters_per_step):
     }
 
 
+def plain_per_body(bodies, args, n_sample=32, steps=10, seed=0):
+    """One plain VBDSolver per body, stepped one body after another (a
+    WarpVBD user without the concatenation trick): mean wall seconds per step
+    of a seeded sample of bodies, scaled to the whole set."""
+    from warpvbd_amd import VBDSolver
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(len(bodies), size=min(n_sample, len(bodies)), replace=False)
+    per_step = []
+    for b in pick:
+        s = VBDSolver(**bodies[b].solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+        for _ in range(3):
+            s.step(s.old_positions, args.dt)
+        w, _, _ = time_steps(s, steps, args.dt)
+        per_step.append(w / steps)
+        del s
+    wall_per_step_all = float(np.mean(per_step)) * len(bodies)
+    return {"aggregate_sim_s_per_wall_s": len(bodies) * args.dt / wall_per_step_all,
+            "ms_per_step_all_bodies": 1e3 * wall_per_step_all,
+            "sample": f"{len(pick)} bodies drawn with seed {seed}, {steps} timed steps each after 3 untimed"}
+
+
 def batched_section(args):
     from warpvbd_amd import BatchedVBDSolver, VBDSolver
     from warpvbd_amd.batched import concat_bodies
@@ -159,7 +182,10 @@ def batched_section(args):
     w_bat, it_bat, ms_bat = time_steps(bat, args.batch_steps, args.dt)
     nb = len(bodies)
     sim = args.batch_steps * args.dt
+    per_body = plain_per_body(bodies, args)
     return {
+        "plain_per_body": per_body,
+        "speedup_vs_plain_per_body": (nb * sim / w_bat) / per_body["aggregate_sim_s_per_wall_s"],
         "workload": f"{nb} mass-spring bodies (cube 6..14, column 3..12 voxels, seed 3): "
                     f"{len(cb['initial_positions'])} vertices, {len(cb['elements'])} tets, SNH E=1e5 nu=0.4, "
                     f"dt=1e-3, tol 1e-7, {args.batch_steps} timed steps",
@@ -296,8 +322,12 @@ def mixed_section(args):
                            "training": f"{args.mixed_samples} samples from the batch's own steps, "
                                        f"{args.warm_epochs} epochs, {t_gen:.1f} s + {t_train:.1f} s",
                            "train_loss_first_last": [float(hist[0]), float(hist[-1])]}
+    out["plain_per_body"] = plain_per_body(bodies, args)
     out["speedup_batched_vs_plain"] = out["batched"]["aggregate_sim_s_per_wall_s"] / \
         out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
+    for k in ("batched", "batched_warm"):
+        out[f"speedup_{k}_vs_plain_per_body"] = out[k]["aggregate_sim_s_per_wall_s"] / \
+            out["plain_per_body"]["aggregate_sim_s_per_wall_s"]
     out["speedup_batched_warm_vs_plain"] = out["batched_warm"]["aggregate_sim_s_per_wall_s"] / \
         out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
     return out
@@ -354,17 +384,20 @@ def main():
         },
     }
     del solver
-    if not args.quick:
-        if args.material == "snh":
-            log("stvk")
-            sc2, s2, w2, it2, ms2, pr2 = headline(argparse.Namespace(**{**vars(args), "steps": 100}), "stvk")
-            out["stvk"] = {"sim_s_per_wall_s": 100 * args.dt / w2, "iterations_per_step": float(it2.mean()),
-                           "sweep_GBps": pr2["bytes"] / (pr2["ms"] * 1e-3) / 1e9 if pr2["ms"] else None}
-            del s2
+    sections = [] if args.quick else [x.strip() for x in args.sections.split(",") if x.strip()]
+    if "stvk" in sections and args.material == "snh":
+        log("stvk")
+        sc2, s2, w2, it2, ms2, pr2 = headline(argparse.Namespace(**{**vars(args), "steps": 100}), "stvk")
+        out["stvk"] = {"sim_s_per_wall_s": 100 * args.dt / w2, "iterations_per_step": float(it2.mean()),
+                       "sweep_GBps": pr2["bytes"] / (pr2["ms"] * 1e-3) / 1e9 if pr2["ms"] else None}
+        del s2
+    if "batched" in sections:
         log("batched")
         out["batched"] = batched_section(args)
+    if "warm_start" in sections:
         log("warm start")
         out["warm_start"] = warm_section(args)
+    if "mixed" in sections:
         log("mixed")
         out["mixed"] = mixed_section(args)
     if not args.no_cpu:
