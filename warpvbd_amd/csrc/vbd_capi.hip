@@ -33,7 +33,7 @@ void vbd_destroy(vbd_solver_t *s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     release_batch(s);
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
-                   s->active, s->slot_off, s->slot_ent, s->slots, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
+                   s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);

@@ -1,18 +1,21 @@
 // vbd_device.h -- per-slot VBD physics, 3x3 solve and lane-group reductions (gfx950).
 //
 // A "slot" is one (vertex, incident tet) pair.  Slots of a vertex are stored
-// contiguously in colour-major vertex order, one 64-byte record each:
+// contiguously in colour-major vertex order, 56 bytes each in two arrays:
 //
-//   q0 = (o0, o1, o2 as int bits, ka)   ids of the tet's other three vertices
-//   q1 = (bs.xyz, kb)                   shape gradient b of the slot's own vertex
-//   q2 = (b0.xyz, kc)                   b of vertex o0
-//   q3 = (b1.xyz, 0)                    b of vertex o1 ; b of o2 = -(bs+b0+b1)
+//   slots  (3 float4 per slot)
+//     qa = (o0, o1, o2 as int bits, ka)   ids of the tet's other three vertices
+//     qb = (bs.xyz, kb)                   shape gradient b of the slot's own vertex
+//     qc = (b0.xyz, b1.x)                 b of vertex o0, first component of b of o1
+//   slots2 (1 float2 per slot)
+//     qd = (b1.y, b1.z)                   ; b of o2 = -(bs+b0+b1)
 //
 // b is the row of invDm belonging to the vertex's local index (minus the row sum
 // for local index 3), so dF/dx_{v,i} = e_i b^T (src/vbd_solver.py:86-94,450-465).
 // Material constants, volume folded in:
 //   SNH : ka = V mu', kb = V lambda', kc = 0.75 ka  (= V lambda'(alpha-1))
 //   StVK: ka = V mu,  kb = V lambda,  kc = 0
+// kc follows from ka and the material, so it is not stored.
 //
 // Positions are carried as FP32 displacements u = x - X from the rest shape, so
 // F = I + D with D = sum_i (u_i - u_self) b_i^T is formed from small
@@ -21,12 +24,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-
-// Slot record storage: 0 = four arrays of n_slots float4 (q0 | q1 | q2 | q3),
-// 1 = interleaved 64-byte records.
-#ifndef VBD_SLOT_AOS
-#define VBD_SLOT_AOS 1
-#endif
 
 // VBD_TET_RECORDS = 1 replaces the per-slot records by one 64-byte record per
 // tet, shared by its four slots, and a 4-byte entry per slot (tet << 2 | local
@@ -45,18 +42,14 @@
 
 namespace vbd {
 
-// index of float4 q (0..3) of slot s among ns slots
-__host__ __device__ __forceinline__ int64_t slot_q(int64_t s, int q, int64_t ns)
-{
-#if VBD_SLOT_AOS
-    (void)ns;
-    return 4 * s + q;
-#else
-    return (int64_t)q * ns + s;
-#endif
-}
-
 enum Material : int { kSNH = 0, kStVK = 1 };
+
+// kc of a slot (see the record layout above)
+template <int MAT>
+__device__ __forceinline__ float slot_kc(float ka)
+{
+    return MAT == kSNH ? 0.75f * ka : 0.f;
+}
 
 // A slot's view of its tet record (VBD_TET_RECORDS layout): the other three
 // vertex ids (ascending local index) and the b vectors of the vertex itself

@@ -134,7 +134,8 @@ struct vbd_solver {
     uint8_t *active = nullptr;    // active flag
     int32_t *slot_off = nullptr;  // [n_swept+1]
     int32_t *slot_ent = nullptr;  // [n_slots] tet << 2 | local index (tet-record layout)
-    float4 *slots = nullptr;      // [n_slots*4]
+    float4 *slots = nullptr;      // [n_slots*3] slot records qa|qb|qc (or tet records, [m*4])
+    float2 *slots2 = nullptr;     // [n_slots] slot records qd
     int32_t *colors = nullptr;    // original order
     int32_t *inc_off = nullptr;   // original order vertex -> incident (tet<<2|local), ascending tet
     int32_t *inc = nullptr;

@@ -350,12 +350,12 @@ __device__ __forceinline__ void brow(const double *iD, int k, double b[3])
     }
 }
 
-// 64-byte slot records (layout documented in vbd_device.h), stored as four
-// arrays of n_slots float4 (q0 | q1 | q2 | q3).
+// 56-byte slot records (layout documented in vbd_device.h): qa|qb|qc in
+// slots, qd in slots2.
 __global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const int32_t *inc_off, const int32_t *inc,
                               const int32_t *tets, const double *invDm, const double *vol, const double *mu,
                               const double *lam, int material, int nswept, const int32_t *slot_off, float4 *slots,
-                              int64_t ns)
+                              float2 *slots2)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nswept) return;
@@ -376,10 +376,11 @@ __global__ void k_build_slots(const int32_t *perm, const int32_t *iperm, const i
         int o0 = iperm[tets[4 * e + other[0]]];
         int o1 = iperm[tets[4 * e + other[1]]];
         int o2 = iperm[tets[4 * e + other[2]]];
-        slots[slot_q(s, 0, ns)] = make_float4(__int_as_float(o0), __int_as_float(o1), __int_as_float(o2), ka);
-        slots[slot_q(s, 1, ns)] = make_float4((float)bs[0], (float)bs[1], (float)bs[2], kb);
-        slots[slot_q(s, 2, ns)] = make_float4((float)b0[0], (float)b0[1], (float)b0[2], kc);
-        slots[slot_q(s, 3, ns)] = make_float4((float)b1[0], (float)b1[1], (float)b1[2], 0.f);
+        (void)kc;  // follows from ka and the material (vbd_device.h)
+        slots[3 * (size_t)s] = make_float4(__int_as_float(o0), __int_as_float(o1), __int_as_float(o2), ka);
+        slots[3 * (size_t)s + 1] = make_float4((float)bs[0], (float)bs[1], (float)bs[2], kb);
+        slots[3 * (size_t)s + 2] = make_float4((float)b0[0], (float)b0[1], (float)b0[2], (float)b1[0]);
+        slots2[s] = make_float2((float)b1[1], (float)b1[2]);
     }
 }
 
@@ -414,7 +415,7 @@ __global__ void k_build_slot_entries(const int32_t *perm, const int32_t *inc_off
 
 // Distinct neighbour vertices gathered by one colour's sweep (for the byte model).
 __global__ void k_mark_boundary(const float4 *slots, const int32_t *ent, const int32_t *slot_off, int v0, int v1,
-                                int tag, int32_t *mark, int32_t *count, int64_t ns)
+                                int tag, int32_t *mark, int32_t *count)
 {
     int i = v0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v1) return;
@@ -422,7 +423,6 @@ __global__ void k_mark_boundary(const float4 *slots, const int32_t *ent, const i
     for (int s = slot_off[i]; s < slot_off[i + 1]; ++s) {
         int o[3];
 #if VBD_TET_RECORDS
-        (void)ns;
         const float4 r0 = slots[4 * (size_t)(ent[s] >> 2)];
         const int loc = ent[s] & 3;
         const int ids[4] = {__float_as_int(r0.x), __float_as_int(r0.y), __float_as_int(r0.z), __float_as_int(r0.w)};
@@ -430,7 +430,7 @@ __global__ void k_mark_boundary(const float4 *slots, const int32_t *ent, const i
             if (k != loc) o[n_o++] = ids[k];
 #else
         (void)ent;
-        float4 q0 = slots[slot_q(s, 0, ns)];
+        const float4 q0 = slots[3 * (size_t)s];
         o[0] = __float_as_int(q0.x);
         o[1] = __float_as_int(q0.y);
         o[2] = __float_as_int(q0.z);
@@ -529,8 +529,7 @@ int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_i
 }
 
 // Scale every record's material constants from one uniform material to
-// another (ns records: slots, or tets in the tet-record layout, whose
-// constants sit in the same positions).
+// another (ns records: slots, or tets in the tet-record layout).
 __global__ void k_rescale_material(int64_t ns, float4 *slots, float fa, float fb, int material)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -540,15 +539,12 @@ __global__ void k_rescale_material(int64_t ns, float4 *slots, float fa, float fb
     r[1].w *= fa;
     r[2].w *= fb;
     r[3].w = material == kSNH ? 0.75f * r[1].w : 0.f;
-    return;
+#else
+    (void)material;
+    float4 *r = slots + 3 * s;  // ka in qa.w, kb in qb.w; kc follows from ka
+    r[0].w *= fa;
+    r[1].w *= fb;
 #endif
-    float4 q0 = slots[slot_q(s, 0, ns)], q1 = slots[slot_q(s, 1, ns)], q2 = slots[slot_q(s, 2, ns)];
-    q0.w *= fa;
-    q1.w *= fb;
-    q2.w = material == kSNH ? 0.75f * q0.w : 0.f;
-    slots[slot_q(s, 0, ns)] = q0;
-    slots[slot_q(s, 1, ns)] = q1;
-    slots[slot_q(s, 2, ns)] = q2;
 }
 
 }  // namespace
@@ -792,11 +788,12 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
         k_build_slot_entries<<<nblk(s->n_swept), kTB, 0, st>>>(s->perm, s->inc_off, s->inc, s->n_swept, s->slot_off,
                                                               s->slot_ent);
 #else
-    VBD_TRY(dmalloc(&s->slots, 4 * (size_t)std::max<int64_t>(ns, 1)));
+    VBD_TRY(dmalloc(&s->slots, 3 * (size_t)std::max<int64_t>(ns, 1)));
+    VBD_TRY(dmalloc(&s->slots2, (size_t)std::max<int64_t>(ns, 1)));
     if (s->n_swept)
         k_build_slots<<<nblk(s->n_swept), kTB, 0, st>>>(s->perm, s->iperm, s->inc_off, s->inc, d_tets, d_invDm, d_vol,
                                                        d_mu, d_lam, s->material, s->n_swept, s->slot_off, s->slots,
-                                                       (int64_t)std::max<int64_t>(ns, 1));
+                                                       s->slots2);
 #endif
 
     // algorithmic bytes per colour sweep: sum_v (40 + 68 k_v) + 12 |boundary|
@@ -810,8 +807,7 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
             int v0 = s->col_start[c], v1 = s->col_start[c + 1];
             if (v1 > v0)
                 k_mark_boundary<<<nblk(v1 - v0), kTB, 0, st>>>(s->slots, s->slot_ent, s->slot_off, v0, v1, c, mark,
-                                                               count + c,
-                                                               std::max<int64_t>(s->n_slots, 1));
+                                                               count + c);
         }
         std::vector<int32_t> hcnt(std::max(1, s->ncolors));
         std::vector<int32_t> hoff(s->n_swept + 1);

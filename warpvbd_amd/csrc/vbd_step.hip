@@ -231,6 +231,7 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
+    a.slots2 = s->slots2;
     a.ent = s->slot_ent;
     a.ns = std::max<int64_t>(s->n_slots, 1);
     a.ctrl = s->ctrl;
@@ -303,6 +304,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
+    a.slots2 = s->slots2;
     a.ent = s->slot_ent;
     a.ns = std::max<int64_t>(s->n_slots, 1);
     a.part = s->part;

@@ -9,29 +9,18 @@ namespace {
 
 inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
 
-#ifndef VBD_STREAM_RECORDS
-#define VBD_STREAM_RECORDS 0
-#endif
 #ifndef VBD_BATCH_L8
 #define VBD_BATCH_L8 1
 #endif
-
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-// 16-byte load that bypasses cache allocation (read-once streams)
-__device__ __forceinline__ float4 load_stream(const float4 *p)
-{
-    const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t *>(p));
-    return make_float4(v.x, v.y, v.z, v.w);
-}
 
 struct SweepArgs {
     float4 *__restrict__ u;
     const float4 *__restrict__ tgt;
     const float4 *__restrict__ uprev;
     const int32_t *__restrict__ slot_off;
-    const float4 *__restrict__ slots;  // slot records, or tet records (VBD_TET_RECORDS)
-    const int32_t *__restrict__ ent;   // per-slot tet << 2 | local index (VBD_TET_RECORDS)
+    const float4 *__restrict__ slots;   // slot records qa|qb|qc (3 per slot), or tet records (VBD_TET_RECORDS)
+    const float2 *__restrict__ slots2;  // slot records qd
+    const int32_t *__restrict__ ent;    // per-slot tet << 2 | local index (VBD_TET_RECORDS)
     int64_t ns;
     uint32_t *part;  // this launch's per-block metric maxima
     Ctrl *ctrl;
@@ -139,32 +128,18 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         // B slots per lane per batch: all records, then all gathers, then the
         // arithmetic, so each batch costs two memory latencies, not 2B.
         for (int s = s0 + lane; s < s1; s += B * L) {
-            float4 q[B][4];
+            float4 q[B][3];
+            float2 qd[B];
             float4 x[B][3];
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 const int sj = s + j * L;
                 if (sj < s1) {
-#if VBD_SLOT_AOS
-                    const float4 *r = a.slots + 4 * (size_t)sj;
-                    const int64_t st = 1;
-#else
-                    const float4 *r = a.slots + sj;
-                    const int64_t st = a.ns;
-#endif
-#if VBD_STREAM_RECORDS
-                    // records are read once per sweep: stream them past the
-                    // caches that hold the gathered positions
-                    q[j][0] = load_stream(r);
-                    q[j][1] = load_stream(r + st);
-                    q[j][2] = load_stream(r + 2 * st);
-                    q[j][3] = load_stream(r + 3 * st);
-#else
+                    const float4 *r = a.slots + 3 * (size_t)sj;
                     q[j][0] = r[0];
-                    q[j][1] = r[st];
-                    q[j][2] = r[2 * st];
-                    q[j][3] = r[3 * st];
-#endif
+                    q[j][1] = r[1];
+                    q[j][2] = r[2];
+                    qd[j] = a.slots2[sj];
                 }
             }
 #pragma unroll
@@ -180,8 +155,8 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                 if (s + j * L < s1) {
                     const float4 *qq = q[j];
                     const float4 *xx = x[j];
-                    slot_eval<MAT>(qq[0].w, qq[1].w, qq[2].w, make_float3(qq[1].x, qq[1].y, qq[1].z),
-                                   make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[3].x, qq[3].y, qq[3].z),
+                    slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), make_float3(qq[1].x, qq[1].y, qq[1].z),
+                                   make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[2].w, qd[j].x, qd[j].y),
                                    make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
                                    make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
                                    make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
