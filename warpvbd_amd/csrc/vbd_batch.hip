@@ -44,6 +44,23 @@ __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_
     if (valid && lane == 0) dxm[v] = m;
 }
 
+// Colour sweep over a task list: block b updates the vertices
+// [tasks[b].x, tasks[b].x + tasks[b].y) (at most blockDim.x / L of them) of
+// bodies still live.  Used once most bodies have converged, so a launch
+// covers only their ranges.
+template <int L, int B, int MAT>
+__global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t *body_new, const BodyState *bs,
+                                                      uint32_t *dxm, const int2 *tasks)
+{
+    const int2 t = tasks[xcd_remap(blockIdx.x, gridDim.x)];
+    const int vi = threadIdx.x / L, lane = threadIdx.x % L;
+    bool valid = vi < t.y;
+    const int v = t.x + (valid ? vi : 0);
+    valid = valid && !bs[body_new[v]].done;
+    const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane);
+    if (valid && lane == 0) dxm[v] = m;
+}
+
 // One workgroup per body: fold the body's per-vertex metrics of this iteration
 // and retire the body when it converged or used its iterations.
 __global__ __launch_bounds__(256) void k_check_bodies(const int2 *ranges, int C, const uint32_t *dxm, BodyState *bs,
@@ -113,6 +130,27 @@ void dispatch_sweep_bodies(int L, const SweepArgs &a, const int32_t *body_new, c
     }
 }
 
+template <int L, int MAT>
+void launch_sweep_tasks(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm,
+                        const int2 *tasks, int ntasks, int block, hipStream_t st)
+{
+    k_sweep_tasks<L, Batch<L>::value, MAT><<<ntasks, block, 0, st>>>(a, body_new, bs, dxm, tasks);
+}
+
+template <int MAT>
+void dispatch_sweep_tasks(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm,
+                          const int2 *tasks, int ntasks, int block, hipStream_t st)
+{
+    switch (L) {
+    case 1: launch_sweep_tasks<1, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
+    case 2: launch_sweep_tasks<2, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
+    case 8: launch_sweep_tasks<8, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
+    case 16: launch_sweep_tasks<16, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
+    case 32: launch_sweep_tasks<32, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
+    default: launch_sweep_tasks<4, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
+    }
+}
+
 SweepArgs sweep_args(vbd_solver *s, float kd_dt)
 {
     SweepArgs a;
@@ -153,6 +191,71 @@ int enqueue_body_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
             dispatch_sweep_bodies<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
         else
             dispatch_sweep_bodies<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
+    }
+    k_check_bodies<<<s->n_bodies, 256, 0, s->stream>>>(s->d_ranges, s->ncolors, s->dxm, s->bstate, max_iter, tol,
+                                                        s->n_live);
+    VBD_HIP(hipGetLastError());
+    return 0;
+}
+
+// Task lists for the bodies not yet done (bstate_host must be current): per
+// colour, the bodies' colour ranges with adjacent ranges merged, cut into
+// blocks of block / L vertices.
+int build_tasks(vbd_solver *s)
+{
+    const int C = s->ncolors, nb = s->n_bodies;
+    const int vpb = std::max(1, s->launch.block / s->launch.lanes);
+    s->tasks_host.clear();
+    s->task_off.assign(C + 1, 0);
+    s->task_vertices = 0;
+    for (int c = 0; c < C; ++c) {
+        s->task_off[c] = (int)s->tasks_host.size();
+        int run0 = -1, run1 = -1;
+        auto flush = [&]() {
+            for (int v = run0; v < run1; v += vpb) s->tasks_host.push_back(make_int2(v, std::min(vpb, run1 - v)));
+            s->task_vertices += run1 - run0;
+        };
+        for (int b = 0; b < nb; ++b) {
+            if (s->bstate_host[b].done) continue;
+            const int2 r = s->ranges_host[(size_t)b * (C + 1) + c];
+            if (r.y <= 0) continue;
+            if (r.x == run1) {
+                run1 += r.y;
+            } else {
+                if (run1 > run0) flush();
+                run0 = r.x;
+                run1 = r.x + r.y;
+            }
+        }
+        if (run1 > run0) flush();
+    }
+    s->task_off[C] = (int)s->tasks_host.size();
+    const size_t nt = std::max<size_t>(1, s->tasks_host.size());
+    if (nt > s->task_cap) {
+        if (s->d_tasks) VBD_HIP(hipFree(s->d_tasks));
+        s->d_tasks = nullptr;
+        VBD_HIP(hipMalloc((void **)&s->d_tasks, sizeof(int2) * nt));
+        s->task_cap = nt;
+    }
+    if (!s->tasks_host.empty())
+        VBD_HIP(hipMemcpyAsync(s->d_tasks, s->tasks_host.data(), sizeof(int2) * s->tasks_host.size(),
+                               hipMemcpyHostToDevice, s->stream));
+    return 0;
+}
+
+// One iteration of the live bodies through the current task lists.
+int enqueue_task_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
+{
+    const SweepArgs a = sweep_args(s, kd_dt);
+    const int L = s->launch.lanes, block = s->launch.block;
+    for (int c = 0; c < s->ncolors; ++c) {
+        const int t0 = s->task_off[c], nt = s->task_off[c + 1] - t0;
+        if (!nt) continue;
+        if (s->material == kSNH)
+            dispatch_sweep_tasks<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->d_tasks + t0, nt, block, s->stream);
+        else
+            dispatch_sweep_tasks<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->d_tasks + t0, nt, block,
+                                        s->stream);
     }
     k_check_bodies<<<s->n_bodies, 256, 0, s->stream>>>(s->d_ranges, s->ncolors, s->dxm, s->bstate, max_iter, tol,
                                                         s->n_live);
@@ -213,11 +316,20 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
         s->bstate, s->n_live);
     VBD_HIP(hipGetLastError());
     if (s->warm) VBD_TRY(mpnn_apply_guess(s->warm, dt));
+    // First chunk: every body, replayed from a graph; its length is the
+    // previous step's largest body iteration count, or with compaction its
+    // median, after which about half the bodies are done and the remaining
+    // chunks launch only the live bodies' ranges (lists rebuilt whenever the
+    // live vertices fall below half of what the current list covers).
+    const bool compact = s->launch.compact && s->n_bodies > 1;
     int launched = 0;
-    int chunk = std::max(1, s->last_iters);
+    int chunk = std::max(1, compact ? s->median_iters : s->last_iters);
+    bool tasks = false;
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);
-        if (s->launch.graph) {
+        if (tasks) {
+            for (int j = 0; j < k; ++j) VBD_TRY(enqueue_task_iteration(s, kd_dt, ftol, max_iter));
+        } else if (s->launch.graph) {
             hipGraphExec_t ex;
             VBD_TRY(get_graph(s, k, kd_dt, ftol, max_iter, &ex));
             VBD_HIP(hipGraphLaunch(ex, s->stream));
@@ -226,8 +338,21 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
         }
         launched += k;
         VBD_HIP(hipMemcpyAsync(s->n_live_host, s->n_live, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+        if (compact)
+            VBD_HIP(hipMemcpyAsync(s->bstate_host, s->bstate, sizeof(BodyState) * s->n_bodies,
+                                   hipMemcpyDeviceToHost, s->stream));
         VBD_HIP(hipStreamSynchronize(s->stream));
         if (*s->n_live_host == 0) break;
+        if (compact) {
+            int64_t live_v = 0;
+            for (int b = 0; b < s->n_bodies; ++b)
+                if (!s->bstate_host[b].done) live_v += s->body_voff_host[b + 1] - s->body_voff_host[b];
+            const int64_t covered = tasks ? s->task_vertices : (int64_t)s->n;
+            if (2 * live_v < covered) {
+                VBD_TRY(build_tasks(s));
+                tasks = true;
+            }
+        }
         chunk = s->launch.chunk > 0 ? s->launch.chunk : 2;
     }
     k_finish_all<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->n, s->u, s->uprev, s->vel, (float)(1.0 / dt));
@@ -236,11 +361,17 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
                            s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
     int mx = 0;
+    std::vector<int> its(s->n_bodies);
     for (int b = 0; b < s->n_bodies; ++b) {
         s->body_out_host[b] = make_int2(s->bstate_host[b].iters, (int)s->bstate_host[b].metric);
         mx = std::max(mx, s->bstate_host[b].iters);
+        its[b] = s->bstate_host[b].iters;
     }
     s->last_iters = mx;
+    if (!its.empty()) {
+        std::nth_element(its.begin(), its.begin() + its.size() / 2, its.end());
+        s->median_iters = its[its.size() / 2];
+    }
     return 0;
 }
 
@@ -292,6 +423,9 @@ int step_bodies(vbd_solver *s, double dt, double tol, int max_iter)
 void release_batch(vbd_solver *s)
 {
     drop_graphs(s);
+    if (s->d_tasks) (void)hipFree(s->d_tasks);
+    s->d_tasks = nullptr;
+    s->task_cap = 0;
     if (s->bstate) (void)hipFree(s->bstate);
     if (s->dxm) (void)hipFree(s->dxm);
     if (s->n_live) (void)hipFree(s->n_live);

@@ -65,6 +65,7 @@ struct Launch {
     int chunk = 0;        // iterations between host waits (0 = adaptive)
     bool resident = true; // allow the one-workgroup whole-step kernel when every colour is small
     bool graph = true;    // batched colour launches: replay captured iterations
+    bool compact = true;  // batched colour launches: after the first chunk, launch only live bodies' ranges
     int resident_block = 1024;  // threads per body in the one-workgroup-per-body form
 };
 
@@ -112,6 +113,14 @@ struct vbd_solver {
     vbd::BodyState *bstate = nullptr, *bstate_host = nullptr;
     uint32_t *dxm = nullptr;         // per-vertex metric bits of the current iteration
     int32_t *n_live = nullptr, *n_live_host = nullptr;
+    // live-range tasks of the batched colour launches: per colour, blocks of
+    // (first vertex, count) covering the bodies live when the list was built
+    std::vector<int2> tasks_host;
+    std::vector<int> task_off;       // per colour: first task, then the total
+    int2 *d_tasks = nullptr;
+    size_t task_cap = 0;
+    int64_t task_vertices = 0;       // vertices the current list covers
+    int median_iters = 0;            // median body iteration count of the last step
     std::vector<vbd::CachedGraph> graphs;
     int32_t *tets_dev = nullptr;     // elements (original ids), kept for the warm-start graph
     double edge_length = 0.01;       // cube root of 6 x mean tet volume
