@@ -307,6 +307,7 @@ def mixed_section(args):
                       "autotuned": {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}}
     used = []
     ms_w, it_w = [], []
+    ws.set_policy(ws.ADAPTIVE)  # skip the network while it keeps losing the first-sweep comparison
     t0 = time.perf_counter()
     for _ in range(steps):
         a = time.perf_counter()
@@ -318,6 +319,8 @@ def mixed_section(args):
     out["batched_warm"] = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
                            "max_body_iterations_per_step": float(np.mean(it_w)),
                            "bodies_using_network": float(np.mean(used)),
+                           "policy": "adaptive (choose per body; network skipped 1..16 steps after a step where "
+                                     "fewer than 2% of bodies took it)",
                            "step_ms_p50_p90_p99": [float(np.percentile(ms_w, q)) for q in (50, 90, 99)],
                            "training": f"{args.mixed_samples} samples from the batch's own steps, "
                                        f"{args.warm_epochs} epochs, {t_gen:.1f} s + {t_train:.1f} s",

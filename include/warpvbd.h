@@ -168,11 +168,15 @@ int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out);
 int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *targets, int32_t *nb_off, int32_t *nb,
                          int32_t *n_edges);
 /* Which initial guess vbd_step_warm takes: per body the one with the smaller
- * first-sweep residual (default), always the network's, or always the plain
- * inertia prediction (the network is then not evaluated). */
+ * first-sweep residual (default), always the network's, always the plain
+ * inertia prediction (the network is then not evaluated), or CHOOSE with
+ * back-off: after a step where fewer than 2% of the bodies took the network
+ * guess, the network is skipped for 1, 2, 4, ... up to 16 steps (plain guess,
+ * no network cost) before it is evaluated again. */
 #define VBD_WARM_CHOOSE 0
 #define VBD_WARM_NETWORK 1
 #define VBD_WARM_PLAIN 2
+#define VBD_WARM_ADAPTIVE 3
 int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy);
 /* vbd_step (or vbd_batch_step on a batched handle) from x_prev with, per body,
  * the network's initial guess when its first-sweep residual (Jacobi

@@ -100,6 +100,41 @@ def resync_errors(ref, solvers, steps, dt=1e-3, warm=None):
     return [np.array(a) for a in its], [np.array(e) for e in errs]
 
 
+@pytest.mark.parametrize("policy", ["plain", "adaptive"])
+def test_policies_without_network_gain_equal_plain(policy):
+    """PLAIN never evaluates the network and ADAPTIVE backs off from a
+    network that keeps losing: both give the plain step bit for bit (same
+    launch shapes on both solvers: a warm step does not use the one-workgroup
+    whole-step kernel)."""
+    sc, s = make_solver()
+    sc2, s2 = make_solver()
+    for sol in (s, s2):
+        sol.set_launch(8, 256, 0, 0)
+    ws = WarmStart(s, capacity=1, seed=4)
+    p = init_params(4)
+    p["b_out"] = np.array([50.0, -80.0, 120.0])
+    ws.set_params(flatten(p))
+    ws.set_policy(ws.PLAIN if policy == "plain" else ws.ADAPTIVE)
+    for k in range(12):
+        x = s.step(s.old_positions, 1e-3, warm_start=ws).numpy()
+        x2 = s2.step(s2.old_positions, 1e-3).numpy()
+        assert not s.last_used_network
+        assert np.array_equal(x, x2) and s.last_iterations == s2.last_iterations
+
+
+def test_network_policy_forces_the_guess():
+    """NETWORK takes the network guess even when it is far off; the step still
+    converges to the tolerance from it."""
+    sc, s = make_solver()
+    ws = WarmStart(s, capacity=1, seed=4)
+    p = init_params(4)
+    p["b_out"] = np.array([5.0, -8.0, 12.0])
+    ws.set_params(flatten(p))
+    ws.set_policy(ws.NETWORK)
+    x = s.step(s.old_positions, 1e-3, warm_start=ws).numpy()
+    assert s.last_used_network and s.last_metric < 1e-7 and np.all(np.isfinite(x))
+
+
 def test_trained_network_gain():
     """Train on solver-made trajectories of a 32x4x4 cantilever, then step
     held-out seeds with and without the warm start to tol 1e-7.  Each step

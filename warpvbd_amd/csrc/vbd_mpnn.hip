@@ -767,6 +767,8 @@ struct vbd_mpnn {
     uint32_t *rm0 = nullptr, *rm1 = nullptr;  // per-vertex first-sweep residuals of the two guesses
     float4 *dxj = nullptr;                     // Jacobi update from the plain guess (a feature)
     int policy = VBD_WARM_CHOOSE;
+    int skip_left = 0, backoff = 0;  // VBD_WARM_ADAPTIVE: steps still to skip, current skip length
+    bool evaluated = false;          // the last step evaluated the network
     int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body
     float *loss_host = nullptr;
     int adam_t = 0;
@@ -927,10 +929,13 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
 {
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
-    if (m->policy == VBD_WARM_PLAIN) {
+    m->evaluated = false;
+    if (m->policy == VBD_WARM_PLAIN || (m->policy == VBD_WARM_ADAPTIVE && m->skip_left > 0)) {
+        if (m->policy == VBD_WARM_ADAPTIVE) --m->skip_left;
         VBD_HIP(hipMemsetAsync(m->use_alt, 0, sizeof(int32_t) * s->n_bodies, st));
         return 0;
     }
+    m->evaluated = true;
     features(m, dt, true, m->f);
     VBD_TRY(mpnn_forward_backward(m, m->f, nullptr, false, 0.f));
     k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, m->out, m->fp.s, m->guess);
@@ -1077,11 +1082,13 @@ int vbd_mpnn_num_params(void) { return kNumParams; }
 
 int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy)
 {
-    if (!m || policy < VBD_WARM_CHOOSE || policy > VBD_WARM_PLAIN) {
-        set_error("vbd_mpnn_set_policy: policy must be VBD_WARM_CHOOSE, VBD_WARM_NETWORK or VBD_WARM_PLAIN");
+    if (!m || policy < VBD_WARM_CHOOSE || policy > VBD_WARM_ADAPTIVE) {
+        set_error("vbd_mpnn_set_policy: policy must be VBD_WARM_CHOOSE, VBD_WARM_NETWORK, VBD_WARM_PLAIN or "
+                  "VBD_WARM_ADAPTIVE");
         return VBD_EINVAL;
     }
     m->policy = policy;
+    m->skip_left = m->backoff = 0;
     return 0;
 }
 
@@ -1316,6 +1323,14 @@ int vbd_step_warm(vbd_solver_t *s, vbd_mpnn_t *m, double dt, double tol, int32_t
     if (rc) return rc;
     int used = 0;
     VBD_TRY(mpnn_used_guess(m, &used));
+    if (m->policy == VBD_WARM_ADAPTIVE && m->evaluated) {
+        if (50 * used < s->n_bodies) {  // fewer than 2% of the bodies took the network guess
+            m->backoff = std::min(16, std::max(1, 2 * m->backoff));
+            m->skip_left = m->backoff;
+        } else {
+            m->backoff = 0;
+        }
+    }
     if (iters_out) *iters_out = it;
     if (metric_out) *metric_out = met;
     if (used_network_out) *used_network_out = used;

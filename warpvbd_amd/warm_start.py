@@ -132,12 +132,14 @@ class WarmStart:
               "vbd_mpnn_set_body_materials")
 
     # initial-guess policy of warm steps
-    CHOOSE, NETWORK, PLAIN = 0, 1, 2
+    CHOOSE, NETWORK, PLAIN, ADAPTIVE = 0, 1, 2, 3
 
     def set_policy(self, policy: int):
         """CHOOSE (default): per body the guess with the smaller first-sweep
         residual; NETWORK: always the network's; PLAIN: always the inertia
-        prediction (the network is not evaluated)."""
+        prediction (the network is not evaluated); ADAPTIVE: CHOOSE, but after
+        a step where fewer than 2% of the bodies took the network guess the
+        network is skipped for 1, 2, 4, ... (at most 16) steps."""
         check(_lib.load().vbd_mpnn_set_policy(self._h, int(policy)), "vbd_mpnn_set_policy")
 
     def save(self, path):
