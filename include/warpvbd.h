@@ -123,11 +123,15 @@ int vbd_batch_step(vbd_solver_t *h, double dt, double tol, int32_t max_iter, int
 /* Per-body iterations and metrics of the last step (after vbd_step or vbd_step_warm). */
 int vbd_batch_results(vbd_solver_t *h, int32_t *iters_out, double *metric_out);
 /* Execution form: VBD_BATCH_LAUNCHES (one launch per colour across all bodies,
- * iterations replayed from a captured hipGraph when use_graph) or
- * VBD_BATCH_RESIDENT (one workgroup of block_size threads per body runs its
- * whole step). */
+ * iterations replayed from a captured hipGraph when use_graph; once most
+ * bodies have converged the launches cover only the live bodies' vertex
+ * ranges), VBD_BATCH_RESIDENT (one workgroup of block_size threads per body
+ * runs its whole step), or VBD_BATCH_LAUNCHES_FULL (colour launches over every
+ * body in every iteration, without the live-range lists).  All forms give the
+ * same result for each body. */
 #define VBD_BATCH_LAUNCHES 1
 #define VBD_BATCH_RESIDENT 2
+#define VBD_BATCH_LAUNCHES_FULL 3
 int vbd_batch_set_mode(vbd_solver_t *h, int32_t mode, int32_t lanes_per_vertex, int32_t block_size,
                        int32_t use_graph);
 int vbd_batch_get_mode(vbd_solver_t *h, int32_t *mode, int32_t *lanes_per_vertex, int32_t *block_size,

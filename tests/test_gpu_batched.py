@@ -65,6 +65,26 @@ def test_batch_tolerance_per_body_convergence(mode):
         assert np.abs(zg - zr).max() <= 1e-3 * np.abs(zr).max()
 
 
+def test_live_range_launches_equal_full_launches():
+    """Colour launches restricted to the live bodies' ranges (after the first
+    chunk of a step) do exactly the per-vertex work of full launches: same
+    iteration counts and bit-identical positions at tol 1e-7."""
+    bs = bodies() + [cantilever((20, 3, 3), youngs_modulus=8e5), mass_spring(5, 3, youngs_modulus=2e4)]
+    a = BatchedVBDSolver(bs, dx_tol=1e-7, max_iter=1000, verbose=False)
+    b = BatchedVBDSolver(bs, dx_tol=1e-7, max_iter=1000, verbose=False)
+    a.set_mode("launches", 8, 256, True)
+    b.set_mode("launches_full", 8, 256, True)
+    assert b.get_mode()["mode"] == "launches_full"
+    spread = 0
+    for k in range(25):
+        xa = a.step(dt=1e-3).numpy()
+        xb = b.step(dt=1e-3).numpy()
+        assert np.array_equal(a.body_iterations, b.body_iterations)
+        assert np.array_equal(xa, xb)
+        spread = max(spread, int(a.body_iterations.max() - a.body_iterations.min()))
+    assert spread > 2  # bodies converge at different counts, so the live lists were used
+
+
 def test_batch_autotune_restores_state_and_keeps_parity(tmp_path, monkeypatch):
     monkeypatch.setenv("WARPVBD_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
     bs = bodies()

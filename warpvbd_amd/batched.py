@@ -21,7 +21,7 @@ from . import _lib
 from ._lib import check, f64, i32, ptr, to_numpy
 from .vbd_solver import VBDSolver, VertexArray
 
-MODES = {"launches": 1, "resident": 2}
+MODES = {"launches": 1, "resident": 2, "launches_full": 3}
 _MODE_NAMES = {v: k for k, v in MODES.items()}
 
 

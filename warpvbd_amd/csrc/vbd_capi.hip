@@ -178,11 +178,13 @@ int vbd_batch_results(vbd_solver_t *s, int32_t *iters_out, double *metric_out)
 int vbd_batch_set_mode(vbd_solver_t *s, int32_t mode, int32_t lanes, int32_t block, int32_t use_graph)
 {
     CHECK_ARG(s && s->batched, "not a batched handle");
-    CHECK_ARG(mode == VBD_BATCH_LAUNCHES || mode == VBD_BATCH_RESIDENT, "unknown batch mode");
+    CHECK_ARG(mode == VBD_BATCH_LAUNCHES || mode == VBD_BATCH_RESIDENT || mode == VBD_BATCH_LAUNCHES_FULL,
+              "unknown batch mode");
     CHECK_ARG(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32,
               "lanes_per_vertex must be 1, 2, 4, 8, 16 or 32");
     CHECK_ARG(block >= 64 && block <= 1024 && block % 64 == 0, "block_size must be a multiple of 64 in [64, 1024]");
-    s->batch_mode = mode;
+    s->batch_mode = mode == VBD_BATCH_RESIDENT ? kBatchResident : kBatchLaunches;
+    s->launch.compact = mode != VBD_BATCH_LAUNCHES_FULL;
     s->launch.lanes = lanes;
     if (mode == VBD_BATCH_RESIDENT) s->launch.resident_block = block;
     else s->launch.block = block;
@@ -193,7 +195,10 @@ int vbd_batch_set_mode(vbd_solver_t *s, int32_t mode, int32_t lanes, int32_t blo
 int vbd_batch_get_mode(vbd_solver_t *s, int32_t *mode, int32_t *lanes, int32_t *block, int32_t *use_graph)
 {
     CHECK_ARG(s && s->batched, "not a batched handle");
-    if (mode) *mode = s->batch_mode;
+    if (mode)
+        *mode = s->batch_mode == kBatchResident ? VBD_BATCH_RESIDENT
+                : s->launch.compact                ? VBD_BATCH_LAUNCHES
+                                                   : VBD_BATCH_LAUNCHES_FULL;
     if (lanes) *lanes = s->launch.lanes;
     if (block) *block = s->batch_mode == VBD_BATCH_RESIDENT ? s->launch.resident_block : s->launch.block;
     if (use_graph) *use_graph = s->launch.graph ? 1 : 0;
