@@ -357,6 +357,58 @@ def test_errors_are_reported():
         g.step(g.old_positions, 0.0)
 
 
+def test_more_than_256_colours_is_an_error():
+    """The reference leaves vertices uncoloured (-1) past 256 colours
+    (src/coloring.py:62-68); here the colouring reports ECOLORS.  A complete
+    graph on 260 vertices needs 260 colours; 256 vertices still colour."""
+    from warpvbd_amd import graph_coloring
+    k256 = {v: set(range(256)) - {v} for v in range(256)}
+    col, groups = graph_coloring(k256, verbose=False)
+    assert np.array_equal(col, np.arange(256)) and len(groups) == 256
+    k260 = {v: set(range(260)) - {v} for v in range(260)}
+    with pytest.raises(VBDError, match="ECOLORS"):
+        graph_coloring(k260, verbose=False)
+
+
+def test_unreferenced_vertex_flies_free_and_matches_oracle():
+    """A vertex that no element references gets no update from the solve (the
+    reference's Cholesky of its empty block would give NaN): it keeps its
+    inertia prediction, i.e. falls freely, and the rest of the mesh steps as
+    without it."""
+    sc = cantilever((6, 2, 2))
+    X = np.vstack([sc.vertices, [[0.5, 0.5, 0.5]]])
+    act = np.append(sc.active_mask, True)
+    k = dict(sc.solver_kwargs(), initial_positions=X, active_mask=act)
+    g = VBDSolver(**k, dx_tol=1e-7, max_iter=1000, verbose=False)
+    r = oracle.OracleSolver(**k, dx_tol=1e-7, max_iter=1000, expanded=False, threads=8)
+    for _ in range(10):
+        x = g.step(g.old_positions, 1e-3).numpy()
+        xr = r.step(1e-3)
+    n = 10
+    assert np.allclose(x[-1], X[-1] + np.array([0.0, 0.0, -9.81e-6 * n * (n + 1) / 2]), rtol=0, atol=1e-9)
+    assert np.abs(x - xr).max() < 1e-5 * diag(sc)
+
+
+def test_iteration_cap_warns_like_reference(capsys):
+    """Hitting max_iter prints the reference's warning (src/vbd_solver.py:581-582)
+    and is not an error."""
+    sc = cantilever((6, 2, 2))
+    g, _ = make(sc, max_iter=1)
+    g.step(g.old_positions, 1e-3)
+    assert g.last_iterations == 1
+    assert "did not converge" in capsys.readouterr().out
+
+
+def test_inf_velocity_runs_to_max_iter():
+    sc = cantilever((4, 2, 2))
+    g, _ = make(sc, max_iter=5)
+    v = np.zeros((sc.n_vertices, 3))
+    v[np.nonzero(sc.active_mask)[0][3], 2] = np.inf
+    g.old_velocities = v
+    g.step(g.old_positions, 1e-3)
+    assert g.last_iterations == 5 and not np.isfinite(g.last_metric)
+
+
 @pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32])
 def test_launch_shapes_agree(lanes):
     sc = cantilever((24, 4, 4))

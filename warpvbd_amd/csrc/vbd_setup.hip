@@ -507,6 +507,10 @@ int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_i
         VBD_HIP(hipStreamSynchronize(st));
         for (int b = 0; b < kBatch; ++b) done |= (hc[b] == 0);
     }
+    // the overflow flag of the rounds above may come from a transient colour;
+    // one more round at the fixed point decides it
+    k_fill_i32<<<1, 1, 0, st>>>(scal + 3, 1, 0);
+    k_color_round_graph<<<nblk(*nref), kTB, 0, st>>>(lo_off, lo, *nref, d_colors, changed, scal + 3);
     VBD_HIP(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
     if (!done) {
@@ -604,6 +608,10 @@ int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_
         VBD_HIP(hipStreamSynchronize(st));
         for (int b = 0; b < kBatch; ++b) done |= (hc[b] == 0);
     }
+    // the overflow flag of the rounds above may come from a transient colour;
+    // one more round at the fixed point decides it
+    k_fill_i32<<<1, 1, 0, st>>>(overflow, 1, 0);
+    k_color_round_graph<<<nblk(n), kTB, 0, st>>>(d_off, d_adj, n, d_colors, changed, overflow);
     int32_t of = 0;
     VBD_HIP(hipMemcpyAsync(&of, overflow, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     std::vector<int32_t> col(n);
