@@ -338,13 +338,29 @@ __global__ __launch_bounds__(256) void k_out_layer(int n, const uint16_t *__rest
     }
 }
 
-// G[W_out, b_out] += sum of the output-layer partials; loss += their loss sum.
-__global__ void k_sum_out(int nparts, const float *partial, float *G, float *loss)
+// Sum of nparts partial rows of len floats, column by column: a block takes
+// 16 columns, its 16 thread rows stride over the partials, LDS folds them.
+__device__ __forceinline__ float sum_column(int nparts, int len, const float *partial, int e, float (*s_red)[17])
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= kOutPart) return;
-    float s = 0.f;
-    for (int b = 0; b < nparts; ++b) s += partial[(size_t)b * kOutPart + e];
+    const int col = threadIdx.x & 15, row = threadIdx.x >> 4;
+    float acc = 0.f;
+    if (e < len)
+        for (int b = row; b < nparts; b += 16) acc += partial[(size_t)b * len + e];
+    s_red[row][col] = acc;
+    __syncthreads();
+    float tot = 0.f;
+    if (row == 0)
+        for (int q = 0; q < 16; ++q) tot += s_red[q][col];
+    return tot;  // valid in thread row 0
+}
+
+// G[W_out, b_out] += sum of the output-layer partials; loss += their loss sum.
+__global__ __launch_bounds__(256) void k_sum_out(int nparts, const float *partial, float *G, float *loss)
+{
+    __shared__ float s_red[16][17];
+    const int e = blockIdx.x * 16 + (threadIdx.x & 15);
+    const float s = sum_column(nparts, kOutPart, partial, e, s_red);
+    if ((threadIdx.x >> 4) != 0 || e >= kOutPart) return;
     if (e < kOutPart - 1) G[kOffWout + e] += s;  // W_out [3][64] then b_out [3], contiguous
     else *loss += s;
 }
@@ -450,13 +466,14 @@ __global__ __launch_bounds__(256) void k_weight_grad(int n, const float *dz, con
     if (kb == 0) p[kW * K + o] = accb;
 }
 
-__global__ void k_sum_partials(int nparts, int len, const float *partial, float *G, int goff_w, int goff_b, int nw)
+__global__ __launch_bounds__(256) void k_sum_partials(int nparts, int len, const float *partial, float *G, int goff_w,
+                                                      int goff_b, int nw)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= len) return;
-    float s = 0.f;
-    for (int b = 0; b < nparts; ++b) s += partial[(size_t)b * len + t];
-    G[t < nw ? goff_w + t : goff_b + (t - nw)] += s;
+    __shared__ float s_red[16][17];
+    const int e = blockIdx.x * 16 + (threadIdx.x & 15);
+    const float s = sum_column(nparts, len, partial, e, s_red);
+    if ((threadIdx.x >> 4) != 0 || e >= len) return;
+    G[e < nw ? goff_w + e : goff_b + (e - nw)] += s;
 }
 
 // BF16 copy of parameter t, plus the transposed copies of the layer weights
@@ -826,15 +843,16 @@ int mpnn_forward_backward(vbd_mpnn *m, const uint16_t *f, const float *target, b
                                                         m->P + off_b(l), m->h[l + 1], m->a[l], m->z[l]);
         k_out_layer<true><<<m->out_blocks, 256, 0, st>>>(n, m->h[kLayers], m->Pb + kOffWout, m->P + kOffBout, target,
                                                          m->act, inv_denom, m->out, m->dh, m->partial);
-        k_sum_out<<<1, 256, 0, st>>>(m->out_blocks, m->partial, m->G, m->loss);
+        k_sum_out<<<nblk(kOutPart, 16), 256, 0, st>>>(m->out_blocks, m->partial, m->G, m->loss);
     } else {
         // inference needs no a_l, z_l: ping-pong between two activation buffers
         k_in_layer<false><<<tiles, 256, 0, st>>>(n, f, m->Pb + kOffWin, m->P + kOffBin, nullptr, m->h[0]);
         for (int l = 0; l < kLayers; ++l)
             k_mp_layer<false><<<mp_blocks, 256, 0, st>>>(n, tiles, m->nb_off, m->nb, m->h[l & 1], m->Pb + off_W(l),
                                                          m->P + off_b(l), m->h[(l + 1) & 1], nullptr, nullptr);
-        k_out_layer<false><<<m->out_blocks, 256, 0, st>>>(n, m->h[kLayers & 1], m->Pb + kOffWout, m->P + kOffBout,
-                                                          nullptr, m->act, 0.f, m->out, nullptr, nullptr);
+        k_out_layer<false><<<std::min(4096, nblk(n, 16)), 256, 0, st>>>(n, m->h[kLayers & 1], m->Pb + kOffWout,
+                                                                        m->P + kOffBout, nullptr, m->act, 0.f, m->out,
+                                                                        nullptr, nullptr);
     }
     VBD_HIP(hipGetLastError());
     if (!with_grad) return 0;
@@ -843,15 +861,15 @@ int mpnn_forward_backward(vbd_mpnn *m, const uint16_t *f, const float *target, b
         k_relu_grad<<<nblk(nw, 256), 256, 0, st>>>(nw, m->dh, m->z[l], m->dz);
         k_weight_grad<2 * kW><<<m->wg_blocks, 256, 0, st>>>(n, m->dz, m->h[l], kW, m->a[l], m->partial);
         const int len = kW * 2 * kW + kW;
-        k_sum_partials<<<nblk(len, 256), 256, 0, st>>>(m->wg_blocks, len, m->partial, m->G, off_W(l), off_b(l),
-                                                        kW * 2 * kW);
+        k_sum_partials<<<nblk(len, 16), 256, 0, st>>>(m->wg_blocks, len, m->partial, m->G, off_W(l), off_b(l),
+                                                       kW * 2 * kW);
         k_dgrad<<<dim3(tiles, 2), 256, 0, st>>>(n, m->dz, m->PbT + (size_t)l * 2 * kW * kW, m->dx);
         k_backprop_mean<<<nblk(nw, 256), 256, 0, st>>>(n, m->nb_off, m->nb, m->dx, m->dh);
     }
     k_relu_grad<<<nblk(nw, 256), 256, 0, st>>>(nw, m->dh, m->z0, m->dz);
     k_weight_grad<kFin><<<m->wg_blocks, 256, 0, st>>>(n, m->dz, f, kFin, nullptr, m->partial);
     const int len = kW * kFin + kW;
-    k_sum_partials<<<nblk(len, 256), 256, 0, st>>>(m->wg_blocks, len, m->partial, m->G, kOffWin, kOffBin, kW * kFin);
+    k_sum_partials<<<nblk(len, 16), 256, 0, st>>>(m->wg_blocks, len, m->partial, m->G, kOffWin, kOffBin, kW * kFin);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -1027,7 +1045,7 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
     for (int l = 0; l < kLayers; ++l)
         if ((rc = dalloc(&m->a[l], nw)) || (rc = dalloc(&m->z[l], nw))) return fail(rc);
     m->wg_blocks = std::max(1, std::min(512, nblk(n, 32)));
-    m->out_blocks = std::max(1, std::min(1024, nblk(n, 16)));
+    m->out_blocks = std::max(1, std::min(2048, nblk(n, 16)));
     const size_t part = std::max((size_t)m->wg_blocks * (kW * 2 * kW + kW), (size_t)m->out_blocks * kOutPart);
     if ((rc = dalloc(&m->partial, part))) return fail(rc);
     if (hipHostMalloc((void **)&m->use_alt_host, sizeof(int32_t) * s->n_bodies, hipHostMallocDefault) != hipSuccess ||
