@@ -676,37 +676,59 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int nv, float4 *dx,
     }
 }
 
-// One workgroup per body: keep the guess with the smaller first-sweep
-// residual (max over the body's vertices) and write it to u; force = 1 takes
-// the network guess without the comparison.
-__global__ __launch_bounds__(256) void k_choose_guess(const int2 *ranges, int C, const uint32_t *m0,
-                                                      const uint32_t *m1, const float4 *alt, float4 *u,
-                                                      int32_t *use_alt, int force)
+// Per body (blockIdx.y) and chunk of kGuessChunk of its swept vertices
+// (blockIdx.x, over the body's colour ranges in colour order): fold the
+// first-sweep residual bits of the plain (m0) and network (m1) guesses into
+// the body's maxima bm[2b], bm[2b+1].  Residual bits are non-negative floats
+// (or NaN bits), so unsigned max orders them like the floats, NaN on top.
+constexpr int kGuessChunk = 4096;
+
+__device__ __forceinline__ int body_chunk_vertex(const int2 *r, int C, int k)
 {
-    __shared__ int s_use;
-    const int2 *r = ranges + (size_t)blockIdx.x * (C + 1);
-    uint32_t a0 = 0, a1 = 0;
-    if (!force) {
-        for (int c = 0; c < C; ++c) {
-            const int2 rc = r[c];
-            for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) {
-                a0 = max(a0, m0[i]);
-                a1 = max(a1, m1[i]);
-            }
-        }
-        a0 = block_max_u32(a0);
-        __syncthreads();
-        a1 = block_max_u32(a1);
-    }
-    if (threadIdx.x == 0) {
-        s_use = force || __uint_as_float(a1) < __uint_as_float(a0) ? 1 : 0;
-        use_alt[blockIdx.x] = s_use;
-    }
-    __syncthreads();
-    if (!s_use) return;
+    // the k-th swept vertex of a body whose colour ranges are r[0..C), or -1
     for (int c = 0; c < C; ++c) {
-        const int2 rc = r[c];
-        for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) u[i] = alt[i];
+        if (k < r[c].y) return r[c].x + k;
+        k -= r[c].y;
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(256) void k_guess_max(const int2 *ranges, int C, const uint32_t *m0, const uint32_t *m1,
+                                                   uint32_t *bm)
+{
+    const int2 *r = ranges + (size_t)blockIdx.y * (C + 1);
+    uint32_t a0 = 0, a1 = 0;
+    const int k0 = blockIdx.x * kGuessChunk;
+    for (int k = k0 + threadIdx.x; k < k0 + kGuessChunk; k += blockDim.x) {
+        const int i = body_chunk_vertex(r, C, k);
+        if (i < 0) break;
+        a0 = max(a0, m0[i]);
+        a1 = max(a1, m1[i]);
+    }
+    a0 = block_max_u32(a0);
+    __syncthreads();
+    a1 = block_max_u32(a1);
+    if (threadIdx.x == 0 && (a0 | a1)) {
+        atomicMax(&bm[2 * blockIdx.y], a0);
+        atomicMax(&bm[2 * blockIdx.y + 1], a1);
+    }
+}
+
+// Keep, per body, the guess with the smaller first-sweep residual (force = 1:
+// the network's) and write it to u; use_alt[b] records the choice.
+__global__ __launch_bounds__(256) void k_guess_apply(const int2 *ranges, int C, const uint32_t *bm, const float4 *alt,
+                                                     float4 *u, int32_t *use_alt, int force)
+{
+    const int b = blockIdx.y;
+    const int2 *r = ranges + (size_t)b * (C + 1);
+    const bool use = force || __uint_as_float(bm[2 * b + 1]) < __uint_as_float(bm[2 * b]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) use_alt[b] = use ? 1 : 0;
+    if (!use) return;
+    const int k0 = blockIdx.x * kGuessChunk;
+    for (int k = k0 + threadIdx.x; k < k0 + kGuessChunk; k += blockDim.x) {
+        const int i = body_chunk_vertex(r, C, k);
+        if (i < 0) break;
+        u[i] = alt[i];
     }
 }
 
@@ -739,6 +761,12 @@ __global__ void k_graph(const int32_t *tets, const int32_t *off, const int32_t *
         }
     }
     if (!FILL) cnt[v] = k;
+}
+
+__global__ void k_fill_u32(int n, uint32_t *p, uint32_t v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // rest positions and the active mask in network (original) order
@@ -787,6 +815,8 @@ struct vbd_mpnn {
     int skip_left = 0, backoff = 0;  // VBD_WARM_ADAPTIVE: steps still to skip, current skip length
     bool evaluated = false;          // the last step evaluated the network
     int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body
+    uint32_t *bmax = nullptr;                             // per body: max residual bits of the two guesses
+    int guess_chunks = 1;                                 // chunks of the largest body's swept vertices
     float *loss_host = nullptr;
     int adam_t = 0;
     FeatParams fp{};
@@ -802,7 +832,7 @@ int mpnn_free(vbd_mpnn *m)
 {
     void *ps[] = {m->nb_off, m->nb, m->Xr, m->U, m->act, m->P, m->G, m->Am, m->Av, m->Pb, m->PbT, m->f, m->z0,
                   m->out, m->dh, m->dx, m->dz, m->partial, m->loss, m->F, m->T, m->guess, m->rm0, m->rm1,
-                  m->use_alt, m->body_geo, m->body_mat, m->dxj};
+                  m->use_alt, m->body_geo, m->body_mat, m->dxj, m->bmax};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (auto p : m->h)
@@ -958,9 +988,13 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
     VBD_TRY(mpnn_forward_backward(m, m->f, nullptr, false, 0.f));
     k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, m->out, m->fp.s, m->guess);
     const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
-    if (!force) jacobi(m, m->guess, dt, nullptr, m->rm1);
-    k_choose_guess<<<s->n_bodies, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, m->rm1, m->guess, s->u, m->use_alt,
-                                                force);
+    const dim3 grid(m->guess_chunks, s->n_bodies);
+    if (!force) {
+        jacobi(m, m->guess, dt, nullptr, m->rm1);
+        k_fill_u32<<<nblk(2 * s->n_bodies, 256), 256, 0, st>>>(2 * s->n_bodies, m->bmax, 0u);
+        k_guess_max<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, m->rm1, m->bmax);
+    }
+    k_guess_apply<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->bmax, m->guess, s->u, m->use_alt, force);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -1037,13 +1071,23 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         (rc = dalloc(&m->dh, nw)) || (rc = dalloc(&m->dx, 2 * nw)) || (rc = dalloc(&m->dz, nw)) ||
         (rc = dalloc(&m->loss, 1)) || (rc = dalloc(&m->guess, n)) || (rc = dalloc(&m->use_alt, s->n_bodies)) ||
         (rc = dalloc(&m->rm0, std::max(1, s->n_swept))) || (rc = dalloc(&m->rm1, std::max(1, s->n_swept))) ||
-        (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) || (rc = dalloc(&m->body_geo, s->n_bodies)) ||
+        (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) || (rc = dalloc(&m->bmax, 2 * (size_t)s->n_bodies)) ||
+        (rc = dalloc(&m->body_geo, s->n_bodies)) ||
         (rc = dalloc(&m->body_mat, s->n_bodies)))
         return fail(rc);
     for (int l = 0; l <= kLayers; ++l)
         if ((rc = dalloc(&m->h[l], nw))) return fail(rc);
     for (int l = 0; l < kLayers; ++l)
         if ((rc = dalloc(&m->a[l], nw)) || (rc = dalloc(&m->z[l], nw))) return fail(rc);
+    {
+        int mx = 0;  // largest body's swept vertex count
+        for (int b = 0; b < s->n_bodies; ++b) {
+            int nbv = 0;
+            for (int c = 0; c < s->ncolors; ++c) nbv += s->ranges_host[(size_t)b * (s->ncolors + 1) + c].y;
+            mx = std::max(mx, nbv);
+        }
+        m->guess_chunks = std::max(1, nblk(mx, kGuessChunk));
+    }
     m->wg_blocks = std::max(1, std::min(512, nblk(n, 32)));
     m->out_blocks = std::max(1, std::min(2048, nblk(n, 16)));
     const size_t part = std::max((size_t)m->wg_blocks * (kW * 2 * kW + kW), (size_t)m->out_blocks * kOutPart);
