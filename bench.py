@@ -216,9 +216,12 @@ def warm_section(args):
     hist = ws.train(epochs=args.warm_epochs, batch=8, lr=2e-3, seed=0)
     t_train = time.perf_counter() - t0
     ref = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    sh = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+    wh = WarmStart(sh, capacity=1)
+    wh.set_policy(WarmStart.HISTORY)
     rng = np.random.default_rng(1000)
-    it_p, it_w, used, err = [], [], [], []
-    w_p = w_w = 0.0
+    it_p, it_w, it_h, used, err, err_h = [], [], [], [], [], []
+    w_p = w_w = w_h = 0.0
     n_eval, eval_steps = 3, 40
     diag = float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
     for _ in range(n_eval):
@@ -226,7 +229,7 @@ def warm_section(args):
         nu = float(rng.uniform(0.3, 0.45))
         g = gravity_in_cone(rng, 30.0)
         v0 = rng.normal(0, 0.05, size=(sc.n_vertices, 3)) * sc.active_mask[:, None]
-        for sol in (s, ref):
+        for sol in (s, ref, sh):
             sol.reset()
             sol.set_material(E, nu)
             sol.set_gravity(g)
@@ -238,12 +241,17 @@ def warm_section(args):
             b = time.perf_counter()
             xp = ref.step(ref.old_positions, args.dt)
             c = time.perf_counter()
+            xh = sh.step(sh.old_positions, args.dt, warm_start=wh)
+            d = time.perf_counter()
             w_w += b - a
             w_p += c - b
+            w_h += d - c
             it_w.append(s.last_iterations)
             it_p.append(ref.last_iterations)
+            it_h.append(sh.last_iterations)
             used.append(s.last_used_network)
         err.append(float(np.abs(xw.numpy() - xp.numpy()).max()))
+        err_h.append(float(np.abs(xh.numpy() - xp.numpy()).max()))
     sim = n_eval * eval_steps * args.dt
     return {
         "workload": f"cantilever 128x16x16 ({sc.n_vertices} vertices); training: {args.warm_trajectories} seeded "
@@ -255,6 +263,8 @@ def warm_section(args):
         "plain": {"iterations_per_step": float(np.mean(it_p)), "sim_s_per_wall_s": sim / w_p},
         "warm": {"iterations_per_step": float(np.mean(it_w)), "sim_s_per_wall_s": sim / w_w,
                  "network_used_fraction": float(np.mean(used))},
+        "history_only": {"iterations_per_step": float(np.mean(it_h)), "sim_s_per_wall_s": sim / w_h,
+                         "max_abs_diff_vs_plain_m": max(err_h)},
         "max_abs_diff_vs_plain_m": max(err), "bbox_diagonal_m": diag,
         "iteration_reduction": 1.0 - float(np.mean(it_w)) / float(np.mean(it_p)),
     }
@@ -300,39 +310,58 @@ def mixed_section(args):
     t_train = time.perf_counter() - t0
     for _ in range(3):
         bat.step(dt=args.dt)
-    w, it, ms = time_steps(bat, steps, args.dt)
-    out["batched"] = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
-                      "max_body_iterations_per_step": float(it.mean()),
-                      "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)],
-                      "autotuned": {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}}
-    used = []
-    ms_w, it_w = [], []
-    ws.set_policy(ws.ADAPTIVE)  # skip the network while it keeps losing the first-sweep comparison
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        a = time.perf_counter()
-        bat.step(dt=args.dt, warm_start=ws)
-        ms_w.append((time.perf_counter() - a) * 1e3)
-        it_w.append(bat.last_iterations)
-        used.append(bat.last_used_network / nb)
-    w = time.perf_counter() - t0
-    out["batched_warm"] = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
-                           "max_body_iterations_per_step": float(np.mean(it_w)),
-                           "bodies_using_network": float(np.mean(used)),
-                           "policy": "adaptive (choose per body; network skipped 1..16 steps after a step where "
-                                     "fewer than 2% of bodies took it)",
-                           "step_ms_p50_p90_p99": [float(np.percentile(ms_w, q)) for q in (50, 90, 99)],
-                           "training": f"{args.mixed_samples} samples from the batch's own steps, "
-                                       f"{args.warm_epochs} epochs, {t_gen:.1f} s + {t_train:.1f} s",
-                           "train_loss_first_last": [float(hist[0]), float(hist[-1])]}
+    # every timed form starts from this state, after one untimed step of its own
+    x0, v0 = bat.old_positions.numpy(), bat.old_velocities.numpy()
+
+    def timed_leg(policy):
+        bat.reset()
+        bat.old_positions = x0
+        bat.old_velocities = v0
+        if policy is not None:
+            ws.set_policy(policy)
+        warm = None if policy is None else ws
+        bat.step(dt=args.dt, warm_start=warm)
+        ms_l, it_l, used_l = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            bat.step(dt=args.dt, warm_start=warm)
+            ms_l.append((time.perf_counter() - a) * 1e3)
+            it_l.append(bat.last_iterations)
+            used_l.append(bat.body_iterations.mean())
+            if warm is not None:
+                used_l[-1] = (used_l[-1], bat.last_used_network / nb)
+        w = time.perf_counter() - t0
+        r = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
+             "max_body_iterations_per_step": float(np.mean(it_l)),
+             "step_ms_p50_p90_p99": [float(np.percentile(ms_l, q)) for q in (50, 90, 99)]}
+        if warm is None:
+            r["mean_body_iterations_per_step"] = float(np.mean(used_l))
+        else:
+            r["mean_body_iterations_per_step"] = float(np.mean([u[0] for u in used_l]))
+            r["bodies_using_warm_guess"] = float(np.mean([u[1] for u in used_l]))
+        return r
+
+    out["batched"] = timed_leg(None)
+    out["batched"]["autotuned"] = {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}
+    out["batched_history"] = timed_leg(WarmStart.HISTORY)
+    out["batched_history"]["policy"] = ("history: per body the better (first-sweep residual) of the plain guess "
+                                        "and x_inertia + the previous step's correction; no network")
+    out["batched_warm"] = timed_leg(WarmStart.CHOOSE)
+    out["batched_warm"]["policy"] = ("choose: per body the better of the plain guess and the network guess "
+                                     "x_inertia + previous correction + network output")
+    out["batched_warm"]["training"] = (f"{args.mixed_samples} samples from the batch's own steps, "
+                                       f"{args.warm_epochs} epochs, {t_gen:.1f} s + {t_train:.1f} s")
+    out["batched_warm"]["train_loss_first_last"] = [float(hist[0]), float(hist[-1])]
+    out["batched_warm_adaptive"] = timed_leg(WarmStart.ADAPTIVE)
+    out["batched_warm_adaptive"]["policy"] = ("adaptive: choose, with the network skipped 1..16 steps after a "
+                                              "step where fewer than 2% of bodies took it")
     out["plain_per_body"] = plain_per_body(bodies, args)
-    out["speedup_batched_vs_plain"] = out["batched"]["aggregate_sim_s_per_wall_s"] / \
-        out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
-    for k in ("batched", "batched_warm"):
-        out[f"speedup_{k}_vs_plain_per_body"] = out[k]["aggregate_sim_s_per_wall_s"] / \
-            out["plain_per_body"]["aggregate_sim_s_per_wall_s"]
-    out["speedup_batched_warm_vs_plain"] = out["batched_warm"]["aggregate_sim_s_per_wall_s"] / \
-        out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
+    base = out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
+    per_body = out["plain_per_body"]["aggregate_sim_s_per_wall_s"]
+    for k in ("batched", "batched_history", "batched_warm", "batched_warm_adaptive"):
+        out[k]["vs_plain_concatenated"] = out[k]["aggregate_sim_s_per_wall_s"] / base
+        out[k]["vs_plain_per_body"] = out[k]["aggregate_sim_s_per_wall_s"] / per_body
     return out
 
 

@@ -144,11 +144,15 @@ int vbd_batch_autotune(vbd_solver_t *h, double dt, double tol, int32_t max_iter,
 
 /* ------------------------------------------------------------------ warm start
  * A 4-layer message-passing network (width 64, BF16 weights on MFMA, FP32
- * accumulation) on the vertex graph of a single-body solver predicts each
- * step's converged correction from the state at the start of the step.  No
- * counterpart in WarpVBD.  Training samples come from the solver's own plain
- * steps (vbd_mpnn_collect) and live in device memory (sample_capacity of them,
- * 140 bytes per vertex each). */
+ * accumulation) on the vertex graph of the solver's mesh predicts each step's
+ * converged correction x_{n+1} - x_inertia from the state at the start of the
+ * step.  No counterpart in WarpVBD.  The network guess is
+ * x_inertia + c_prev + dt^2 |g| out, where c_prev is the previous step's
+ * converged correction (scaled by (dt / dt_prev)^2; zero right after create,
+ * vbd_reset or vbd_load_snapshot), so the network predicts how the correction
+ * changes from one step to the next.  Training samples come from the solver's
+ * own plain steps (vbd_mpnn_collect) and live in device memory
+ * (sample_capacity of them, 76 bytes per vertex each). */
 typedef struct vbd_mpnn vbd_mpnn_t;
 int vbd_mpnn_create(vbd_solver_t *h, int32_t sample_capacity, vbd_mpnn_t **out);
 void vbd_mpnn_destroy(vbd_mpnn_t *m);
@@ -159,28 +163,32 @@ int vbd_mpnn_get_params(vbd_mpnn_t *m, float *params);
 /* material features: one (E, nu) for every body, or one per body (arrays of n_bodies) */
 int vbd_mpnn_set_material(vbd_mpnn_t *m, double youngs, double poisson);
 int vbd_mpnn_set_body_materials(vbd_mpnn_t *m, const double *youngs, const double *poisson);
-/* one plain solver step that stores (features before, correction after) in `slot` */
+/* one plain solver step that stores (features before, correction after minus c_prev) in `slot` */
 int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t max_iter, int32_t *iters_out);
 /* loss of a stored sample; grads (nullable, vbd_mpnn_num_params floats) receives its gradient */
 int vbd_mpnn_loss_grad(vbd_mpnn_t *m, int32_t slot, float *loss_out, float *grads);
 /* Adam over stored samples [0, n_samples) */
 int vbd_mpnn_train(vbd_mpnn_t *m, int32_t n_samples, int32_t epochs, int32_t batch, float lr, uint64_t seed,
                    float *loss_per_epoch);
-/* prediction [n,3] (original vertex order, units of dt^2 |g|) for sample `slot`, or the current state if slot < 0 */
+/* network output [n,3] (original vertex order, units of dt^2 |g|; the guess adds c_prev) for sample `slot`, or
+ * the current state if slot < 0 */
 int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out);
 /* a stored sample and the graph, in original vertex order (each pointer nullable) */
 int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *targets, int32_t *nb_off, int32_t *nb,
                          int32_t *n_edges);
 /* Which initial guess vbd_step_warm takes: per body the one with the smaller
  * first-sweep residual (default), always the network's, always the plain
- * inertia prediction (the network is then not evaluated), or CHOOSE with
- * back-off: after a step where fewer than 2% of the bodies took the network
- * guess, the network is skipped for 1, 2, 4, ... up to 16 steps (plain guess,
- * no network cost) before it is evaluated again. */
+ * inertia prediction (the network is then not evaluated), CHOOSE with
+ * back-off (after a step where fewer than 2% of the bodies took the network
+ * guess, the network is skipped for 1, 2, 4, ... up to 16 steps: plain guess,
+ * no network cost), or HISTORY: per body the smaller-residual one of the
+ * plain guess and x_inertia + c_prev, without the network (the method with
+ * the network's output taken as zero). */
 #define VBD_WARM_CHOOSE 0
 #define VBD_WARM_NETWORK 1
 #define VBD_WARM_PLAIN 2
 #define VBD_WARM_ADAPTIVE 3
+#define VBD_WARM_HISTORY 4
 int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy);
 /* vbd_step (or vbd_batch_step on a batched handle) from x_prev with, per body,
  * the network's initial guess when its first-sweep residual (Jacobi

@@ -61,6 +61,8 @@ def lib() -> ctypes.CDLL:
                                  d, ctypes.c_int, ctypes.c_int, P(ctypes.c_int)]
         L.orc_step.restype = ctypes.c_int
         L.orc_step.argtypes = [ctypes.c_void_p, P(d), d, d, ctypes.c_int, P(d), P(ctypes.c_int), P(d)]
+        L.orc_step_guess.restype = ctypes.c_int
+        L.orc_step_guess.argtypes = [ctypes.c_void_p, P(d), P(d), d, d, ctypes.c_int, P(d), P(ctypes.c_int), P(d)]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_reset.argtypes = [ctypes.c_void_p]
         L.orc_get_state.argtypes = [ctypes.c_void_p, P(d), P(d)]
@@ -225,14 +227,18 @@ class OracleSolver:
         lib().orc_set_state(self._h, None if x is None else _p(x, ctypes.c_double),
                             None if v is None else _p(v, ctypes.c_double))
 
-    def step(self, dt, dx_tol=None, max_iter=None, positions=None):
+    def step(self, dt, dx_tol=None, max_iter=None, positions=None, guess=None):
+        """One step; guess (optional [n,3]) replaces the inertia prediction as
+        the starting positions of the active vertices."""
         tol = self.dx_tol if dx_tol is None else dx_tol
         mi = self.max_iter if max_iter is None else max_iter
         out = np.empty((self.n, 3))
         it, met = ctypes.c_int(0), ctypes.c_double(0.0)
         pos = None if positions is None else _f64(positions).reshape(self.n, 3)
-        rc = lib().orc_step(self._h, None if pos is None else _p(pos, ctypes.c_double), float(dt), float(tol),
-                            int(mi), _p(out, ctypes.c_double), ctypes.byref(it), ctypes.byref(met))
+        gs = None if guess is None else _f64(guess).reshape(self.n, 3)
+        rc = lib().orc_step_guess(self._h, None if pos is None else _p(pos, ctypes.c_double),
+                                  None if gs is None else _p(gs, ctypes.c_double), float(dt), float(tol),
+                                  int(mi), _p(out, ctypes.c_double), ctypes.byref(it), ctypes.byref(met))
         if rc != 0:
             raise ValueError("oracle step: invalid dt or max_iter")
         self.last_iters, self.last_metric = it.value, met.value

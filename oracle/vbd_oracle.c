@@ -632,6 +632,9 @@ static void update_vertex(orc_solver *s, int v, double dt)
     }
 }
 
+int orc_step_guess(orc_solver *s, const double *positions, const double *guess, double dt, double tol, int max_iter,
+                   double *out_x, int *iters, double *metric);
+
 /* src/vbd_solver.py:509-612 (VBDSolver.step).  positions: the x_n the caller
  * passes (NULL = the solver's own old_positions).  Writes the new positions to
  * out_x (may be NULL), the number of iterations executed (loop index + 1) to
@@ -640,6 +643,17 @@ static void update_vertex(orc_solver *s, int v, double dt)
 int orc_step(orc_solver *s, const double *positions, double dt, double tol, int max_iter,
              double *out_x, int *iters, double *metric)
 {
+    return orc_step_guess(s, positions, NULL, dt, tol, max_iter, out_x, iters, metric);
+}
+
+/* orc_step with the iterations started from `guess` (positions [n,3]) at the
+ * active vertices instead of the inertia prediction (NULL: the inertia
+ * prediction, as the reference).  The inertia term of the energy still uses
+ * x_prev and v_prev.  No counterpart in the reference: it checks a solver
+ * started from a warm guess. */
+int orc_step_guess(orc_solver *s, const double *positions, const double *guess, double dt, double tol, int max_iter,
+                   double *out_x, int *iters, double *metric)
+{
     const int n = s->n;
     if (max_iter < 1 || !(dt > 0.0)) return -1;
     if (positions) memcpy(s->xprev, positions, sizeof(double) * 3 * (size_t)n); /* :535 */
@@ -647,7 +661,8 @@ int orc_step(orc_solver *s, const double *positions, double dt, double tol, int 
     for (int v = 0; v < n; ++v) {                                                /* :277-292 */
         if (!s->active[v]) continue;
         for (int r = 0; r < 3; ++r)
-            s->x[3 * v + r] = s->xprev[3 * v + r] + s->vprev[3 * v + r] * dt + s->grav[r] * dt * dt;
+            s->x[3 * v + r] = guess ? guess[3 * v + r]
+                                    : s->xprev[3 * v + r] + s->vprev[3 * v + r] * dt + s->grav[r] * dt * dt;
     }
     memset(s->dxs, 0, sizeof(double) * 3 * (size_t)n); /* :545 */
     int it = 0;

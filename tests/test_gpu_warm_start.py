@@ -84,8 +84,10 @@ def test_fallback_keeps_plain_guess_bit_exact():
 
 def resync_errors(ref, solvers, steps, dt=1e-3, warm=None):
     """Per-step resync against a converged oracle trajectory: from each oracle
-    state, one step of every solver (the first with the warm start `warm`);
-    returns per-solver arrays of iterations and max |x - x_oracle| per step."""
+    state, one step of every solver (warm: the warm start of the first solver,
+    or a list with one entry, possibly None, per solver); returns per-solver
+    arrays of iterations and max |x - x_oracle| per step."""
+    warms = list(warm) if isinstance(warm, (list, tuple)) else [warm] + [None] * (len(solvers) - 1)
     its = [[] for _ in solvers]
     errs = [[] for _ in solvers]
     for k in range(steps):
@@ -94,7 +96,7 @@ def resync_errors(ref, solvers, steps, dt=1e-3, warm=None):
         for i, sol in enumerate(solvers):
             sol.old_positions = x0
             sol.old_velocities = v0
-            x = sol.step(sol.old_positions, dt, warm_start=warm if i == 0 else None).numpy()
+            x = sol.step(sol.old_positions, dt, warm_start=warms[i]).numpy()
             its[i].append(sol.last_iterations)
             errs[i].append(np.abs(x - xr).max())
     return [np.array(a) for a in its], [np.array(e) for e in errs]
@@ -154,26 +156,90 @@ def test_trained_network_gain():
     print("train loss", hist[0], "->", hist[-1])
     rng = np.random.default_rng(1000)
     sc_b, sb = make_solver(nx)
-    it_w, it_p, e_w, e_p, used = [], [], [], [], []
+    sc_h, sh = make_solver(nx)
+    wh = WarmStart(sh, capacity=1)
+    wh.set_policy(wh.HISTORY)
+    it_w, it_p, it_h, e_w, e_p = [], [], [], [], []
     for seed in range(3):
         E = float(10 ** rng.uniform(4, 6))
         nu = float(rng.uniform(0.3, 0.45))
         g = gravity_in_cone(rng, 30.0)
         v0 = rng.normal(0, 0.05, size=(sc.n_vertices, 3)) * sc.active_mask[:, None]
-        for sol in (s, sb):
+        for sol in (s, sb, sh):
             sol.set_material(E, nu)
             sol.set_gravity(g)
         ws.set_material(E, nu)
         ref = oracle.OracleSolver(**dict(sc.solver_kwargs(), youngs_modulus=E, poissons_ratio=nu, gravity=g),
                                   dx_tol=1e-11, max_iter=100000, expanded=False, threads=8)
         ref.set_state(v=v0)
-        (iw, ip), (ew, ep) = resync_errors(ref, [s, sb], 25, warm=ws)
-        it_w.append(iw); it_p.append(ip); e_w.append(ew); e_p.append(ep)
-        print(f"E {E:.3g} nu {nu:.3f}: iterations warm {iw.mean():.2f} plain {ip.mean():.2f}; "
-              f"step error warm {ew.mean():.3e} (max {ew.max():.3e}) plain {ep.mean():.3e} (max {ep.max():.3e})")
-    it_w, it_p, e_w, e_p = (np.concatenate(a) for a in (it_w, it_p, e_w, e_p))
+        (iw, ip, ih), (ew, ep, _) = resync_errors(ref, [s, sb, sh], 25, warm=[ws, None, wh])
+        it_w.append(iw); it_p.append(ip); it_h.append(ih); e_w.append(ew); e_p.append(ep)
+        print(f"E {E:.3g} nu {nu:.3f}: iterations warm {iw.mean():.2f} plain {ip.mean():.2f} "
+              f"history only {ih.mean():.2f}; step error warm {ew.mean():.3e} (max {ew.max():.3e}) "
+              f"plain {ep.mean():.3e} (max {ep.max():.3e})")
+    it_w, it_p, it_h, e_w, e_p = (np.concatenate(a) for a in (it_w, it_p, it_h, e_w, e_p))
+    print(f"all seeds: iterations warm {it_w.mean():.2f} plain {it_p.mean():.2f} history only {it_h.mean():.2f}")
     assert it_w.mean() < it_p.mean()
     assert e_w.mean() <= 1.5 * e_p.mean() and e_w.max() <= 1.5 * e_p.max()
+
+
+def test_history_guess_matches_oracle():
+    """With the network's output layer zeroed the warm guess is x_inertia +
+    c_prev, c_prev being the previous step's converged correction.  Forced
+    (NETWORK policy) at a fixed iteration count, each step from the FP64
+    oracle's state matches the oracle started from that same guess to 1e-6 of
+    the bounding-box diagonal; the first step after a reset has no c_prev and
+    matches the oracle's plain step."""
+    import oracle
+    sc, s = make_solver((16, 3, 3))
+    ws = WarmStart(s, capacity=1, seed=3)
+    p = init_params(3)
+    p["W_out"][:] = 0.0
+    p["b_out"][:] = 0.0
+    ws.set_params(flatten(p))
+    ws.set_policy(ws.NETWORK)
+    ref = oracle.OracleSolver(**sc.solver_kwargs(), dx_tol=0.0, max_iter=10, expanded=False)
+    dt, K = 1e-3, 6
+    act = sc.active_mask[:, None].astype(np.float64)
+    g = np.asarray(sc.gravity, np.float64)
+    diag = float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
+    c = np.zeros((sc.n_vertices, 3))
+    for k in range(8):
+        x0, v0 = ref.old_positions, ref.old_velocities
+        y = x0 + (dt * v0 + dt * dt * g) * act
+        xr = ref.step(dt, dx_tol=0.0, max_iter=K, guess=y + c)
+        s.old_positions = x0
+        s.old_velocities = v0
+        x = s.step(s.old_positions, dt, dx_tol=0.0, max_iter=K, warm_start=ws).numpy()
+        err = np.abs(x - xr).max()
+        print(f"step {k}: |c_prev| max {np.abs(c).max():.3e}, error {err:.3e} (bar {1e-6 * diag:.3e})")
+        assert s.last_used_network and s.last_iterations == K
+        assert err <= 1e-6 * diag
+        c = (xr - y) * act
+
+
+def test_history_policy_cuts_iterations():
+    """HISTORY (per body the better of the plain guess and x_inertia + c_prev,
+    no network) on a stiff 32x4x4 cantilever, each step from the state of a
+    converged FP64 oracle trajectory (tol 1e-11): fewer iterations than the
+    plain step, and the step error against the converged step within 1.5x of
+    the plain step's in mean and max."""
+    import oracle
+    nx = (32, 4, 4)
+    sc, s = make_solver(nx)
+    sc_b, sb = make_solver(nx)
+    E = 1e6
+    for sol in (s, sb):
+        sol.set_material(E, 0.4)
+    ws = WarmStart(s, capacity=1)
+    ws.set_policy(ws.HISTORY)
+    ref = oracle.OracleSolver(**dict(sc.solver_kwargs(), youngs_modulus=E), dx_tol=1e-11, max_iter=100000,
+                              expanded=False, threads=8)
+    (ih, ip), (eh, ep) = resync_errors(ref, [s, sb], 25, warm=ws)
+    print(f"iterations history {ih.mean():.2f} plain {ip.mean():.2f}; step error history {eh.mean():.3e} "
+          f"(max {eh.max():.3e}) plain {ep.mean():.3e} (max {ep.max():.3e})")
+    assert ih[1:].mean() < 0.7 * ip[1:].mean()
+    assert eh.mean() <= 1.5 * ep.mean() and eh.max() <= 1.5 * ep.max()
 
 
 @pytest.mark.parametrize("mode", ["launches", "resident"])

@@ -416,8 +416,12 @@ int step_bodies_resident(vbd_solver *s, double dt, double tol, int max_iter)
 
 int step_bodies(vbd_solver *s, double dt, double tol, int max_iter)
 {
-    if (s->batch_mode == kBatchResident) return step_bodies_resident(s, dt, tol, max_iter);
-    return step_bodies_launches(s, dt, tol, max_iter);
+    if (s->warm) VBD_TRY(mpnn_before_init(s->warm, dt));
+    const int rc = s->batch_mode == kBatchResident ? step_bodies_resident(s, dt, tol, max_iter)
+                                                   : step_bodies_launches(s, dt, tol, max_iter);
+    s->hist_valid = rc == 0;
+    s->hist_dt = dt;
+    return rc;
 }
 
 void release_batch(vbd_solver *s)
@@ -443,11 +447,20 @@ void release_batch(vbd_solver *s)
 int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int trial_steps, double *best_ms)
 {
     const int n = s->n;
-    float4 *save_u = nullptr, *save_v = nullptr;
+    float4 *save_u = nullptr, *save_v = nullptr, *save_t = nullptr;
     VBD_HIP(hipMalloc((void **)&save_u, sizeof(float4) * n));
     VBD_HIP(hipMalloc((void **)&save_v, sizeof(float4) * n));
+    if (hipMalloc((void **)&save_t, sizeof(float4) * n) != hipSuccess) {
+        (void)hipFree(save_u);
+        (void)hipFree(save_v);
+        set_error("hipMalloc failed (autotune)");
+        return VBD_ENOMEM;
+    }
     VBD_HIP(hipMemcpyAsync(save_u, s->uprev, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
     VBD_HIP(hipMemcpyAsync(save_v, s->vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
+    VBD_HIP(hipMemcpyAsync(save_t, s->tgt, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
+    const bool keep_hist = s->hist_valid;
+    const double keep_hist_dt = s->hist_dt;
     struct Cand {
         int mode, lanes, block;
     };
@@ -494,9 +507,13 @@ int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int tria
     }
     (void)hipMemcpyAsync(s->uprev, save_u, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
     (void)hipMemcpyAsync(s->vel, save_v, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+    (void)hipMemcpyAsync(s->tgt, save_t, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
     (void)hipStreamSynchronize(s->stream);
     (void)hipFree(save_u);
     (void)hipFree(save_v);
+    (void)hipFree(save_t);
+    s->hist_valid = keep_hist;
+    s->hist_dt = keep_hist_dt;
     s->last_iters = 0;
     if (best_ms) *best_ms = best;
     return rc;

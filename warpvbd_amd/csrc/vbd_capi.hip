@@ -247,6 +247,7 @@ int vbd_reset(vbd_solver_t *s)
     VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(float4) * s->n, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
     s->last_iters = 0;
+    s->hist_valid = false;
     return 0;
 }
 
@@ -358,6 +359,7 @@ int vbd_load_snapshot(vbd_solver_t *s, int32_t slot)
     VBD_HIP(hipSetDevice(s->device));
     VBD_HIP(hipMemcpyAsync(s->uprev, s->snap[slot], sizeof(float4) * s->n, hipMemcpyDeviceToDevice, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
+    s->hist_valid = false;
     return 0;
 }
 

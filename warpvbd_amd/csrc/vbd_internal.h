@@ -125,6 +125,11 @@ struct vbd_solver {
     int32_t *tets_dev = nullptr;     // elements (original ids), kept for the warm-start graph
     double edge_length = 0.01;       // cube root of 6 x mean tet volume
     vbd_mpnn *warm = nullptr;        // warm-start network used by the running step
+    // tgt still holds the inertia target of the step that produced uprev
+    // (set by every finished step, cleared by create, reset and snapshot
+    // loads), so uprev - tgt is that step's converged correction
+    bool hist_valid = false;
+    double hist_dt = 0.0;            // dt of that step
     double uniform_E = 0.0, uniform_nu = 0.0;  // the material when every element shares it (else NaN)
     float4 *d_grav = nullptr;        // per body gravity
     int2 *d_body_out = nullptr;      // per body (iterations, metric bits) of the last step

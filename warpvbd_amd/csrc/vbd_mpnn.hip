@@ -541,15 +541,17 @@ __device__ __forceinline__ float3 inertia_disp(int i, const float4 *uprev, const
 // Per-vertex input features, written in the network's (original) vertex
 // order; the network is defined in oracle/mpnn_ref.py.  From the solver
 // state at the start of a step (colour-major vertex i, original v = perm[i]):
-// rest position relative to its body, inertia-prediction displacement, its
-// increment, velocity, pin flag, the body's material and gravity direction,
-// and the update a Jacobi pass from the inertia prediction would take (dxj,
-// swept vertices [0, nsw) only).  Features 12..20, a local deformation
-// estimate from the neighbours' inertia predictions, follow in
-// k_features_neighbours from U (the inertia displacement in network order).
+// rest position relative to its body, inertia-prediction displacement, the
+// previous step's converged correction c_prev (times hs = (dt / dt_prev)^2;
+// zero and flag 0 when there is none), velocity, pin flag, the body's
+// material and gravity direction, and the update a Jacobi pass from the
+// inertia prediction would take (dxj, swept vertices [0, nsw) only).
+// Features 12..20, a local deformation estimate from the neighbours' inertia
+// predictions, follow in k_features_neighbours from U (the inertia
+// displacement in network order).
 __global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, const float4 *uprev,
                                   const float4 *vel, const uint8_t *active, FeatParams p, BodyFeat bf,
-                                  const float4 *dxj, int nsw, float4 *U, uint16_t *f)
+                                  const float4 *dxj, int nsw, const float4 *cprev, float hs, float4 *U, uint16_t *f)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -560,7 +562,8 @@ __global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, 
     const float4 g = bf.grav[b];
     const float4 X = Xr[v];
     const float3 uy = inertia_disp(i, uprev, vel, active, p.dt, g);
-    const float4 up = uprev[i], vv = vel[i];
+    const float4 vv = vel[i];
+    const float4 c = hs > 0.f ? cprev[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     U[v] = make_float4(uy.x, uy.y, uy.z, 0.f);
     float o[kFin];
     o[0] = (X.x - geo.x) * geo.w;
@@ -569,9 +572,9 @@ __global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, 
     o[3] = uy.x * p.inv_h;
     o[4] = uy.y * p.inv_h;
     o[5] = uy.z * p.inv_h;
-    o[6] = (uy.x - up.x) / p.s;
-    o[7] = (uy.y - up.y) / p.s;
-    o[8] = (uy.z - up.z) / p.s;
+    o[6] = hs * c.x / p.s;
+    o[7] = hs * c.y / p.s;
+    o[8] = hs * c.z / p.s;
     o[9] = vv.x * p.dt * p.inv_h;
     o[10] = vv.y * p.dt * p.inv_h;
     o[11] = vv.z * p.dt * p.inv_h;
@@ -580,7 +583,7 @@ __global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, 
     o[21] = active[i] ? 0.f : 1.f;
     o[22] = mat.x - 5.f;
     o[23] = (mat.y - 0.375f) * 10.f;
-    o[24] = 1.f;
+    o[24] = hs > 0.f ? 1.f : 0.f;
     const float gn = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z) + 1e-30f;
     o[25] = g.x / gn;
     o[26] = g.y / gn;
@@ -631,32 +634,54 @@ __global__ void k_plain_guess(int n, float4 *u, const float4 *uprev, const float
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body[i]]);
 }
 
-// training target of the step just taken, in network order: (x_{n+1} - x_inertia) / s
+// The previous step's converged correction, read before the step's init
+// overwrites tgt: uprev (that step's result) - tgt (its inertia target).
+__global__ void k_history(int n, const float4 *uprev, const float4 *tgt, const uint8_t *active, float4 *c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = uprev[i], y = tgt[i];
+    c[i] = active[i] ? make_float4(x.x - y.x, x.y - y.y, x.z - y.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// training target of the step just taken, in network order:
+// (x_{n+1} - x_inertia - hs c_prev) / s
 __global__ void k_target(int n, const int32_t *perm, const float4 *u, const float4 *tgt, const uint8_t *active,
-                         float inv_s, float *t)
+                         const float4 *cprev, float hs, float inv_s, float *t)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 x = u[i], y = tgt[i];
+    const float4 c = hs > 0.f ? cprev[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool a = active[i];
     float *o = t + (size_t)perm[i] * kFout;
-    o[0] = a ? (x.x - y.x) * inv_s : 0.f;
-    o[1] = a ? (x.y - y.y) * inv_s : 0.f;
-    o[2] = a ? (x.z - y.z) * inv_s : 0.f;
+    o[0] = a ? (x.x - y.x - hs * c.x) * inv_s : 0.f;
+    o[1] = a ? (x.y - y.y - hs * c.y) * inv_s : 0.f;
+    o[2] = a ? (x.z - y.z - hs * c.z) * inv_s : 0.f;
 }
 
-// network guess (colour-major): x_inertia + s out[perm[i]] for active vertices
+// warm guess (colour-major) for active vertices: x_inertia + hs c_prev
+// (+ s out[perm[i]] when out is given); pinned vertices keep x_inertia
 __global__ void k_guess(int n, const int32_t *perm, const float4 *tgt, const uint8_t *active, const float *out,
-                        float s, float4 *dst)
+                        float s, const float4 *cprev, float hs, float4 *dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 y = tgt[i];
-    const float *o = out + (size_t)perm[i] * kFout;
-    if (active[i])
-        dst[i] = make_float4(y.x + s * o[0], y.y + s * o[1], y.z + s * o[2], 0.f);
-    else
+    if (!active[i]) {
         dst[i] = make_float4(y.x, y.y, y.z, 0.f);
+        return;
+    }
+    float3 g = make_float3(y.x, y.y, y.z);
+    if (hs > 0.f) {
+        const float4 c = cprev[i];
+        g = make_float3(g.x + hs * c.x, g.y + hs * c.y, g.z + hs * c.z);
+    }
+    if (out) {
+        const float *o = out + (size_t)perm[i] * kFout;
+        g = make_float3(g.x + s * o[0], g.y + s * o[1], g.z + s * o[2]);
+    }
+    dst[i] = make_float4(g.x, g.y, g.z, 0.f);
 }
 
 // The update a Jacobi pass from the positions in a.u would take at each swept
@@ -811,6 +836,9 @@ struct vbd_mpnn {
     float4 *guess = nullptr;
     uint32_t *rm0 = nullptr, *rm1 = nullptr;  // per-vertex first-sweep residuals of the two guesses
     float4 *dxj = nullptr;                     // Jacobi update from the plain guess (a feature)
+    float4 *cprev = nullptr;                   // previous step's converged correction
+    float4 *ytmp = nullptr;                    // inertia target of a guess formed outside a step
+    float hs = 0.f;                            // (dt / dt_prev)^2, or 0: no previous correction
     int policy = VBD_WARM_CHOOSE;
     int skip_left = 0, backoff = 0;  // VBD_WARM_ADAPTIVE: steps still to skip, current skip length
     bool evaluated = false;          // the last step evaluated the network
@@ -832,7 +860,7 @@ int mpnn_free(vbd_mpnn *m)
 {
     void *ps[] = {m->nb_off, m->nb, m->Xr, m->U, m->act, m->P, m->G, m->Am, m->Av, m->Pb, m->PbT, m->f, m->z0,
                   m->out, m->dh, m->dx, m->dz, m->partial, m->loss, m->F, m->T, m->guess, m->rm0, m->rm1,
-                  m->use_alt, m->body_geo, m->body_mat, m->dxj, m->bmax};
+                  m->use_alt, m->body_geo, m->body_mat, m->dxj, m->bmax, m->cprev, m->ytmp};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (auto p : m->h)
@@ -917,11 +945,11 @@ BodyFeat body_feat(vbd_mpnn *m)
     return BodyFeat{m->s->body_new, m->body_geo, m->body_mat, m->s->d_grav};
 }
 
-SweepArgs sweep_args_at(vbd_solver *s, float4 *u, double dt)
+SweepArgs sweep_args_at(vbd_solver *s, float4 *u, const float4 *tgt, double dt)
 {
     SweepArgs a;
     a.u = u;
-    a.tgt = s->tgt;
+    a.tgt = tgt;
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
@@ -934,34 +962,38 @@ SweepArgs sweep_args_at(vbd_solver *s, float4 *u, double dt)
     return a;
 }
 
-// Jacobi update from the guess in u (and its residual bits into res).
-void jacobi(vbd_mpnn *m, float4 *u, double dt, float4 *dx, uint32_t *res)
+// Jacobi update from the guess in u against the inertia target tgt (and its
+// residual bits into res).
+void jacobi(vbd_mpnn *m, float4 *u, const float4 *tgt, double dt, float4 *dx, uint32_t *res)
 {
     vbd_solver *s = m->s;
     constexpr int L = 8;
-    const SweepArgs a = sweep_args_at(s, u, dt);
+    const SweepArgs a = sweep_args_at(s, u, tgt, dt);
     if (s->material == kSNH) launch_jacobi<L, kSNH>(a, s->n_swept, dx, res, s->stream);
     else launch_jacobi<L, kStVK>(a, s->n_swept, dx, res, s->stream);
 }
 
 // Features of the solver's state at the start of a step of size dt into f
-// (network order).  plain_in_u: s->u and s->tgt already hold the plain guess
-// and the inertia target (the step's init ran); otherwise the plain guess is
-// formed in m->guess and the solver's positions are left alone.
+// (network order); mpnn_before_init has set the previous correction.
+// plain_in_u: s->u and s->tgt already hold the plain guess and the inertia
+// target (the step's init ran); otherwise both are formed in m->guess and
+// m->ytmp and the solver's arrays are left alone.
 void features(vbd_mpnn *m, double dt, bool plain_in_u, uint16_t *f)
 {
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     set_feature_params(m, dt);
     float4 *u = s->u;
+    const float4 *tgt = s->tgt;
     if (!plain_in_u) {
         u = m->guess;
-        k_plain_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, u, s->uprev, s->vel, s->tgt, s->mass, s->active,
+        tgt = m->ytmp;
+        k_plain_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, u, s->uprev, s->vel, m->ytmp, s->mass, s->active,
                                                        s->body_new, s->d_grav, (float)dt);
     }
-    jacobi(m, u, dt, m->dxj, m->rm0);
+    jacobi(m, u, tgt, dt, m->dxj, m->rm0);
     k_features_vertex<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, m->Xr, s->uprev, s->vel, s->active, m->fp,
-                                                       body_feat(m), m->dxj, s->n_swept, m->U, f);
+                                                       body_feat(m), m->dxj, s->n_swept, m->cprev, m->hs, m->U, f);
     k_features_neighbours<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, m->U, m->nb_off, m->nb, f);
 }
 
@@ -970,7 +1002,19 @@ void features(vbd_mpnn *m, double dt, bool plain_in_u, uint16_t *f)
 // ----------------------------------------------------------------- host hooks used by vbd_step.hip
 namespace vbd {
 
-// Replace the plain initial guess in s->u by the network guess when its
+int mpnn_before_init(vbd_mpnn *m, double dt)
+{
+    vbd_solver *s = m->s;
+    m->hs = 0.f;
+    if (!s->hist_valid || !(s->hist_dt > 0.0)) return 0;
+    k_history<<<nblk(m->n, 256), 256, 0, s->stream>>>(m->n, s->uprev, s->tgt, s->active, m->cprev);
+    VBD_HIP(hipGetLastError());
+    const double r = dt / s->hist_dt;
+    m->hs = (float)(r * r);
+    return 0;
+}
+
+// Replace the plain initial guess in s->u by the warm guess when its
 // first-sweep residual is smaller (per body; the policy may force either
 // guess).  Called after the step's init.
 int mpnn_apply_guess(vbd_mpnn *m, double dt)
@@ -978,19 +1022,30 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     m->evaluated = false;
-    if (m->policy == VBD_WARM_PLAIN || (m->policy == VBD_WARM_ADAPTIVE && m->skip_left > 0)) {
+    const bool history = m->policy == VBD_WARM_HISTORY;
+    if (m->policy == VBD_WARM_PLAIN || (m->policy == VBD_WARM_ADAPTIVE && m->skip_left > 0) ||
+        (history && !(m->hs > 0.f))) {
         if (m->policy == VBD_WARM_ADAPTIVE) --m->skip_left;
         VBD_HIP(hipMemsetAsync(m->use_alt, 0, sizeof(int32_t) * s->n_bodies, st));
         return 0;
     }
-    m->evaluated = true;
-    features(m, dt, true, m->f);
-    VBD_TRY(mpnn_forward_backward(m, m->f, nullptr, false, 0.f));
-    k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, m->out, m->fp.s, m->guess);
+    if (history) {
+        // x_inertia + hs c_prev against the plain guess, no network
+        set_feature_params(m, dt);
+        jacobi(m, s->u, s->tgt, dt, nullptr, m->rm0);
+        k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev, m->hs,
+                                                 m->guess);
+    } else {
+        m->evaluated = true;
+        features(m, dt, true, m->f);
+        VBD_TRY(mpnn_forward_backward(m, m->f, nullptr, false, 0.f));
+        k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, m->out, m->fp.s, m->cprev,
+                                                 m->hs, m->guess);
+    }
     const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
     const dim3 grid(m->guess_chunks, s->n_bodies);
     if (!force) {
-        jacobi(m, m->guess, dt, nullptr, m->rm1);
+        jacobi(m, m->guess, s->tgt, dt, nullptr, m->rm1);
         k_fill_u32<<<nblk(2 * s->n_bodies, 256), 256, 0, st>>>(2 * s->n_bodies, m->bmax, 0u);
         k_guess_max<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, m->rm1, m->bmax);
     }
@@ -1072,6 +1127,7 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         (rc = dalloc(&m->loss, 1)) || (rc = dalloc(&m->guess, n)) || (rc = dalloc(&m->use_alt, s->n_bodies)) ||
         (rc = dalloc(&m->rm0, std::max(1, s->n_swept))) || (rc = dalloc(&m->rm1, std::max(1, s->n_swept))) ||
         (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) || (rc = dalloc(&m->bmax, 2 * (size_t)s->n_bodies)) ||
+        (rc = dalloc(&m->cprev, n)) || (rc = dalloc(&m->ytmp, n)) ||
         (rc = dalloc(&m->body_geo, s->n_bodies)) ||
         (rc = dalloc(&m->body_mat, s->n_bodies)))
         return fail(rc);
@@ -1144,9 +1200,9 @@ int vbd_mpnn_num_params(void) { return kNumParams; }
 
 int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy)
 {
-    if (!m || policy < VBD_WARM_CHOOSE || policy > VBD_WARM_ADAPTIVE) {
-        set_error("vbd_mpnn_set_policy: policy must be VBD_WARM_CHOOSE, VBD_WARM_NETWORK, VBD_WARM_PLAIN or "
-                  "VBD_WARM_ADAPTIVE");
+    if (!m || policy < VBD_WARM_CHOOSE || policy > VBD_WARM_HISTORY) {
+        set_error("vbd_mpnn_set_policy: policy must be VBD_WARM_CHOOSE, VBD_WARM_NETWORK, VBD_WARM_PLAIN, "
+                  "VBD_WARM_ADAPTIVE or VBD_WARM_HISTORY");
         return VBD_EINVAL;
     }
     m->policy = policy;
@@ -1223,6 +1279,7 @@ int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     VBD_HIP(hipSetDevice(s->device));
+    VBD_TRY(mpnn_before_init(m, dt));
     features(m, dt, false, m->F + (size_t)slot * m->n * kFin);
     int it = 0;
     double met = 0.0;
@@ -1232,8 +1289,8 @@ int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t
     } else {
         VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
     }
-    k_target<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->uprev, s->tgt, s->active, 1.f / m->fp.s,
-                                              m->T + (size_t)slot * m->n * 3);
+    k_target<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->uprev, s->tgt, s->active, m->cprev, m->hs,
+                                              1.f / m->fp.s, m->T + (size_t)slot * m->n * 3);
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(st));
     if (iters_out) *iters_out = it;
@@ -1315,8 +1372,12 @@ int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out_host)
     }
     hipStream_t st = m->s->stream;
     const uint16_t *f = m->f;
-    if (slot >= 0) f = m->F + (size_t)slot * m->n * kFin;
-    else features(m, dt, false, m->f);
+    if (slot >= 0) {
+        f = m->F + (size_t)slot * m->n * kFin;
+    } else {
+        VBD_TRY(mpnn_before_init(m, dt));
+        features(m, dt, false, m->f);
+    }
     VBD_TRY(mpnn_forward_backward(m, f, nullptr, false, 0.f));
     VBD_HIP(hipMemcpyAsync(out_host, m->out, sizeof(float) * 3 * m->n, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));

@@ -333,7 +333,8 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     return 0;
 }
 
-int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric)
+namespace {
+int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric)
 {
     if (s->resident && !s->warm) return step_resident(s, dt, tol, max_iter, iters, metric);
     VBD_TRY(ensure_partials(s));
@@ -384,6 +385,16 @@ int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, 
         s->ev_pending.clear();
     }
     return 0;
+}
+}  // namespace
+
+int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric)
+{
+    if (s->warm) VBD_TRY(mpnn_before_init(s->warm, dt));
+    const int rc = step_device_run(s, dt, tol, max_iter, iters, metric);
+    s->hist_valid = rc == 0;
+    s->hist_dt = dt;
+    return rc;
 }
 
 int export_positions(vbd_solver *s, const float4 *src, double *host_out)

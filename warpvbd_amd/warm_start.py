@@ -3,10 +3,14 @@
 A 4-layer message-passing network on the mesh's vertex graph (width 64, BF16
 weights on MFMA with FP32 accumulation, hand-written forward and training
 kernels in csrc/vbd_mpnn.hip) predicts each step's converged correction
-x_{n+1} - x_inertia from the state at the start of the step.  The solver then
-iterates from that guess to the same tolerance, so its accuracy is still
-decided by VBD; a step whose network guess has a larger first-sweep residual
-than the plain inertia guess starts from the plain guess instead.
+x_{n+1} - x_inertia from the state at the start of the step.  The guess is
+x_inertia + c_prev + dt^2 |g| out: c_prev, the previous step's converged
+correction (scaled by (dt / dt_prev)^2, zero after a reset), carries the part
+of the correction that persists from step to step, and the network predicts
+how it changes.  The solver then iterates from that guess to the same
+tolerance, so its accuracy is still decided by VBD; a body whose warm guess
+has a larger first-sweep residual than the plain inertia guess starts from
+the plain guess instead.
 
 Typical use::
 
@@ -132,14 +136,17 @@ class WarmStart:
               "vbd_mpnn_set_body_materials")
 
     # initial-guess policy of warm steps
-    CHOOSE, NETWORK, PLAIN, ADAPTIVE = 0, 1, 2, 3
+    CHOOSE, NETWORK, PLAIN, ADAPTIVE, HISTORY = 0, 1, 2, 3, 4
 
     def set_policy(self, policy: int):
         """CHOOSE (default): per body the guess with the smaller first-sweep
         residual; NETWORK: always the network's; PLAIN: always the inertia
         prediction (the network is not evaluated); ADAPTIVE: CHOOSE, but after
         a step where fewer than 2% of the bodies took the network guess the
-        network is skipped for 1, 2, 4, ... (at most 16) steps."""
+        network is skipped for 1, 2, 4, ... (at most 16) steps; HISTORY: per
+        body the smaller-residual one of the plain guess and x_inertia +
+        c_prev, without evaluating the network (the warm guess with the
+        network's output taken as zero)."""
         check(_lib.load().vbd_mpnn_set_policy(self._h, int(policy)), "vbd_mpnn_set_policy")
 
     def save(self, path):
@@ -218,6 +225,8 @@ class WarmStart:
         return hist
 
     def predict(self, dt, slot: int = -1) -> np.ndarray:
+        """The network's output [n, 3] in units of dt^2 |g| (the warm guess
+        adds c_prev) for stored sample `slot`, or for the solver's state."""
         out = np.empty((self.solver.n_vertices, 3), np.float32)
         check(_lib.load().vbd_mpnn_predict(self._h, int(slot), float(dt), ptr(out, ctypes.c_float)),
               "vbd_mpnn_predict")
