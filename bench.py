@@ -220,7 +220,7 @@ def warm_section(args):
     wh = WarmStart(sh, capacity=1)
     wh.set_policy(WarmStart.HISTORY)
     rng = np.random.default_rng(1000)
-    it_p, it_w, it_h, used, err, err_h = [], [], [], [], [], []
+    it_p, it_w, it_h, used, used_h, err, err_h = [], [], [], [], [], [], []
     w_p = w_w = w_h = 0.0
     n_eval, eval_steps = 3, 40
     diag = float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
@@ -250,6 +250,7 @@ def warm_section(args):
             it_p.append(ref.last_iterations)
             it_h.append(sh.last_iterations)
             used.append(s.last_used_network)
+            used_h.append(ws.last_guess_counts()["history"])
         err.append(float(np.abs(xw.numpy() - xp.numpy()).max()))
         err_h.append(float(np.abs(xh.numpy() - xp.numpy()).max()))
     sim = n_eval * eval_steps * args.dt
@@ -262,7 +263,7 @@ def warm_section(args):
         "generate_s": t_gen, "train_s": t_train, "train_loss_first_last": [float(hist[0]), float(hist[-1])],
         "plain": {"iterations_per_step": float(np.mean(it_p)), "sim_s_per_wall_s": sim / w_p},
         "warm": {"iterations_per_step": float(np.mean(it_w)), "sim_s_per_wall_s": sim / w_w,
-                 "network_used_fraction": float(np.mean(used))},
+                 "network_used_fraction": float(np.mean(used)), "history_used_fraction": float(np.mean(used_h))},
         "history_only": {"iterations_per_step": float(np.mean(it_h)), "sim_s_per_wall_s": sim / w_h,
                          "max_abs_diff_vs_plain_m": max(err_h)},
         "max_abs_diff_vs_plain_m": max(err), "bbox_diagonal_m": diag,
@@ -321,25 +322,24 @@ def mixed_section(args):
             ws.set_policy(policy)
         warm = None if policy is None else ws
         bat.step(dt=args.dt, warm_start=warm)
-        ms_l, it_l, used_l = [], [], []
+        ms_l, it_l, mean_l, guess_l = [], [], [], []
         t0 = time.perf_counter()
         for _ in range(steps):
             a = time.perf_counter()
             bat.step(dt=args.dt, warm_start=warm)
             ms_l.append((time.perf_counter() - a) * 1e3)
             it_l.append(bat.last_iterations)
-            used_l.append(bat.body_iterations.mean())
+            mean_l.append(bat.body_iterations.mean())
             if warm is not None:
-                used_l[-1] = (used_l[-1], bat.last_used_network / nb)
+                guess_l.append(ws.last_guess_counts())
         w = time.perf_counter() - t0
         r = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
              "max_body_iterations_per_step": float(np.mean(it_l)),
+             "mean_body_iterations_per_step": float(np.mean(mean_l)),
              "step_ms_p50_p90_p99": [float(np.percentile(ms_l, q)) for q in (50, 90, 99)]}
-        if warm is None:
-            r["mean_body_iterations_per_step"] = float(np.mean(used_l))
-        else:
-            r["mean_body_iterations_per_step"] = float(np.mean([u[0] for u in used_l]))
-            r["bodies_using_warm_guess"] = float(np.mean([u[1] for u in used_l]))
+        if warm is not None:
+            r["bodies_per_guess"] = {k: float(np.mean([g[k] for g in guess_l])) / nb
+                                     for k in ("plain", "network", "history")}
         return r
 
     out["batched"] = timed_leg(None)
@@ -348,14 +348,15 @@ def mixed_section(args):
     out["batched_history"]["policy"] = ("history: per body the better (first-sweep residual) of the plain guess "
                                         "and x_inertia + the previous step's correction; no network")
     out["batched_warm"] = timed_leg(WarmStart.CHOOSE)
-    out["batched_warm"]["policy"] = ("choose: per body the better of the plain guess and the network guess "
-                                     "x_inertia + previous correction + network output")
+    out["batched_warm"]["policy"] = ("choose: per body the smallest first-sweep residual among the plain guess, "
+                                     "the history guess and the network guess x_inertia + previous correction + "
+                                     "network output")
     out["batched_warm"]["training"] = (f"{args.mixed_samples} samples from the batch's own steps, "
                                        f"{args.warm_epochs} epochs, {t_gen:.1f} s + {t_train:.1f} s")
     out["batched_warm"]["train_loss_first_last"] = [float(hist[0]), float(hist[-1])]
     out["batched_warm_adaptive"] = timed_leg(WarmStart.ADAPTIVE)
-    out["batched_warm_adaptive"]["policy"] = ("adaptive: choose, with the network skipped 1..16 steps after a "
-                                              "step where fewer than 2% of bodies took it")
+    out["batched_warm_adaptive"]["policy"] = ("adaptive: choose, with the network skipped 1..16 steps (history "
+                                              "or plain guess) after a step where fewer than 2% of bodies took it")
     out["plain_per_body"] = plain_per_body(bodies, args)
     base = out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
     per_body = out["plain_per_body"]["aggregate_sim_s_per_wall_s"]

@@ -176,24 +176,28 @@ int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out);
 /* a stored sample and the graph, in original vertex order (each pointer nullable) */
 int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *targets, int32_t *nb_off, int32_t *nb,
                          int32_t *n_edges);
-/* Which initial guess vbd_step_warm takes: per body the one with the smaller
- * first-sweep residual (default), always the network's, always the plain
- * inertia prediction (the network is then not evaluated), CHOOSE with
- * back-off (after a step where fewer than 2% of the bodies took the network
- * guess, the network is skipped for 1, 2, 4, ... up to 16 steps: plain guess,
- * no network cost), or HISTORY: per body the smaller-residual one of the
- * plain guess and x_inertia + c_prev, without the network (the method with
- * the network's output taken as zero). */
+/* Which initial guess vbd_step_warm takes.  CHOOSE (default): per body the
+ * one with the smallest first-sweep residual among the plain inertia
+ * prediction, the network guess and the history guess x_inertia + c_prev
+ * (when there is a c_prev).  NETWORK: always the network's.  PLAIN: always the
+ * plain one (the network is not evaluated).  ADAPTIVE: CHOOSE, but after a
+ * step where fewer than 2% of the bodies took the network guess the network
+ * is skipped for 1, 2, 4, ... up to 16 steps, which then choose between the
+ * plain and the history guess only.  HISTORY: that choice on every step,
+ * without the network. */
 #define VBD_WARM_CHOOSE 0
 #define VBD_WARM_NETWORK 1
 #define VBD_WARM_PLAIN 2
 #define VBD_WARM_ADAPTIVE 3
 #define VBD_WARM_HISTORY 4
 int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy);
+/* bodies that started the last vbd_step_warm from the plain, network and
+ * history guess (counts[3]) */
+int vbd_mpnn_guess_counts(vbd_mpnn_t *m, int32_t *counts);
 /* vbd_step (or vbd_batch_step on a batched handle) from x_prev with, per body,
- * the network's initial guess when its first-sweep residual (Jacobi
- * max|dx_x+dx_y+dx_z| over the body) is below the plain guess's;
- * *used_network_out: the number of bodies that started from the network. */
+ * the initial guess the policy picks; first-sweep residual = Jacobi
+ * max|dx_x+dx_y+dx_z| over the body; *used_network_out: the number of bodies
+ * that started from the network guess. */
 int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t max_iter, int32_t *iters_out,
                   double *metric_out, int32_t *used_network_out);
 

@@ -65,21 +65,29 @@ def test_training_reduces_loss():
 
 
 def test_fallback_keeps_plain_guess_bit_exact():
-    """A network whose guess is far off must lose the first-sweep comparison,
-    and the step must then equal the plain step exactly."""
+    """A network whose guess is far off must lose the first-sweep comparison:
+    the first step (no previous correction yet) equals the plain step exactly,
+    and every step equals the step of the HISTORY policy (the same choice
+    between the plain and history guesses, without the network)."""
     sc, s = make_solver()
     sc2, s2 = make_solver()
-    for sol in (s, s2):
+    sc3, s3 = make_solver()
+    for sol in (s, s2, s3):
         sol.set_launch(8, 256, 0, 0)
     ws = WarmStart(s, capacity=1, seed=4)
     p = init_params(4)
     p["b_out"] = np.array([50.0, -80.0, 120.0])  # a guess 100s of dt^2 g away
     ws.set_params(flatten(p))
+    wh = WarmStart(s3, capacity=1)
+    wh.set_policy(wh.HISTORY)
     for k in range(10):
         x = s.step(s.old_positions, 1e-3, warm_start=ws).numpy()
         assert not s.last_used_network
-        x2 = s2.step(s2.old_positions, 1e-3).numpy()
-        assert np.array_equal(x, x2) and s.last_iterations == s2.last_iterations
+        x3 = s3.step(s3.old_positions, 1e-3, warm_start=wh).numpy()
+        assert np.array_equal(x, x3) and s.last_iterations == s3.last_iterations
+        if k == 0:
+            x2 = s2.step(s2.old_positions, 1e-3).numpy()
+            assert np.array_equal(x, x2) and s.last_iterations == s2.last_iterations
 
 
 def resync_errors(ref, solvers, steps, dt=1e-3, warm=None):
@@ -102,12 +110,10 @@ def resync_errors(ref, solvers, steps, dt=1e-3, warm=None):
     return [np.array(a) for a in its], [np.array(e) for e in errs]
 
 
-@pytest.mark.parametrize("policy", ["plain", "adaptive"])
-def test_policies_without_network_gain_equal_plain(policy):
-    """PLAIN never evaluates the network and ADAPTIVE backs off from a
-    network that keeps losing: both give the plain step bit for bit (same
-    launch shapes on both solvers: a warm step does not use the one-workgroup
-    whole-step kernel)."""
+def test_plain_policy_equals_plain_step():
+    """PLAIN never evaluates the network: the warm step equals the plain step
+    bit for bit (same launch shapes on both solvers: a warm step does not use
+    the one-workgroup whole-step kernel)."""
     sc, s = make_solver()
     sc2, s2 = make_solver()
     for sol in (s, s2):
@@ -116,12 +122,39 @@ def test_policies_without_network_gain_equal_plain(policy):
     p = init_params(4)
     p["b_out"] = np.array([50.0, -80.0, 120.0])
     ws.set_params(flatten(p))
-    ws.set_policy(ws.PLAIN if policy == "plain" else ws.ADAPTIVE)
+    ws.set_policy(ws.PLAIN)
     for k in range(12):
         x = s.step(s.old_positions, 1e-3, warm_start=ws).numpy()
         x2 = s2.step(s2.old_positions, 1e-3).numpy()
-        assert not s.last_used_network
+        assert not s.last_used_network and ws.last_guess_counts()["plain"] == 1
         assert np.array_equal(x, x2) and s.last_iterations == s2.last_iterations
+
+
+def test_adaptive_policy_with_a_losing_network_equals_history():
+    """ADAPTIVE with a network that always loses the first-sweep comparison:
+    no body takes the network guess, the network is skipped with growing
+    back-off, and every step equals the HISTORY policy's step bit for bit
+    (both choose between the same plain and history guesses)."""
+    sc, s = make_solver()
+    sc2, s2 = make_solver()
+    for sol in (s, s2):
+        sol.set_launch(8, 256, 0, 0)
+    p = init_params(4)
+    p["b_out"] = np.array([50.0, -80.0, 120.0])
+    wa = WarmStart(s, capacity=1, seed=4)
+    wa.set_params(flatten(p))
+    wa.set_policy(wa.ADAPTIVE)
+    wh = WarmStart(s2, capacity=1, seed=4)
+    wh.set_policy(wh.HISTORY)
+    hist_taken = 0
+    for k in range(12):
+        x = s.step(s.old_positions, 1e-3, warm_start=wa).numpy()
+        x2 = s2.step(s2.old_positions, 1e-3, warm_start=wh).numpy()
+        ca, ch = wa.last_guess_counts(), wh.last_guess_counts()
+        assert not s.last_used_network and ca == ch
+        hist_taken += ca["history"]
+        assert np.array_equal(x, x2) and s.last_iterations == s2.last_iterations
+    assert hist_taken > 0
 
 
 def test_network_policy_forces_the_guess():
@@ -218,6 +251,65 @@ def test_history_guess_matches_oracle():
         c = (xr - y) * act
 
 
+def test_history_features_and_targets():
+    """Stored samples carry c_prev (features 6..8, flag 24), its change over
+    the last two steps (features 3..5, flag 31: only when the two steps were
+    consecutive) and the target (x_{n+1} - x_inertia - c_prev) / (dt^2 |g|),
+    all checked against corrections computed on the host from the exported
+    states; a plain step in between makes the next sample's change feature
+    absent, and a reset clears both."""
+    sc, s = make_solver((12, 3, 3))
+    dt = 1e-3
+    g = np.asarray(sc.gravity, np.float64)
+    S = dt * dt * np.linalg.norm(g)
+    act = sc.active_mask[:, None].astype(np.float64)
+    ws = WarmStart(s, capacity=8)
+    rng = np.random.default_rng(3)
+    s.old_velocities = rng.normal(0, 0.05, size=(sc.n_vertices, 3)) * act
+    plan = ["collect", "collect", "collect", "plain", "collect", "collect", "reset", "collect", "collect"]
+    corr = []  # host corrections of finished steps, None across a reset
+    slot = 0
+    for what in plan:
+        if what == "reset":
+            s.reset()
+            s.old_velocities = rng.normal(0, 0.05, size=(sc.n_vertices, 3)) * act
+            corr.append(None)
+            continue
+        x0, v0 = s.old_positions.numpy(), s.old_velocities.numpy()
+        y = x0 + (dt * v0 + dt * dt * g) * act
+        c1 = corr[-1] if corr else None
+        c2 = corr[-2] if len(corr) > 1 and c1 is not None else None
+        if what == "collect":
+            ws.collect(dt, 1e-7)
+            F, T, _, _ = ws.sample(slot)
+            slot += 1
+        else:
+            s.step(s.old_positions, dt)
+            F = None
+        x1 = s.old_positions.numpy()
+        if F is not None:
+            tol = lambda e: np.abs(e).max() * 2.0 ** -7 + 1e-3
+            if c1 is None:
+                assert np.all(F[:, 6:9] == 0) and np.all(F[:, 24] == 0) and np.all(F[:, 31] == 0)
+                t_exp = (x1 - y) * act / S
+            else:
+                assert np.all(F[:, 24] == 1)
+                assert np.abs(F[:, 6:9] - c1 / S).max() <= tol(c1 / S)
+                t_exp = (x1 - y - c1) * act / S
+                if c2 is None:
+                    assert np.all(F[:, 31] == 0) and np.all(F[:, 3:6] == 0)
+                else:
+                    assert np.all(F[:, 31] == 1)
+                    d = (c1 - c2) / S
+                    assert np.abs(F[:, 3:6] - d).max() <= tol(d)
+            # FP32 displacements on the device against FP64 on the host
+            assert np.abs(T - t_exp).max() <= 2e-3 * np.abs(t_exp).max() + 2e-3
+        corr.append((x1 - y) * act)
+        if what == "plain" and len(corr) > 1:
+            corr[-2] = None  # the network did not see that step: no change feature across it
+    assert slot == 7
+
+
 def test_history_policy_cuts_iterations():
     """HISTORY (per body the better of the plain guess and x_inertia + c_prev,
     no network) on a stiff 32x4x4 cantilever, each step from the state of a
@@ -244,30 +336,39 @@ def test_history_policy_cuts_iterations():
 
 @pytest.mark.parametrize("mode", ["launches", "resident"])
 def test_batched_fallback_bit_exact(mode):
-    """Per-body fallback: with a far-off network every body keeps its plain
-    guess and the batched step equals the plain batched step exactly."""
+    """Per-body fallback: with a far-off network no body takes the network
+    guess; the first batched step equals the plain batched step exactly and
+    every step equals the HISTORY policy's."""
     from warpvbd_amd import BatchedVBDSolver
     from warpvbd_amd.scenes import mass_spring
     bodies = [cantilever((10, 2, 2)), mass_spring(6, 3), cantilever((14, 3, 2), youngs_modulus=4e4)]
     a = BatchedVBDSolver(bodies, dx_tol=1e-7, max_iter=1000, verbose=False)
     b = BatchedVBDSolver(bodies, dx_tol=1e-7, max_iter=1000, verbose=False)
-    for sol in (a, b):
+    c = BatchedVBDSolver(bodies, dx_tol=1e-7, max_iter=1000, verbose=False)
+    for sol in (a, b, c):
         sol.set_mode(mode, 8, 256 if mode == "launches" else 1024, True)
     ws = WarmStart(a, capacity=1, seed=4)
     p = init_params(4)
     p["b_out"] = np.array([50.0, -80.0, 120.0])
     ws.set_params(flatten(p))
+    wh = WarmStart(c, capacity=1)
+    wh.set_policy(wh.HISTORY)
     for k in range(8):
         xa = a.step(dt=1e-3, warm_start=ws).numpy()
-        xb = b.step(dt=1e-3).numpy()
         assert a.last_used_network == 0
-        assert np.array_equal(xa, xb) and np.array_equal(a.body_iterations, b.body_iterations)
+        xc = c.step(dt=1e-3, warm_start=wh).numpy()
+        assert np.array_equal(xa, xc) and np.array_equal(a.body_iterations, c.body_iterations)
+        if k == 0:
+            xb = b.step(dt=1e-3).numpy()
+            assert np.array_equal(xa, xb) and np.array_equal(a.body_iterations, b.body_iterations)
 
 
 def test_batched_warm_start_per_body_choice():
     """A network trained on one body, applied to a batch: the step chooses per
     body, and each body's one-step error against its converged FP64 oracle
-    step (tol 1e-11, per-step resync) stays that of its plain step."""
+    step (tol 1e-11, per-step resync) stays that of its plain step (within
+    1.5x in mean and max, as in the single-body gain test: both stop once the
+    update falls below the tolerance, with errors of the tolerance's order)."""
     import oracle
     from warpvbd_amd import BatchedVBDSolver
     sc, s = make_solver((24, 4, 4))
@@ -304,4 +405,4 @@ def test_batched_warm_start_per_body_choice():
     print("mean step error warm", e_w.mean(0), "plain", e_p.mean(0))
     assert used > 0
     for i in range(nb):
-        assert e_w[:, i].mean() <= 1.2 * e_p[:, i].mean() and e_w[:, i].max() <= 1.5 * e_p[:, i].max()
+        assert e_w[:, i].mean() <= 1.5 * e_p[:, i].mean() and e_w[:, i].max() <= 1.5 * e_p[:, i].max()

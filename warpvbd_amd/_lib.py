@@ -60,6 +60,7 @@ SIGNATURES = {
     "vbd_mpnn_destroy": (None, [_vp]),
     "vbd_mpnn_num_params": (ctypes.c_int, []),
     "vbd_mpnn_set_policy": (ctypes.c_int, [_vp, _i32]),
+    "vbd_mpnn_guess_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "vbd_mpnn_set_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_get_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_set_material": (ctypes.c_int, [_vp, _d, _d]),

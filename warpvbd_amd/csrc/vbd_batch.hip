@@ -421,6 +421,7 @@ int step_bodies(vbd_solver *s, double dt, double tol, int max_iter)
                                                    : step_bodies_launches(s, dt, tol, max_iter);
     s->hist_valid = rc == 0;
     s->hist_dt = dt;
+    if (rc == 0) ++s->step_serial;
     return rc;
 }
 
@@ -461,6 +462,7 @@ int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int tria
     VBD_HIP(hipMemcpyAsync(save_t, s->tgt, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
     const bool keep_hist = s->hist_valid;
     const double keep_hist_dt = s->hist_dt;
+    const int64_t keep_steps = s->step_serial;
     struct Cand {
         int mode, lanes, block;
     };
@@ -514,6 +516,7 @@ int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int tria
     (void)hipFree(save_t);
     s->hist_valid = keep_hist;
     s->hist_dt = keep_hist_dt;
+    s->step_serial = keep_steps;
     s->last_iters = 0;
     if (best_ms) *best_ms = best;
     return rc;

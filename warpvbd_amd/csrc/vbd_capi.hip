@@ -248,6 +248,7 @@ int vbd_reset(vbd_solver_t *s)
     VBD_HIP(hipStreamSynchronize(s->stream));
     s->last_iters = 0;
     s->hist_valid = false;
+    s->step_serial += 2;
     return 0;
 }
 
@@ -360,6 +361,7 @@ int vbd_load_snapshot(vbd_solver_t *s, int32_t slot)
     VBD_HIP(hipMemcpyAsync(s->uprev, s->snap[slot], sizeof(float4) * s->n, hipMemcpyDeviceToDevice, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
     s->hist_valid = false;
+    s->step_serial += 2;
     return 0;
 }
 

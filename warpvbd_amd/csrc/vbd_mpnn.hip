@@ -541,17 +541,19 @@ __device__ __forceinline__ float3 inertia_disp(int i, const float4 *uprev, const
 // Per-vertex input features, written in the network's (original) vertex
 // order; the network is defined in oracle/mpnn_ref.py.  From the solver
 // state at the start of a step (colour-major vertex i, original v = perm[i]):
-// rest position relative to its body, inertia-prediction displacement, the
-// previous step's converged correction c_prev (times hs = (dt / dt_prev)^2;
-// zero and flag 0 when there is none), velocity, pin flag, the body's
-// material and gravity direction, and the update a Jacobi pass from the
-// inertia prediction would take (dxj, swept vertices [0, nsw) only).
+// rest position relative to its body, the change of the converged correction
+// over the last two steps (hs c_prev - hs2 c_prev2, flag 31), the previous
+// step's converged correction c_prev (times hs = (dt / dt_prev)^2; zero and
+// flag 24 = 0 when there is none), velocity, pin flag, the body's material and
+// gravity direction, and the update a Jacobi pass from the inertia prediction
+// would take (dxj, swept vertices [0, nsw) only).
 // Features 12..20, a local deformation estimate from the neighbours' inertia
 // predictions, follow in k_features_neighbours from U (the inertia
 // displacement in network order).
 __global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, const float4 *uprev,
                                   const float4 *vel, const uint8_t *active, FeatParams p, BodyFeat bf,
-                                  const float4 *dxj, int nsw, const float4 *cprev, float hs, float4 *U, uint16_t *f)
+                                  const float4 *dxj, int nsw, const float4 *cprev, float hs, const float4 *cprev2,
+                                  float hs2, float4 *U, uint16_t *f)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -564,14 +566,15 @@ __global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, 
     const float3 uy = inertia_disp(i, uprev, vel, active, p.dt, g);
     const float4 vv = vel[i];
     const float4 c = hs > 0.f ? cprev[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 c2 = hs2 > 0.f ? cprev2[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     U[v] = make_float4(uy.x, uy.y, uy.z, 0.f);
     float o[kFin];
     o[0] = (X.x - geo.x) * geo.w;
     o[1] = (X.y - geo.y) * geo.w;
     o[2] = (X.z - geo.z) * geo.w;
-    o[3] = uy.x * p.inv_h;
-    o[4] = uy.y * p.inv_h;
-    o[5] = uy.z * p.inv_h;
+    o[3] = hs2 > 0.f ? (hs * c.x - hs2 * c2.x) / p.s : 0.f;
+    o[4] = hs2 > 0.f ? (hs * c.y - hs2 * c2.y) / p.s : 0.f;
+    o[5] = hs2 > 0.f ? (hs * c.z - hs2 * c2.z) / p.s : 0.f;
     o[6] = hs * c.x / p.s;
     o[7] = hs * c.y / p.s;
     o[8] = hs * c.z / p.s;
@@ -592,7 +595,7 @@ __global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, 
     o[28] = d.x / p.s;
     o[29] = d.y / p.s;
     o[30] = d.z / p.s;
-    o[31] = 0.f;
+    o[31] = hs2 > 0.f ? 1.f : 0.f;
     uint4 *dst = reinterpret_cast<uint4 *>(f + (size_t)v * kFin);
 #pragma unroll
     for (int q = 0; q < kFin / 8; ++q)
@@ -703,10 +706,12 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int nv, float4 *dx,
 
 // Per body (blockIdx.y) and chunk of kGuessChunk of its swept vertices
 // (blockIdx.x, over the body's colour ranges in colour order): fold the
-// first-sweep residual bits of the plain (m0) and network (m1) guesses into
-// the body's maxima bm[2b], bm[2b+1].  Residual bits are non-negative floats
-// (or NaN bits), so unsigned max orders them like the floats, NaN on top.
+// first-sweep residual bits of the plain (m0), network (m1) and history (m2)
+// guesses into the body's maxima bm[3b .. 3b+2]; a missing guess (null) gets
+// all bits set.  Residual bits are non-negative floats or positive NaN bits,
+// so unsigned order is float order with NaN above everything finite.
 constexpr int kGuessChunk = 4096;
+constexpr int kGuesses = 3;  // plain, network, history
 
 __device__ __forceinline__ int body_chunk_vertex(const int2 *r, int C, int k)
 {
@@ -719,41 +724,60 @@ __device__ __forceinline__ int body_chunk_vertex(const int2 *r, int C, int k)
 }
 
 __global__ __launch_bounds__(256) void k_guess_max(const int2 *ranges, int C, const uint32_t *m0, const uint32_t *m1,
-                                                   uint32_t *bm)
+                                                   const uint32_t *m2, uint32_t *bm)
 {
     const int2 *r = ranges + (size_t)blockIdx.y * (C + 1);
-    uint32_t a0 = 0, a1 = 0;
+    uint32_t a0 = 0, a1 = m1 ? 0u : 0xffffffffu, a2 = m2 ? 0u : 0xffffffffu;
     const int k0 = blockIdx.x * kGuessChunk;
     for (int k = k0 + threadIdx.x; k < k0 + kGuessChunk; k += blockDim.x) {
         const int i = body_chunk_vertex(r, C, k);
         if (i < 0) break;
         a0 = max(a0, m0[i]);
-        a1 = max(a1, m1[i]);
+        if (m1) a1 = max(a1, m1[i]);
+        if (m2) a2 = max(a2, m2[i]);
     }
     a0 = block_max_u32(a0);
     __syncthreads();
     a1 = block_max_u32(a1);
-    if (threadIdx.x == 0 && (a0 | a1)) {
-        atomicMax(&bm[2 * blockIdx.y], a0);
-        atomicMax(&bm[2 * blockIdx.y + 1], a1);
+    __syncthreads();
+    a2 = block_max_u32(a2);
+    if (threadIdx.x == 0) {
+        uint32_t *b = bm + (size_t)kGuesses * blockIdx.y;
+        if (a0) atomicMax(&b[0], a0);
+        if (a1) atomicMax(&b[1], a1);
+        if (a2) atomicMax(&b[2], a2);
     }
 }
 
-// Keep, per body, the guess with the smaller first-sweep residual (force = 1:
-// the network's) and write it to u; use_alt[b] records the choice.
-__global__ __launch_bounds__(256) void k_guess_apply(const int2 *ranges, int C, const uint32_t *bm, const float4 *alt,
-                                                     float4 *u, int32_t *use_alt, int force)
+// Per body the guess with the smallest first-sweep residual (ties: plain,
+// then network; force = 1: the network's), written to u; choice[b] records
+// it (0 plain, 1 network, 2 history).
+__global__ __launch_bounds__(256) void k_guess_apply(const int2 *ranges, int C, const uint32_t *bm,
+                                                     const float4 *net, const float4 *hist, float4 *u,
+                                                     int32_t *choice, int force)
 {
     const int b = blockIdx.y;
     const int2 *r = ranges + (size_t)b * (C + 1);
-    const bool use = force || __uint_as_float(bm[2 * b + 1]) < __uint_as_float(bm[2 * b]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) use_alt[b] = use ? 1 : 0;
-    if (!use) return;
+    const uint32_t *q = bm + (size_t)kGuesses * b;
+    int use = 0;
+    if (force) {
+        use = 1;
+    } else {
+        uint32_t best = q[0];
+        if (q[1] < best) {
+            best = q[1];
+            use = 1;
+        }
+        if (q[2] < best) use = 2;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) choice[b] = use;
+    if (use == 0) return;
+    const float4 *src = use == 1 ? net : hist;
     const int k0 = blockIdx.x * kGuessChunk;
     for (int k = k0 + threadIdx.x; k < k0 + kGuessChunk; k += blockDim.x) {
         const int i = body_chunk_vertex(r, C, k);
         if (i < 0) break;
-        u[i] = alt[i];
+        u[i] = src[i];
     }
 }
 
@@ -834,16 +858,22 @@ struct vbd_mpnn {
     float *T = nullptr;
     // colour-major (solver order)
     float4 *guess = nullptr;
-    uint32_t *rm0 = nullptr, *rm1 = nullptr;  // per-vertex first-sweep residuals of the two guesses
+    uint32_t *rm0 = nullptr, *rm1 = nullptr, *rm2 = nullptr;  // first-sweep residuals: plain, network, history
+    float4 *hguess = nullptr;                                   // history guess x_inertia + hs c_prev
     float4 *dxj = nullptr;                     // Jacobi update from the plain guess (a feature)
     float4 *cprev = nullptr;                   // previous step's converged correction
+    float4 *cprev2 = nullptr;                  // the one of the step before
     float4 *ytmp = nullptr;                    // inertia target of a guess formed outside a step
-    float hs = 0.f;                            // (dt / dt_prev)^2, or 0: no previous correction
+    float hs = 0.f, hs2 = 0.f;                 // (dt / dt of that step)^2, or 0: none
+    bool have_c = false;                       // cprev holds the correction of step number c_step
+    int64_t c_step = -1;
+    double c_dt = 0.0, c2_dt = 0.0;
     int policy = VBD_WARM_CHOOSE;
     int skip_left = 0, backoff = 0;  // VBD_WARM_ADAPTIVE: steps still to skip, current skip length
     bool evaluated = false;          // the last step evaluated the network
-    int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body
-    uint32_t *bmax = nullptr;                             // per body: max residual bits of the two guesses
+    int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body: 0 plain, 1 network, 2 history guess
+    uint32_t *bmax = nullptr;                             // per body: max residual bits of the three guesses
+    int counts[kGuesses] = {};                            // bodies per guess in the last warm step
     int guess_chunks = 1;                                 // chunks of the largest body's swept vertices
     float *loss_host = nullptr;
     int adam_t = 0;
@@ -860,7 +890,8 @@ int mpnn_free(vbd_mpnn *m)
 {
     void *ps[] = {m->nb_off, m->nb, m->Xr, m->U, m->act, m->P, m->G, m->Am, m->Av, m->Pb, m->PbT, m->f, m->z0,
                   m->out, m->dh, m->dx, m->dz, m->partial, m->loss, m->F, m->T, m->guess, m->rm0, m->rm1,
-                  m->use_alt, m->body_geo, m->body_mat, m->dxj, m->bmax, m->cprev, m->ytmp};
+                  m->use_alt, m->body_geo, m->body_mat, m->dxj, m->bmax, m->cprev, m->cprev2, m->ytmp,
+                  m->rm2, m->hguess};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (auto p : m->h)
@@ -993,7 +1024,8 @@ void features(vbd_mpnn *m, double dt, bool plain_in_u, uint16_t *f)
     }
     jacobi(m, u, tgt, dt, m->dxj, m->rm0);
     k_features_vertex<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, m->Xr, s->uprev, s->vel, s->active, m->fp,
-                                                       body_feat(m), m->dxj, s->n_swept, m->cprev, m->hs, m->U, f);
+                                                       body_feat(m), m->dxj, s->n_swept, m->cprev, m->hs, m->cprev2,
+                                                       m->hs2, m->U, f);
     k_features_neighbours<<<nblk(m->n, 256), 256, 0, st>>>(m->n, m->Xr, m->U, m->nb_off, m->nb, f);
 }
 
@@ -1002,54 +1034,90 @@ void features(vbd_mpnn *m, double dt, bool plain_in_u, uint16_t *f)
 // ----------------------------------------------------------------- host hooks used by vbd_step.hip
 namespace vbd {
 
+// The corrections of the last two finished steps, kept when the solver still
+// holds the last one; the one before counts only when its step came right
+// before (no step in between that this network did not see).
 int mpnn_before_init(vbd_mpnn *m, double dt)
 {
     vbd_solver *s = m->s;
-    m->hs = 0.f;
-    if (!s->hist_valid || !(s->hist_dt > 0.0)) return 0;
+    m->hs = m->hs2 = 0.f;
+    if (!s->hist_valid || !(s->hist_dt > 0.0)) {
+        m->have_c = false;
+        return 0;
+    }
+    bool have_c2;
+    if (m->have_c && m->c_step == s->step_serial) {
+        // the same step's correction again (a prediction before the step): keep cprev2
+        have_c2 = m->c2_dt > 0.0;
+    } else {
+        have_c2 = m->have_c && m->c_step == s->step_serial - 1;
+        if (have_c2) {
+            std::swap(m->cprev, m->cprev2);
+            m->c2_dt = m->c_dt;
+        } else {
+            m->c2_dt = 0.0;
+        }
+    }
     k_history<<<nblk(m->n, 256), 256, 0, s->stream>>>(m->n, s->uprev, s->tgt, s->active, m->cprev);
     VBD_HIP(hipGetLastError());
-    const double r = dt / s->hist_dt;
+    m->have_c = true;
+    m->c_step = s->step_serial;
+    m->c_dt = s->hist_dt;
+    const double r = dt / m->c_dt;
     m->hs = (float)(r * r);
+    if (have_c2) {
+        const double r2 = dt / m->c2_dt;
+        m->hs2 = (float)(r2 * r2);
+    }
     return 0;
 }
 
-// Replace the plain initial guess in s->u by the warm guess when its
-// first-sweep residual is smaller (per body; the policy may force either
-// guess).  Called after the step's init.
+// Replace the plain initial guess in s->u, per body, by the warm guess with
+// the smallest first-sweep residual when it beats the plain guess's: the
+// network guess x_inertia + hs c_prev + s out and, when there is a previous
+// correction, the history guess x_inertia + hs c_prev (the policy may force
+// the network's or keep the plain one).  Called after the step's init.
 int mpnn_apply_guess(vbd_mpnn *m, double dt)
 {
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     m->evaluated = false;
-    const bool history = m->policy == VBD_WARM_HISTORY;
-    if (m->policy == VBD_WARM_PLAIN || (m->policy == VBD_WARM_ADAPTIVE && m->skip_left > 0) ||
-        (history && !(m->hs > 0.f))) {
-        if (m->policy == VBD_WARM_ADAPTIVE) --m->skip_left;
-        VBD_HIP(hipMemsetAsync(m->use_alt, 0, sizeof(int32_t) * s->n_bodies, st));
+    const bool skipping = m->policy == VBD_WARM_ADAPTIVE && m->skip_left > 0;  // network skipped this step
+    if (skipping) --m->skip_left;
+    const bool history = m->policy == VBD_WARM_HISTORY || skipping;
+    const bool have_hist = m->hs > 0.f;
+    if (m->policy == VBD_WARM_PLAIN || (history && !have_hist)) {
+        k_fill_u32<<<nblk(s->n_bodies, 256), 256, 0, st>>>(s->n_bodies, reinterpret_cast<uint32_t *>(m->use_alt),
+                                                           0u);
+        VBD_HIP(hipGetLastError());
         return 0;
     }
+    const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
     if (history) {
-        // x_inertia + hs c_prev against the plain guess, no network
         set_feature_params(m, dt);
         jacobi(m, s->u, s->tgt, dt, nullptr, m->rm0);
-        k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev, m->hs,
-                                                 m->guess);
     } else {
         m->evaluated = true;
-        features(m, dt, true, m->f);
+        features(m, dt, true, m->f);  // the plain guess's residual into rm0
         VBD_TRY(mpnn_forward_backward(m, m->f, nullptr, false, 0.f));
         k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, m->out, m->fp.s, m->cprev,
                                                  m->hs, m->guess);
+        if (!force) jacobi(m, m->guess, s->tgt, dt, nullptr, m->rm1);
     }
-    const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
+    const bool use_hist = have_hist && !force;
+    if (use_hist) {
+        k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev, m->hs,
+                                                 m->hguess);
+        jacobi(m, m->hguess, s->tgt, dt, nullptr, m->rm2);
+    }
     const dim3 grid(m->guess_chunks, s->n_bodies);
     if (!force) {
-        jacobi(m, m->guess, s->tgt, dt, nullptr, m->rm1);
-        k_fill_u32<<<nblk(2 * s->n_bodies, 256), 256, 0, st>>>(2 * s->n_bodies, m->bmax, 0u);
-        k_guess_max<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, m->rm1, m->bmax);
+        k_fill_u32<<<nblk(kGuesses * s->n_bodies, 256), 256, 0, st>>>(kGuesses * s->n_bodies, m->bmax, 0u);
+        k_guess_max<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, history ? nullptr : m->rm1,
+                                          use_hist ? m->rm2 : nullptr, m->bmax);
     }
-    k_guess_apply<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->bmax, m->guess, s->u, m->use_alt, force);
+    k_guess_apply<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->bmax, m->guess, m->hguess, s->u, m->use_alt,
+                                        force);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -1059,9 +1127,12 @@ int mpnn_used_guess(vbd_mpnn *m, int *used)
     const int nb = m->s->n_bodies;
     VBD_HIP(hipMemcpyAsync(m->use_alt_host, m->use_alt, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, m->s->stream));
     VBD_HIP(hipStreamSynchronize(m->s->stream));
-    int c = 0;
-    for (int b = 0; b < nb; ++b) c += m->use_alt_host[b] ? 1 : 0;
-    *used = c;
+    for (int g = 0; g < kGuesses; ++g) m->counts[g] = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int g = m->use_alt_host[b];
+        if (g >= 0 && g < kGuesses) ++m->counts[g];
+    }
+    *used = m->counts[1];
     return 0;
 }
 
@@ -1126,8 +1197,10 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         (rc = dalloc(&m->dh, nw)) || (rc = dalloc(&m->dx, 2 * nw)) || (rc = dalloc(&m->dz, nw)) ||
         (rc = dalloc(&m->loss, 1)) || (rc = dalloc(&m->guess, n)) || (rc = dalloc(&m->use_alt, s->n_bodies)) ||
         (rc = dalloc(&m->rm0, std::max(1, s->n_swept))) || (rc = dalloc(&m->rm1, std::max(1, s->n_swept))) ||
-        (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) || (rc = dalloc(&m->bmax, 2 * (size_t)s->n_bodies)) ||
-        (rc = dalloc(&m->cprev, n)) || (rc = dalloc(&m->ytmp, n)) ||
+        (rc = dalloc(&m->rm2, std::max(1, s->n_swept))) || (rc = dalloc(&m->hguess, n)) ||
+        (rc = dalloc(&m->dxj, std::max(1, s->n_swept))) ||
+        (rc = dalloc(&m->bmax, (size_t)kGuesses * s->n_bodies)) ||
+        (rc = dalloc(&m->cprev, n)) || (rc = dalloc(&m->cprev2, n)) || (rc = dalloc(&m->ytmp, n)) ||
         (rc = dalloc(&m->body_geo, s->n_bodies)) ||
         (rc = dalloc(&m->body_mat, s->n_bodies)))
         return fail(rc);
@@ -1207,6 +1280,16 @@ int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy)
     }
     m->policy = policy;
     m->skip_left = m->backoff = 0;
+    return 0;
+}
+
+int vbd_mpnn_guess_counts(vbd_mpnn_t *m, int32_t *counts)
+{
+    if (!m || !counts) {
+        set_error("null argument");
+        return VBD_EINVAL;
+    }
+    for (int g = 0; g < kGuesses; ++g) counts[g] = m->counts[g];
     return 0;
 }
 

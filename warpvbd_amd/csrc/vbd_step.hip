@@ -394,6 +394,7 @@ int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, 
     const int rc = step_device_run(s, dt, tol, max_iter, iters, metric);
     s->hist_valid = rc == 0;
     s->hist_dt = dt;
+    if (rc == 0) ++s->step_serial;
     return rc;
 }
 

@@ -243,9 +243,10 @@ class VBDSolver:
 
         Returns the new positions; the input is not overwritten.  ``warm_start``
         (a :class:`warpvbd_amd.warm_start.WarmStart` built on this solver)
-        starts the iterations from the network's guess when its first-sweep
-        residual beats the plain inertia guess; ``last_used_network`` records
-        which one the step used."""
+        starts the iterations from the initial guess its policy picks (by
+        default the one of the plain, network and history guesses with the
+        smallest first-sweep residual); ``last_used_network`` records whether
+        that was the network's."""
         if dx_tol is None:
             dx_tol = self.dx_tol
         if max_iter is None:

@@ -80,6 +80,7 @@ class WarmStart:
         self.capacity = int(capacity)
         self.n_samples = 0
         self.n_params = L.vbd_mpnn_num_params()
+        self.policy = self.CHOOSE
         self.set_params(flatten(init_params(seed)))
         self.refresh_materials()
 
@@ -139,15 +140,22 @@ class WarmStart:
     CHOOSE, NETWORK, PLAIN, ADAPTIVE, HISTORY = 0, 1, 2, 3, 4
 
     def set_policy(self, policy: int):
-        """CHOOSE (default): per body the guess with the smaller first-sweep
-        residual; NETWORK: always the network's; PLAIN: always the inertia
-        prediction (the network is not evaluated); ADAPTIVE: CHOOSE, but after
-        a step where fewer than 2% of the bodies took the network guess the
-        network is skipped for 1, 2, 4, ... (at most 16) steps; HISTORY: per
-        body the smaller-residual one of the plain guess and x_inertia +
-        c_prev, without evaluating the network (the warm guess with the
-        network's output taken as zero)."""
+        """CHOOSE (default): per body the guess with the smallest first-sweep
+        residual among the plain inertia prediction, the network guess and the
+        history guess x_inertia + c_prev; NETWORK: always the network's;
+        PLAIN: always the inertia prediction (the network is not evaluated);
+        ADAPTIVE: CHOOSE, but after a step where fewer than 2% of the bodies
+        took the network guess the network is skipped for 1, 2, 4, ... (at
+        most 16) steps, which choose between the plain and history guesses
+        only; HISTORY: that choice on every step, without the network."""
         check(_lib.load().vbd_mpnn_set_policy(self._h, int(policy)), "vbd_mpnn_set_policy")
+        self.policy = int(policy)
+
+    def last_guess_counts(self) -> dict:
+        """Bodies that started the last warm step from each guess."""
+        c = (ctypes.c_int32 * 3)()
+        check(_lib.load().vbd_mpnn_guess_counts(self._h, c), "vbd_mpnn_guess_counts")
+        return {"plain": int(c[0]), "network": int(c[1]), "history": int(c[2])}
 
     def save(self, path):
         np.save(path, self.get_params())
@@ -156,6 +164,19 @@ class WarmStart:
         self.set_params(np.load(path))
 
     # ------------------------------------------------------------------ data
+    def plain_step(self, dt, tol, max_iter=1000) -> int:
+        """A step from the plain guess that still passes through the warm
+        start, so that the next sample's features see its correction as the
+        step right before (a solver step without the warm start breaks that
+        chain)."""
+        keep = self.policy
+        self.set_policy(self.PLAIN)
+        try:
+            self.solver.step(self.solver.old_positions, dt, dx_tol=tol, max_iter=max_iter, warm_start=self)
+        finally:
+            self.set_policy(keep)
+        return self.solver.last_iterations
+
     def collect(self, dt, tol, max_iter=1000) -> int:
         """One plain solver step that also stores a training sample."""
         if self.n_samples >= self.capacity:
@@ -191,8 +212,7 @@ class WarmStart:
                 if k % sample_every == 0 and self.n_samples < self.capacity:
                     iters.append(self.collect(dt, tol, max_iter))
                 else:
-                    s.step(s.old_positions, dt, dx_tol=tol, max_iter=max_iter)
-                    iters.append(s.last_iterations)
+                    iters.append(self.plain_step(dt, tol, max_iter))
         return np.array(iters)
 
     def sample(self, slot):
