@@ -20,14 +20,18 @@ WRITE_SIZE, calibrated against a copy kernel).
 
 Extra entries (each bounded to seconds):
   stvk        the same beam with the StVK material
+  warm_big    the same beam plain and with the warm start's history guess
+              (x_inertia + the previous step's converged correction)
   batched     512 mass-spring bodies (seed 3): one BatchedVBDSolver (per-body
               convergence, autotuned form, hipGraph) vs the reference's form,
               one VBDSolver on the concatenated mesh
   warm_start  128x16x16 cantilever: the network trained on solver-made
-              trajectories (reduced budget, stated), then held-out seeds with
-              and without the warm start
-  mixed       1024 bodies (512 cantilevers + 512 mass-springs, seed 5):
-              batched + warm start vs plain batched, per-step latency percentiles
+              trajectories (reduced budget, stated), then held-out seeds
+              plain, with the warm start, and with the history guess only
+  mixed       1024 bodies (512 cantilevers + 512 mass-springs, seed 5): one
+              concatenated solver, one solver per body, batched, and batched
+              with the history guess / warm start (CHOOSE, ADAPTIVE), each
+              from the same state, with per-step latency percentiles
 cpu_baseline: the FP64 CPU oracle (C restatement of the reference algorithm,
 1 thread) on a bounded sample of the headline workload.
 
@@ -58,7 +62,7 @@ def parse():
     p.add_argument("--max-iter", type=int, default=1000)
     p.add_argument("--dt", type=float, default=1e-3)
     p.add_argument("--quick", action="store_true", help="headline workload only")
-    p.add_argument("--sections", default="stvk,batched,warm_start,mixed",
+    p.add_argument("--sections", default="stvk,warm_big,batched,warm_start,mixed",
                    help="comma list of the sections to run after the headline (ignored with --quick)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
@@ -197,6 +201,40 @@ def batched_section(args):
                     "autotuned": {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}},
         "speedup": w_plain / w_bat,
     }
+
+
+def warm_big_section(args):
+    """The headline 256x32x32 beam stepped plainly and with the warm start's
+    history guess (HISTORY policy: per step the better of the inertia
+    prediction and x_inertia + the previous step's correction; no network,
+    no training), from the same rest state for the same steps."""
+    from warpvbd_amd import VBDSolver
+    from warpvbd_amd.scenes import cantilever
+    from warpvbd_amd.warm_start import WarmStart
+
+    sc = cantilever((256, 32, 32))
+    out = {"workload": "cantilever 256x32x32 SNH, E=1e5, dt=1e-3, tol 1e-7, from rest: "
+                       f"{args.warmup} untimed + {args.steps} timed steps per form"}
+    xs = {}
+    for form in ("plain", "history"):
+        s = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+        ws = None
+        if form == "history":
+            ws = WarmStart(s, capacity=1)
+            ws.set_policy(WarmStart.HISTORY)
+        for _ in range(args.warmup):
+            s.step(s.old_positions, args.dt, warm_start=ws)
+        w, it, ms = time_steps(s, args.steps, args.dt, warm_start=ws)
+        xs[form] = s.old_positions.numpy()
+        out[form] = {"sim_s_per_wall_s": args.steps * args.dt / w, "ms_per_step": 1e3 * w / args.steps,
+                     "iterations_per_step": float(it.mean()),
+                     "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)]}
+        if ws is not None:
+            ws.close()
+        del s
+    out["speedup_history_vs_plain"] = out["history"]["sim_s_per_wall_s"] / out["plain"]["sim_s_per_wall_s"]
+    out["max_abs_diff_history_vs_plain_m"] = float(np.abs(xs["history"] - xs["plain"]).max())
+    return out
 
 
 def warm_section(args):
@@ -427,6 +465,9 @@ def main():
     if "batched" in sections:
         log("batched")
         out["batched"] = batched_section(args)
+    if "warm_big" in sections:
+        log("warm big")
+        out["warm_big"] = warm_big_section(args)
     if "warm_start" in sections:
         log("warm start")
         out["warm_start"] = warm_section(args)

@@ -334,6 +334,44 @@ def test_history_policy_cuts_iterations():
     assert eh.mean() <= 1.5 * ep.mean() and eh.max() <= 1.5 * ep.max()
 
 
+@pytest.mark.parametrize("E", [1e5, 1e6])
+def test_history_trajectory_error(E):
+    """Free-running trajectories (no resync) of a 24x4x4 cantilever over 100
+    steps at tol 1e-7, plain and with the HISTORY policy, against a converged
+    FP64 oracle trajectory (tol 1e-11): the history trajectory's tip error
+    stays within 1.5x of the plain trajectory's (the FP64 oracle run the same
+    way gives 1.6e-3 vs 1.3e-3 of the tip range at E = 1e5 and 6.5e-2 vs
+    1.6e-1 at E = 1e6 over 300 steps: stopping on the update size leaves an
+    error that grows with stiffness, and the history start leaves no more)."""
+    import oracle
+    nx = (24, 4, 4)
+    sc, s = make_solver(nx)
+    sc_b, sb = make_solver(nx)
+    for sol in (s, sb):
+        sol.set_material(E, 0.4)
+    wh = WarmStart(s, capacity=1)
+    wh.set_policy(wh.HISTORY)
+    ref = oracle.OracleSolver(**dict(sc.solver_kwargs(), youngs_modulus=E), dx_tol=1e-11, max_iter=100000,
+                              expanded=False, threads=8)
+    tip = sc.vertices[:, 0] >= sc.vertices[:, 0].max() - 1e-9
+    th, tp, tr, ih, ip = [], [], [], [], []
+    for k in range(100):
+        xr = ref.step(1e-3)
+        xh = s.step(s.old_positions, 1e-3, warm_start=wh).numpy()
+        xp = sb.step(sb.old_positions, 1e-3).numpy()
+        ih.append(s.last_iterations)
+        ip.append(sb.last_iterations)
+        for t, x in ((th, xh), (tp, xp), (tr, xr)):
+            t.append((x[tip] - sc.vertices[tip]).mean(0))
+    th, tp, tr = (np.array(a) for a in (th, tp, tr))
+    rng = np.abs(tr).max()
+    eh, ep = np.abs(th - tr).max(), np.abs(tp - tr).max()
+    print(f"E {E:.0e}: iterations history {np.mean(ih):.2f} plain {np.mean(ip):.2f}; tip error history "
+          f"{eh:.3e} plain {ep:.3e} (tip range {rng:.3e})")
+    assert np.mean(ih) < np.mean(ip)
+    assert eh <= 1.5 * ep + 1e-6 * rng
+
+
 @pytest.mark.parametrize("mode", ["launches", "resident"])
 def test_batched_fallback_bit_exact(mode):
     """Per-body fallback: with a far-off network no body takes the network

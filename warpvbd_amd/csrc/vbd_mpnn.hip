@@ -710,7 +710,7 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int nv, float4 *dx,
 // guesses into the body's maxima bm[3b .. 3b+2]; a missing guess (null) gets
 // all bits set.  Residual bits are non-negative floats or positive NaN bits,
 // so unsigned order is float order with NaN above everything finite.
-constexpr int kGuessChunk = 4096;
+constexpr int kGuessChunk = 1024;
 constexpr int kGuesses = 3;  // plain, network, history
 
 __device__ __forceinline__ int body_chunk_vertex(const int2 *r, int C, int k)
