@@ -131,6 +131,23 @@ def cpu_baseline(scene, args, gpu_iters_per_step):
     t0 = time.perf_counter()
     s.step(args.dt, dx_tol=0.0, max_iter=n_it)
     t_it = (time.perf_counter() - t0) / n_it
+    # the same sample on every host CPU this process may use (OpenMP over the
+    # vertices of a colour), a quarter of the budget
+    ncpu = len(os.sched_getaffinity(0))
+    s.set_threads(ncpu)
+    s.reset()
+    s.step(args.dt, dx_tol=0.0, max_iter=1)
+    n_all = max(1, min(50, int(0.25 * args.cpu_seconds * ncpu / max(t_one, 1e-9))))
+    s.reset()
+    t0 = time.perf_counter()
+    s.step(args.dt, dx_tol=0.0, max_iter=n_all)
+    t_all = (time.perf_counter() - t0) / n_all
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": args.dt / (t_it * gpu_iters_per_step),
         "unit": "sim-s/wall-s",
@@ -140,7 +157,11 @@ def cpu_baseline(scene, args, gpu_iters_per_step):
                    f"{n_it} colour-sweep iterations of one time step at {t_it * 1e3:.1f} ms/iteration, scaled by "
                    f"the GPU's {gpu_iters_per_step:.2f} iterations/step; setup {setup:.1f} s excluded"),
         "ms_per_iteration": t_it * 1e3,
-        "host_cpus": len(os.sched_getaffinity(0)),
+        "host_cpus": ncpu,
+        "cpu_model": model,
+        "all_cores": {"value": args.dt / (t_all * gpu_iters_per_step), "unit": "sim-s/wall-s", "threads": ncpu,
+                      "ms_per_iteration": t_all * 1e3,
+                      "sample": f"{n_all} iterations of the same step, OpenMP over each colour's vertices"},
     }
 
 
