@@ -201,6 +201,24 @@ int vbd_mpnn_guess_counts(vbd_mpnn_t *m, int32_t *counts);
 int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t max_iter, int32_t *iters_out,
                   double *metric_out, int32_t *used_network_out);
 
+/* Slot record layout.  Each slot is a 16-byte compact record (the tet's other
+ * three vertices and a dictionary entry); the entry holds its 44 bytes of
+ * constants.  Meshes whose slots share constants (voxel meshes: 5 rest
+ * shapes x 4 local indices per material) share entries, so a sweep reads 16
+ * bytes per slot from HBM; otherwise each slot has its own entry (64 bytes).
+ * The values, and so the results, are the same either way.
+ * vbd_use_record_dictionary(0) gives solvers created afterwards one entry per
+ * slot (process-wide; default on).  vbd_records reports the number of entries
+ * and the HBM bytes per slot a sweep reads.  No counterpart in WarpVBD (its
+ * per-tet arrays, src/vbd_solver.py:295-343, are what the records hold). */
+void vbd_use_record_dictionary(int32_t enable);
+int vbd_records(vbd_solver_t *h, int32_t *n_dictionary, int32_t *bytes_per_slot);
+/* bytes[n_colors]: the HBM bytes one sweep of each colour reads and writes
+ * with this layout, sum_v (40 + bytes_per_slot k_v) + 12 |gathered neighbours|
+ * (vbd_color_bytes is the same sum with the 68 bytes per slot of the
+ * reference's per-tet arrays). */
+int vbd_color_layout_bytes(vbd_solver_t *h, double *bytes);
+
 /* Mesh facts computed at create. */
 int vbd_info(vbd_solver_t *h, int32_t *n_vertices, int32_t *n_tets, int32_t *n_colors, int32_t *max_incident,
              int32_t *max_degree, int64_t *n_slots);

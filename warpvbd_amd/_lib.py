@@ -60,6 +60,9 @@ SIGNATURES = {
     "vbd_mpnn_destroy": (None, [_vp]),
     "vbd_mpnn_num_params": (ctypes.c_int, []),
     "vbd_mpnn_set_policy": (ctypes.c_int, [_vp, _i32]),
+    "vbd_use_record_dictionary": (None, [_i32]),
+    "vbd_color_layout_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_d)]),
+    "vbd_records": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "vbd_mpnn_guess_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "vbd_mpnn_set_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_get_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
@@ -142,6 +145,13 @@ def to_numpy(a):
     if hasattr(a, "numpy") and callable(a.numpy):
         return a.numpy()
     return np.asarray(a)
+
+
+def use_record_dictionary(enable: bool = True) -> None:
+    """Share record dictionary entries between slots with equal constants
+    (default on; off: one entry per slot); applies to solvers created
+    afterwards."""
+    load().vbd_use_record_dictionary(int(bool(enable)))
 
 
 def use_device(device) -> int:

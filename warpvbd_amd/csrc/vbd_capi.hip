@@ -34,7 +34,7 @@ void vbd_destroy(vbd_solver_t *s)
     release_batch(s);
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
-                   s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev};
+                   s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (float4 *p : s->snap)
@@ -540,6 +540,36 @@ int vbd_profile_read(vbd_solver_t *s, double *ms, int64_t *launches, double *byt
     if (bytes) *bytes = s->prof_bytes;
     s->prof_ms = s->prof_bytes = 0.0;
     s->prof_launches = 0;
+    return 0;
+}
+
+void vbd_use_record_dictionary(int32_t enable) { g_record_dictionary = enable != 0; }
+
+int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
+{
+    CHECK_ARG(s && bytes, "null argument");
+#if VBD_TET_RECORDS
+    const double per_slot = 20.0;
+#else
+    const double per_slot = s->dict_shared ? 16.0 : 64.0;
+#endif
+    for (int c = 0; c < s->ncolors; ++c) {
+        const double nv = (double)(s->col_start[c + 1] - s->col_start[c]);
+        bytes[c] = 40.0 * nv + per_slot * s->col_slots[c] + 12.0 * s->col_boundary[c];
+    }
+    return 0;
+}
+
+int vbd_records(vbd_solver_t *s, int32_t *n_dictionary, int32_t *bytes_per_slot)
+{
+    CHECK_ARG(s, "null handle");
+#if VBD_TET_RECORDS
+    if (n_dictionary) *n_dictionary = 0;
+    if (bytes_per_slot) *bytes_per_slot = 20;  // 64-byte tet record shared by 4 slots + 4-byte entry
+#else
+    if (n_dictionary) *n_dictionary = s->n_dict;
+    if (bytes_per_slot) *bytes_per_slot = s->dict_shared ? 16 : 64;
+#endif
     return 0;
 }
 

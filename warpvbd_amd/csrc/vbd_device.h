@@ -1,14 +1,20 @@
 // vbd_device.h -- per-slot VBD physics, 3x3 solve and lane-group reductions (gfx950).
 //
-// A "slot" is one (vertex, incident tet) pair.  Slots of a vertex are stored
-// contiguously in colour-major vertex order, 56 bytes each in two arrays:
-//
-//   slots  (3 float4 per slot)
-//     qa = (o0, o1, o2 as int bits, ka)   ids of the tet's other three vertices
+// A "slot" is one (vertex, incident tet) pair.  Its constants are
+//     ka                                  material constant (below)
 //     qb = (bs.xyz, kb)                   shape gradient b of the slot's own vertex
 //     qc = (b0.xyz, b1.x)                 b of vertex o0, first component of b of o1
-//   slots2 (1 float2 per slot)
 //     qd = (b1.y, b1.z)                   ; b of o2 = -(bs+b0+b1)
+// with o0, o1, o2 the tet's other three vertices.  Slots of a vertex are stored
+// contiguously in colour-major vertex order as compact 16-byte records
+//     slots  (1 float4 per slot)  c = (o0, o1, o2, entry as int bits)
+// and the constants live in a record dictionary
+//     dict   (3 float4 per entry) d0 = qb, d1 = qc, d2 = (qd.x, qd.y, ka, 0).
+// Meshes built from voxels have a handful of distinct rest shapes (5 for the
+// 5-tet split of a uniform grid), so their slots share few entries (20 per
+// material) and a sweep reads 16 bytes per slot from HBM, the dictionary from
+// cache.  Otherwise every slot has its own entry, in slot order (64 bytes per
+// slot, read as two streams).
 //
 // b is the row of invDm belonging to the vertex's local index (minus the row sum
 // for local index 3), so dF/dx_{v,i} = e_i b^T (src/vbd_solver.py:86-94,450-465).
@@ -174,12 +180,36 @@ __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const fl
     }
 }
 
-// Butterfly sum over aligned groups of L lanes (L a power of two <= 64).
+// x from another lane of the same 16-lane row by a DPP lane pattern
+// (quad_perm, row mirror), without a trip through LDS.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF,
+                                                                 false));
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;  // lane i -> 7-i within each 8 lanes
+constexpr int kDppMirror = 0x140;      // lane i -> 15-i within each 16 lanes
+
+// Sum over aligned groups of L lanes (L a power of two <= 64), the same value
+// in every lane of a group: pairs within quads, then quads within 8 and 8s
+// within 16 by mirroring, then shuffles across rows.
 template <int L>
 __device__ __forceinline__ float group_sum(float x)
 {
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, L);
+    if constexpr (L >= 2) x += dpp_f32<kDppXor1>(x);
+    if constexpr (L >= 4) x += dpp_f32<kDppXor2>(x);
+    if constexpr (L >= 8) x += dpp_f32<kDppHalfMirror>(x);
+    if constexpr (L >= 16) x += dpp_f32<kDppMirror>(x);
+    if constexpr (L >= 32) x += __shfl_xor(x, 16, L);
+    if constexpr (L >= 64) x += __shfl_xor(x, 32, 64);
     return x;
 }
 
@@ -236,14 +266,18 @@ __device__ __forceinline__ uint32_t metric_bits(float3 dx)
     return __float_as_uint(s);
 }
 
+// Max over the wave (every lane active); lane 0 gets the result.
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        uint32_t y = __shfl_xor(x, off, 64);
-        x = x > y ? x : y;
-    }
-    return x;
+    x = max(x, dpp_u32<kDppXor1>(x));
+    x = max(x, dpp_u32<kDppXor2>(x));
+    x = max(x, dpp_u32<kDppHalfMirror>(x));
+    x = max(x, dpp_u32<kDppMirror>(x));
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 0);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 16);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 32);
+    const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
+    return max(max(r0, r1), max(r2, r3));
 }
 
 } // namespace vbd

@@ -98,6 +98,7 @@ struct vbd_solver {
     std::vector<int> col_start;   // colour-major start of each colour's active range (size ncolors+1)
     std::vector<int> col_size;    // active vertices per colour
     std::vector<double> col_bytes;  // algorithmic bytes of one sweep of each colour
+    std::vector<double> col_slots, col_boundary;  // per colour: slots, distinct gathered neighbours
     std::vector<vbd::Segment> segments;
     int32_t *d_col_start = nullptr;  // device copy of col_start
     float kd_dt = 0.f;               // damping / dt of the running step
@@ -151,8 +152,11 @@ struct vbd_solver {
     uint8_t *active = nullptr;    // active flag
     int32_t *slot_off = nullptr;  // [n_swept+1]
     int32_t *slot_ent = nullptr;  // [n_slots] tet << 2 | local index (tet-record layout)
-    float4 *slots = nullptr;      // [n_slots*3] slot records qa|qb|qc (or tet records, [m*4])
-    float2 *slots2 = nullptr;     // [n_slots] slot records qd
+    float4 *slots = nullptr;      // [n_slots] compact slot records (ids, entry), or tet records [m*4]
+    float2 *slots2 = nullptr;     // full records' qd while they are built; null afterwards
+    float4 *dict = nullptr;       // [n_dict*3] record dictionary (vbd_device.h)
+    int32_t n_dict = 0;
+    bool dict_shared = false;     // entries shared between slots (else one per slot)
     int32_t *colors = nullptr;    // original order
     int32_t *inc_off = nullptr;   // original order vertex -> incident (tet<<2|local), ascending tet
     int32_t *inc = nullptr;
@@ -195,6 +199,7 @@ int import_positions_dev(vbd_solver *s, const double *dev_in, float4 *dst);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);
+extern bool g_record_dictionary;  // share record dictionary entries in later creates (default on)
 hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,
                                       hipStream_t st);
 int step_bodies(vbd_solver *s, double dt, double tol, int max_iter);

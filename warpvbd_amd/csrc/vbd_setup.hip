@@ -413,6 +413,92 @@ __global__ void k_build_slot_entries(const int32_t *perm, const int32_t *inc_off
     for (int q = inc_off[v]; q < inc_off[v + 1]; ++q, ++s) ent[s] = inc[q];
 }
 
+// ----------------------------------------------------------------- record dictionary
+// 64-bit hash of a slot record's 11 constant words (ka, qb, qc, qd).
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z ^= z >> 30;
+    z *= 0xbf58476d1ce4e5b9ULL;
+    z ^= z >> 27;
+    z *= 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void record_words(const float4 *slots, const float2 *slots2, int64_t s, uint32_t w[11])
+{
+    const float4 qa = slots[3 * s], qb = slots[3 * s + 1], qc = slots[3 * s + 2];
+    const float2 qd = slots2[s];
+    const float f[11] = {qa.w, qb.x, qb.y, qb.z, qb.w, qc.x, qc.y, qc.z, qc.w, qd.x, qd.y};
+#pragma unroll
+    for (int k = 0; k < 11; ++k) w[k] = __float_as_uint(f[k]);
+}
+
+__global__ void k_record_hash(int64_t ns, const float4 *slots, const float2 *slots2, uint64_t *h, int32_t *idx)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    uint32_t w[11];
+    record_words(slots, slots2, s, w);
+    uint64_t x = 0x9e3779b97f4a7c15ULL;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) x = mix64(x ^ ((uint64_t)w[k] + 0x632be59bd9b4e019ULL * (uint64_t)(k + 1)));
+    h[s] = x;
+    idx[s] = (int32_t)s;
+}
+
+// In hash order: head[i] = 1 where a new hash starts; a record equal in hash
+// but not in content to its predecessor sets *collide.
+__global__ void k_record_heads(int64_t ns, const uint64_t *hs, const int32_t *is, const float4 *slots,
+                               const float2 *slots2, int32_t *head, int32_t *collide)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const bool h = i == 0 || hs[i] != hs[i - 1];
+    head[i] = h ? 1 : 0;
+    if (!h) {
+        uint32_t a[11], b[11];
+        record_words(slots, slots2, is[i], a);
+        record_words(slots, slots2, is[i - 1], b);
+        bool same = true;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) same = same && a[k] == b[k];
+        if (!same) atomicExch(collide, 1);
+    }
+}
+
+// Compact records and shared dictionary entries (entry of hash position i:
+// pos[i] + head[i] - 1, pos the exclusive scan of head).
+__global__ void k_record_dict(int64_t ns, const int32_t *is, const int32_t *head, const int32_t *pos,
+                              const float4 *slots, const float2 *slots2, float4 *compact, float4 *dict)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const int64_t s = is[i];
+    const int k = pos[i] + head[i] - 1;
+    const float4 qa = slots[3 * s];
+    compact[s] = make_float4(qa.x, qa.y, qa.z, __int_as_float(k));
+    if (head[i]) {
+        const float2 qd = slots2[s];
+        dict[3 * (size_t)k] = slots[3 * s + 1];
+        dict[3 * (size_t)k + 1] = slots[3 * s + 2];
+        dict[3 * (size_t)k + 2] = make_float4(qd.x, qd.y, qa.w, 0.f);
+    }
+}
+
+// Compact records with one dictionary entry per slot, in slot order.
+__global__ void k_record_identity(int64_t ns, const float4 *slots, const float2 *slots2, float4 *compact,
+                                  float4 *dict)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const float4 qa = slots[3 * s];
+    const float2 qd = slots2[s];
+    compact[s] = make_float4(qa.x, qa.y, qa.z, __int_as_float((int)s));
+    dict[3 * s] = slots[3 * s + 1];
+    dict[3 * s + 1] = slots[3 * s + 2];
+    dict[3 * s + 2] = make_float4(qd.x, qd.y, qa.w, 0.f);
+}
+
 // Distinct neighbour vertices gathered by one colour's sweep (for the byte model).
 __global__ void k_mark_boundary(const float4 *slots, const int32_t *ent, const int32_t *slot_off, int v0, int v1,
                                 int tag, int32_t *mark, int32_t *count)
@@ -439,6 +525,77 @@ __global__ void k_mark_boundary(const float4 *slots, const int32_t *ent, const i
             if (atomicExch(&mark[o[k]], tag) != tag) ++local;
     }
     if (local) atomicAdd(count, local);
+}
+
+int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, hipStream_t st, Scratch &scr);
+
+// Turn the full slot records (3 float4 + 1 float2 per slot, as built) into
+// compact records and a record dictionary: shared entries when the distinct
+// constant sets are at most a quarter of the slots, no hash collides and
+// `share` is set; one entry per slot otherwise.
+int build_record_dictionary(vbd_solver *s, bool share, hipStream_t st, Scratch &scr)
+{
+    const int64_t ns = s->n_slots;
+    float4 *compact = nullptr, *dict = nullptr;
+    auto install = [&](int64_t nd) {
+        (void)hipFree(s->slots);
+        (void)hipFree(s->slots2);
+        s->slots = compact;
+        s->slots2 = nullptr;
+        s->dict = dict;
+        s->n_dict = (int32_t)nd;
+        s->dict_shared = nd < ns;
+    };
+    auto identity = [&]() -> int {
+        VBD_TRY(dmalloc(&compact, (size_t)std::max<int64_t>(ns, 1)));
+        if (dmalloc(&dict, 3 * (size_t)std::max<int64_t>(ns, 1))) {
+            (void)hipFree(compact);
+            return VBD_ENOMEM;
+        }
+        if (ns > 0) k_record_identity<<<nblk(ns), kTB, 0, st>>>(ns, s->slots, s->slots2, compact, dict);
+        VBD_HIP(hipGetLastError());
+        VBD_HIP(hipStreamSynchronize(st));
+        install(ns);
+        return 0;
+    };
+    if (!share || ns < 4) return identity();
+    uint64_t *h, *hs;
+    int32_t *is0, *is, *head, *pos, *collide;
+    VBD_TRY(scr.get(&h, ns));
+    VBD_TRY(scr.get(&hs, ns));
+    VBD_TRY(scr.get(&is0, ns));
+    VBD_TRY(scr.get(&is, ns));
+    VBD_TRY(scr.get(&head, ns));
+    VBD_TRY(scr.get(&pos, ns));
+    VBD_TRY(scr.get(&collide, 1));
+    k_fill_i32<<<1, kTB, 0, st>>>(collide, 1, 0);
+    k_record_hash<<<nblk(ns), kTB, 0, st>>>(ns, s->slots, s->slots2, h, is0);
+    {
+        size_t tb = 0;
+        VBD_HIP(rocprim::radix_sort_pairs(nullptr, tb, h, hs, is0, is, (size_t)ns, 0, 64, st));
+        void *tmp;
+        VBD_TRY(scr.get((char **)&tmp, tb));
+        VBD_HIP(rocprim::radix_sort_pairs(tmp, tb, h, hs, is0, is, (size_t)ns, 0, 64, st));
+    }
+    k_record_heads<<<nblk(ns), kTB, 0, st>>>(ns, hs, is, s->slots, s->slots2, head, collide);
+    VBD_TRY(exclusive_scan_i32(head, pos, (int)ns, st, scr));
+    int32_t last[2] = {0, 0}, col = 0;
+    VBD_HIP(hipMemcpyAsync(&last[0], pos + ns - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipMemcpyAsync(&last[1], head + ns - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipMemcpyAsync(&col, collide, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    const int64_t nd = (int64_t)last[0] + last[1];
+    if (col || nd < 1 || 4 * nd > ns) return identity();
+    VBD_TRY(dmalloc(&compact, (size_t)ns));
+    if (dmalloc(&dict, 3 * (size_t)nd)) {
+        (void)hipFree(compact);
+        return VBD_ENOMEM;
+    }
+    k_record_dict<<<nblk(ns), kTB, 0, st>>>(ns, is, head, pos, s->slots, s->slots2, compact, dict);
+    VBD_HIP(hipGetLastError());
+    VBD_HIP(hipStreamSynchronize(st));
+    install(nd);
+    return 0;
 }
 
 int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, hipStream_t st, Scratch &scr)
@@ -532,26 +689,32 @@ int build_incidence_and_colors(const int32_t *d_tets, int m, int n, int32_t *d_i
     return 0;
 }
 
-// Scale every record's material constants from one uniform material to
-// another (ns records: slots, or tets in the tet-record layout).
+#if VBD_TET_RECORDS
+// Scale every tet record's material constants from one uniform material to
+// another.
 __global__ void k_rescale_material(int64_t ns, float4 *slots, float fa, float fb, int material)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-#if VBD_TET_RECORDS
     float4 *r = slots + 4 * s;
     r[1].w *= fa;
     r[2].w *= fb;
     r[3].w = material == kSNH ? 0.75f * r[1].w : 0.f;
-#else
-    (void)material;
-    float4 *r = slots + 3 * s;  // ka in qa.w, kb in qb.w; kc follows from ka
-    r[0].w *= fa;
-    r[1].w *= fb;
+}
 #endif
+
+// Scale the material constants of dictionary entries: ka in d2.z, kb in d0.w.
+__global__ void k_rescale_dict(int64_t nd, float4 *dict, float fa, float fb)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nd) return;
+    dict[3 * k + 2].z *= fa;
+    dict[3 * k].w *= fb;
 }
 
 }  // namespace
+
+bool g_record_dictionary = true;  // vbd_use_record_dictionary
 
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1)
 {
@@ -575,8 +738,14 @@ int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1
 #else
     const int64_t nrec = s->n_slots;
 #endif
+#if VBD_TET_RECORDS
     if (nrec > 0)
         k_rescale_material<<<nblk(nrec), kTB, 0, s->stream>>>(nrec, s->slots, (float)fa, (float)fb, s->material);
+#else
+    (void)nrec;
+    if (s->n_dict > 0)
+        k_rescale_dict<<<nblk(s->n_dict), kTB, 0, s->stream>>>(s->n_dict, s->dict, (float)fa, (float)fb);
+#endif
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(s->stream));
     return 0;
@@ -827,8 +996,14 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
             int v0 = s->col_start[c], v1 = s->col_start[c + 1];
             double k = (double)(hoff[v1] - hoff[v0]);
             s->col_bytes[c] = 40.0 * (v1 - v0) + 68.0 * k + 12.0 * hcnt[c];
+            s->col_slots.push_back(k);
+            s->col_boundary.push_back((double)hcnt[c]);
         }
     }
+
+#if !VBD_TET_RECORDS
+    VBD_TRY(build_record_dictionary(s, g_record_dictionary, st, scr));
+#endif
 
     // state buffers
     VBD_TRY(dmalloc(&s->u, n));

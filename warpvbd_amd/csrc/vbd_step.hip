@@ -37,8 +37,8 @@ namespace {
 // partial maxima of the iteration and decides convergence
 // (src/vbd_solver.py:578), so no separate check launch is needed.
 template <int L, int B, int MAT>
-__global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, int v0, int nv, int close, const uint32_t *part_all,
-                                                int np_all, float tol)
+__global__ __launch_bounds__(1024) void k_sweep(
+    SweepArgs a, int v0, int nv, int close, const uint32_t *part_all, int np_all, float tol)
 {
     __shared__ int s_last;
     Ctrl *c = a.ctrl;
@@ -232,6 +232,7 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
     a.slot_off = s->slot_off;
     a.slots = s->slots;
     a.slots2 = s->slots2;
+    a.dict = s->dict;
     a.ent = s->slot_ent;
     a.ns = std::max<int64_t>(s->n_slots, 1);
     a.ctrl = s->ctrl;
@@ -305,6 +306,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     a.slot_off = s->slot_off;
     a.slots = s->slots;
     a.slots2 = s->slots2;
+    a.dict = s->dict;
     a.ent = s->slot_ent;
     a.ns = std::max<int64_t>(s->n_slots, 1);
     a.part = s->part;

@@ -18,8 +18,9 @@ struct SweepArgs {
     const float4 *__restrict__ tgt;
     const float4 *__restrict__ uprev;
     const int32_t *__restrict__ slot_off;
-    const float4 *__restrict__ slots;   // slot records qa|qb|qc (3 per slot), or tet records (VBD_TET_RECORDS)
-    const float2 *__restrict__ slots2;  // slot records qd
+    const float4 *__restrict__ slots;   // compact slot records (ids, entry), or tet records (VBD_TET_RECORDS)
+    const float2 *__restrict__ slots2;  // unused (null)
+    const float4 *__restrict__ dict;    // record dictionary
     const int32_t *__restrict__ ent;    // per-slot tet << 2 | local index (VBD_TET_RECORDS)
     int64_t ns;
     uint32_t *part;  // this launch's per-block metric maxima
@@ -131,15 +132,18 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             float4 q[B][3];
             float2 qd[B];
             float4 x[B][3];
+            // compact record (ids, entry), then the entry's constants
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 const int sj = s + j * L;
                 if (sj < s1) {
-                    const float4 *r = a.slots + 3 * (size_t)sj;
-                    q[j][0] = r[0];
-                    q[j][1] = r[1];
-                    q[j][2] = r[2];
-                    qd[j] = a.slots2[sj];
+                    const float4 c = a.slots[sj];
+                    const float4 *d = a.dict + 3 * (size_t)__float_as_int(c.w);
+                    const float4 d2 = d[2];
+                    q[j][0] = make_float4(c.x, c.y, c.z, d2.z);
+                    q[j][1] = d[0];
+                    q[j][2] = d[1];
+                    qd[j] = make_float2(d2.x, d2.y);
                 }
             }
 #pragma unroll

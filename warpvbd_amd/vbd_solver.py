@@ -338,6 +338,21 @@ class VBDSolver:
         check(_lib.load().vbd_get_color_sizes(self._h, ptr(out, ctypes.c_int32)), "vbd_get_color_sizes")
         return out[: self.n_colors]
 
+    def record_layout(self) -> dict:
+        """Slot records of this solver: record dictionary entries (as many as
+        slots when no constants are shared) and HBM bytes per slot read by a
+        sweep (16 with shared entries, else 64)."""
+        n, b = ctypes.c_int32(), ctypes.c_int32()
+        check(_lib.load().vbd_records(self._h, ctypes.byref(n), ctypes.byref(b)), "vbd_records")
+        return {"dictionary_entries": n.value, "bytes_per_slot": b.value}
+
+    def color_layout_bytes(self) -> np.ndarray:
+        """HBM bytes of one sweep of each colour with this solver's record
+        layout (color_bytes: with 68 bytes per slot, the reference's arrays)."""
+        out = np.empty(max(1, self.n_colors))
+        check(_lib.load().vbd_color_layout_bytes(self._h, ptr(out, ctypes.c_double)), "vbd_color_layout_bytes")
+        return out[: self.n_colors]
+
     def color_bytes(self) -> np.ndarray:
         out = np.empty(max(1, self.n_colors))
         check(_lib.load().vbd_color_bytes(self._h, ptr(out, ctypes.c_double)), "vbd_color_bytes")
