@@ -123,19 +123,22 @@ __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const fl
     if constexpr (MAT == kSNH) {
         const float Ic = F00 * F00 + F01 * F01 + F02 * F02 + F10 * F10 + F11 * F11 + F12 * F12 + F20 * F20 +
                          F21 * F21 + F22 * F22;
-        // cofactor matrix of F (cof(F) = J F^-T)
-        const float C00 = F11 * F22 - F12 * F21, C01 = F12 * F20 - F10 * F22, C02 = F10 * F21 - F11 * F20;
-        const float C10 = F02 * F21 - F01 * F22, C11 = F00 * F22 - F02 * F20, C12 = F01 * F20 - F00 * F21;
-        const float C20 = F01 * F12 - F02 * F11, C21 = F02 * F10 - F00 * F12, C22 = F00 * F11 - F01 * F10;
         // J - 1 = tr D + (principal 2x2 minors of D) + det D, accurate for small D
         const float m2 = (D00 * D11 - D01 * D10) + (D00 * D22 - D02 * D20) + (D11 * D22 - D12 * D21);
         const float detD = D00 * (D11 * D22 - D12 * D21) - D01 * (D10 * D22 - D12 * D20) +
                            D02 * (D10 * D21 - D11 * D20);
         const float Jm1 = (D00 + D11 + D22) + m2 + detD;
-        const float cb0 = C00 * bs.x + C01 * bs.y + C02 * bs.z;
-        const float cb1 = C10 * bs.x + C11 * bs.y + C12 * bs.z;
-        const float cb2 = C20 * bs.x + C21 * bs.y + C22 * bs.z;
-        const float r = 1.f / (Ic + 1.f);
+        // cof(F) b.  Row i of cof(F) = J F^-T is F_j x F_k (rows of F, (i,j,k)
+        // cyclic).  With p = F_1 x b the scalar triple products give
+        //   cb0 = (F_1 x F_2) . b = -F_2 . p,   cb2 = (F_0 x F_1) . b = F_0 . p,
+        // and cb1 = (F_2 x F_0) . b directly.
+        const float p0 = F11 * bs.z - F12 * bs.y, p1 = F12 * bs.x - F10 * bs.z, p2 = F10 * bs.y - F11 * bs.x;
+        const float cb0 = -(F20 * p0 + F21 * p1 + F22 * p2);
+        const float cb2 = F00 * p0 + F01 * p1 + F02 * p2;
+        const float cb1 = (F21 * F02 - F22 * F01) * bs.x + (F22 * F00 - F20 * F02) * bs.y +
+                          (F20 * F01 - F21 * F00) * bs.z;
+        // hardware reciprocal (1 ulp): the SNH weights need no correctly rounded divide
+        const float r = __builtin_amdgcn_rcpf(Ic + 1.f);
         const float s1 = ka * (Ic * r);        // V mu'(1 - 1/(Ic+1))
         const float s2 = 2.f * ka * r * r;     // V 2mu'/(Ic+1)^2
         const float sj = kb * Jm1 - kc;        // V lambda'(J - alpha)

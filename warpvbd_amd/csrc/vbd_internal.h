@@ -59,7 +59,7 @@ struct CachedGraph {
 };
 
 struct Launch {
-    int lanes = 8;        // lanes per vertex
+    int lanes = 4;        // lanes per vertex
     int block = 256;      // threads per block
     int small_color = 2048;  // colours below this many vertices run fused in one workgroup
     int chunk = 0;        // iterations between host waits (0 = adaptive)

@@ -9,6 +9,9 @@ namespace {
 
 inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
 
+#ifndef VBD_BATCH_L4
+#define VBD_BATCH_L4 1
+#endif
 #ifndef VBD_BATCH_L8
 #define VBD_BATCH_L8 1
 #endif
@@ -218,7 +221,7 @@ __device__ __forceinline__ float3 eval_vertex(const SweepArgs &a, int v, bool va
 // slots per lane per load batch, by lanes per vertex
 template <int L>
 struct Batch {
-    static constexpr int value = L == 1 ? 4 : L == 2 ? 4 : L == 4 ? 5 : L == 8 ? VBD_BATCH_L8 : L == 16 ? 2 : 1;
+    static constexpr int value = L == 1 ? 4 : L == 2 ? 4 : L == 4 ? VBD_BATCH_L4 : L == 8 ? VBD_BATCH_L8 : L == 16 ? 2 : 1;
 };
 
 struct StepParams {
