@@ -129,6 +129,30 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             }
         }
 #else
+        if constexpr (B == 1) {
+            // One slot per lane per batch, software-pipelined: the next compact
+            // record is requested after the current slot's constants and
+            // gathers, so each batch waits for one memory latency, not two.
+            int s = s0 + lane;
+            float4 c = s < s1 ? a.slots[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (; s < s1; s += L) {
+                const float4 *d = a.dict + 3 * (size_t)__float_as_int(c.w);
+                const float4 d0 = d[0], d1 = d[1], d2 = d[2];
+                const float4 x0 = a.u[__float_as_int(c.x)];
+                const float4 x1 = a.u[__float_as_int(c.y)];
+                const float4 x2 = a.u[__float_as_int(c.z)];
+                // keep the request for the next record behind this slot's loads
+                __builtin_amdgcn_sched_barrier(0);
+                const float4 cn = a.slots[min(s + L, s1 - 1)];  // the last one reloads a record in cache
+                __builtin_amdgcn_sched_barrier(0);
+                slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
+                               make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y),
+                               make_float3(x0.x - us.x, x0.y - us.y, x0.z - us.z),
+                               make_float3(x1.x - us.x, x1.y - us.y, x1.z - us.z),
+                               make_float3(x2.x - us.x, x2.y - us.y, x2.z - us.z), acc);
+                c = cn;
+            }
+        } else
         // B slots per lane per batch: all records, then all gathers, then the
         // arithmetic, so each batch costs two memory latencies, not 2B.
         for (int s = s0 + lane; s < s1; s += B * L) {
