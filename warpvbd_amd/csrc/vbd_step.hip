@@ -42,13 +42,18 @@ __global__ __launch_bounds__(1024) void k_sweep(
 {
     __shared__ int s_last;
     Ctrl *c = a.ctrl;
-    if (c->converged) return;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
     const int vi = t / L, lane = t % L;
     const bool valid = vi < nv;
     const int v = v0 + (valid ? vi : 0);
-    const uint32_t mb = block_max_u32(update_vertex<L, B, MAT>(a, v, valid, lane));
+    // The vertex's own loads and the converged flag are requested together: a
+    // launch after convergence sweeps nothing (every lane without a vertex)
+    // and stores nothing, instead of branching on the flag before its loads.
+    const VertexPre pre = vertex_pre(a, v, valid);
+    const bool live = !c->converged;
+    const uint32_t mb = block_max_u32(update_vertex<L, B, MAT>(a, v, valid && live, lane, pre));
+    if (!live) return;
     if (threadIdx.x == 0) a.part[blockIdx.x] = mb;
     if (!close) return;
     if (threadIdx.x == 0) {

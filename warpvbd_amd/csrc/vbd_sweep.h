@@ -73,19 +73,40 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// A vertex's own data, loaded before anything that depends on other loads so
+// that a kernel can request it ahead of its early-exit test.
+struct VertexPre {
+    int s0, s1;  // slot range (empty for lanes without a vertex)
+    float4 uv;   // current displacement
+    float4 tg;   // inertia target, M/dt^2
+};
+
+__device__ __forceinline__ VertexPre vertex_pre(const SweepArgs &a, int v, bool valid)
+{
+    VertexPre p;
+    p.uv = a.u[v];
+    p.tg = a.tgt[v];
+    // v is a swept vertex for every lane (lanes without one use the range's first)
+    p.s0 = a.slot_off[v];
+    const int e = a.slot_off[v + 1];
+    p.s1 = valid ? e : p.s0;
+    return p;
+}
+
 // Gradient and Hessian of one vertex's local energy (all incident tets plus
 // inertia and damping) accumulated by a group of L lanes, and the block solve:
 // the update dx (zero for lanes without a vertex).  us receives the vertex's
 // current displacement.
 template <int L, int B, int MAT>
-__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, float3 &us)
+__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, float3 &us,
+                                            const VertexPre &pre)
 {
     Acc acc;
     acc_zero(acc);
-    const float4 uv4 = a.u[v];
+    const float4 uv4 = pre.uv;
     us = make_float3(uv4.x, uv4.y, uv4.z);
     if (valid) {
-        const int s0 = a.slot_off[v], s1 = a.slot_off[v + 1];
+        const int s0 = pre.s0, s1 = pre.s1;
 #if VBD_TET_RECORDS
         // B slots per lane per batch: entries, tet records, gathers, arithmetic
         for (int s = s0 + lane; s < s1; s += B * L) {
@@ -210,7 +231,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
     }
     // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
-    const float4 tg = a.tgt[v];
+    const float4 tg = pre.tg;
     const float w = tg.w;
     acc.g0 += w * (us.x - tg.x);
     acc.g1 += w * (us.y - tg.y);
@@ -224,6 +245,23 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
 
 // Evaluate and apply one vertex update for a lane group.  Returns the metric
 // bits (0 for lanes without a vertex).
+template <int L, int B, int MAT>
+__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, float3 &us)
+{
+    return vertex_dx<L, B, MAT>(a, v, valid, lane, us, vertex_pre(a, v, valid));
+}
+
+template <int L, int B, int MAT>
+__device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane,
+                                                  const VertexPre &pre)
+{
+    float3 us;
+    const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us, pre);
+    if (!valid) return 0u;
+    if (lane == 0) a.u[v] = make_float4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.f);
+    return metric_bits(dx);
+}
+
 template <int L, int B, int MAT>
 __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane)
 {
