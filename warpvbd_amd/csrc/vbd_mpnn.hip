@@ -999,7 +999,7 @@ SweepArgs sweep_args_at(vbd_solver *s, float4 *u, const float4 *tgt, double dt)
 void jacobi(vbd_mpnn *m, float4 *u, const float4 *tgt, double dt, float4 *dx, uint32_t *res)
 {
     vbd_solver *s = m->s;
-    constexpr int L = 8;
+    constexpr int L = 4;  // as the big-colour sweeps
     const SweepArgs a = sweep_args_at(s, u, tgt, dt);
     if (s->material == kSNH) launch_jacobi<L, kSNH>(a, s->n_swept, dx, res, s->stream);
     else launch_jacobi<L, kStVK>(a, s->n_swept, dx, res, s->stream);
