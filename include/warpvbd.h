@@ -42,7 +42,8 @@ enum {
     VBD_EINVAL = -1,      /* bad argument (null pointer, size, dt <= 0, max_iter < 1) */
     VBD_EHIP = -2,        /* a HIP runtime call failed */
     VBD_ENOMEM = -3,      /* device allocation failed */
-    VBD_EMESH = -4,       /* element index outside [0, n_vertices) */
+    VBD_EMESH = -4,       /* element index outside [0, n_vertices), or a mesh above 2^26 - 1 vertices or
+                             vertex-tet incidences (16.7 M tets: the sweep's 32-bit offsets) */
     VBD_ECOLORS = -5,     /* a vertex needs more than 256 colours (src/coloring.py:50-68) */
     VBD_EVALENCE = -6,    /* a vertex has more than 32 incident elements (src/vbd_solver.py:12,529) */
     VBD_EDEGENERATE = -7  /* an element has a singular rest shape (src/vbd_solver.py:336-343) */

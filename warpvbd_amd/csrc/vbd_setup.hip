@@ -952,6 +952,12 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_HIP(hipMemcpyAsync(&ns, s->slot_off + s->n_swept, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
     s->n_slots = ns;
+    if (ns >= (1 << 26) || n >= (1 << 26)) {
+        // the sweep addresses records, dictionary entries and vertices with
+        // 32-bit byte offsets
+        set_error("mesh too large: at most 2^26 - 1 vertices and vertex-tet incidences (16.7 M tets)");
+        return VBD_EMESH;
+    }
     VBD_TRY(dmalloc(&s->mass, n));
     VBD_TRY(dmalloc(&s->active, n));
     k_vertex_mass<<<nblk(n), kTB, 0, st>>>(s->perm, s->inc_off, s->inc, d_mass, n, s->mass, d_act_orig, s->active);

@@ -73,6 +73,15 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// &base[i] for a 32-bit unsigned element index (arrays here hold fewer than
+// 2^26 float4: create rejects larger meshes), formed as base + 16 i in 32 bits
+// so that a load can take the base from scalar registers and the offset from
+// one vector register.
+__device__ __forceinline__ const float4 *at(const float4 *base, uint32_t i)
+{
+    return reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + (uint32_t)(i << 4));
+}
+
 // A vertex's own data, loaded before anything that depends on other loads so
 // that a kernel can request it ahead of its early-exit test.
 struct VertexPre {
@@ -157,14 +166,18 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             int s = s0 + lane;
             float4 c = s < s1 ? a.slots[s] : make_float4(0.f, 0.f, 0.f, 0.f);
             for (; s < s1; s += L) {
-                const float4 *d = a.dict + 3 * (size_t)__float_as_int(c.w);
+                // 32-bit unsigned byte offsets from the arrays' (uniform) bases:
+                // address = base + offset, no 64-bit index arithmetic per load
+                // entry * 48 bytes in 32 bits (create admits fewer than 2^26 slots)
+                const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) +
+                                                                   __float_as_uint(c.w) * 48u);
                 const float4 d0 = d[0], d1 = d[1], d2 = d[2];
-                const float4 x0 = a.u[__float_as_int(c.x)];
-                const float4 x1 = a.u[__float_as_int(c.y)];
-                const float4 x2 = a.u[__float_as_int(c.z)];
+                const float4 x0 = *at(a.u, __float_as_uint(c.x));
+                const float4 x1 = *at(a.u, __float_as_uint(c.y));
+                const float4 x2 = *at(a.u, __float_as_uint(c.z));
                 // keep the request for the next record behind this slot's loads
                 __builtin_amdgcn_sched_barrier(0);
-                const float4 cn = a.slots[min(s + L, s1 - 1)];  // the last one reloads a record in cache
+                const float4 cn = *at(a.slots, (uint32_t)min(s + L, s1 - 1));  // the last one reloads a record in cache
                 __builtin_amdgcn_sched_barrier(0);
                 slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
                                make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y),
