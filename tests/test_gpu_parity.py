@@ -1,13 +1,14 @@
 """GPU parity tests: the HIP solver (through the C ABI) against the FP64 CPU
 oracle and the committed golden trajectories.
 
-Gates (FP32 solver vs FP64 oracle):
+Gates (GPU solver: FP32 slot arithmetic, FP64 vertex state; vs the FP64
+oracle):
 * colouring, colour groups and vertex->element adjacency: bit-exact;
 * positions at a fixed iteration count: within 1e-6 x bounding-box diagonal,
   every step;
 * tolerance-terminated runs (tol 1e-7): per-step iteration counts within +-1 of
-  the oracle's, tip-displacement curve within 1e-4 of the curve's range plus
-  the accumulated effect of the +-1 iteration differences (stated per test);
+  the oracle's (at least 98% equal), tip-displacement curve within 1e-4 of the
+  curve's range;
 * exact cases: free fall, pinned vertices.
 """
 import glob
@@ -187,13 +188,15 @@ def test_tolerance_cantilever_10x2x2_matches_golden():
     it_g, z_g = np.array(it_g), np.array(z_g)
     it_r, z_r = d["iters"], d["tipz"]
     assert np.all(np.abs(it_g - it_r) <= 1)
-    assert np.mean(it_g == it_r) >= 0.95
+    assert np.mean(it_g == it_r) >= 0.98
     disp_g, disp_r = z_g - z_r[0], z_r - z_r[0]
     rel = np.abs(disp_g - disp_r).max() / np.abs(disp_r).max()
-    print(f"iterations {it_g.sum()} vs {it_r.sum()}; tip curve max rel err {rel:.2e}")
-    # tol 1e-7 allows each step's solution to differ by O(tol); a +-1 iteration
-    # difference at a borderline step propagates through the velocity.
-    assert rel < 5e-4
+    print(f"iterations {it_g.sum()} vs {it_r.sum()} ({np.sum(it_g != it_r)} steps differ); "
+          f"tip curve max rel err {rel:.2e}")
+    # tol 1e-7 allows each step's solution to differ by O(tol): a step whose
+    # last metric is within rounding of the tolerance can take one iteration
+    # more or less than in FP64 (measured: 3 of 300 steps, 3.3e-5).
+    assert rel < 1e-4
 
 
 def test_tolerance_mass_spring_matches_golden():
@@ -208,7 +211,10 @@ def test_tolerance_mass_spring_matches_golden():
     assert np.all(np.abs(np.array(it_g) - d["iters"]) <= 1)
     disp_r = d["tipz"] - d["tipz"][0]
     rel = np.abs((np.array(z_g) - d["tipz"][0]) - disp_r).max() / np.abs(disp_r).max()
-    assert rel < 5e-4
+    print(f"iterations {np.sum(it_g)} vs {d['iters'].sum()} ({np.sum(np.array(it_g) != d['iters'])} steps differ); "
+          f"tip curve max rel err {rel:.2e}")
+    assert np.mean(np.array(it_g) == d["iters"]) >= 0.98
+    assert rel < 1e-4
 
 
 def test_golden_fixed_iteration_fixtures():

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_check_bodies(const int2 *ranges, int C,
     }
 }
 
-__global__ void k_init_bodies(int n, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt, const float *mass,
+__global__ void k_init_bodies(int n, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
                               const uint8_t *active, const int32_t *body_new, const float4 *grav, float dt, int nb,
                               BodyState *bs, int32_t *n_live)
 {
@@ -99,12 +99,12 @@ __global__ void k_init_bodies(int n, float4 *u, const float4 *uprev, const float
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body_new[i]]);
 }
 
-__global__ void k_finish_all(int n, const float4 *u, float4 *uprev, float4 *vel, float inv_dt)
+__global__ void k_finish_all(int n, const double4 *u, double4 *uprev, float4 *vel, float dt)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 x = u[i], p = uprev[i];
-    vel[i] = make_float4((x.x - p.x) * inv_dt, (x.y - p.y) * inv_dt, (x.z - p.z) * inv_dt, 0.f);
+    const double4 x = u[i], p = uprev[i];
+    vel[i] = step_velocity(x, p, dt);
     uprev[i] = x;
 }
 
@@ -356,7 +356,7 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
         }
         chunk = s->launch.chunk > 0 ? s->launch.chunk : 2;
     }
-    k_finish_all<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->n, s->u, s->uprev, s->vel, (float)(1.0 / dt));
+    k_finish_all<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->n, s->u, s->uprev, s->vel, (float)dt);
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipMemcpyAsync(s->bstate_host, s->bstate, sizeof(BodyState) * s->n_bodies, hipMemcpyDeviceToHost,
                            s->stream));
@@ -449,18 +449,19 @@ void release_batch(vbd_solver *s)
 int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int trial_steps, double *best_ms)
 {
     const int n = s->n;
-    float4 *save_u = nullptr, *save_v = nullptr, *save_t = nullptr;
-    VBD_HIP(hipMalloc((void **)&save_u, sizeof(float4) * n));
+    double4 *save_u = nullptr, *save_t = nullptr;
+    float4 *save_v = nullptr;
+    VBD_HIP(hipMalloc((void **)&save_u, sizeof(double4) * n));
     VBD_HIP(hipMalloc((void **)&save_v, sizeof(float4) * n));
-    if (hipMalloc((void **)&save_t, sizeof(float4) * n) != hipSuccess) {
+    if (hipMalloc((void **)&save_t, sizeof(double4) * n) != hipSuccess) {
         (void)hipFree(save_u);
         (void)hipFree(save_v);
         set_error("hipMalloc failed (autotune)");
         return VBD_ENOMEM;
     }
-    VBD_HIP(hipMemcpyAsync(save_u, s->uprev, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
+    VBD_HIP(hipMemcpyAsync(save_u, s->uprev, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
     VBD_HIP(hipMemcpyAsync(save_v, s->vel, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
-    VBD_HIP(hipMemcpyAsync(save_t, s->tgt, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream));
+    VBD_HIP(hipMemcpyAsync(save_t, s->tgt, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream));
     const bool keep_hist = s->hist_valid;
     const double keep_hist_dt = s->hist_dt;
     const int64_t keep_steps = s->step_serial;
@@ -485,7 +486,7 @@ int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int tria
         s->last_iters = 0;
         // restore the start state, one untimed step, then the timed ones
         for (int rep = 0; rep < 2 && rc == 0; ++rep) {
-            (void)hipMemcpyAsync(s->uprev, save_u, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+            (void)hipMemcpyAsync(s->uprev, save_u, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream);
             (void)hipMemcpyAsync(s->vel, save_v, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
             const int steps = rep == 0 ? 1 : trial_steps;
             auto t0 = std::chrono::steady_clock::now();
@@ -508,9 +509,9 @@ int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int tria
         s->batch_mode = keep_mode;
         s->launch = keep;
     }
-    (void)hipMemcpyAsync(s->uprev, save_u, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+    (void)hipMemcpyAsync(s->uprev, save_u, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream);
     (void)hipMemcpyAsync(s->vel, save_v, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
-    (void)hipMemcpyAsync(s->tgt, save_t, sizeof(float4) * n, hipMemcpyDeviceToDevice, s->stream);
+    (void)hipMemcpyAsync(s->tgt, save_t, sizeof(double4) * n, hipMemcpyDeviceToDevice, s->stream);
     (void)hipStreamSynchronize(s->stream);
     (void)hipFree(save_u);
     (void)hipFree(save_v);

@@ -37,7 +37,7 @@ void vbd_destroy(vbd_solver_t *s)
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict};
     for (void *p : dev)
         if (p) (void)hipFree(p);
-    for (float4 *p : s->snap)
+    for (double4 *p : s->snap)
         if (p) (void)hipFree(p);
     if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
     if (s->xio_host) (void)hipHostFree(s->xio_host);
@@ -242,9 +242,9 @@ int vbd_reset(vbd_solver_t *s)
 {
     CHECK_ARG(s, "null handle");
     VBD_HIP(hipSetDevice(s->device));
-    VBD_HIP(hipMemsetAsync(s->uprev, 0, sizeof(float4) * s->n, s->stream));
+    VBD_HIP(hipMemsetAsync(s->uprev, 0, sizeof(double4) * s->n, s->stream));
     VBD_HIP(hipMemsetAsync(s->vel, 0, sizeof(float4) * s->n, s->stream));
-    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(float4) * s->n, s->stream));
+    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(double4) * s->n, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
     s->last_iters = 0;
     s->hist_valid = false;
@@ -340,8 +340,8 @@ int vbd_snapshot(vbd_solver_t *s, int32_t slot)
     CHECK_ARG(s, "null handle");
     CHECK_ARG(slot >= 0 && slot < VBD_MAX_SNAPSHOTS, "snapshot slot out of range");
     VBD_HIP(hipSetDevice(s->device));
-    if (!s->snap[slot]) VBD_HIP(hipMalloc((void **)&s->snap[slot], sizeof(float4) * s->n));
-    VBD_HIP(hipMemcpyAsync(s->snap[slot], s->uprev, sizeof(float4) * s->n, hipMemcpyDeviceToDevice, s->stream));
+    if (!s->snap[slot]) VBD_HIP(hipMalloc((void **)&s->snap[slot], sizeof(double4) * s->n));
+    VBD_HIP(hipMemcpyAsync(s->snap[slot], s->uprev, sizeof(double4) * s->n, hipMemcpyDeviceToDevice, s->stream));
     return 0;
 }
 
@@ -358,7 +358,7 @@ int vbd_load_snapshot(vbd_solver_t *s, int32_t slot)
     CHECK_ARG(s, "null handle");
     CHECK_ARG(slot >= 0 && slot < VBD_MAX_SNAPSHOTS && s->snap[slot], "empty snapshot slot");
     VBD_HIP(hipSetDevice(s->device));
-    VBD_HIP(hipMemcpyAsync(s->uprev, s->snap[slot], sizeof(float4) * s->n, hipMemcpyDeviceToDevice, s->stream));
+    VBD_HIP(hipMemcpyAsync(s->uprev, s->snap[slot], sizeof(double4) * s->n, hipMemcpyDeviceToDevice, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
     s->hist_valid = false;
     s->step_serial += 2;

@@ -144,10 +144,13 @@ struct vbd_solver {
     int32_t *perm = nullptr;      // new -> original
     int32_t *iperm = nullptr;     // original -> new
     double *X = nullptr;          // rest positions, original order, f64 [n*3]
-    float4 *u = nullptr;          // displacement x - X
-    float4 *uprev = nullptr;      // displacement at the start of the step
+    // Vertex state in FP64 (the per-slot arithmetic is FP32): an FP32
+    // displacement has a 1e-9 m ulp at 1 cm, which moves a tolerance-
+    // terminated trajectory off the FP64 one by iteration-count flips.
+    double4 *u = nullptr;         // displacement x - X
+    double4 *uprev = nullptr;     // displacement at the start of the step
     float4 *vel = nullptr;        // velocity
-    float4 *tgt = nullptr;        // xyz: inertia target (x_prev + dt v + dt^2 g) - X ; w: M_v / dt^2
+    double4 *tgt = nullptr;       // xyz: inertia target (x_prev + dt v + dt^2 g) - X ; w: M_v / dt^2
     float *mass = nullptr;        // lumped vertex mass sum_e m_e / 4
     uint8_t *active = nullptr;    // active flag
     int32_t *slot_off = nullptr;  // [n_swept+1]
@@ -166,7 +169,7 @@ struct vbd_solver {
     vbd::Ctrl *ctrl_host = nullptr;  // pinned
     double *xio = nullptr;        // f64 [n*3] staging for host transfers (device)
     double *xio_host = nullptr;   // pinned f64 [n*3]
-    float4 *snap[VBD_MAX_SNAPSHOTS] = {};
+    double4 *snap[VBD_MAX_SNAPSHOTS] = {};
 
     vbd::Launch launch;
     int last_iters = 0;
@@ -190,12 +193,12 @@ int color_tets_device(const int32_t *d_tets, int m, int n, int32_t *d_colors, in
 int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_t *d_colors, int *ncolors,
                        hipStream_t stream);
 int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric);
-int export_positions(vbd_solver *s, const float4 *src, double *host_out);
-int import_positions(vbd_solver *s, const double *host_in, float4 *dst);
+int export_positions(vbd_solver *s, const double4 *src, double *host_out);
+int import_positions(vbd_solver *s, const double *host_in, double4 *dst);
 int import_velocities(vbd_solver *s, const double *host_in);
 // device f64 [n*3] buffers in original vertex order, enqueued on s->stream
-int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out);
-int import_positions_dev(vbd_solver *s, const double *dev_in, float4 *dst);
+int export_positions_dev(vbd_solver *s, const double4 *src, double *dev_out);
+int import_positions_dev(vbd_solver *s, const double *dev_in, double4 *dst);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);

@@ -528,14 +528,15 @@ struct BodyFeat {
     const float4 *grav;
 };
 
-__device__ __forceinline__ float3 inertia_disp(int i, const float4 *uprev, const float4 *vel, const uint8_t *active,
+__device__ __forceinline__ float3 inertia_disp(int i, const double4 *uprev, const float4 *vel, const uint8_t *active,
                                                float dt, float4 g)
 {
-    const float4 u = uprev[i];
-    if (!active[i]) return make_float3(u.x, u.y, u.z);
+    const double4 u = uprev[i];
+    if (!active[i]) return make_float3((float)u.x, (float)u.y, (float)u.z);
     const float4 v = vel[i];
-    const float dt2 = dt * dt;
-    return make_float3(u.x + dt * v.x + dt2 * g.x, u.y + dt * v.y + dt2 * g.y, u.z + dt * v.z + dt2 * g.z);
+    const double h = dt, h2 = h * h;
+    return make_float3((float)(u.x + h * v.x + h2 * g.x), (float)(u.y + h * v.y + h2 * g.y),
+                       (float)(u.z + h * v.z + h2 * g.z));
 }
 
 // Per-vertex input features, written in the network's (original) vertex
@@ -550,7 +551,7 @@ __device__ __forceinline__ float3 inertia_disp(int i, const float4 *uprev, const
 // Features 12..20, a local deformation estimate from the neighbours' inertia
 // predictions, follow in k_features_neighbours from U (the inertia
 // displacement in network order).
-__global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, const float4 *uprev,
+__global__ void k_features_vertex(int n, const int32_t *perm, const float4 *Xr, const double4 *uprev,
                                   const float4 *vel, const uint8_t *active, FeatParams p, BodyFeat bf,
                                   const float4 *dxj, int nsw, const float4 *cprev, float hs, const float4 *cprev2,
                                   float hs2, float4 *U, uint16_t *f)
@@ -629,7 +630,7 @@ __global__ void k_features_neighbours(int n, const float4 *Xr, const float4 *U, 
 
 // The plain initial guess (inertia prediction, per-body gravity) in u and the
 // inertia target in tgt, as the step's own init writes them.
-__global__ void k_plain_guess(int n, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt,
+__global__ void k_plain_guess(int n, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
                               const float *mass, const uint8_t *active, const int32_t *body, const float4 *grav,
                               float dt)
 {
@@ -639,52 +640,54 @@ __global__ void k_plain_guess(int n, float4 *u, const float4 *uprev, const float
 
 // The previous step's converged correction, read before the step's init
 // overwrites tgt: uprev (that step's result) - tgt (its inertia target).
-__global__ void k_history(int n, const float4 *uprev, const float4 *tgt, const uint8_t *active, float4 *c)
+__global__ void k_history(int n, const double4 *uprev, const double4 *tgt, const uint8_t *active, float4 *c)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 x = uprev[i], y = tgt[i];
-    c[i] = active[i] ? make_float4(x.x - y.x, x.y - y.y, x.z - y.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double4 x = uprev[i], y = tgt[i];
+    c[i] = active[i] ? make_float4((float)(x.x - y.x), (float)(x.y - y.y), (float)(x.z - y.z), 0.f)
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // training target of the step just taken, in network order:
 // (x_{n+1} - x_inertia - hs c_prev) / s
-__global__ void k_target(int n, const int32_t *perm, const float4 *u, const float4 *tgt, const uint8_t *active,
+__global__ void k_target(int n, const int32_t *perm, const double4 *u, const double4 *tgt, const uint8_t *active,
                          const float4 *cprev, float hs, float inv_s, float *t)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 x = u[i], y = tgt[i];
+    const double4 x = u[i], y = tgt[i];
     const float4 c = hs > 0.f ? cprev[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool a = active[i];
     float *o = t + (size_t)perm[i] * kFout;
-    o[0] = a ? (x.x - y.x - hs * c.x) * inv_s : 0.f;
-    o[1] = a ? (x.y - y.y - hs * c.y) * inv_s : 0.f;
-    o[2] = a ? (x.z - y.z - hs * c.z) * inv_s : 0.f;
+    o[0] = a ? ((float)(x.x - y.x) - hs * c.x) * inv_s : 0.f;
+    o[1] = a ? ((float)(x.y - y.y) - hs * c.y) * inv_s : 0.f;
+    o[2] = a ? ((float)(x.z - y.z) - hs * c.z) * inv_s : 0.f;
 }
 
 // warm guess (colour-major) for active vertices: x_inertia + hs c_prev
 // (+ s out[perm[i]] when out is given); pinned vertices keep x_inertia
-__global__ void k_guess(int n, const int32_t *perm, const float4 *tgt, const uint8_t *active, const float *out,
-                        float s, const float4 *cprev, float hs, float4 *dst)
+__global__ void k_guess(int n, const int32_t *perm, const double4 *tgt, const uint8_t *active, const float *out,
+                        float s, const float4 *cprev, float hs, double4 *dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 y = tgt[i];
+    const double4 y = tgt[i];
     if (!active[i]) {
-        dst[i] = make_float4(y.x, y.y, y.z, 0.f);
+        dst[i] = make_double4(y.x, y.y, y.z, 0.0);
         return;
     }
-    float3 g = make_float3(y.x, y.y, y.z);
+    // the correction (float) is formed first and added to the FP64 target once
+    float3 g = make_float3(0.f, 0.f, 0.f);
     if (hs > 0.f) {
         const float4 c = cprev[i];
-        g = make_float3(g.x + hs * c.x, g.y + hs * c.y, g.z + hs * c.z);
+        g = make_float3(hs * c.x, hs * c.y, hs * c.z);
     }
     if (out) {
         const float *o = out + (size_t)perm[i] * kFout;
         g = make_float3(g.x + s * o[0], g.y + s * o[1], g.z + s * o[2]);
     }
-    dst[i] = make_float4(g.x, g.y, g.z, 0.f);
+    dst[i] = make_double4(y.x + g.x, y.y + g.y, y.z + g.z, 0.0);
 }
 
 // The update a Jacobi pass from the positions in a.u would take at each swept
@@ -753,7 +756,7 @@ __global__ __launch_bounds__(256) void k_guess_max(const int2 *ranges, int C, co
 // then network; force = 1: the network's), written to u; choice[b] records
 // it (0 plain, 1 network, 2 history).
 __global__ __launch_bounds__(256) void k_guess_apply(const int2 *ranges, int C, const uint32_t *bm,
-                                                     const float4 *net, const float4 *hist, float4 *u,
+                                                     const double4 *net, const double4 *hist, double4 *u,
                                                      int32_t *choice, int force)
 {
     const int b = blockIdx.y;
@@ -772,7 +775,7 @@ __global__ __launch_bounds__(256) void k_guess_apply(const int2 *ranges, int C, 
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) choice[b] = use;
     if (use == 0) return;
-    const float4 *src = use == 1 ? net : hist;
+    const double4 *src = use == 1 ? net : hist;
     const int k0 = blockIdx.x * kGuessChunk;
     for (int k = k0 + threadIdx.x; k < k0 + kGuessChunk; k += blockDim.x) {
         const int i = body_chunk_vertex(r, C, k);
@@ -857,13 +860,13 @@ struct vbd_mpnn {
     uint16_t *F = nullptr;
     float *T = nullptr;
     // colour-major (solver order)
-    float4 *guess = nullptr;
+    double4 *guess = nullptr;
     uint32_t *rm0 = nullptr, *rm1 = nullptr, *rm2 = nullptr;  // first-sweep residuals: plain, network, history
-    float4 *hguess = nullptr;                                   // history guess x_inertia + hs c_prev
+    double4 *hguess = nullptr;                                   // history guess x_inertia + hs c_prev
     float4 *dxj = nullptr;                     // Jacobi update from the plain guess (a feature)
     float4 *cprev = nullptr;                   // previous step's converged correction
     float4 *cprev2 = nullptr;                  // the one of the step before
-    float4 *ytmp = nullptr;                    // inertia target of a guess formed outside a step
+    double4 *ytmp = nullptr;                    // inertia target of a guess formed outside a step
     float hs = 0.f, hs2 = 0.f;                 // (dt / dt of that step)^2, or 0: none
     bool have_c = false;                       // cprev holds the correction of step number c_step
     int64_t c_step = -1;
@@ -976,7 +979,7 @@ BodyFeat body_feat(vbd_mpnn *m)
     return BodyFeat{m->s->body_new, m->body_geo, m->body_mat, m->s->d_grav};
 }
 
-SweepArgs sweep_args_at(vbd_solver *s, float4 *u, const float4 *tgt, double dt)
+SweepArgs sweep_args_at(vbd_solver *s, double4 *u, const double4 *tgt, double dt)
 {
     SweepArgs a;
     a.u = u;
@@ -996,7 +999,7 @@ SweepArgs sweep_args_at(vbd_solver *s, float4 *u, const float4 *tgt, double dt)
 
 // Jacobi update from the guess in u against the inertia target tgt (and its
 // residual bits into res).
-void jacobi(vbd_mpnn *m, float4 *u, const float4 *tgt, double dt, float4 *dx, uint32_t *res)
+void jacobi(vbd_mpnn *m, double4 *u, const double4 *tgt, double dt, float4 *dx, uint32_t *res)
 {
     vbd_solver *s = m->s;
     constexpr int L = 4;  // as the big-colour sweeps
@@ -1015,8 +1018,8 @@ void features(vbd_mpnn *m, double dt, bool plain_in_u, uint16_t *f)
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     set_feature_params(m, dt);
-    float4 *u = s->u;
-    const float4 *tgt = s->tgt;
+    double4 *u = s->u;
+    const double4 *tgt = s->tgt;
     if (!plain_in_u) {
         u = m->guess;
         tgt = m->ytmp;

@@ -84,7 +84,7 @@ __global__ void k_check_empty(Ctrl *c, float tol)
     close_iteration(c, c->it, 0u, tol);
 }
 
-__global__ void k_step_init(int n, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt, const float *mass,
+__global__ void k_step_init(int n, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
                             const uint8_t *active, float dt, float4 g, Ctrl *ctrl)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,33 +98,32 @@ __global__ void k_step_init(int n, float4 *u, const float4 *uprev, const float4 
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, g);
 }
 
-__global__ void k_step_finish(int n, const float4 *u, float4 *uprev, float4 *vel, float inv_dt)
+__global__ void k_step_finish(int n, const double4 *u, double4 *uprev, float4 *vel, float dt)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 x = u[i], p = uprev[i];
-    vel[i] = make_float4((x.x - p.x) * inv_dt, (x.y - p.y) * inv_dt, (x.z - p.z) * inv_dt, 0.f);
+    const double4 x = u[i], p = uprev[i];
+    vel[i] = step_velocity(x, p, dt);
     uprev[i] = x;
 }
 
 // x (original order, f64) = X + u[iperm]
-__global__ void k_export(int n, const float4 *src, const int32_t *iperm, const double *X, double *out)
+__global__ void k_export(int n, const double4 *src, const int32_t *iperm, const double *X, double *out)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const float4 d = src[iperm[v]];
-    out[3 * v] = X[3 * v] + (double)d.x;
-    out[3 * v + 1] = X[3 * v + 1] + (double)d.y;
-    out[3 * v + 2] = X[3 * v + 2] + (double)d.z;
+    const double4 d = src[iperm[v]];
+    out[3 * v] = X[3 * v] + d.x;
+    out[3 * v + 1] = X[3 * v + 1] + d.y;
+    out[3 * v + 2] = X[3 * v + 2] + d.z;
 }
 
-__global__ void k_import(int n, const double *in, const int32_t *perm, const double *X, float4 *dst)
+__global__ void k_import(int n, const double *in, const int32_t *perm, const double *X, double4 *dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int v = perm[i];
-    dst[i] = make_float4((float)(in[3 * v] - X[3 * v]), (float)(in[3 * v + 1] - X[3 * v + 1]),
-                         (float)(in[3 * v + 2] - X[3 * v + 2]), 0.f);
+    dst[i] = make_double4(in[3 * v] - X[3 * v], in[3 * v + 1] - X[3 * v + 1], in[3 * v + 2] - X[3 * v + 2], 0.0);
 }
 
 __global__ void k_export_vec(int n, const float4 *src, const int32_t *iperm, double *out)
@@ -372,7 +371,7 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
         conv = s->ctrl_host->converged != 0;
         chunk = s->launch.chunk > 0 ? s->launch.chunk : 2;
     }
-    k_step_finish<<<nblk(n, 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, (float)(1.0 / dt));
+    k_step_finish<<<nblk(n, 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, (float)dt);
     VBD_HIP(hipGetLastError());
     const int it = s->ctrl_host->iters;
     s->last_iters = it;
@@ -405,7 +404,7 @@ int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, 
     return rc;
 }
 
-int export_positions(vbd_solver *s, const float4 *src, double *host_out)
+int export_positions(vbd_solver *s, const double4 *src, double *host_out)
 {
     const int n = s->n;
     k_export<<<nblk(n, 256), 256, 0, s->stream>>>(n, src, s->iperm, s->X, s->xio);
@@ -416,7 +415,7 @@ int export_positions(vbd_solver *s, const float4 *src, double *host_out)
     return 0;
 }
 
-int import_positions(vbd_solver *s, const double *host_in, float4 *dst)
+int import_positions(vbd_solver *s, const double *host_in, double4 *dst)
 {
     const int n = s->n;
     std::copy(host_in, host_in + 3 * (size_t)n, s->xio_host);
@@ -427,7 +426,7 @@ int import_positions(vbd_solver *s, const double *host_in, float4 *dst)
     return 0;
 }
 
-int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out)
+int export_positions_dev(vbd_solver *s, const double4 *src, double *dev_out)
 {
     const int n = s->n;
     k_export<<<nblk(n, 256), 256, 0, s->stream>>>(n, src, s->iperm, s->X, dev_out);
@@ -435,7 +434,7 @@ int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out)
     return 0;
 }
 
-int import_positions_dev(vbd_solver *s, const double *dev_in, float4 *dst)
+int import_positions_dev(vbd_solver *s, const double *dev_in, double4 *dst)
 {
     const int n = s->n;
     k_import<<<nblk(n, 256), 256, 0, s->stream>>>(n, dev_in, s->perm, s->X, dst);

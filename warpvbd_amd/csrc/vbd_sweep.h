@@ -17,9 +17,9 @@ inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
 #endif
 
 struct SweepArgs {
-    float4 *__restrict__ u;
-    const float4 *__restrict__ tgt;
-    const float4 *__restrict__ uprev;
+    double4 *__restrict__ u;
+    const double4 *__restrict__ tgt;
+    const double4 *__restrict__ uprev;
     const int32_t *__restrict__ slot_off;
     const float4 *__restrict__ slots;   // compact slot records (ids, entry), or tet records (VBD_TET_RECORDS)
     const float2 *__restrict__ slots2;  // unused (null)
@@ -82,12 +82,24 @@ __device__ __forceinline__ const float4 *at(const float4 *base, uint32_t i)
     return reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + (uint32_t)(i << 4));
 }
 
+__device__ __forceinline__ const double4 *at(const double4 *base, uint32_t i)
+{
+    return reinterpret_cast<const double4 *>(reinterpret_cast<const char *>(base) + (uint32_t)(i << 5));
+}
+
+// x - us, the displacement of a tet vertex relative to the swept vertex: the
+// difference is formed in FP64 and rounded once to FP32 for the slot arithmetic.
+__device__ __forceinline__ float3 rel(const double4 &x, const double3 &us)
+{
+    return make_float3((float)(x.x - us.x), (float)(x.y - us.y), (float)(x.z - us.z));
+}
+
 // A vertex's own data, loaded before anything that depends on other loads so
 // that a kernel can request it ahead of its early-exit test.
 struct VertexPre {
     int s0, s1;  // slot range (empty for lanes without a vertex)
-    float4 uv;   // current displacement
-    float4 tg;   // inertia target, M/dt^2
+    double4 uv;  // current displacement
+    double4 tg;  // inertia target, M/dt^2
 };
 
 __device__ __forceinline__ VertexPre vertex_pre(const SweepArgs &a, int v, bool valid)
@@ -107,13 +119,21 @@ __device__ __forceinline__ VertexPre vertex_pre(const SweepArgs &a, int v, bool 
 // the update dx (zero for lanes without a vertex).  us receives the vertex's
 // current displacement.
 template <int L, int B, int MAT>
-__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, float3 &us,
+__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, double3 &us,
                                             const VertexPre &pre)
 {
     Acc acc;
     acc_zero(acc);
-    const float4 uv4 = pre.uv;
-    us = make_float3(uv4.x, uv4.y, uv4.z);
+    const double4 uv4 = pre.uv;
+    us = make_double3(uv4.x, uv4.y, uv4.z);
+    // the inertia term needs only x - t and M/dt^2: formed here in FP64, so
+    // the slot loop carries four FP32 values instead of the FP64 target
+    const double4 tg = pre.tg;
+    float w = (float)tg.w;
+    float t0 = (float)(us.x - tg.x), t1 = (float)(us.y - tg.y), t2 = (float)(us.z - tg.z);
+    // an empty asm that reads and writes them keeps the compiler from sinking
+    // the subtraction (and the FP64 target with it) past the loop
+    asm volatile("" : "+v"(t0), "+v"(t1), "+v"(t2), "+v"(w));
     if (valid) {
         const int s0 = pre.s0, s1 = pre.s1;
 #if VBD_TET_RECORDS
@@ -121,7 +141,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         for (int s = s0 + lane; s < s1; s += B * L) {
             int e[B];
             float4 q[B][4];
-            float4 x[B][3];
+            double4 x[B][3];
 #pragma unroll
             for (int j = 0; j < B; ++j)
                 if (s + j * L < s1) e[j] = a.ent[s + j * L];
@@ -150,11 +170,11 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
-                    const float4 *xx = x[j];
+                    const double4 *xx = x[j];
                     slot_eval<MAT>(ka[j], kb[j], kc[j], bs[j], b0[j], b1[j],
-                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
-                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
-                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
+                                   rel(xx[0], us),
+                                   rel(xx[1], us),
+                                   rel(xx[2], us), acc);
                 }
             }
         }
@@ -172,18 +192,18 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                 const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) +
                                                                    __float_as_uint(c.w) * 48u);
                 const float4 d0 = d[0], d1 = d[1], d2 = d[2];
-                const float4 x0 = *at(a.u, __float_as_uint(c.x));
-                const float4 x1 = *at(a.u, __float_as_uint(c.y));
-                const float4 x2 = *at(a.u, __float_as_uint(c.z));
+                const double4 x0 = *at(a.u, __float_as_uint(c.x));
+                const double4 x1 = *at(a.u, __float_as_uint(c.y));
+                const double4 x2 = *at(a.u, __float_as_uint(c.z));
                 // keep the request for the next record behind this slot's loads
                 __builtin_amdgcn_sched_barrier(0);
                 const float4 cn = *at(a.slots, (uint32_t)min(s + L, s1 - 1));  // the last one reloads a record in cache
                 __builtin_amdgcn_sched_barrier(0);
                 slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
                                make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y),
-                               make_float3(x0.x - us.x, x0.y - us.y, x0.z - us.z),
-                               make_float3(x1.x - us.x, x1.y - us.y, x1.z - us.z),
-                               make_float3(x2.x - us.x, x2.y - us.y, x2.z - us.z), acc);
+                               rel(x0, us),
+                               rel(x1, us),
+                               rel(x2, us), acc);
                 c = cn;
             }
         } else
@@ -192,7 +212,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         for (int s = s0 + lane; s < s1; s += B * L) {
             float4 q[B][3];
             float2 qd[B];
-            float4 x[B][3];
+            double4 x[B][3];
             // compact record (ids, entry), then the entry's constants
 #pragma unroll
             for (int j = 0; j < B; ++j) {
@@ -219,12 +239,12 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
                     const float4 *qq = q[j];
-                    const float4 *xx = x[j];
+                    const double4 *xx = x[j];
                     slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), make_float3(qq[1].x, qq[1].y, qq[1].z),
                                    make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[2].w, qd[j].x, qd[j].y),
-                                   make_float3(xx[0].x - us.x, xx[0].y - us.y, xx[0].z - us.z),
-                                   make_float3(xx[1].x - us.x, xx[1].y - us.y, xx[1].z - us.z),
-                                   make_float3(xx[2].x - us.x, xx[2].y - us.y, xx[2].z - us.z), acc);
+                                   rel(xx[0], us),
+                                   rel(xx[1], us),
+                                   rel(xx[2], us), acc);
                 }
             }
         }
@@ -234,8 +254,8 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
     if (!valid) return make_float3(0.f, 0.f, 0.f);
     if (a.kd_dt != 0.f) {
         // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
-        const float4 up = a.uprev[v];
-        const float e0 = us.x - up.x, e1 = us.y - up.y, e2 = us.z - up.z;
+        const double4 up = a.uprev[v];
+        const float e0 = (float)(us.x - up.x), e1 = (float)(us.y - up.y), e2 = (float)(us.z - up.z);
         const float k = a.kd_dt;
         acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
         acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
@@ -244,11 +264,9 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
     }
     // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
-    const float4 tg = pre.tg;
-    const float w = tg.w;
-    acc.g0 += w * (us.x - tg.x);
-    acc.g1 += w * (us.y - tg.y);
-    acc.g2 += w * (us.z - tg.z);
+    acc.g0 += w * t0;
+    acc.g1 += w * t1;
+    acc.g2 += w * t2;
     acc.h00 += w;
     acc.h11 += w;
     acc.h22 += w;
@@ -259,7 +277,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
 // Evaluate and apply one vertex update for a lane group.  Returns the metric
 // bits (0 for lanes without a vertex).
 template <int L, int B, int MAT>
-__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, float3 &us)
+__device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, double3 &us)
 {
     return vertex_dx<L, B, MAT>(a, v, valid, lane, us, vertex_pre(a, v, valid));
 }
@@ -268,20 +286,20 @@ template <int L, int B, int MAT>
 __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane,
                                                   const VertexPre &pre)
 {
-    float3 us;
+    double3 us;
     const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us, pre);
     if (!valid) return 0u;
-    if (lane == 0) a.u[v] = make_float4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.f);
+    if (lane == 0) a.u[v] = make_double4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.0);
     return metric_bits(dx);
 }
 
 template <int L, int B, int MAT>
 __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane)
 {
-    float3 us;
+    double3 us;
     const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us);
     if (!valid) return 0u;
-    if (lane == 0) a.u[v] = make_float4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.f);
+    if (lane == 0) a.u[v] = make_double4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.0);
     return metric_bits(dx);
 }
 
@@ -289,7 +307,7 @@ __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, boo
 template <int L, int B, int MAT>
 __device__ __forceinline__ float3 eval_vertex(const SweepArgs &a, int v, bool valid, int lane)
 {
-    float3 us;
+    double3 us;
     return vertex_dx<L, B, MAT>(a, v, valid, lane, us);
 }
 
@@ -305,22 +323,30 @@ struct StepParams {
     int skip_init;  // the initial guess is already in u (warm start)
 };
 
-__device__ __forceinline__ void init_vertex(int i, float4 *u, const float4 *uprev, const float4 *vel, float4 *tgt,
+__device__ __forceinline__ void init_vertex(int i, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
                                             const float *mass, const uint8_t *active, float dt, float4 g)
 {
-    const float4 p = uprev[i];
+    const double4 p = uprev[i];
     if (active[i]) {
         const float4 v = vel[i];
-        const float dt2 = dt * dt;
+        const double h = dt, h2 = h * h;
         // explicit fused forms: every kernel that inlines this rounds alike
-        const float4 t = make_float4(fmaf(dt2, g.x, fmaf(dt, v.x, p.x)), fmaf(dt2, g.y, fmaf(dt, v.y, p.y)),
-                                     fmaf(dt2, g.z, fmaf(dt, v.z, p.z)), mass[i] / dt2);
-        u[i] = make_float4(t.x, t.y, t.z, 0.f);
+        const double4 t = make_double4(fma(h2, (double)g.x, fma(h, (double)v.x, p.x)),
+                                       fma(h2, (double)g.y, fma(h, (double)v.y, p.y)),
+                                       fma(h2, (double)g.z, fma(h, (double)v.z, p.z)), (double)mass[i] / h2);
+        u[i] = make_double4(t.x, t.y, t.z, 0.0);
         tgt[i] = t;
     } else {
         u[i] = p;
-        tgt[i] = make_float4(p.x, p.y, p.z, 0.f);
+        tgt[i] = make_double4(p.x, p.y, p.z, 0.0);
     }
+}
+
+// v = (x - x_prev) / dt from the FP64 displacements, stored in FP32
+__device__ __forceinline__ float4 step_velocity(const double4 x, const double4 q, float dt)
+{
+    const double r = 1.0 / (double)dt;
+    return make_float4((float)((x.x - q.x) * r), (float)((x.y - q.y) * r), (float)((x.z - q.z) * r), 0.f);
 }
 
 // A whole time step of one body in one workgroup: initial guess, colour
@@ -331,7 +357,7 @@ __device__ __forceinline__ void init_vertex(int i, float4 *u, const float4 *upre
 // colour ranges then the range of its other vertices, as (start, count)).
 template <int L, int B, int MAT>
 __global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 *ranges, int C, const float4 *grav,
-                                                        const float *mass, const uint8_t *active, float4 *uprev,
+                                                        const float *mass, const uint8_t *active, double4 *uprev,
                                                         float4 *vel, StepParams p, int2 *out)
 {
     __shared__ uint32_t s_bits;
@@ -341,7 +367,7 @@ __global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 
         for (int c = 0; c <= C; ++c) {
             const int2 rc = r[c];
             for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x)
-                init_vertex(i, a.u, uprev, vel, const_cast<float4 *>(a.tgt), mass, active, p.dt, g);
+                init_vertex(i, a.u, uprev, vel, const_cast<double4 *>(a.tgt), mass, active, p.dt, g);
         }
         __syncthreads();
     }
@@ -371,8 +397,8 @@ __global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 
     for (int c = 0; c <= C; ++c) {
         const int2 rc = r[c];
         for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) {
-            const float4 x = a.u[i], q = uprev[i];
-            vel[i] = make_float4((x.x - q.x) * p.inv_dt, (x.y - q.y) * p.inv_dt, (x.z - q.z) * p.inv_dt, 0.f);
+            const double4 x = a.u[i], q = uprev[i];
+            vel[i] = step_velocity(x, q, p.dt);
             uprev[i] = x;
         }
     }
@@ -381,7 +407,7 @@ __global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 
 
 template <int L, int MAT>
 void launch_resident(const SweepArgs &a, const int2 *ranges, int C, const float4 *grav, const float *mass,
-                     const uint8_t *active, float4 *uprev, float4 *vel, StepParams p, int2 *out, int n_bodies,
+                     const uint8_t *active, double4 *uprev, float4 *vel, StepParams p, int2 *out, int n_bodies,
                      int block, hipStream_t st)
 {
     k_step_resident<L, Batch<L>::value, MAT><<<n_bodies, block, 0, st>>>(a, ranges, C, grav, mass, active, uprev, vel,
@@ -390,7 +416,7 @@ void launch_resident(const SweepArgs &a, const int2 *ranges, int C, const float4
 
 template <int MAT>
 void dispatch_resident(int L, const SweepArgs &a, const int2 *ranges, int C, const float4 *grav, const float *mass,
-                       const uint8_t *active, float4 *uprev, float4 *vel, StepParams p, int2 *out, int n_bodies,
+                       const uint8_t *active, double4 *uprev, float4 *vel, StepParams p, int2 *out, int n_bodies,
                        int block, hipStream_t st)
 {
     switch (L) {
