@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_check_bodies(const int2 *ranges, int C,
     }
 }
 
-__global__ void k_init_bodies(int n, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
+__global__ void k_init_bodies(int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
                               const uint8_t *active, const int32_t *body_new, const float4 *grav, float dt, int nb,
                               BodyState *bs, int32_t *n_live)
 {
@@ -99,11 +99,11 @@ __global__ void k_init_bodies(int n, double4 *u, const double4 *uprev, const flo
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body_new[i]]);
 }
 
-__global__ void k_finish_all(int n, const double4 *u, double4 *uprev, float4 *vel, float dt)
+__global__ void k_finish_all(int n, const float4 *u, double4 *uprev, float4 *vel, float dt)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double4 x = u[i], p = uprev[i];
+    const double4 x = unpack_disp4(u[i]), p = uprev[i];
     vel[i] = step_velocity(x, p, dt);
     uprev[i] = x;
 }

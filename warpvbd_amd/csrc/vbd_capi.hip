@@ -244,7 +244,7 @@ int vbd_reset(vbd_solver_t *s)
     VBD_HIP(hipSetDevice(s->device));
     VBD_HIP(hipMemsetAsync(s->uprev, 0, sizeof(double4) * s->n, s->stream));
     VBD_HIP(hipMemsetAsync(s->vel, 0, sizeof(float4) * s->n, s->stream));
-    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(double4) * s->n, s->stream));
+    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(float4) * s->n, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
     s->last_iters = 0;
     s->hist_valid = false;

@@ -144,10 +144,12 @@ struct vbd_solver {
     int32_t *perm = nullptr;      // new -> original
     int32_t *iperm = nullptr;     // original -> new
     double *X = nullptr;          // rest positions, original order, f64 [n*3]
-    // Vertex state in FP64 (the per-slot arithmetic is FP32): an FP32
+    // Vertex state beyond FP32 (the per-slot arithmetic is FP32): an FP32
     // displacement has a 1e-9 m ulp at 1 cm, which moves a tolerance-
-    // terminated trajectory off the FP64 one by iteration-count flips.
-    double4 *u = nullptr;         // displacement x - X
+    // terminated trajectory off the FP64 one by iteration-count flips.  The
+    // iterate u is packed to 16 bytes (one load per neighbour gather); the
+    // step's start and target, read once per vertex, are FP64.
+    float4 *u = nullptr;          // displacement x - X, packed (32 significant bits, pack_disp in vbd_sweep.h)
     double4 *uprev = nullptr;     // displacement at the start of the step
     float4 *vel = nullptr;        // velocity
     double4 *tgt = nullptr;       // xyz: inertia target (x_prev + dt v + dt^2 g) - X ; w: M_v / dt^2
@@ -194,10 +196,12 @@ int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_
                        hipStream_t stream);
 int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric);
 int export_positions(vbd_solver *s, const double4 *src, double *host_out);
+int export_positions(vbd_solver *s, const float4 *src, double *host_out);  // packed displacements
 int import_positions(vbd_solver *s, const double *host_in, double4 *dst);
 int import_velocities(vbd_solver *s, const double *host_in);
 // device f64 [n*3] buffers in original vertex order, enqueued on s->stream
 int export_positions_dev(vbd_solver *s, const double4 *src, double *dev_out);
+int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out);
 int import_positions_dev(vbd_solver *s, const double *dev_in, double4 *dst);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);

@@ -630,7 +630,7 @@ __global__ void k_features_neighbours(int n, const float4 *Xr, const float4 *U, 
 
 // The plain initial guess (inertia prediction, per-body gravity) in u and the
 // inertia target in tgt, as the step's own init writes them.
-__global__ void k_plain_guess(int n, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
+__global__ void k_plain_guess(int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
                               const float *mass, const uint8_t *active, const int32_t *body, const float4 *grav,
                               float dt)
 {
@@ -668,13 +668,13 @@ __global__ void k_target(int n, const int32_t *perm, const double4 *u, const dou
 // warm guess (colour-major) for active vertices: x_inertia + hs c_prev
 // (+ s out[perm[i]] when out is given); pinned vertices keep x_inertia
 __global__ void k_guess(int n, const int32_t *perm, const double4 *tgt, const uint8_t *active, const float *out,
-                        float s, const float4 *cprev, float hs, double4 *dst)
+                        float s, const float4 *cprev, float hs, float4 *dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 y = tgt[i];
     if (!active[i]) {
-        dst[i] = make_double4(y.x, y.y, y.z, 0.0);
+        dst[i] = pack_disp(y.x, y.y, y.z);
         return;
     }
     // the correction (float) is formed first and added to the FP64 target once
@@ -687,7 +687,7 @@ __global__ void k_guess(int n, const int32_t *perm, const double4 *tgt, const ui
         const float *o = out + (size_t)perm[i] * kFout;
         g = make_float3(g.x + s * o[0], g.y + s * o[1], g.z + s * o[2]);
     }
-    dst[i] = make_double4(y.x + g.x, y.y + g.y, y.z + g.z, 0.0);
+    dst[i] = pack_disp(y.x + g.x, y.y + g.y, y.z + g.z);
 }
 
 // The update a Jacobi pass from the positions in a.u would take at each swept
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(256) void k_guess_max(const int2 *ranges, int C, co
 // then network; force = 1: the network's), written to u; choice[b] records
 // it (0 plain, 1 network, 2 history).
 __global__ __launch_bounds__(256) void k_guess_apply(const int2 *ranges, int C, const uint32_t *bm,
-                                                     const double4 *net, const double4 *hist, double4 *u,
+                                                     const float4 *net, const float4 *hist, float4 *u,
                                                      int32_t *choice, int force)
 {
     const int b = blockIdx.y;
@@ -775,7 +775,7 @@ __global__ __launch_bounds__(256) void k_guess_apply(const int2 *ranges, int C, 
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) choice[b] = use;
     if (use == 0) return;
-    const double4 *src = use == 1 ? net : hist;
+    const float4 *src = use == 1 ? net : hist;
     const int k0 = blockIdx.x * kGuessChunk;
     for (int k = k0 + threadIdx.x; k < k0 + kGuessChunk; k += blockDim.x) {
         const int i = body_chunk_vertex(r, C, k);
@@ -860,9 +860,9 @@ struct vbd_mpnn {
     uint16_t *F = nullptr;
     float *T = nullptr;
     // colour-major (solver order)
-    double4 *guess = nullptr;
+    float4 *guess = nullptr;  // packed, as the solver's iterate
     uint32_t *rm0 = nullptr, *rm1 = nullptr, *rm2 = nullptr;  // first-sweep residuals: plain, network, history
-    double4 *hguess = nullptr;                                   // history guess x_inertia + hs c_prev
+    float4 *hguess = nullptr;                                    // history guess x_inertia + hs c_prev
     float4 *dxj = nullptr;                     // Jacobi update from the plain guess (a feature)
     float4 *cprev = nullptr;                   // previous step's converged correction
     float4 *cprev2 = nullptr;                  // the one of the step before
@@ -979,7 +979,7 @@ BodyFeat body_feat(vbd_mpnn *m)
     return BodyFeat{m->s->body_new, m->body_geo, m->body_mat, m->s->d_grav};
 }
 
-SweepArgs sweep_args_at(vbd_solver *s, double4 *u, const double4 *tgt, double dt)
+SweepArgs sweep_args_at(vbd_solver *s, float4 *u, const double4 *tgt, double dt)
 {
     SweepArgs a;
     a.u = u;
@@ -999,7 +999,7 @@ SweepArgs sweep_args_at(vbd_solver *s, double4 *u, const double4 *tgt, double dt
 
 // Jacobi update from the guess in u against the inertia target tgt (and its
 // residual bits into res).
-void jacobi(vbd_mpnn *m, double4 *u, const double4 *tgt, double dt, float4 *dx, uint32_t *res)
+void jacobi(vbd_mpnn *m, float4 *u, const double4 *tgt, double dt, float4 *dx, uint32_t *res)
 {
     vbd_solver *s = m->s;
     constexpr int L = 4;  // as the big-colour sweeps
@@ -1018,7 +1018,7 @@ void features(vbd_mpnn *m, double dt, bool plain_in_u, uint16_t *f)
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     set_feature_params(m, dt);
-    double4 *u = s->u;
+    float4 *u = s->u;
     const double4 *tgt = s->tgt;
     if (!plain_in_u) {
         u = m->guess;

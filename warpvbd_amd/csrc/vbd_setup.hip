@@ -1020,7 +1020,7 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_TRY(dmalloc(&s->ctrl, 1));
     VBD_HIP(hipHostMalloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
     VBD_HIP(hipHostMalloc((void **)&s->xio_host, sizeof(double) * 3 * std::max(n, 1), hipHostMallocDefault));
-    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(double4) * n, st));
+    VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(float4) * n, st));
     VBD_HIP(hipMemsetAsync(s->uprev, 0, sizeof(double4) * n, st));
     VBD_HIP(hipMemsetAsync(s->vel, 0, sizeof(float4) * n, st));
     VBD_HIP(hipMemsetAsync(s->tgt, 0, sizeof(double4) * n, st));

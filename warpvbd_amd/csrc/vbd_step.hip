@@ -84,7 +84,7 @@ __global__ void k_check_empty(Ctrl *c, float tol)
     close_iteration(c, c->it, 0u, tol);
 }
 
-__global__ void k_step_init(int n, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
+__global__ void k_step_init(int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
                             const uint8_t *active, float dt, float4 g, Ctrl *ctrl)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,21 +98,25 @@ __global__ void k_step_init(int n, double4 *u, const double4 *uprev, const float
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, g);
 }
 
-__global__ void k_step_finish(int n, const double4 *u, double4 *uprev, float4 *vel, float dt)
+__global__ void k_step_finish(int n, const float4 *u, double4 *uprev, float4 *vel, float dt)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double4 x = u[i], p = uprev[i];
+    const double4 x = unpack_disp4(u[i]), p = uprev[i];
     vel[i] = step_velocity(x, p, dt);
     uprev[i] = x;
 }
 
-// x (original order, f64) = X + u[iperm]
-__global__ void k_export(int n, const double4 *src, const int32_t *iperm, const double *X, double *out)
+__device__ __forceinline__ double4 load_disp(const double4 *p, int i) { return p[i]; }
+__device__ __forceinline__ double4 load_disp(const float4 *p, int i) { return unpack_disp4(p[i]); }
+
+// x (original order, f64) = X + u[iperm] (u in FP64 or packed)
+template <typename T>
+__global__ void k_export(int n, const T *src, const int32_t *iperm, const double *X, double *out)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const double4 d = src[iperm[v]];
+    const double4 d = load_disp(src, iperm[v]);
     out[3 * v] = X[3 * v] + d.x;
     out[3 * v + 1] = X[3 * v + 1] + d.y;
     out[3 * v + 2] = X[3 * v + 2] + d.z;
@@ -404,7 +408,8 @@ int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, 
     return rc;
 }
 
-int export_positions(vbd_solver *s, const double4 *src, double *host_out)
+template <typename T>
+static int export_positions_t(vbd_solver *s, const T *src, double *host_out)
 {
     const int n = s->n;
     k_export<<<nblk(n, 256), 256, 0, s->stream>>>(n, src, s->iperm, s->X, s->xio);
@@ -426,12 +431,24 @@ int import_positions(vbd_solver *s, const double *host_in, double4 *dst)
     return 0;
 }
 
-int export_positions_dev(vbd_solver *s, const double4 *src, double *dev_out)
+template <typename T>
+static int export_positions_dev_t(vbd_solver *s, const T *src, double *dev_out)
 {
     const int n = s->n;
     k_export<<<nblk(n, 256), 256, 0, s->stream>>>(n, src, s->iperm, s->X, dev_out);
     VBD_HIP(hipGetLastError());
     return 0;
+}
+
+int export_positions(vbd_solver *s, const double4 *src, double *host_out) { return export_positions_t(s, src, host_out); }
+int export_positions(vbd_solver *s, const float4 *src, double *host_out) { return export_positions_t(s, src, host_out); }
+int export_positions_dev(vbd_solver *s, const double4 *src, double *dev_out)
+{
+    return export_positions_dev_t(s, src, dev_out);
+}
+int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out)
+{
+    return export_positions_dev_t(s, src, dev_out);
 }
 
 int import_positions_dev(vbd_solver *s, const double *dev_in, double4 *dst)

@@ -17,7 +17,7 @@ inline int nblk(int64_t n, int tb) { return (int)((n + tb - 1) / tb); }
 #endif
 
 struct SweepArgs {
-    double4 *__restrict__ u;
+    float4 *__restrict__ u;  // packed displacements (pack_disp)
     const double4 *__restrict__ tgt;
     const double4 *__restrict__ uprev;
     const int32_t *__restrict__ slot_off;
@@ -82,23 +82,71 @@ __device__ __forceinline__ const float4 *at(const float4 *base, uint32_t i)
     return reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + (uint32_t)(i << 4));
 }
 
-__device__ __forceinline__ const double4 *at(const double4 *base, uint32_t i)
+// Packed displacement, one float4 per vertex (16 bytes, one load per gather):
+//   xyz  the FP32 values nearest to the displacement's components
+//   w    bits 0-7, 8-15, 16-23: the remainders x - xyz of x, y, z as integers
+//        in [-128, 127] (stored + 128) times a quantum q = 2^(E-31), E the
+//        exponent of the largest |xyz|; bits 24-31: q's biased FP32 exponent
+//        (0: q = 0, used when that largest |xyz| is below 2^-95)
+// so the displacement is held to 2^-8 of an FP32 ulp of its largest component
+// (32 significant bits).  The remainder of each component is at most half its
+// own ulp <= 128 q, so only an exact half-ulp remainder of the largest one is
+// clamped (by q).  All-zero bits are the zero displacement.
+__device__ __forceinline__ float4 pack_disp(double x, double y, double z)
 {
-    return reinterpret_cast<const double4 *>(reinterpret_cast<const char *>(base) + (uint32_t)(i << 5));
+    const float hx = (float)x, hy = (float)y, hz = (float)z;
+    const float m = fmaxf(fabsf(hx), fmaxf(fabsf(hy), fabsf(hz)));
+    const int em = (int)(__float_as_uint(m) >> 23);  // biased exponent of m
+    const int eb = em > 31 ? em - 31 : 0;            // biased exponent of q
+    uint32_t w = 0x808080u;
+    if (eb) {
+        // 1/q = 2^(127 - eb), as an FP64 power of two
+        const double iq = __longlong_as_double((long long)(1023 + 127 - eb) << 52);
+        const int qx = min(127, max(-128, __double2int_rn((x - (double)hx) * iq)));
+        const int qy = min(127, max(-128, __double2int_rn((y - (double)hy) * iq)));
+        const int qz = min(127, max(-128, __double2int_rn((z - (double)hz) * iq)));
+        w = (uint32_t)(qx + 128) | (uint32_t)(qy + 128) << 8 | (uint32_t)(qz + 128) << 16;
+    }
+    return make_float4(hx, hy, hz, __uint_as_float(w | (uint32_t)eb << 24));
 }
 
-// x - us, the displacement of a tet vertex relative to the swept vertex: the
-// difference is formed in FP64 and rounded once to FP32 for the slot arithmetic.
-__device__ __forceinline__ float3 rel(const double4 &x, const double3 &us)
+// the remainders of a packed displacement (exact in FP32)
+__device__ __forceinline__ float3 disp_lo(const float4 &p)
 {
-    return make_float3((float)(x.x - us.x), (float)(x.y - us.y), (float)(x.z - us.z));
+    const uint32_t w = __float_as_uint(p.w);
+    const float q = __uint_as_float((w >> 24) << 23), nq = -128.f * q;
+    return make_float3(fmaf((float)(w & 0xFFu), q, nq), fmaf((float)((w >> 8) & 0xFFu), q, nq),
+                       fmaf((float)((w >> 16) & 0xFFu), q, nq));
+}
+
+// the displacement in FP64 (exact)
+__device__ __forceinline__ double3 unpack_disp(const float4 &p)
+{
+    const float3 lo = disp_lo(p);
+    return make_double3((double)p.x + (double)lo.x, (double)p.y + (double)lo.y, (double)p.z + (double)lo.z);
+}
+
+__device__ __forceinline__ double4 unpack_disp4(const float4 &p)
+{
+    const double3 d = unpack_disp(p);
+    return make_double4(d.x, d.y, d.z, 0.0);
+}
+
+// The displacement of a tet vertex (packed, x) relative to the swept vertex
+// (FP32 parts hi and remainders lo), in FP32: the FP32 parts and the
+// remainders are differenced separately, so the result carries the rounding
+// of one FP32 operation on the difference, not that of the stored values.
+__device__ __forceinline__ float3 rel(const float4 &x, const float3 &hi, const float3 &lo)
+{
+    const float3 l = disp_lo(x);
+    return make_float3((x.x - hi.x) + (l.x - lo.x), (x.y - hi.y) + (l.y - lo.y), (x.z - hi.z) + (l.z - lo.z));
 }
 
 // A vertex's own data, loaded before anything that depends on other loads so
 // that a kernel can request it ahead of its early-exit test.
 struct VertexPre {
     int s0, s1;  // slot range (empty for lanes without a vertex)
-    double4 uv;  // current displacement
+    float4 uv;   // current displacement (packed)
     double4 tg;  // inertia target, M/dt^2
 };
 
@@ -124,13 +172,14 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
 {
     Acc acc;
     acc_zero(acc);
-    const double4 uv4 = pre.uv;
-    us = make_double3(uv4.x, uv4.y, uv4.z);
+    const float4 uv4 = pre.uv;
+    float3 hi = make_float3(uv4.x, uv4.y, uv4.z), lo = disp_lo(uv4);
     // the inertia term needs only x - t and M/dt^2: formed here in FP64, so
     // the slot loop carries four FP32 values instead of the FP64 target
     const double4 tg = pre.tg;
+    const double3 u0 = unpack_disp(uv4);
     float w = (float)tg.w;
-    float t0 = (float)(us.x - tg.x), t1 = (float)(us.y - tg.y), t2 = (float)(us.z - tg.z);
+    float t0 = (float)(u0.x - tg.x), t1 = (float)(u0.y - tg.y), t2 = (float)(u0.z - tg.z);
     // an empty asm that reads and writes them keeps the compiler from sinking
     // the subtraction (and the FP64 target with it) past the loop
     asm volatile("" : "+v"(t0), "+v"(t1), "+v"(t2), "+v"(w));
@@ -141,7 +190,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         for (int s = s0 + lane; s < s1; s += B * L) {
             int e[B];
             float4 q[B][4];
-            double4 x[B][3];
+            float4 x[B][3];
 #pragma unroll
             for (int j = 0; j < B; ++j)
                 if (s + j * L < s1) e[j] = a.ent[s + j * L];
@@ -170,11 +219,9 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
-                    const double4 *xx = x[j];
-                    slot_eval<MAT>(ka[j], kb[j], kc[j], bs[j], b0[j], b1[j],
-                                   rel(xx[0], us),
-                                   rel(xx[1], us),
-                                   rel(xx[2], us), acc);
+                    const float4 *xx = x[j];
+                    slot_eval<MAT>(ka[j], kb[j], kc[j], bs[j], b0[j], b1[j], rel(xx[0], hi, lo), rel(xx[1], hi, lo),
+                                   rel(xx[2], hi, lo), acc);
                 }
             }
         }
@@ -192,18 +239,16 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                 const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) +
                                                                    __float_as_uint(c.w) * 48u);
                 const float4 d0 = d[0], d1 = d[1], d2 = d[2];
-                const double4 x0 = *at(a.u, __float_as_uint(c.x));
-                const double4 x1 = *at(a.u, __float_as_uint(c.y));
-                const double4 x2 = *at(a.u, __float_as_uint(c.z));
+                const float4 x0 = *at(a.u, __float_as_uint(c.x));
+                const float4 x1 = *at(a.u, __float_as_uint(c.y));
+                const float4 x2 = *at(a.u, __float_as_uint(c.z));
                 // keep the request for the next record behind this slot's loads
                 __builtin_amdgcn_sched_barrier(0);
                 const float4 cn = *at(a.slots, (uint32_t)min(s + L, s1 - 1));  // the last one reloads a record in cache
                 __builtin_amdgcn_sched_barrier(0);
                 slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
                                make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y),
-                               rel(x0, us),
-                               rel(x1, us),
-                               rel(x2, us), acc);
+                               rel(x0, hi, lo), rel(x1, hi, lo), rel(x2, hi, lo), acc);
                 c = cn;
             }
         } else
@@ -212,7 +257,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         for (int s = s0 + lane; s < s1; s += B * L) {
             float4 q[B][3];
             float2 qd[B];
-            double4 x[B][3];
+            float4 x[B][3];
             // compact record (ids, entry), then the entry's constants
 #pragma unroll
             for (int j = 0; j < B; ++j) {
@@ -239,17 +284,19 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
                     const float4 *qq = q[j];
-                    const double4 *xx = x[j];
+                    const float4 *xx = x[j];
                     slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), make_float3(qq[1].x, qq[1].y, qq[1].z),
                                    make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[2].w, qd[j].x, qd[j].y),
-                                   rel(xx[0], us),
-                                   rel(xx[1], us),
-                                   rel(xx[2], us), acc);
+                                   rel(xx[0], hi, lo), rel(xx[1], hi, lo), rel(xx[2], hi, lo), acc);
                 }
             }
         }
 #endif
     }
+    // the FP64 displacement again from its parts, which the loop holds anyway
+    // (the empty asm keeps the compiler from carrying u0 through the loop)
+    asm volatile("" : "+v"(hi.x), "+v"(hi.y), "+v"(hi.z), "+v"(lo.x), "+v"(lo.y), "+v"(lo.z));
+    us = make_double3((double)hi.x + (double)lo.x, (double)hi.y + (double)lo.y, (double)hi.z + (double)lo.z);
     acc_group_sum<L>(acc);
     if (!valid) return make_float3(0.f, 0.f, 0.f);
     if (a.kd_dt != 0.f) {
@@ -289,7 +336,7 @@ __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, boo
     double3 us;
     const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us, pre);
     if (!valid) return 0u;
-    if (lane == 0) a.u[v] = make_double4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.0);
+    if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
     return metric_bits(dx);
 }
 
@@ -299,7 +346,7 @@ __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, boo
     double3 us;
     const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us);
     if (!valid) return 0u;
-    if (lane == 0) a.u[v] = make_double4(us.x + dx.x, us.y + dx.y, us.z + dx.z, 0.0);
+    if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
     return metric_bits(dx);
 }
 
@@ -323,7 +370,7 @@ struct StepParams {
     int skip_init;  // the initial guess is already in u (warm start)
 };
 
-__device__ __forceinline__ void init_vertex(int i, double4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
+__device__ __forceinline__ void init_vertex(int i, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
                                             const float *mass, const uint8_t *active, float dt, float4 g)
 {
     const double4 p = uprev[i];
@@ -334,10 +381,10 @@ __device__ __forceinline__ void init_vertex(int i, double4 *u, const double4 *up
         const double4 t = make_double4(fma(h2, (double)g.x, fma(h, (double)v.x, p.x)),
                                        fma(h2, (double)g.y, fma(h, (double)v.y, p.y)),
                                        fma(h2, (double)g.z, fma(h, (double)v.z, p.z)), (double)mass[i] / h2);
-        u[i] = make_double4(t.x, t.y, t.z, 0.0);
+        u[i] = pack_disp(t.x, t.y, t.z);
         tgt[i] = t;
     } else {
-        u[i] = p;
+        u[i] = pack_disp(p.x, p.y, p.z);
         tgt[i] = make_double4(p.x, p.y, p.z, 0.0);
     }
 }
@@ -397,7 +444,7 @@ __global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 
     for (int c = 0; c <= C; ++c) {
         const int2 rc = r[c];
         for (int i = rc.x + threadIdx.x; i < rc.x + rc.y; i += blockDim.x) {
-            const double4 x = a.u[i], q = uprev[i];
+            const double4 x = unpack_disp4(a.u[i]), q = uprev[i];
             vel[i] = step_velocity(x, q, p.dt);
             uprev[i] = x;
         }
