@@ -457,6 +457,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "state": "displacements: iterate packed to 32 significant bits, step start and inertia target FP64",
         "data": "synthetic (voxel meshes built in-process; no dataset)",
         "config": {
             "workload": f"plain VBD cantilever 256x32x32 voxels of 1 cm ({sc.n_vertices} vertices, "

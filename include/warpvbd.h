@@ -215,9 +215,11 @@ int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t
 void vbd_use_record_dictionary(int32_t enable);
 int vbd_records(vbd_solver_t *h, int32_t *n_dictionary, int32_t *bytes_per_slot);
 /* bytes[n_colors]: the HBM bytes one sweep of each colour reads and writes
- * with this layout, sum_v (40 + bytes_per_slot k_v) + 12 |gathered neighbours|
- * (vbd_color_bytes is the same sum with the 68 bytes per slot of the
- * reference's per-tet arrays). */
+ * with this layout, sum_v (68 + bytes_per_slot k_v) + 16 |gathered neighbours|:
+ * per swept vertex its 16-byte packed position, 32-byte FP64 inertia target,
+ * 4-byte slot offset and 16-byte write-back; per gathered neighbour its packed
+ * position (vbd_color_bytes is the algorithmic model: 40 bytes per vertex, the
+ * 68 bytes per slot of the reference's per-tet arrays, 12 per neighbour). */
 int vbd_color_layout_bytes(vbd_solver_t *h, double *bytes);
 
 /* Mesh facts computed at create. */

@@ -555,7 +555,8 @@ int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
 #endif
     for (int c = 0; c < s->ncolors; ++c) {
         const double nv = (double)(s->col_start[c + 1] - s->col_start[c]);
-        bytes[c] = 40.0 * nv + per_slot * s->col_slots[c] + 12.0 * s->col_boundary[c];
+        // packed position 16 + FP64 target 32 + slot offset 4 + write-back 16
+        bytes[c] = 68.0 * nv + per_slot * s->col_slots[c] + 16.0 * s->col_boundary[c];
     }
     return 0;
 }

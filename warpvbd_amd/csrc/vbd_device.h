@@ -23,10 +23,13 @@
 //   StVK: ka = V mu,  kb = V lambda,  kc = 0
 // kc follows from ka and the material, so it is not stored.
 //
-// Positions are carried as FP32 displacements u = x - X from the rest shape, so
+// Positions are carried as displacements u = x - X from the rest shape, so
 // F = I + D with D = sum_i (u_i - u_self) b_i^T is formed from small
 // differences; FP32 absolute coordinates cannot resolve a 1e-7 m tolerance on a
-// 2.56 m beam.
+// 2.56 m beam.  The iterate u is packed to 32 significant bits (pack_disp in
+// vbd_sweep.h): FP32 displacements alone (a 1e-9 m ulp at 1 cm) move
+// tolerance-terminated trajectories off the FP64 ones by iteration-count flips.
+// Each difference u_i - u_self reaches the slot arithmetic rounded once to FP32.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

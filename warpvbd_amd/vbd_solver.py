@@ -9,8 +9,10 @@ Differences a caller can see (all additive):
   positions, like the ``wp.array`` the reference returns.  The array keeps its
   values after later steps (a device snapshot is taken when needed).
 * ``last_iterations`` / ``last_metric`` report what the reference only prints.
-* Positions are stored on the device as FP32 displacements from the rest shape
-  (an FP32 solve, as the workloads ask; the reference computes in FP64).
+* Positions are stored on the device as displacements from the rest shape:
+  the iterate packed to 32 significant bits (16 bytes per vertex), the step's
+  start and inertia target in FP64.  The per-element arithmetic is FP32 (the
+  reference computes in FP64).
 """
 from __future__ import annotations
 
@@ -347,8 +349,10 @@ class VBDSolver:
         return {"dictionary_entries": n.value, "bytes_per_slot": b.value}
 
     def color_layout_bytes(self) -> np.ndarray:
-        """HBM bytes of one sweep of each colour with this solver's record
-        layout (color_bytes: with 68 bytes per slot, the reference's arrays)."""
+        """HBM bytes of one sweep of each colour with this solver's layout:
+        68 per swept vertex, 16 or 64 per slot, 16 per gathered neighbour
+        (color_bytes: the algorithmic model with the reference's 68 bytes per
+        slot)."""
         out = np.empty(max(1, self.n_colors))
         check(_lib.load().vbd_color_layout_bytes(self._h, ptr(out, ctypes.c_double)), "vbd_color_layout_bytes")
         return out[: self.n_colors]
