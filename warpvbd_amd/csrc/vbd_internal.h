@@ -61,7 +61,7 @@ struct CachedGraph {
 struct Launch {
     int lanes = 4;        // lanes per vertex
     int block = 256;      // threads per block
-    int small_color = 2048;  // colours below this many vertices run fused in one workgroup
+    int small_color = 2048;  // colours below this many vertices get 32 lanes per vertex
     int chunk = 0;        // iterations between host waits (0 = adaptive)
     bool resident = true; // allow the one-workgroup whole-step kernel when every colour is small
     bool graph = true;    // batched colour launches: replay captured iterations
@@ -69,10 +69,9 @@ struct Launch {
     int resident_block = 1024;  // threads per body in the one-workgroup-per-body form
 };
 
-// A run of consecutive colours executed by one launch.
+// One colour's launch.
 struct Segment {
     int c0, c1;    // colour range [c0, c1)
-    bool fused;    // one workgroup, barrier between colours
     int part_off;  // first per-block metric partial of this launch
     int nparts;    // blocks of this launch
     int lanes;     // lanes per vertex of this launch

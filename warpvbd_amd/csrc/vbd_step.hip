@@ -189,7 +189,7 @@ void plan_segments(vbd_solver *s)
         if (nv == 0) continue;
         const int lanes = nv < s->launch.small_color ? 32 : s->launch.lanes;
         const int64_t threads = (int64_t)nv * lanes;
-        s->segments.push_back({c, c + 1, false, 0, nblk(threads, s->launch.block), lanes});
+        s->segments.push_back({c, c + 1, 0, nblk(threads, s->launch.block), lanes});
     }
     int off = 0;
     for (auto &g : s->segments) {
