@@ -242,6 +242,28 @@ def test_free_fall_exact_and_one_iteration():
         assert np.allclose(x[:, 2], sc.vertices[:, 2] - 9.81e-6 * n * (n + 1) / 2, rtol=0, atol=2e-7)
 
 
+def test_translated_free_fall_keeps_32_bit_displacements():
+    """Free fall of an unpinned body 2.9 m from its rest shape, positions passed
+    in each step.  The iterate holds 32 significant bits of the displacement:
+    each step's result is off the exact FP64 fall by at most |u| 2^-32
+    (0.7 nm here), and the velocity carries that on, so after 20 steps the
+    error is at most 210 such roundings, 1.4e-7 m.  An FP32 iterate rounds by
+    1.2e-7 m per step at this distance and leaves the 2e-7 m gate within a few
+    steps."""
+    sc = cantilever((4, 2, 2))
+    k = sc.solver_kwargs()
+    k["active_mask"] = None
+    g = VBDSolver(**k, dx_tol=1e-7, max_iter=50, verbose=False)
+    off = np.array([1.25, -0.75, 2.5])
+    x = sc.vertices + off
+    for n in range(1, 21):
+        x = g.step(x, 1e-3).numpy()
+        assert g.last_iterations == 1
+        exact = sc.vertices + off
+        exact[:, 2] -= 9.81e-6 * n * (n + 1) / 2
+        assert np.abs(x - exact).max() < 2e-7, n
+
+
 def test_pinned_vertices_bit_unchanged_and_velocity():
     sc = cantilever((10, 2, 2))
     g, _ = make(sc)
