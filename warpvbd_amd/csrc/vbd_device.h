@@ -94,7 +94,9 @@ __device__ __forceinline__ void acc_zero(Acc &a)
 }
 
 // Elastic gradient and Hessian block of one slot, added into a.
-//   d0, d1, d2: u_o0 - u_self, u_o1 - u_self, u_o2 - u_self.
+//   e0, e1, d2: u_o0 - u_o2, u_o1 - u_o2, u_o2 - u_self, so that with
+//   b2 = -(bs + b0 + b1) the displacement gradient
+//   D = sum_k (u_ok - u_self) b_k^T = e0 b0^T + e1 b1^T - d2 bs^T.
 // SNH (src/vbd_solver.py:101-124) in the equivalent compact form
 //   g = V P b,  P = mu'(1 - 1/(Ic+1)) F + lambda'(J - alpha) cof(F)
 //   H = V[2mu'/(Ic+1)^2 (Fb)(Fb)^T + mu'(1 - 1/(Ic+1))|b|^2 I + lambda'(cof(F) b)(cof(F) b)^T]
@@ -103,20 +105,19 @@ __device__ __forceinline__ void acc_zero(Acc &a)
 //   H = V[(lambda+mu)(Fb)(Fb)^T + mu |b|^2 F F^T + (lambda tr(E)|b|^2 + 2 mu b^T E b) I]
 template <int MAT>
 __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const float3 bs, const float3 b0,
-                                          const float3 b1, const float3 d0, const float3 d1, const float3 d2,
+                                          const float3 b1, const float3 e0, const float3 e1, const float3 d2,
                                           Acc &a)
 {
-    const float3 b2 = make_float3(-(bs.x + b0.x + b1.x), -(bs.y + b0.y + b1.y), -(bs.z + b0.z + b1.z));
-    // D = d0 b0^T + d1 b1^T + d2 b2^T  (row r, column c)
-    const float D00 = d0.x * b0.x + d1.x * b1.x + d2.x * b2.x;
-    const float D01 = d0.x * b0.y + d1.x * b1.y + d2.x * b2.y;
-    const float D02 = d0.x * b0.z + d1.x * b1.z + d2.x * b2.z;
-    const float D10 = d0.y * b0.x + d1.y * b1.x + d2.y * b2.x;
-    const float D11 = d0.y * b0.y + d1.y * b1.y + d2.y * b2.y;
-    const float D12 = d0.y * b0.z + d1.y * b1.z + d2.y * b2.z;
-    const float D20 = d0.z * b0.x + d1.z * b1.x + d2.z * b2.x;
-    const float D21 = d0.z * b0.y + d1.z * b1.y + d2.z * b2.y;
-    const float D22 = d0.z * b0.z + d1.z * b1.z + d2.z * b2.z;
+    // D (row r, column c)
+    const float D00 = e0.x * b0.x + e1.x * b1.x - d2.x * bs.x;
+    const float D01 = e0.x * b0.y + e1.x * b1.y - d2.x * bs.y;
+    const float D02 = e0.x * b0.z + e1.x * b1.z - d2.x * bs.z;
+    const float D10 = e0.y * b0.x + e1.y * b1.x - d2.y * bs.x;
+    const float D11 = e0.y * b0.y + e1.y * b1.y - d2.y * bs.y;
+    const float D12 = e0.y * b0.z + e1.y * b1.z - d2.y * bs.z;
+    const float D20 = e0.z * b0.x + e1.z * b1.x - d2.z * bs.x;
+    const float D21 = e0.z * b0.y + e1.z * b1.y - d2.z * bs.y;
+    const float D22 = e0.z * b0.z + e1.z * b1.z - d2.z * bs.z;
     const float F00 = 1.f + D00, F11 = 1.f + D11, F22 = 1.f + D22;
     const float F01 = D01, F02 = D02, F10 = D10, F12 = D12, F20 = D20, F21 = D21;
     const float bb = bs.x * bs.x + bs.y * bs.y + bs.z * bs.z;

@@ -132,14 +132,30 @@ __device__ __forceinline__ double4 unpack_disp4(const float4 &p)
     return make_double4(d.x, d.y, d.z, 0.0);
 }
 
-// The displacement of a tet vertex (packed, x) relative to the swept vertex
-// (FP32 parts hi and remainders lo), in FP32: the FP32 parts and the
-// remainders are differenced separately, so the result carries the rounding
-// of one FP32 operation on the difference, not that of the stored values.
-__device__ __forceinline__ float3 rel(const float4 &x, const float3 &hi, const float3 &lo)
+// The differences slot_eval takes, from the packed displacements x0, x1, x2
+// of a slot's other vertices and the swept vertex's FP32 parts hi and
+// remainders lo: e0 = u_o0 - u_o2, e1 = u_o1 - u_o2, d2 = u_o2 - u_self.  The
+// FP32 parts and the remainders are differenced separately, so each result
+// carries the rounding of one FP32 operation on the difference, not that of
+// the stored values.
+struct SlotRel {
+    float3 e0, e1, d2;
+};
+
+__device__ __forceinline__ float3 diff_packed(const float4 &x, const float3 &lx, const float4 &y, const float3 &ly)
 {
-    const float3 l = disp_lo(x);
-    return make_float3((x.x - hi.x) + (l.x - lo.x), (x.y - hi.y) + (l.y - lo.y), (x.z - hi.z) + (l.z - lo.z));
+    return make_float3((x.x - y.x) + (lx.x - ly.x), (x.y - y.y) + (lx.y - ly.y), (x.z - y.z) + (lx.z - ly.z));
+}
+
+__device__ __forceinline__ SlotRel slot_rel(const float4 &x0, const float4 &x1, const float4 &x2, const float3 &hi,
+                                            const float3 &lo)
+{
+    const float3 l0 = disp_lo(x0), l1 = disp_lo(x1), l2 = disp_lo(x2);
+    SlotRel r;
+    r.e0 = diff_packed(x0, l0, x2, l2);
+    r.e1 = diff_packed(x1, l1, x2, l2);
+    r.d2 = make_float3((x2.x - hi.x) + (l2.x - lo.x), (x2.y - hi.y) + (l2.y - lo.y), (x2.z - hi.z) + (l2.z - lo.z));
+    return r;
 }
 
 // A vertex's own data, loaded before anything that depends on other loads so
@@ -219,9 +235,8 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
-                    const float4 *xx = x[j];
-                    slot_eval<MAT>(ka[j], kb[j], kc[j], bs[j], b0[j], b1[j], rel(xx[0], hi, lo), rel(xx[1], hi, lo),
-                                   rel(xx[2], hi, lo), acc);
+                    const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
+                    slot_eval<MAT>(ka[j], kb[j], kc[j], bs[j], b0[j], b1[j], r.e0, r.e1, r.d2, acc);
                 }
             }
         }
@@ -246,9 +261,10 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                 __builtin_amdgcn_sched_barrier(0);
                 const float4 cn = *at(a.slots, (uint32_t)min(s + L, s1 - 1));  // the last one reloads a record in cache
                 __builtin_amdgcn_sched_barrier(0);
+                const SlotRel sr = slot_rel(x0, x1, x2, hi, lo);
                 slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
-                               make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y),
-                               rel(x0, hi, lo), rel(x1, hi, lo), rel(x2, hi, lo), acc);
+                               make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2,
+                               acc);
                 c = cn;
             }
         } else
@@ -284,10 +300,10 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
                     const float4 *qq = q[j];
-                    const float4 *xx = x[j];
+                    const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
                     slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), make_float3(qq[1].x, qq[1].y, qq[1].z),
                                    make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[2].w, qd[j].x, qd[j].y),
-                                   rel(xx[0], hi, lo), rel(xx[1], hi, lo), rel(xx[2], hi, lo), acc);
+                                   r.e0, r.e1, r.d2, acc);
                 }
             }
         }
