@@ -217,12 +217,14 @@ def test_trained_network_gain():
 
 
 def test_history_guess_matches_oracle():
-    """With the network's output layer zeroed the warm guess is x_inertia +
-    c_prev, c_prev being the previous step's converged correction.  Forced
-    (NETWORK policy) at a fixed iteration count, each step from the FP64
-    oracle's state matches the oracle started from that same guess to 1e-6 of
-    the bounding-box diagonal; the first step after a reset has no c_prev and
-    matches the oracle's plain step."""
+    """With the network's output layer zeroed the warm guess is x_inertia plus
+    the history part: c_prev, the previous step's converged correction, after
+    one step, and the linear extrapolation 2 c_prev - c_prev2 once the two
+    steps before were consecutive.  Forced (NETWORK policy) at a fixed
+    iteration count, each step from the FP64 oracle's state matches the oracle
+    started from that same guess to 1e-6 of the bounding-box diagonal; the
+    first step after a reset has no c_prev and matches the oracle's plain
+    step."""
     import oracle
     sc, s = make_solver((16, 3, 3))
     ws = WarmStart(s, capacity=1, seed=3)
@@ -237,10 +239,11 @@ def test_history_guess_matches_oracle():
     g = np.asarray(sc.gravity, np.float64)
     diag = float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
     c = np.zeros((sc.n_vertices, 3))
+    c2 = None
     for k in range(8):
         x0, v0 = ref.old_positions, ref.old_velocities
         y = x0 + (dt * v0 + dt * dt * g) * act
-        xr = ref.step(dt, dx_tol=0.0, max_iter=K, guess=y + c)
+        xr = ref.step(dt, dx_tol=0.0, max_iter=K, guess=y + (c if c2 is None else 2.0 * c - c2))
         s.old_positions = x0
         s.old_velocities = v0
         x = s.step(s.old_positions, dt, dx_tol=0.0, max_iter=K, warm_start=ws).numpy()
@@ -248,16 +251,19 @@ def test_history_guess_matches_oracle():
         print(f"step {k}: |c_prev| max {np.abs(c).max():.3e}, error {err:.3e} (bar {1e-6 * diag:.3e})")
         assert s.last_used_network and s.last_iterations == K
         assert err <= 1e-6 * diag
+        if k > 0:
+            c2 = c
         c = (xr - y) * act
 
 
 def test_history_features_and_targets():
     """Stored samples carry c_prev (features 6..8, flag 24), its change over
     the last two steps (features 3..5, flag 31: only when the two steps were
-    consecutive) and the target (x_{n+1} - x_inertia - c_prev) / (dt^2 |g|),
-    all checked against corrections computed on the host from the exported
-    states; a plain step in between makes the next sample's change feature
-    absent, and a reset clears both."""
+    consecutive) and the target (x_{n+1} - x_inertia - h) / (dt^2 |g|), with
+    the history part h = c_prev, or 2 c_prev - c_prev2 after two consecutive
+    steps, all checked against corrections computed on the host from the
+    exported states; a plain step in between makes the next sample's change
+    feature absent (and h first order), and a reset clears both."""
     sc, s = make_solver((12, 3, 3))
     dt = 1e-3
     g = np.asarray(sc.gravity, np.float64)
@@ -295,14 +301,14 @@ def test_history_features_and_targets():
             else:
                 assert np.all(F[:, 24] == 1)
                 assert np.abs(F[:, 6:9] - c1 / S).max() <= tol(c1 / S)
-                t_exp = (x1 - y - c1) * act / S
+                t_exp = (x1 - y - (c1 if c2 is None else 2.0 * c1 - c2)) * act / S
                 if c2 is None:
                     assert np.all(F[:, 31] == 0) and np.all(F[:, 3:6] == 0)
                 else:
                     assert np.all(F[:, 31] == 1)
                     d = (c1 - c2) / S
                     assert np.abs(F[:, 3:6] - d).max() <= tol(d)
-            # FP32 displacements on the device against FP64 on the host
+            # FP32 corrections on the device against FP64 on the host
             assert np.abs(T - t_exp).max() <= 2e-3 * np.abs(t_exp).max() + 2e-3
         corr.append((x1 - y) * act)
         if what == "plain" and len(corr) > 1:
@@ -311,8 +317,8 @@ def test_history_features_and_targets():
 
 
 def test_history_policy_cuts_iterations():
-    """HISTORY (per body the better of the plain guess and x_inertia + c_prev,
-    no network) on a stiff 32x4x4 cantilever, each step from the state of a
+    """HISTORY (per body the better of the plain guess and x_inertia + the
+    history part, no network) on a stiff 32x4x4 cantilever, each step from the state of a
     converged FP64 oracle trajectory (tol 1e-11): fewer iterations than the
     plain step, and the step error against the converged step within 1.5x of
     the plain step's in mean and max."""
@@ -340,9 +346,10 @@ def test_history_trajectory_error(E):
     steps at tol 1e-7, plain and with the HISTORY policy, against a converged
     FP64 oracle trajectory (tol 1e-11): the history trajectory's tip error
     stays within 1.5x of the plain trajectory's (the FP64 oracle run the same
-    way gives 1.6e-3 vs 1.3e-3 of the tip range at E = 1e5 and 6.5e-2 vs
-    1.6e-1 at E = 1e6 over 300 steps: stopping on the update size leaves an
-    error that grows with stiffness, and the history start leaves no more)."""
+    way over 200 steps gives 2.3e-4 vs 1.0e-3 of the tip range at E = 1e5 and
+    1.4e-3 vs 1.6e-1 at E = 1e6 from the extrapolated start: stopping on the
+    update size leaves an error that grows with stiffness, and a start nearer
+    the solution leaves less)."""
     import oracle
     nx = (24, 4, 4)
     sc, s = make_solver(nx)

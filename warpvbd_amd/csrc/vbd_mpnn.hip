@@ -649,26 +649,43 @@ __global__ void k_history(int n, const double4 *uprev, const double4 *tgt, const
                      : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// The history part of the warm guesses: the previous step's converged
+// correction scaled to this step's dt (hs c_prev), extrapolated linearly over
+// the last two steps when the one before was consecutive
+// (2 hs c_prev - hs2 c_prev2); zero without a previous correction.  On the
+// FP64 oracle the extrapolated start cuts a 24x4x4 cantilever at E = 1e5 from
+// 4.8 to 1.8 iterations per step, and its tolerance-terminated trajectory ends
+// nearer a converged one than the first-order start's.
+__device__ __forceinline__ float3 history_part(int i, const float4 *cprev, float hs, const float4 *cprev2, float hs2)
+{
+    if (!(hs > 0.f)) return make_float3(0.f, 0.f, 0.f);
+    const float4 c = cprev[i];
+    if (!(hs2 > 0.f)) return make_float3(hs * c.x, hs * c.y, hs * c.z);
+    const float4 d = cprev2[i];
+    const float h = 2.f * hs;
+    return make_float3(h * c.x - hs2 * d.x, h * c.y - hs2 * d.y, h * c.z - hs2 * d.z);
+}
+
 // training target of the step just taken, in network order:
-// (x_{n+1} - x_inertia - hs c_prev) / s
+// (x_{n+1} - x_inertia - history part) / s
 __global__ void k_target(int n, const int32_t *perm, const double4 *u, const double4 *tgt, const uint8_t *active,
-                         const float4 *cprev, float hs, float inv_s, float *t)
+                         const float4 *cprev, float hs, const float4 *cprev2, float hs2, float inv_s, float *t)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 x = u[i], y = tgt[i];
-    const float4 c = hs > 0.f ? cprev[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float3 c = history_part(i, cprev, hs, cprev2, hs2);
     const bool a = active[i];
     float *o = t + (size_t)perm[i] * kFout;
-    o[0] = a ? ((float)(x.x - y.x) - hs * c.x) * inv_s : 0.f;
-    o[1] = a ? ((float)(x.y - y.y) - hs * c.y) * inv_s : 0.f;
-    o[2] = a ? ((float)(x.z - y.z) - hs * c.z) * inv_s : 0.f;
+    o[0] = a ? ((float)(x.x - y.x) - c.x) * inv_s : 0.f;
+    o[1] = a ? ((float)(x.y - y.y) - c.y) * inv_s : 0.f;
+    o[2] = a ? ((float)(x.z - y.z) - c.z) * inv_s : 0.f;
 }
 
-// warm guess (colour-major) for active vertices: x_inertia + hs c_prev
+// warm guess (colour-major) for active vertices: x_inertia + the history part
 // (+ s out[perm[i]] when out is given); pinned vertices keep x_inertia
 __global__ void k_guess(int n, const int32_t *perm, const double4 *tgt, const uint8_t *active, const float *out,
-                        float s, const float4 *cprev, float hs, float4 *dst)
+                        float s, const float4 *cprev, float hs, const float4 *cprev2, float hs2, float4 *dst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -678,11 +695,7 @@ __global__ void k_guess(int n, const int32_t *perm, const double4 *tgt, const ui
         return;
     }
     // the correction (float) is formed first and added to the FP64 target once
-    float3 g = make_float3(0.f, 0.f, 0.f);
-    if (hs > 0.f) {
-        const float4 c = cprev[i];
-        g = make_float3(hs * c.x, hs * c.y, hs * c.z);
-    }
+    float3 g = history_part(i, cprev, hs, cprev2, hs2);
     if (out) {
         const float *o = out + (size_t)perm[i] * kFout;
         g = make_float3(g.x + s * o[0], g.y + s * o[1], g.z + s * o[2]);
@@ -1105,13 +1118,13 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
         features(m, dt, true, m->f);  // the plain guess's residual into rm0
         VBD_TRY(mpnn_forward_backward(m, m->f, nullptr, false, 0.f));
         k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, m->out, m->fp.s, m->cprev,
-                                                 m->hs, m->guess);
+                                                 m->hs, m->cprev2, m->hs2, m->guess);
         if (!force) jacobi(m, m->guess, s->tgt, dt, nullptr, m->rm1);
     }
     const bool use_hist = have_hist && !force;
     if (use_hist) {
         k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev, m->hs,
-                                                 m->hguess);
+                                                 m->cprev2, m->hs2, m->hguess);
         jacobi(m, m->hguess, s->tgt, dt, nullptr, m->rm2);
     }
     const dim3 grid(m->guess_chunks, s->n_bodies);
@@ -1377,7 +1390,7 @@ int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t
         VBD_TRY(step_device(s, dt, tol, max_iter, &it, &met));
     }
     k_target<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->uprev, s->tgt, s->active, m->cprev, m->hs,
-                                              1.f / m->fp.s, m->T + (size_t)slot * m->n * 3);
+                                              m->cprev2, m->hs2, 1.f / m->fp.s, m->T + (size_t)slot * m->n * 3);
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(st));
     if (iters_out) *iters_out = it;
