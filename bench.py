@@ -21,7 +21,8 @@ WRITE_SIZE, calibrated against a copy kernel).
 Extra entries (each bounded to seconds):
   stvk        the same beam with the StVK material
   warm_big    the same beam plain and with the warm start's history guess
-              (x_inertia + the previous step's converged correction)
+              (x_inertia + the previous steps' converged corrections,
+              extrapolated linearly over the last two)
   batched     512 mass-spring bodies (seed 3): one BatchedVBDSolver (per-body
               convergence, autotuned form, hipGraph) vs the reference's form,
               one VBDSolver on the concatenated mesh
@@ -232,8 +233,9 @@ def batched_section(args):
 def warm_big_section(args):
     """The headline 256x32x32 beam stepped plainly and with the warm start's
     history guess (HISTORY policy: per step the better of the inertia
-    prediction and x_inertia + the previous step's correction; no network,
-    no training), from the same rest state for the same steps."""
+    prediction and x_inertia + the linear extrapolation of the last two
+    steps' corrections; no network, no training), from the same rest state
+    for the same steps."""
     from warpvbd_amd import VBDSolver
     from warpvbd_amd.scenes import cantilever
     from warpvbd_amd.warm_start import WarmStart
@@ -410,11 +412,11 @@ def mixed_section(args):
     out["batched"]["autotuned"] = {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}
     out["batched_history"] = timed_leg(WarmStart.HISTORY)
     out["batched_history"]["policy"] = ("history: per body the better (first-sweep residual) of the plain guess "
-                                        "and x_inertia + the previous step's correction; no network")
+                                        "and x_inertia + the last two steps' corrections extrapolated; no network")
     out["batched_warm"] = timed_leg(WarmStart.CHOOSE)
     out["batched_warm"]["policy"] = ("choose: per body the smallest first-sweep residual among the plain guess, "
-                                     "the history guess and the network guess x_inertia + previous correction + "
-                                     "network output")
+                                     "the history guess and the network guess x_inertia + extrapolated correction "
+                                     "+ network output")
     out["batched_warm"]["training"] = (f"{args.mixed_samples} samples from the batch's own steps, "
                                        f"{args.warm_epochs} epochs, {t_gen:.1f} s + {t_train:.1f} s")
     out["batched_warm"]["train_loss_first_last"] = [float(hist[0]), float(hist[-1])]

@@ -148,10 +148,12 @@ int vbd_batch_autotune(vbd_solver_t *h, double dt, double tol, int32_t max_iter,
  * accumulation) on the vertex graph of the solver's mesh predicts each step's
  * converged correction x_{n+1} - x_inertia from the state at the start of the
  * step.  No counterpart in WarpVBD.  The network guess is
- * x_inertia + c_prev + dt^2 |g| out, where c_prev is the previous step's
- * converged correction (scaled by (dt / dt_prev)^2; zero right after create,
- * vbd_reset or vbd_load_snapshot), so the network predicts how the correction
- * changes from one step to the next.  Training samples come from the solver's
+ * x_inertia + h + dt^2 |g| out, where the history part h is c_prev, the
+ * previous step's converged correction, or 2 c_prev - c_prev2, its linear
+ * extrapolation, when the step before that one was consecutive (each scaled
+ * by (dt / dt of its step)^2; h = 0 right after create, vbd_reset or
+ * vbd_load_snapshot), so the network predicts how the correction departs from
+ * that extrapolation.  Training samples come from the solver's
  * own plain steps (vbd_mpnn_collect) and live in device memory
  * (sample_capacity of them, 76 bytes per vertex each). */
 typedef struct vbd_mpnn vbd_mpnn_t;
@@ -164,14 +166,14 @@ int vbd_mpnn_get_params(vbd_mpnn_t *m, float *params);
 /* material features: one (E, nu) for every body, or one per body (arrays of n_bodies) */
 int vbd_mpnn_set_material(vbd_mpnn_t *m, double youngs, double poisson);
 int vbd_mpnn_set_body_materials(vbd_mpnn_t *m, const double *youngs, const double *poisson);
-/* one plain solver step that stores (features before, correction after minus c_prev) in `slot` */
+/* one plain solver step that stores (features before, correction after minus h) in `slot` */
 int vbd_mpnn_collect(vbd_mpnn_t *m, int32_t slot, double dt, double tol, int32_t max_iter, int32_t *iters_out);
 /* loss of a stored sample; grads (nullable, vbd_mpnn_num_params floats) receives its gradient */
 int vbd_mpnn_loss_grad(vbd_mpnn_t *m, int32_t slot, float *loss_out, float *grads);
 /* Adam over stored samples [0, n_samples) */
 int vbd_mpnn_train(vbd_mpnn_t *m, int32_t n_samples, int32_t epochs, int32_t batch, float lr, uint64_t seed,
                    float *loss_per_epoch);
-/* network output [n,3] (original vertex order, units of dt^2 |g|; the guess adds c_prev) for sample `slot`, or
+/* network output [n,3] (original vertex order, units of dt^2 |g|; the guess adds h) for sample `slot`, or
  * the current state if slot < 0 */
 int vbd_mpnn_predict(vbd_mpnn_t *m, int32_t slot, double dt, float *out);
 /* a stored sample and the graph, in original vertex order (each pointer nullable) */
@@ -179,7 +181,7 @@ int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *ta
                          int32_t *n_edges);
 /* Which initial guess vbd_step_warm takes.  CHOOSE (default): per body the
  * one with the smallest first-sweep residual among the plain inertia
- * prediction, the network guess and the history guess x_inertia + c_prev
+ * prediction, the network guess and the history guess x_inertia + h
  * (when there is a c_prev).  NETWORK: always the network's.  PLAIN: always the
  * plain one (the network is not evaluated).  ADAPTIVE: CHOOSE, but after a
  * step where fewer than 2% of the bodies took the network guess the network

@@ -4,10 +4,12 @@ A 4-layer message-passing network on the mesh's vertex graph (width 64, BF16
 weights on MFMA with FP32 accumulation, hand-written forward and training
 kernels in csrc/vbd_mpnn.hip) predicts each step's converged correction
 x_{n+1} - x_inertia from the state at the start of the step.  The guess is
-x_inertia + c_prev + dt^2 |g| out: c_prev, the previous step's converged
-correction (scaled by (dt / dt_prev)^2, zero after a reset), carries the part
-of the correction that persists from step to step, and the network predicts
-how it changes.  The solver then iterates from that guess to the same
+x_inertia + h + dt^2 |g| out.  The history part h is c_prev, the previous
+step's converged correction, or its linear extrapolation 2 c_prev - c_prev2
+once the two steps before were consecutive (each scaled by (dt / dt of its
+step)^2; h = 0 after a reset).  It carries the part of the correction that
+evolves smoothly from step to step, and the network predicts how the
+correction departs from it.  The solver then iterates from that guess to the same
 tolerance, so its accuracy is still decided by VBD; a body whose warm guess
 has a larger first-sweep residual than the plain inertia guess starts from
 the plain guess instead.
@@ -142,7 +144,7 @@ class WarmStart:
     def set_policy(self, policy: int):
         """CHOOSE (default): per body the guess with the smallest first-sweep
         residual among the plain inertia prediction, the network guess and the
-        history guess x_inertia + c_prev; NETWORK: always the network's;
+        history guess x_inertia + h; NETWORK: always the network's;
         PLAIN: always the inertia prediction (the network is not evaluated);
         ADAPTIVE: CHOOSE, but after a step where fewer than 2% of the bodies
         took the network guess the network is skipped for 1, 2, 4, ... (at
@@ -246,7 +248,8 @@ class WarmStart:
 
     def predict(self, dt, slot: int = -1) -> np.ndarray:
         """The network's output [n, 3] in units of dt^2 |g| (the warm guess
-        adds c_prev) for stored sample `slot`, or for the solver's state."""
+        adds the history part h) for stored sample `slot`, or for the solver's
+        state."""
         out = np.empty((self.solver.n_vertices, 3), np.float32)
         check(_lib.load().vbd_mpnn_predict(self._h, int(slot), float(dt), ptr(out, ctypes.c_float)),
               "vbd_mpnn_predict")
