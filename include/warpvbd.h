@@ -187,7 +187,13 @@ int vbd_mpnn_read_sample(vbd_mpnn_t *m, int32_t slot, float *features, float *ta
  * step where fewer than 2% of the bodies took the network guess the network
  * is skipped for 1, 2, 4, ... up to 16 steps, which then choose between the
  * plain and the history guess only.  HISTORY: that choice on every step,
- * without the network. */
+ * without the network.  In CHOOSE, ADAPTIVE and HISTORY, on steps with a
+ * history guess the plain guess is compared only on check steps: after a
+ * check where no body took it, the next 1, 2, 4, ... up to 16 such steps
+ * leave it out (a HISTORY step then takes the history guess without a
+ * comparison); any change from outside a step (state set or passed to a
+ * step, material, gravity, reset, snapshot load) makes the next step a check
+ * step. */
 #define VBD_WARM_CHOOSE 0
 #define VBD_WARM_NETWORK 1
 #define VBD_WARM_PLAIN 2

@@ -223,6 +223,7 @@ int vbd_set_uniform_material(vbd_solver_t *s, double youngs, double poisson)
     VBD_TRY(rescale_material(s, s->uniform_E, s->uniform_nu, youngs, poisson));
     s->uniform_E = youngs;
     s->uniform_nu = poisson;
+    ++s->edits;
     return 0;
 }
 
@@ -235,6 +236,7 @@ int vbd_set_gravity(vbd_solver_t *s, const double *gravity)
     const float4 g = make_float4((float)gravity[0], (float)gravity[1], (float)gravity[2], 0.f);
     VBD_HIP(hipMemcpyAsync(s->d_grav, &g, sizeof(float4), hipMemcpyHostToDevice, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
+    ++s->edits;
     return 0;
 }
 
@@ -249,6 +251,7 @@ int vbd_reset(vbd_solver_t *s)
     s->last_iters = 0;
     s->hist_valid = false;
     s->step_serial += 2;
+    ++s->edits;
     return 0;
 }
 
@@ -259,7 +262,10 @@ int vbd_step(vbd_solver_t *s, const double *positions, double dt, double tol, in
     CHECK_ARG(dt > 0.0, "dt must be > 0");
     CHECK_ARG(max_iter >= 1, "max_iter must be >= 1");
     VBD_HIP(hipSetDevice(s->device));
-    if (positions) VBD_TRY(import_positions(s, positions, s->uprev));
+    if (positions) {
+        VBD_TRY(import_positions(s, positions, s->uprev));
+        ++s->edits;
+    }
     int it = 0;
     double met = 0.0;
     if (s->batched) {
@@ -293,7 +299,10 @@ int vbd_step_dev(vbd_solver_t *s, const double *positions_dev, double dt, double
     // the step runs after what the caller queued on its stream ...
     VBD_HIP(hipEventRecord(s->ev_in, cs));
     VBD_HIP(hipStreamWaitEvent(s->stream, s->ev_in, 0));
-    if (positions_dev) VBD_TRY(import_positions_dev(s, positions_dev, s->uprev));
+    if (positions_dev) {
+        VBD_TRY(import_positions_dev(s, positions_dev, s->uprev));
+        ++s->edits;
+    }
     int it = 0;
     double met = 0.0;
     if (s->batched) {
@@ -332,6 +341,7 @@ int vbd_set_state(vbd_solver_t *s, const double *x_prev, const double *v_prev)
     VBD_HIP(hipSetDevice(s->device));
     if (x_prev) VBD_TRY(import_positions(s, x_prev, s->uprev));
     if (v_prev) VBD_TRY(import_velocities(s, v_prev));
+    if (x_prev || v_prev) ++s->edits;
     return 0;
 }
 
@@ -362,6 +372,7 @@ int vbd_load_snapshot(vbd_solver_t *s, int32_t slot)
     VBD_HIP(hipStreamSynchronize(s->stream));
     s->hist_valid = false;
     s->step_serial += 2;
+    ++s->edits;
     return 0;
 }
 

@@ -133,6 +133,9 @@ struct vbd_solver {
     // serial number of the last finished step; reset and snapshot loads skip
     // one, so that no step before them is taken as the one right before the next
     int64_t step_serial = 0;
+    // changes made from outside a step: positions or velocities set (or
+    // passed to a step), material, gravity, reset, snapshot loads
+    uint64_t edits = 0;
     double uniform_E = 0.0, uniform_nu = 0.0;  // the material when every element shares it (else NaN)
     float4 *d_grav = nullptr;        // per body gravity
     int2 *d_body_out = nullptr;      // per body (iterations, metric bits) of the last step

@@ -743,12 +743,13 @@ __global__ __launch_bounds__(256) void k_guess_max(const int2 *ranges, int C, co
                                                    const uint32_t *m2, uint32_t *bm)
 {
     const int2 *r = ranges + (size_t)blockIdx.y * (C + 1);
-    uint32_t a0 = 0, a1 = m1 ? 0u : 0xffffffffu, a2 = m2 ? 0u : 0xffffffffu;
+    // a guess without residuals (null) is out of the comparison
+    uint32_t a0 = m0 ? 0u : 0xffffffffu, a1 = m1 ? 0u : 0xffffffffu, a2 = m2 ? 0u : 0xffffffffu;
     const int k0 = blockIdx.x * kGuessChunk;
     for (int k = k0 + threadIdx.x; k < k0 + kGuessChunk; k += blockDim.x) {
         const int i = body_chunk_vertex(r, C, k);
         if (i < 0) break;
-        a0 = max(a0, m0[i]);
+        if (m0) a0 = max(a0, m0[i]);
         if (m1) a1 = max(a1, m1[i]);
         if (m2) a2 = max(a2, m2[i]);
     }
@@ -887,6 +888,11 @@ struct vbd_mpnn {
     int policy = VBD_WARM_CHOOSE;
     int skip_left = 0, backoff = 0;  // VBD_WARM_ADAPTIVE: steps still to skip, current skip length
     bool evaluated = false;          // the last step evaluated the network
+    // the plain guess on steps with a history guess: steps still to leave it
+    // out, current skip length, and whether the last step compared it
+    int plain_skip_left = 0, plain_backoff = 0;
+    bool plain_checked = false;
+    uint64_t seen_edits = 0;  // the solver's edit count at the last warm step
     int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body: 0 plain, 1 network, 2 history guess
     uint32_t *bmax = nullptr;                             // per body: max residual bits of the three guesses
     int counts[kGuesses] = {};                            // bodies per guess in the last warm step
@@ -1099,10 +1105,15 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
     m->evaluated = false;
+    m->plain_checked = false;
     const bool skipping = m->policy == VBD_WARM_ADAPTIVE && m->skip_left > 0;  // network skipped this step
     if (skipping) --m->skip_left;
     const bool history = m->policy == VBD_WARM_HISTORY || skipping;
     const bool have_hist = m->hs > 0.f;
+    // the history chain broke, or the state or parameters were changed from
+    // outside: compare the plain guess again
+    if (!have_hist || s->edits != m->seen_edits) m->plain_skip_left = m->plain_backoff = 0;
+    m->seen_edits = s->edits;
     if (m->policy == VBD_WARM_PLAIN || (history && !have_hist)) {
         k_fill_u32<<<nblk(s->n_bodies, 256), 256, 0, st>>>(s->n_bodies, reinterpret_cast<uint32_t *>(m->use_alt),
                                                            0u);
@@ -1110,6 +1121,22 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
         return 0;
     }
     const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
+    // With a history guess the plain guess is compared only on check steps:
+    // after a check that no body took it, the next 1, 2, 4, ... 16 such steps
+    // leave it out (the solver iterates to the same tolerance from any start,
+    // so this only saves the comparison).
+    const bool plain_out = have_hist && !force && m->plain_skip_left > 0;
+    if (plain_out) --m->plain_skip_left;
+    m->plain_checked = have_hist && !force && !plain_out;
+    if (history && plain_out) {
+        // the history guess is the only candidate: straight into u
+        k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev, m->hs,
+                                                 m->cprev2, m->hs2, s->u);
+        k_fill_u32<<<nblk(s->n_bodies, 256), 256, 0, st>>>(s->n_bodies, reinterpret_cast<uint32_t *>(m->use_alt),
+                                                           2u);
+        VBD_HIP(hipGetLastError());
+        return 0;
+    }
     if (history) {
         set_feature_params(m, dt);
         jacobi(m, s->u, s->tgt, dt, nullptr, m->rm0);
@@ -1130,8 +1157,8 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
     const dim3 grid(m->guess_chunks, s->n_bodies);
     if (!force) {
         k_fill_u32<<<nblk(kGuesses * s->n_bodies, 256), 256, 0, st>>>(kGuesses * s->n_bodies, m->bmax, 0u);
-        k_guess_max<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->rm0, history ? nullptr : m->rm1,
-                                          use_hist ? m->rm2 : nullptr, m->bmax);
+        k_guess_max<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, plain_out ? nullptr : m->rm0,
+                                          history ? nullptr : m->rm1, use_hist ? m->rm2 : nullptr, m->bmax);
     }
     k_guess_apply<<<grid, 256, 0, st>>>(s->d_ranges, s->ncolors, m->bmax, m->guess, m->hguess, s->u, m->use_alt,
                                         force);
@@ -1297,6 +1324,7 @@ int vbd_mpnn_set_policy(vbd_mpnn_t *m, int32_t policy)
     }
     m->policy = policy;
     m->skip_left = m->backoff = 0;
+    m->plain_skip_left = m->plain_backoff = 0;
     return 0;
 }
 
@@ -1552,6 +1580,14 @@ int vbd_step_warm(vbd_solver_t *s, vbd_mpnn_t *m, double dt, double tol, int32_t
             m->skip_left = m->backoff;
         } else {
             m->backoff = 0;
+        }
+    }
+    if (m->plain_checked) {
+        if (m->counts[0] == 0) {  // no body took the plain guess
+            m->plain_backoff = std::min(16, std::max(1, 2 * m->plain_backoff));
+            m->plain_skip_left = m->plain_backoff;
+        } else {
+            m->plain_backoff = 0;
         }
     }
     if (iters_out) *iters_out = it;

@@ -149,7 +149,11 @@ class WarmStart:
         ADAPTIVE: CHOOSE, but after a step where fewer than 2% of the bodies
         took the network guess the network is skipped for 1, 2, 4, ... (at
         most 16) steps, which choose between the plain and history guesses
-        only; HISTORY: that choice on every step, without the network."""
+        only; HISTORY: that choice on every step, without the network.  On
+        steps with a history guess the plain guess is compared only on check
+        steps: after a check where no body took it, the next 1, 2, 4, ... (at
+        most 16) such steps leave it out; setting the state, material or
+        gravity makes the next step a check step again."""
         check(_lib.load().vbd_mpnn_set_policy(self._h, int(policy)), "vbd_mpnn_set_policy")
         self.policy = int(policy)
 
