@@ -426,9 +426,12 @@ def mixed_section(args):
     out["plain_per_body"] = plain_per_body(bodies, args)
     base = out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
     per_body = out["plain_per_body"]["aggregate_sim_s_per_wall_s"]
+    batched = out["batched"]["aggregate_sim_s_per_wall_s"]
     for k in ("batched", "batched_history", "batched_warm", "batched_warm_adaptive"):
         out[k]["vs_plain_concatenated"] = out[k]["aggregate_sim_s_per_wall_s"] / base
         out[k]["vs_plain_per_body"] = out[k]["aggregate_sim_s_per_wall_s"] / per_body
+        if k != "batched":
+            out[k]["vs_plain_batched"] = out[k]["aggregate_sim_s_per_wall_s"] / batched
     return out
 
 
