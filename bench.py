@@ -67,7 +67,7 @@ def parse():
                    help="comma list of the sections to run after the headline (ignored with --quick)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
-    p.add_argument("--batch-steps", type=int, default=30)
+    p.add_argument("--batch-steps", type=int, default=100)
     p.add_argument("--warm-trajectories", type=int, default=16)
     p.add_argument("--warm-steps", type=int, default=60)
     p.add_argument("--warm-epochs", type=int, default=12)

@@ -37,10 +37,12 @@ __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
     const int vi = t / L, lane = t % L;
-    bool valid = vi < nv;
-    const int v = v0 + (valid ? vi : 0);
-    valid = valid && !bs[body_new[v]].done;
-    const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane);
+    const bool in = vi < nv;
+    const int v = v0 + (in ? vi : 0);
+    // the vertex's own loads go out with the body lookup, not after it
+    const VertexPre pre = vertex_pre(a, v, in);
+    const bool valid = in && !bs[body_new[v]].done;
+    const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane, pre);
     if (valid && lane == 0) dxm[v] = m;
 }
 
@@ -54,10 +56,12 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t
 {
     const int2 t = tasks[xcd_remap(blockIdx.x, gridDim.x)];
     const int vi = threadIdx.x / L, lane = threadIdx.x % L;
-    bool valid = vi < t.y;
-    const int v = t.x + (valid ? vi : 0);
-    valid = valid && !bs[body_new[v]].done;
-    const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane);
+    const bool in = vi < t.y;
+    const int v = t.x + (in ? vi : 0);
+    // the vertex's own loads go out with the body lookup, not after it
+    const VertexPre pre = vertex_pre(a, v, in);
+    const bool valid = in && !bs[body_new[v]].done;
+    const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane, pre);
     if (valid && lane == 0) dxm[v] = m;
 }
 
