@@ -248,7 +248,7 @@ int vbd_reset(vbd_solver_t *s)
     VBD_HIP(hipMemsetAsync(s->vel, 0, sizeof(float4) * s->n, s->stream));
     VBD_HIP(hipMemsetAsync(s->u, 0, sizeof(float4) * s->n, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
-    s->last_iters = 0;
+    s->last_iters = s->prev_iters = 0;
     s->hist_valid = false;
     s->step_serial += 2;
     ++s->edits;

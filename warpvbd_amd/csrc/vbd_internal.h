@@ -177,6 +177,7 @@ struct vbd_solver {
 
     vbd::Launch launch;
     int last_iters = 0;
+    int prev_iters = 0;  // iterations of the step before the last (single-body launches)
 
     // profiling
     bool prof = false;

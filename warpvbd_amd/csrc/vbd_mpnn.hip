@@ -894,6 +894,7 @@ struct vbd_mpnn {
     bool plain_checked = false;
     uint64_t seen_edits = 0;  // the solver's edit count at the last warm step
     int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body: 0 plain, 1 network, 2 history guess
+    hipEvent_t use_alt_ready = nullptr;  // the copy of use_alt queued by the last guess is complete
     uint32_t *bmax = nullptr;                             // per body: max residual bits of the three guesses
     int counts[kGuesses] = {};                            // bodies per guess in the last warm step
     int guess_chunks = 1;                                 // chunks of the largest body's swept vertices
@@ -924,6 +925,7 @@ int mpnn_free(vbd_mpnn *m)
         if (p) (void)hipFree(p);
     if (m->use_alt_host) (void)hipHostFree(m->use_alt_host);
     if (m->loss_host) (void)hipHostFree(m->loss_host);
+    if (m->use_alt_ready) (void)hipEventDestroy(m->use_alt_ready);
     delete m;
     return 0;
 }
@@ -1100,7 +1102,7 @@ int mpnn_before_init(vbd_mpnn *m, double dt)
 // network guess x_inertia + hs c_prev + s out and, when there is a previous
 // correction, the history guess x_inertia + hs c_prev (the policy may force
 // the network's or keep the plain one).  Called after the step's init.
-int mpnn_apply_guess(vbd_mpnn *m, double dt)
+static int apply_guess_kernels(vbd_mpnn *m, double dt)
 {
     vbd_solver *s = m->s;
     hipStream_t st = s->stream;
@@ -1166,11 +1168,23 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
     return 0;
 }
 
+// The guess kernels, then the per-body choice queued for the host: the step's
+// own wait for convergence completes the copy, so reading it needs no extra
+// round trip.
+int mpnn_apply_guess(vbd_mpnn *m, double dt)
+{
+    VBD_TRY(apply_guess_kernels(m, dt));
+    hipStream_t st = m->s->stream;
+    VBD_HIP(hipMemcpyAsync(m->use_alt_host, m->use_alt, sizeof(int32_t) * m->s->n_bodies, hipMemcpyDeviceToHost,
+                           st));
+    VBD_HIP(hipEventRecord(m->use_alt_ready, st));
+    return 0;
+}
+
 int mpnn_used_guess(vbd_mpnn *m, int *used)
 {
     const int nb = m->s->n_bodies;
-    VBD_HIP(hipMemcpyAsync(m->use_alt_host, m->use_alt, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, m->s->stream));
-    VBD_HIP(hipStreamSynchronize(m->s->stream));
+    VBD_HIP(hipEventSynchronize(m->use_alt_ready));
     for (int g = 0; g < kGuesses; ++g) m->counts[g] = 0;
     for (int b = 0; b < nb; ++b) {
         const int g = m->use_alt_host[b];
@@ -1268,6 +1282,7 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
     if (hipHostMalloc((void **)&m->use_alt_host, sizeof(int32_t) * s->n_bodies, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&m->loss_host, sizeof(float), hipHostMallocDefault) != hipSuccess)
         return fail(VBD_ENOMEM);
+    if (hipEventCreateWithFlags(&m->use_alt_ready, hipEventDisableTiming) != hipSuccess) return fail(VBD_EHIP);
     // samples
     m->cap = sample_capacity;
     if (m->cap > 0) {

@@ -337,6 +337,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     VBD_HIP(hipMemcpyAsync(s->body_out_host, s->d_body_out, sizeof(int2), hipMemcpyDeviceToHost, s->stream));
     VBD_HIP(hipStreamSynchronize(s->stream));
     const int it = s->body_out_host[0].x;
+    s->prev_iters = s->last_iters;
     s->last_iters = it;
     if (iters) *iters = it;
     if (metric) *metric = (double)__builtin_bit_cast(float, (uint32_t)s->body_out_host[0].y);
@@ -358,7 +359,9 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
     // A tolerance is compared in FP32 on the device; tol = 0 never converges.
     const float ftol = (float)tol;
     int launched = 0;
-    int chunk = s->launch.chunk > 0 ? s->launch.chunk : std::max(1, s->last_iters);
+    // first chunk: the larger of the last two steps' iteration counts, so a
+    // step one iteration longer than the last needs no second host round trip
+    int chunk = s->launch.chunk > 0 ? s->launch.chunk : std::max(1, std::max(s->last_iters, s->prev_iters));
     bool conv = false;
     while (launched < max_iter && !conv) {
         const int k = std::min(chunk, max_iter - launched);
@@ -378,6 +381,7 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
     k_step_finish<<<nblk(n, 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, (float)dt);
     VBD_HIP(hipGetLastError());
     const int it = s->ctrl_host->iters;
+    s->prev_iters = s->last_iters;
     s->last_iters = it;
     if (iters) *iters = it;
     if (metric) *metric = (double)__builtin_bit_cast(float, s->ctrl_host->metric);
