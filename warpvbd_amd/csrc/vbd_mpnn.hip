@@ -656,14 +656,58 @@ __global__ void k_history(int n, const double4 *uprev, const double4 *tgt, const
 // FP64 oracle the extrapolated start cuts a 24x4x4 cantilever at E = 1e5 from
 // 4.8 to 1.8 iterations per step, and its tolerance-terminated trajectory ends
 // nearer a converged one than the first-order start's.
-__device__ __forceinline__ float3 history_part(int i, const float4 *cprev, float hs, const float4 *cprev2, float hs2)
+__device__ __forceinline__ float3 history_part_of(const float4 c, int i, float hs, const float4 *cprev2, float hs2)
 {
     if (!(hs > 0.f)) return make_float3(0.f, 0.f, 0.f);
-    const float4 c = cprev[i];
     if (!(hs2 > 0.f)) return make_float3(hs * c.x, hs * c.y, hs * c.z);
     const float4 d = cprev2[i];
     const float h = 2.f * hs;
     return make_float3(h * c.x - hs2 * d.x, h * c.y - hs2 * d.y, h * c.z - hs2 * d.z);
+}
+
+__device__ __forceinline__ float3 history_part(int i, const float4 *cprev, float hs, const float4 *cprev2, float hs2)
+{
+    return hs > 0.f ? history_part_of(cprev[i], i, hs, cprev2, hs2) : make_float3(0.f, 0.f, 0.f);
+}
+
+// The step's init and the history guess in one pass (a step that takes the
+// history guess without any comparison): with fresh, first the previous
+// step's correction c_prev = uprev - tgt (the last step's target) into cprev,
+// as k_history; then the inertia target into tgt, as the plain init; then
+// u = x_inertia + the history part, as k_guess.  Thread 0 resets the control
+// block, as k_step_init.
+__global__ void k_init_history(int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
+                               const float *mass, const uint8_t *active, float dt, float4 g, Ctrl *ctrl,
+                               float4 *cprev, int fresh, float hs, const float4 *cprev2, float hs2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctrl->it = 0;
+        ctrl->converged = 0;
+        ctrl->iters = 0;
+        ctrl->metric = 0u;
+        ctrl->blocks_done = 0u;
+    }
+    if (i >= n) return;
+    const double4 p = uprev[i];
+    if (!active[i]) {
+        if (fresh) cprev[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        u[i] = pack_disp(p.x, p.y, p.z);
+        tgt[i] = make_double4(p.x, p.y, p.z, 0.0);
+        return;
+    }
+    float4 c;
+    if (fresh) {
+        const double4 y = tgt[i];
+        c = make_float4((float)(p.x - y.x), (float)(p.y - y.y), (float)(p.z - y.z), 0.f);
+        cprev[i] = c;
+    } else {
+        c = cprev[i];
+    }
+    const double4 t = inertia_target(p, vel[i], mass[i], dt, g);
+    tgt[i] = t;
+    const float3 h = history_part_of(c, i, hs, cprev2, hs2);
+    u[i] = pack_disp(t.x + h.x, t.y + h.y, t.z + h.z);
 }
 
 // training target of the step just taken, in network order:
@@ -893,6 +937,9 @@ struct vbd_mpnn {
     int plain_skip_left = 0, plain_backoff = 0;
     bool plain_checked = false;
     uint64_t seen_edits = 0;  // the solver's edit count at the last warm step
+    bool hist_pending = false;  // the correction pass was deferred to mpnn_init
+    // this step's guess decision (made once, before or at the guess)
+    bool decided = false, dec_skipping = false, dec_plain_out = false, dec_direct_done = false;
     int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body: 0 plain, 1 network, 2 history guess
     hipEvent_t use_alt_ready = nullptr;  // the copy of use_alt queued by the last guess is complete
     uint32_t *bmax = nullptr;                             // per body: max residual bits of the three guesses
@@ -1062,10 +1109,11 @@ namespace vbd {
 // The corrections of the last two finished steps, kept when the solver still
 // holds the last one; the one before counts only when its step came right
 // before (no step in between that this network did not see).
-int mpnn_before_init(vbd_mpnn *m, double dt)
+int mpnn_before_init(vbd_mpnn *m, double dt, bool defer)
 {
     vbd_solver *s = m->s;
     m->hs = m->hs2 = 0.f;
+    m->hist_pending = false;
     if (!s->hist_valid || !(s->hist_dt > 0.0)) {
         m->have_c = false;
         return 0;
@@ -1083,8 +1131,12 @@ int mpnn_before_init(vbd_mpnn *m, double dt)
             m->c2_dt = 0.0;
         }
     }
-    k_history<<<nblk(m->n, 256), 256, 0, s->stream>>>(m->n, s->uprev, s->tgt, s->active, m->cprev);
-    VBD_HIP(hipGetLastError());
+    if (defer) {
+        m->hist_pending = true;
+    } else {
+        k_history<<<nblk(m->n, 256), 256, 0, s->stream>>>(m->n, s->uprev, s->tgt, s->active, m->cprev);
+        VBD_HIP(hipGetLastError());
+    }
     m->have_c = true;
     m->c_step = s->step_serial;
     m->c_dt = s->hist_dt;
@@ -1097,25 +1149,78 @@ int mpnn_before_init(vbd_mpnn *m, double dt)
     return 0;
 }
 
+static bool decide(vbd_mpnn *m);
+
+int mpnn_init(vbd_mpnn *m, int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
+              const float *mass, const uint8_t *active, float dt, float4 g, Ctrl *ctrl, bool *done)
+{
+    vbd_solver *s = m->s;
+    *done = false;
+    if (decide(m)) {
+        k_init_history<<<nblk(std::max(n, 1), 256), 256, 0, s->stream>>>(
+            n, u, uprev, vel, tgt, mass, active, dt, g, ctrl, m->cprev, m->hist_pending ? 1 : 0, m->hs, m->cprev2,
+            m->hs2);
+        VBD_HIP(hipGetLastError());
+        m->hist_pending = false;
+        m->dec_direct_done = true;
+        *done = true;
+        return 0;
+    }
+    if (m->hist_pending) {
+        k_history<<<nblk(m->n, 256), 256, 0, s->stream>>>(m->n, s->uprev, s->tgt, s->active, m->cprev);
+        VBD_HIP(hipGetLastError());
+        m->hist_pending = false;
+    }
+    return 0;
+}
+
 // Replace the plain initial guess in s->u, per body, by the warm guess with
 // the smallest first-sweep residual when it beats the plain guess's: the
 // network guess x_inertia + hs c_prev + s out and, when there is a previous
 // correction, the history guess x_inertia + hs c_prev (the policy may force
 // the network's or keep the plain one).  Called after the step's init.
-static int apply_guess_kernels(vbd_mpnn *m, double dt)
+// This step's guess decision, taken once per step (the skip counters count
+// down here).  Returns whether the step takes the history guess without any
+// comparison.
+static bool decide(vbd_mpnn *m)
 {
     vbd_solver *s = m->s;
-    hipStream_t st = s->stream;
+    m->decided = true;
+    m->dec_direct_done = false;
     m->evaluated = false;
     m->plain_checked = false;
     const bool skipping = m->policy == VBD_WARM_ADAPTIVE && m->skip_left > 0;  // network skipped this step
     if (skipping) --m->skip_left;
+    m->dec_skipping = skipping;
     const bool history = m->policy == VBD_WARM_HISTORY || skipping;
     const bool have_hist = m->hs > 0.f;
     // the history chain broke, or the state or parameters were changed from
     // outside: compare the plain guess again
     if (!have_hist || s->edits != m->seen_edits) m->plain_skip_left = m->plain_backoff = 0;
     m->seen_edits = s->edits;
+    m->dec_plain_out = false;
+    if (m->policy == VBD_WARM_PLAIN || (history && !have_hist)) return false;
+    const bool force = m->policy == VBD_WARM_NETWORK;
+    // With a history guess the plain guess is compared only on check steps:
+    // after a check that no body took it, the next 1, 2, 4, ... 16 such steps
+    // leave it out (the solver iterates to the same tolerance from any start,
+    // so this only saves the comparison).
+    const bool plain_out = have_hist && !force && m->plain_skip_left > 0;
+    if (plain_out) --m->plain_skip_left;
+    m->dec_plain_out = plain_out;
+    m->plain_checked = have_hist && !force && !plain_out;
+    return history && plain_out;
+}
+
+static int apply_guess_kernels(vbd_mpnn *m, double dt)
+{
+    vbd_solver *s = m->s;
+    hipStream_t st = s->stream;
+    if (!m->decided) decide(m);
+    m->decided = false;
+    const bool skipping = m->dec_skipping;
+    const bool history = m->policy == VBD_WARM_HISTORY || skipping;
+    const bool have_hist = m->hs > 0.f;
     if (m->policy == VBD_WARM_PLAIN || (history && !have_hist)) {
         k_fill_u32<<<nblk(s->n_bodies, 256), 256, 0, st>>>(s->n_bodies, reinterpret_cast<uint32_t *>(m->use_alt),
                                                            0u);
@@ -1123,17 +1228,13 @@ static int apply_guess_kernels(vbd_mpnn *m, double dt)
         return 0;
     }
     const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
-    // With a history guess the plain guess is compared only on check steps:
-    // after a check that no body took it, the next 1, 2, 4, ... 16 such steps
-    // leave it out (the solver iterates to the same tolerance from any start,
-    // so this only saves the comparison).
-    const bool plain_out = have_hist && !force && m->plain_skip_left > 0;
-    if (plain_out) --m->plain_skip_left;
-    m->plain_checked = have_hist && !force && !plain_out;
+    const bool plain_out = m->dec_plain_out;
     if (history && plain_out) {
-        // the history guess is the only candidate: straight into u
-        k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev, m->hs,
-                                                 m->cprev2, m->hs2, s->u);
+        // the history guess is the only candidate: straight into u (unless
+        // the fused init already wrote it)
+        if (!m->dec_direct_done)
+            k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev,
+                                                     m->hs, m->cprev2, m->hs2, s->u);
         k_fill_u32<<<nblk(s->n_bodies, 256), 256, 0, st>>>(s->n_bodies, reinterpret_cast<uint32_t *>(m->use_alt),
                                                            2u);
         VBD_HIP(hipGetLastError());

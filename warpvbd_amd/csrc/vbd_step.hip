@@ -352,8 +352,13 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
     const int n = s->n;
     s->kd_dt = (float)(s->damping / dt);
     const float4 g = make_float4((float)s->grav[0], (float)s->grav[1], (float)s->grav[2], 0.f);
-    k_step_init<<<nblk(std::max(n, 1), 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, s->tgt, s->mass,
-                                                                   s->active, (float)dt, g, s->ctrl);
+    bool fused = false;
+    if (s->warm)
+        VBD_TRY(mpnn_init(s->warm, n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, (float)dt, g, s->ctrl,
+                          &fused));
+    if (!fused)
+        k_step_init<<<nblk(std::max(n, 1), 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, s->tgt, s->mass,
+                                                                       s->active, (float)dt, g, s->ctrl);
     VBD_HIP(hipGetLastError());
     if (s->warm) VBD_TRY(mpnn_apply_guess(s->warm, dt));
     // A tolerance is compared in FP32 on the device; tol = 0 never converges.
@@ -404,7 +409,8 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
 
 int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric)
 {
-    if (s->warm) VBD_TRY(mpnn_before_init(s->warm, dt));
+    // the correction pass is deferred to the init (mpnn_init), which may fold it in
+    if (s->warm) VBD_TRY(mpnn_before_init(s->warm, dt, true));
     const int rc = step_device_run(s, dt, tol, max_iter, iters, metric);
     s->hist_valid = rc == 0;
     s->hist_dt = dt;

@@ -386,17 +386,21 @@ struct StepParams {
     int skip_init;  // the initial guess is already in u (warm start)
 };
 
+// inertia target x_prev + dt v + dt^2 g of an active vertex (xyz) and M/dt^2 (w)
+__device__ __forceinline__ double4 inertia_target(const double4 p, const float4 v, float mass, float dt, float4 g)
+{
+    const double h = dt, h2 = h * h;
+    // explicit fused forms: every kernel that inlines this rounds alike
+    return make_double4(fma(h2, (double)g.x, fma(h, (double)v.x, p.x)), fma(h2, (double)g.y, fma(h, (double)v.y, p.y)),
+                        fma(h2, (double)g.z, fma(h, (double)v.z, p.z)), (double)mass / h2);
+}
+
 __device__ __forceinline__ void init_vertex(int i, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt,
                                             const float *mass, const uint8_t *active, float dt, float4 g)
 {
     const double4 p = uprev[i];
     if (active[i]) {
-        const float4 v = vel[i];
-        const double h = dt, h2 = h * h;
-        // explicit fused forms: every kernel that inlines this rounds alike
-        const double4 t = make_double4(fma(h2, (double)g.x, fma(h, (double)v.x, p.x)),
-                                       fma(h2, (double)g.y, fma(h, (double)v.y, p.y)),
-                                       fma(h2, (double)g.z, fma(h, (double)v.z, p.z)), (double)mass[i] / h2);
+        const double4 t = inertia_target(p, vel[i], mass[i], dt, g);
         u[i] = pack_disp(t.x, t.y, t.z);
         tgt[i] = t;
     } else {
