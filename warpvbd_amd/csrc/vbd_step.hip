@@ -56,6 +56,23 @@ __global__ __launch_bounds__(1024) void k_sweep(
     if (!live) return;
     if (threadIdx.x == 0) a.part[blockIdx.x] = mb;
     if (!close) return;
+    if (gridDim.x == 1) {
+        // The only block of the iteration's last launch: the other launches'
+        // partial maxima were written by kernels that finished before this one
+        // started, and this block's own is mb (thread 0), so no device-scope
+        // fence or block counter is needed.  Its own slot still holds an older
+        // value for the other threads, so it is skipped.
+        const int own = (int)(a.part - part_all);
+        uint32_t x = 0;
+        for (int i = threadIdx.x; i < np_all; i += blockDim.x) {
+            const uint32_t y = i == own ? 0u : part_all[i];
+            x = x > y ? x : y;
+        }
+        __syncthreads();  // block_max_u32's staging is free again
+        x = block_max_u32(x);
+        if (threadIdx.x == 0) close_iteration(c, c->it, x > mb ? x : mb, tol);
+        return;
+    }
     if (threadIdx.x == 0) {
         __threadfence();
         const uint32_t prev = atomicAdd(&c->blocks_done, 1u);
