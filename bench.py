@@ -207,9 +207,10 @@ def batched_section(args):
     w_plain, it_plain, ms_plain = time_steps(plain, args.batch_steps, args.dt)
     del plain
     bat = BatchedVBDSolver(bodies, dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
-    tune = bat.autotune(args.dt, trial_steps=2)
     for _ in range(3):
         bat.step(dt=args.dt)
+    # tuned on steps like the timed ones (trials start from this state, which is restored)
+    tune = bat.autotune(args.dt, trial_steps=30, cache=False)
     w_bat, it_bat, ms_bat = time_steps(bat, args.batch_steps, args.dt)
     nb = len(bodies)
     sim = args.batch_steps * args.dt
@@ -366,7 +367,6 @@ def mixed_section(args):
                                  "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)]}
     del plain
     bat = BatchedVBDSolver(bodies, dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
-    tune = bat.autotune(args.dt, trial_steps=2)
     ws = WarmStart(bat, capacity=args.mixed_samples, seed=0)
     t0 = time.perf_counter()
     for _ in range(args.mixed_samples):
@@ -377,6 +377,8 @@ def mixed_section(args):
     t_train = time.perf_counter() - t0
     for _ in range(3):
         bat.step(dt=args.dt)
+    # tuned on steps like the timed ones (trials start from this state, which is restored)
+    tune = bat.autotune(args.dt, trial_steps=30, cache=False)
     # every timed form starts from this state, after one untimed step of its own
     x0, v0 = bat.old_positions.numpy(), bat.old_velocities.numpy()
 

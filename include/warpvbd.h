@@ -137,9 +137,12 @@ int vbd_batch_set_mode(vbd_solver_t *h, int32_t mode, int32_t lanes_per_vertex, 
                        int32_t use_graph);
 int vbd_batch_get_mode(vbd_solver_t *h, int32_t *mode, int32_t *lanes_per_vertex, int32_t *block_size,
                        int32_t *use_graph);
-/* Autotuner: times every (form, lanes per vertex, block size) candidate for
- * trial_steps steps from the current state, restores the state, keeps the
- * fastest and returns its ms per step. */
+/* Autotuner: times every (form, lanes per vertex 2/4/8, block size) candidate
+ * for trial_steps steps from the current state (each run twice, the second
+ * timed, so that graph captures stay out of the timing), restores the state,
+ * keeps the fastest and returns its ms per step.  Short trials favour the
+ * one-workgroup-per-body form early in a trajectory; trials as long as the
+ * run to be tuned pick best. */
 int vbd_batch_autotune(vbd_solver_t *h, double dt, double tol, int32_t max_iter, int32_t trial_steps,
                        double *best_ms_per_step);
 
