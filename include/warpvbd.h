@@ -225,12 +225,20 @@ int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t
  * per-tet arrays, src/vbd_solver.py:295-343, are what the records hold). */
 void vbd_use_record_dictionary(int32_t enable);
 int vbd_records(vbd_solver_t *h, int32_t *n_dictionary, int32_t *bytes_per_slot);
+/* Neighbour staging of single-body solvers with a shared dictionary: colours
+ * of at least small_color vertices swept with 4 lanes per vertex and 256
+ * threads read each block's distinct neighbours and the dictionary once into
+ * LDS and an 8-byte record per slot (the results are the same bit for bit).
+ * *n_colors: the colours that use it.  No counterpart in WarpVBD. */
+int vbd_staged_colors(vbd_solver_t *h, int32_t *n_colors);
 /* bytes[n_colors]: the HBM bytes one sweep of each colour reads and writes
  * with this layout, sum_v (68 + bytes_per_slot k_v) + 16 |gathered neighbours|:
  * per swept vertex its 16-byte packed position, 32-byte FP64 inertia target,
  * 4-byte slot offset and 16-byte write-back; per gathered neighbour its packed
- * position (vbd_color_bytes is the algorithmic model: 40 bytes per vertex, the
- * 68 bytes per slot of the reference's per-tet arrays, 12 per neighbour). */
+ * position; with neighbour staging (vbd_staged_colors) 8 bytes per slot plus
+ * 4 per staged neighbour id (vbd_color_bytes is the algorithmic model: 40
+ * bytes per vertex, the 68 bytes per slot of the reference's per-tet arrays,
+ * 12 per neighbour). */
 int vbd_color_layout_bytes(vbd_solver_t *h, double *bytes);
 
 /* Mesh facts computed at create. */

@@ -63,6 +63,7 @@ SIGNATURES = {
     "vbd_use_record_dictionary": (None, [_i32]),
     "vbd_color_layout_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_d)]),
     "vbd_records": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "vbd_staged_colors": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "vbd_mpnn_guess_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "vbd_mpnn_set_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_get_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),

@@ -34,7 +34,8 @@ void vbd_destroy(vbd_solver_t *s)
     release_batch(s);
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
-                   s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict};
+                   s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict,
+                   s->st_off, s->st_nb, s->st_rec};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (double4 *p : s->snap)
@@ -87,6 +88,7 @@ int vbd_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, 
             break;
         }
     r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active, nullptr, 1, nullptr);
+    if (r == 0) r = build_staging(s);
     if (r != 0) {
         vbd_destroy(s);
         return r;
@@ -568,7 +570,22 @@ int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
         const double nv = (double)(s->col_start[c + 1] - s->col_start[c]);
         // packed position 16 + FP64 target 32 + slot offset 4 + write-back 16
         bytes[c] = 68.0 * nv + per_slot * s->col_slots[c] + 16.0 * s->col_boundary[c];
+        // staged sweep (4 lanes, 256 threads, the default launch): 8-byte
+        // slot records, plus each block's list of neighbour ids
+        if (s->st_ok && c < (int)s->st_blk0.size() && s->st_blk0[c] >= 0 && s->launch.lanes == 4 &&
+            s->launch.block == 256)
+            bytes[c] = 68.0 * nv + 8.0 * s->col_slots[c] + 16.0 * s->col_boundary[c] + 4.0 * (double)s->st_ids[c];
     }
+    return 0;
+}
+
+int vbd_staged_colors(vbd_solver_t *s, int32_t *n_colors)
+{
+    CHECK_ARG(s && n_colors, "null argument");
+    int k = 0;
+    if (s->st_ok)
+        for (int b : s->st_blk0) k += b >= 0;
+    *n_colors = k;
     return 0;
 }
 

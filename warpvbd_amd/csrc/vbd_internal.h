@@ -14,6 +14,10 @@ namespace vbd {
 
 constexpr int kMaxColors = 256;            // src/coloring.py:50
 constexpr int kMaxElementsPerVertex = 32;  // src/vbd_solver.py:12
+// neighbour staging of the big-colour sweep (4 lanes per vertex, 256 threads)
+constexpr int kStageRows = 64;    // vertices per block
+constexpr int kStageMax = 1024;   // distinct neighbours a block may stage (LDS: 24 bytes each)
+constexpr int kStageDict = 64;    // dictionary entries staged with them (48 bytes each)
 
 void set_error(const std::string &msg);
 
@@ -162,6 +166,16 @@ struct vbd_solver {
     float4 *slots = nullptr;      // [n_slots] compact slot records (ids, entry), or tet records [m*4]
     float2 *slots2 = nullptr;     // full records' qd while they are built; null afterwards
     float4 *dict = nullptr;       // [n_dict*3] record dictionary (vbd_device.h)
+    // Neighbour staging for the 4-lane, 256-thread sweep of colours of at least
+    // small_color vertices (single-body solvers with a shared dictionary): per
+    // block of kStageRows vertices its distinct neighbours, and per slot an
+    // 8-byte record of local indices into that list plus the dictionary entry.
+    bool st_ok = false;
+    std::vector<int32_t> st_blk0;  // per colour: its first block, or -1 (not staged)
+    std::vector<int64_t> st_ids;   // per colour: neighbour ids its blocks stage (sum over blocks)
+    int32_t *st_off = nullptr;     // [blocks + 1] start of each block's list in st_nb
+    int32_t *st_nb = nullptr;      // neighbour ids (colour-major)
+    uint2 *st_rec = nullptr;       // [n_slots] (l0 | l1 << 16, l2 | entry << 16)
     int32_t n_dict = 0;
     bool dict_shared = false;     // entries shared between slots (else one per slot)
     int32_t *colors = nullptr;    // original order
@@ -209,6 +223,7 @@ int import_positions_dev(vbd_solver *s, const double *dev_in, double4 *dst);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);
+int build_staging(vbd_solver *s);  // neighbour staging tables (st_*), when the mesh allows
 extern bool g_record_dictionary;  // share record dictionary entries in later creates (default on)
 hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,
                                       hipStream_t st);

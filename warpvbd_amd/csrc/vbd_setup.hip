@@ -1044,4 +1044,72 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     return 0;
 }
 
+// Neighbour staging tables (the st_* fields).  Colours of at least
+// small_color vertices are cut into blocks of kStageRows consecutive vertices,
+// as the 4-lane, 256-thread sweep assigns them; each block's distinct
+// neighbour ids are listed in order of first appearance, and each of its slots
+// gets (l0 | l1 << 16, l2 | entry << 16) with l_k the local indices of the
+// slot's other vertices.  Needs a shared dictionary (entries below 2^16); a
+// block with more than kStageMax neighbours turns staging off for the solver.
+int build_staging(vbd_solver *s)
+{
+    s->st_ok = false;
+    s->st_blk0.assign(s->ncolors, -1);
+    if (!s->dict_shared || s->n_dict > kStageDict || s->n_slots <= 0) return 0;
+    hipStream_t st = s->stream;
+    const int nsw = s->n_swept;
+    std::vector<int32_t> off(nsw + 1);
+    std::vector<float4> rec(s->n_slots);
+    VBD_HIP(hipMemcpyAsync(off.data(), s->slot_off, sizeof(int32_t) * (nsw + 1), hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipMemcpyAsync(rec.data(), s->slots, sizeof(float4) * s->n_slots, hipMemcpyDeviceToHost, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    std::vector<int32_t> blk_off(1, 0), nb;
+    std::vector<uint2> srec(s->n_slots, make_uint2(0u, 0u));
+    std::vector<int32_t> stamp(s->n, -1), local(s->n, 0);
+    std::vector<int32_t> blk0(s->ncolors, -1);
+    std::vector<int64_t> ids(s->ncolors, 0);
+    int nblocks = 0;
+    for (int c = 0; c < s->ncolors; ++c) {
+        const int nv = s->col_size[c];
+        if (nv < s->launch.small_color) continue;
+        blk0[c] = nblocks;
+        const int c0 = s->col_start[c];
+        for (int j = 0; j < nv; j += kStageRows, ++nblocks) {
+            const int va = c0 + j, vb = c0 + std::min(nv, j + kStageRows);
+            int cnt = 0;
+            for (int64_t q = off[va]; q < off[vb]; ++q) {
+                uint32_t w[4];
+                std::memcpy(w, &rec[q], sizeof(w));
+                uint32_t l[3];
+                for (int k = 0; k < 3; ++k) {
+                    const int id = (int)w[k];
+                    if (stamp[id] != nblocks) {
+                        stamp[id] = nblocks;
+                        local[id] = cnt++;
+                        nb.push_back(id);
+                    }
+                    l[k] = (uint32_t)local[id];
+                }
+                srec[q] = make_uint2(l[0] | l[1] << 16, l[2] | w[3] << 16);
+            }
+            if (cnt > kStageMax) return 0;
+            ids[c] += cnt;
+            blk_off.push_back((int32_t)nb.size());
+        }
+    }
+    if (nblocks == 0) return 0;
+    VBD_TRY(dmalloc(&s->st_off, blk_off.size()));
+    VBD_TRY(dmalloc(&s->st_nb, std::max<size_t>(1, nb.size())));
+    VBD_TRY(dmalloc(&s->st_rec, srec.size()));
+    VBD_HIP(hipMemcpyAsync(s->st_off, blk_off.data(), sizeof(int32_t) * blk_off.size(), hipMemcpyHostToDevice, st));
+    if (!nb.empty())
+        VBD_HIP(hipMemcpyAsync(s->st_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice, st));
+    VBD_HIP(hipMemcpyAsync(s->st_rec, srec.data(), sizeof(uint2) * srec.size(), hipMemcpyHostToDevice, st));
+    VBD_HIP(hipStreamSynchronize(st));
+    s->st_blk0 = blk0;
+    s->st_ids = ids;
+    s->st_ok = true;
+    return 0;
+}
+
 }  // namespace vbd

@@ -36,24 +36,13 @@ namespace {
 // the block that finishes last (device counter in Ctrl) folds all np_all
 // partial maxima of the iteration and decides convergence
 // (src/vbd_solver.py:578), so no separate check launch is needed.
-template <int L, int B, int MAT>
-__global__ __launch_bounds__(1024) void k_sweep(
-    SweepArgs a, int v0, int nv, int close, const uint32_t *part_all, int np_all, float tol)
+// A launch's end: the block's metric maximum mb (thread 0) into
+// a.part[blockIdx.x]; with close, the iteration's last launch folds every
+// partial maximum and decides convergence.
+__device__ __forceinline__ void finish_sweep(const SweepArgs &a, Ctrl *c, uint32_t mb, int close,
+                                             const uint32_t *part_all, int np_all, float tol)
 {
     __shared__ int s_last;
-    Ctrl *c = a.ctrl;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int t = b * blockDim.x + threadIdx.x;
-    const int vi = t / L, lane = t % L;
-    const bool valid = vi < nv;
-    const int v = v0 + (valid ? vi : 0);
-    // The vertex's own loads and the converged flag are requested together: a
-    // launch after convergence sweeps nothing (every lane without a vertex)
-    // and stores nothing, instead of branching on the flag before its loads.
-    const VertexPre pre = vertex_pre(a, v, valid);
-    const bool live = !c->converged;
-    const uint32_t mb = block_max_u32(update_vertex<L, B, MAT>(a, v, valid && live, lane, pre));
-    if (!live) return;
     if (threadIdx.x == 0) a.part[blockIdx.x] = mb;
     if (!close) return;
     if (gridDim.x == 1) {
@@ -92,6 +81,72 @@ __global__ __launch_bounds__(1024) void k_sweep(
         c->blocks_done = 0u;
         close_iteration(c, it, x, tol);
     }
+}
+
+// One colour: a group of L lanes per vertex.  Each block leaves its metric
+// maximum in a.part[blockIdx.x].  close != 0 on the iteration's last launch:
+// the block that finishes last (device counter in Ctrl) folds all np_all
+// partial maxima of the iteration and decides convergence
+// (src/vbd_solver.py:578), so no separate check launch is needed.
+template <int L, int B, int MAT>
+__global__ __launch_bounds__(1024) void k_sweep(
+    SweepArgs a, int v0, int nv, int close, const uint32_t *part_all, int np_all, float tol)
+{
+    Ctrl *c = a.ctrl;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = b * blockDim.x + threadIdx.x;
+    const int vi = t / L, lane = t % L;
+    const bool valid = vi < nv;
+    const int v = v0 + (valid ? vi : 0);
+    // The vertex's own loads and the converged flag are requested together: a
+    // launch after convergence sweeps nothing (every lane without a vertex)
+    // and stores nothing, instead of branching on the flag before its loads.
+    const VertexPre pre = vertex_pre(a, v, valid);
+    const bool live = !c->converged;
+    const uint32_t mb = block_max_u32(update_vertex<L, B, MAT>(a, v, valid && live, lane, pre));
+    if (!live) return;
+    finish_sweep(a, c, mb, close, part_all, np_all, tol);
+}
+
+// k_sweep<4, 1, MAT> with neighbour staging (256 threads, kStageRows
+// vertices per block): the block first unpacks its distinct neighbours
+// (st_nb[st_off[blk0 + b] ..]) and the record dictionary into LDS, so a slot
+// reads one 8-byte record from memory instead of a 16-byte record, three
+// dictionary rows and three positions.  Neighbours belong to other colours,
+// which this launch does not write, so staging them at the start is exact.
+template <int MAT>
+__global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int nv, int close,
+                                                      const uint32_t *part_all, int np_all, float tol,
+                                                      const int32_t *st_off, const int32_t *st_nb,
+                                                      const uint2 *st_rec, int blk0, int n_dict)
+{
+    constexpr int L = 4;
+    __shared__ float4 s_na[kStageMax];
+    __shared__ float2 s_nb[kStageMax];
+    __shared__ float4 s_dict[3 * kStageDict];
+    Ctrl *c = a.ctrl;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = b * blockDim.x + threadIdx.x;
+    const int vi = t / L, lane = t % L;
+    const bool valid = vi < nv;
+    const int v = v0 + (valid ? vi : 0);
+    const VertexPre pre = vertex_pre(a, v, valid);
+    const bool live = !c->converged;  // the same for every thread of the block
+    if (live) {
+        const int n0 = st_off[blk0 + b], cnt = st_off[blk0 + b + 1] - n0;
+        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+            const float4 p = a.u[st_nb[n0 + k]];
+            const float3 lo = disp_lo(p);
+            s_na[k] = make_float4(p.x, p.y, p.z, lo.x);
+            s_nb[k] = make_float2(lo.y, lo.z);
+        }
+        for (int k = threadIdx.x; k < 3 * n_dict; k += blockDim.x) s_dict[k] = a.dict[k];
+        __syncthreads();
+    }
+    const Staged stg{st_rec, s_na, s_nb, s_dict};
+    const uint32_t mb = block_max_u32(update_vertex<L, 1, MAT, true>(a, v, valid && live, lane, pre, &stg));
+    if (!live) return;
+    finish_sweep(a, c, mb, close, part_all, np_all, tol);
 }
 
 // Iteration close when no colour has a vertex to sweep.
@@ -174,6 +229,19 @@ void launch_sweep(const SweepArgs &a, int v0, int nv, int block, int close, cons
     const int64_t threads = (int64_t)nv * L;
     hipExtLaunchKernelGGL(k_sweep<L, Batch<L>::value, MAT>, dim3(nblk(threads, block)), dim3(block), 0, st, e0, e1, 0,
                           a, v0, nv, close, part_all, np_all, tol);
+}
+
+void launch_staged(int material, const SweepArgs &a, int v0, int nv, int close, const uint32_t *part_all,
+                   int np_all, float tol, const int32_t *st_off, const int32_t *st_nb, const uint2 *st_rec, int blk0,
+                   int n_dict, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
+{
+    const dim3 grid(nblk(nv, kStageRows));
+    if (material == kSNH)
+        hipExtLaunchKernelGGL(k_sweep_staged<kSNH>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, close, part_all, np_all,
+                              tol, st_off, st_nb, st_rec, blk0, n_dict);
+    else
+        hipExtLaunchKernelGGL(k_sweep_staged<kStVK>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, close, part_all,
+                              np_all, tol, st_off, st_nb, st_rec, blk0, n_dict);
 }
 
 template <int MAT>
@@ -270,7 +338,14 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
         const int v0 = s->col_start[g.c0], nv = s->col_size[g.c0];
         hipEvent_t ev[2] = {nullptr, nullptr};
         if (profile && nv >= s->prof_min) VBD_TRY(record_event(s, g.c0, iter, ev));
-        if (s->material == kSNH)
+        // neighbour staging: built for 4 lanes per vertex and 256 threads
+        const int blk0 = s->st_ok && g.lanes == 4 && block == 256 && g.c0 < (int)s->st_blk0.size()
+                             ? s->st_blk0[g.c0]
+                             : -1;
+        if (blk0 >= 0)
+            launch_staged(s->material, a, v0, nv, close, s->part, s->n_parts, tol, s->st_off, s->st_nb,
+                          s->st_rec, blk0, s->n_dict, ev[0], ev[1], s->stream);
+        else if (s->material == kSNH)
             dispatch_sweep<kSNH>(g.lanes, a, v0, nv, block, close, s->part, s->n_parts, tol, ev[0], ev[1], s->stream);
         else
             dispatch_sweep<kStVK>(g.lanes, a, v0, nv, block, close, s->part, s->n_parts, tol, ev[0], ev[1], s->stream);

@@ -178,13 +178,24 @@ __device__ __forceinline__ VertexPre vertex_pre(const SweepArgs &a, int v, bool 
     return p;
 }
 
+// A block's staged sweep inputs (the big-colour sweep with neighbour
+// staging): per slot an 8-byte record (l0 | l1 << 16, l2 | entry << 16) in
+// memory, and in LDS the block's distinct neighbours unpacked (FP32 parts
+// and remainders) and the record dictionary.
+struct Staged {
+    const uint2 *rec;
+    const float4 *na;    // (x, y, z, remainder x) per neighbour
+    const float2 *nb;    // (remainder y, remainder z)
+    const float4 *dict;  // 3 rows per entry
+};
+
 // Gradient and Hessian of one vertex's local energy (all incident tets plus
 // inertia and damping) accumulated by a group of L lanes, and the block solve:
 // the update dx (zero for lanes without a vertex).  us receives the vertex's
-// current displacement.
-template <int L, int B, int MAT>
+// current displacement.  STAGED: slots read through stg (B == 1).
+template <int L, int B, int MAT, bool STAGED = false>
 __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, double3 &us,
-                                            const VertexPre &pre)
+                                            const VertexPre &pre, const Staged *stg = nullptr)
 {
     Acc acc;
     acc_zero(acc);
@@ -241,7 +252,29 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             }
         }
 #else
-        if constexpr (B == 1) {
+        if constexpr (STAGED) {
+            // One slot per lane per batch from the staged neighbours and
+            // dictionary; only the 8-byte record comes from memory, and the
+            // next one is requested before this slot's LDS reads.  The
+            // differences are formed exactly as slot_rel forms them.
+            int s = s0 + lane;
+            uint2 r = s < s1 ? stg->rec[s] : make_uint2(0u, 0u);
+            for (; s < s1; s += L) {
+                const uint2 rn = stg->rec[min(s + L, s1 - 1)];  // the last one reloads a record in cache
+                const uint32_t e = r.y >> 16, l0 = r.x & 0xffffu, l1 = r.x >> 16, l2 = r.y & 0xffffu;
+                const float4 d0 = stg->dict[3 * e], d1 = stg->dict[3 * e + 1], d2 = stg->dict[3 * e + 2];
+                const float4 p0 = stg->na[l0], p1 = stg->na[l1], p2 = stg->na[l2];
+                const float2 q0 = stg->nb[l0], q1 = stg->nb[l1], q2 = stg->nb[l2];
+                const float3 m0 = make_float3(p0.w, q0.x, q0.y), m1 = make_float3(p1.w, q1.x, q1.y),
+                             m2 = make_float3(p2.w, q2.x, q2.y);
+                const float3 e0 = diff_packed(p0, m0, p2, m2), e1 = diff_packed(p1, m1, p2, m2);
+                const float3 f2 = make_float3((p2.x - hi.x) + (m2.x - lo.x), (p2.y - hi.y) + (m2.y - lo.y),
+                                              (p2.z - hi.z) + (m2.z - lo.z));
+                slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
+                               make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), e0, e1, f2, acc);
+                r = rn;
+            }
+        } else if constexpr (B == 1) {
             // One slot per lane per batch, software-pipelined: the next compact
             // record is requested after the current slot's constants and
             // gathers, so each batch waits for one memory latency, not two.
@@ -345,12 +378,12 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
     return vertex_dx<L, B, MAT>(a, v, valid, lane, us, vertex_pre(a, v, valid));
 }
 
-template <int L, int B, int MAT>
+template <int L, int B, int MAT, bool STAGED = false>
 __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane,
-                                                  const VertexPre &pre)
+                                                  const VertexPre &pre, const Staged *stg = nullptr)
 {
     double3 us;
-    const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us, pre);
+    const float3 dx = vertex_dx<L, B, MAT, STAGED>(a, v, valid, lane, us, pre, stg);
     if (!valid) return 0u;
     if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
     return metric_bits(dx);

@@ -340,6 +340,13 @@ class VBDSolver:
         check(_lib.load().vbd_get_color_sizes(self._h, ptr(out, ctypes.c_int32)), "vbd_get_color_sizes")
         return out[: self.n_colors]
 
+    def staged_colors(self) -> int:
+        """Colours whose sweep reads its neighbours and constants through
+        LDS staging (4 lanes per vertex, 256 threads; same results)."""
+        n = ctypes.c_int32()
+        check(_lib.load().vbd_staged_colors(self._h, ctypes.byref(n)), "vbd_staged_colors")
+        return n.value
+
     def record_layout(self) -> dict:
         """Slot records of this solver: record dictionary entries (as many as
         slots when no constants are shared) and HBM bytes per slot read by a
