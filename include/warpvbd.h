@@ -226,11 +226,13 @@ int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t
 void vbd_use_record_dictionary(int32_t enable);
 int vbd_records(vbd_solver_t *h, int32_t *n_dictionary, int32_t *bytes_per_slot);
 /* Neighbour staging of single-body solvers with a shared dictionary: colours
- * of at least small_color vertices swept with 4 lanes per vertex and 256
+ * of at least 40,960 vertices swept with 4 lanes per vertex and 256
  * threads read each block's distinct neighbours and the dictionary once into
  * LDS and an 8-byte record per slot (the results are the same bit for bit).
  * *n_colors: the colours that use it.  No counterpart in WarpVBD. */
 int vbd_staged_colors(vbd_solver_t *h, int32_t *n_colors);
+/* Neighbour staging in solvers created afterwards (process-wide; default on). */
+void vbd_use_neighbour_staging(int32_t enable);
 /* bytes[n_colors]: the HBM bytes one sweep of each colour reads and writes
  * with this layout, sum_v (68 + bytes_per_slot k_v) + 16 |gathered neighbours|:
  * per swept vertex its 16-byte packed position, 32-byte FP64 inertia target,

@@ -244,16 +244,19 @@ def test_free_fall_exact_and_one_iteration():
 
 @pytest.mark.parametrize("material", ["snh", "stvk"])
 def test_neighbour_staging_is_bit_exact(material):
-    """The big-colour sweep with neighbour staging (4 lanes, 256 threads: the
-    default launch) against the same sweep without it (128-thread blocks do
-    not stage) on the 128x16x16 beam: the per-slot arithmetic is the same and
-    only where neighbours and constants are read from differs, so positions
-    and iteration counts agree bit for bit."""
-    sc = cantilever((128, 16, 16))
+    """The big-colour sweep with neighbour staging against the same launch
+    without it on the 224x28x28 beam (colours of 47k vertices): the per-slot
+    arithmetic is the same and only where neighbours and constants are read
+    from differs, so positions and iteration counts agree bit for bit."""
+    from warpvbd_amd import _lib
+    sc = cantilever((224, 28, 28))
     a = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
-    b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
-    b.set_launch(4, 128)
-    assert a.staged_colors() == 4
+    _lib.use_neighbour_staging(False)
+    try:
+        b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
+    finally:
+        _lib.use_neighbour_staging(True)
+    assert a.staged_colors() == 4 and b.staged_colors() == 0
     for _ in range(6):
         xa = a.step(a.old_positions, 1e-3).numpy()
         xb = b.step(b.old_positions, 1e-3).numpy()

@@ -61,6 +61,7 @@ SIGNATURES = {
     "vbd_mpnn_num_params": (ctypes.c_int, []),
     "vbd_mpnn_set_policy": (ctypes.c_int, [_vp, _i32]),
     "vbd_use_record_dictionary": (None, [_i32]),
+    "vbd_use_neighbour_staging": (None, [_i32]),
     "vbd_color_layout_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_d)]),
     "vbd_records": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "vbd_staged_colors": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
@@ -153,6 +154,13 @@ def use_record_dictionary(enable: bool = True) -> None:
     (default on; off: one entry per slot); applies to solvers created
     afterwards."""
     load().vbd_use_record_dictionary(int(bool(enable)))
+
+
+def use_neighbour_staging(enable: bool = True) -> None:
+    """Stage each sweep block's neighbours and the dictionary in LDS
+    (default on; same results either way); applies to solvers created
+    afterwards."""
+    load().vbd_use_neighbour_staging(int(bool(enable)))
 
 
 def use_device(device) -> int:

@@ -557,6 +557,7 @@ int vbd_profile_read(vbd_solver_t *s, double *ms, int64_t *launches, double *byt
 }
 
 void vbd_use_record_dictionary(int32_t enable) { g_record_dictionary = enable != 0; }
+void vbd_use_neighbour_staging(int32_t enable) { g_staging = enable != 0; }
 
 int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
 {

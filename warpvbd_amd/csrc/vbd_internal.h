@@ -18,6 +18,12 @@ constexpr int kMaxElementsPerVertex = 32;  // src/vbd_solver.py:12
 constexpr int kStageRows = 64;    // vertices per block
 constexpr int kStageMax = 1024;   // distinct neighbours a block may stage (LDS: 24 bytes each)
 constexpr int kStageDict = 64;    // dictionary entries staged with them (48 bytes each)
+// Colours below this many vertices are not staged: the staging pass adds a
+// fixed latency per block, repaid only when a colour gives each of the 256
+// CUs two or more blocks (measured per big-colour launch at 256 threads: 9.3k
+// vertices 7.35 vs 6.31 us unstaged, 30k 9.14 vs 7.83, 47k 10.35 vs 10.67,
+// 70k 13.05 vs 14.64).
+constexpr int kStageMinColor = 40960;
 
 void set_error(const std::string &msg);
 
@@ -167,7 +173,7 @@ struct vbd_solver {
     float2 *slots2 = nullptr;     // full records' qd while they are built; null afterwards
     float4 *dict = nullptr;       // [n_dict*3] record dictionary (vbd_device.h)
     // Neighbour staging for the 4-lane, 256-thread sweep of colours of at least
-    // small_color vertices (single-body solvers with a shared dictionary): per
+    // kStageMinColor vertices (single-body solvers with a shared dictionary): per
     // block of kStageRows vertices its distinct neighbours, and per slot an
     // 8-byte record of local indices into that list plus the dictionary entry.
     bool st_ok = false;
@@ -225,6 +231,7 @@ void plan_segments(vbd_solver *s);
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);
 int build_staging(vbd_solver *s);  // neighbour staging tables (st_*), when the mesh allows
 extern bool g_record_dictionary;  // share record dictionary entries in later creates (default on)
+extern bool g_staging;            // neighbour staging in later creates (default on)
 hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,
                                       hipStream_t st);
 int step_bodies(vbd_solver *s, double dt, double tol, int max_iter);

@@ -715,6 +715,7 @@ __global__ void k_rescale_dict(int64_t nd, float4 *dict, float fa, float fb)
 }  // namespace
 
 bool g_record_dictionary = true;  // vbd_use_record_dictionary
+bool g_staging = true;            // vbd_use_neighbour_staging
 
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1)
 {
@@ -1045,7 +1046,7 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
 }
 
 // Neighbour staging tables (the st_* fields).  Colours of at least
-// small_color vertices are cut into blocks of kStageRows consecutive vertices,
+// kStageMinColor vertices are cut into blocks of kStageRows consecutive vertices,
 // as the 4-lane, 256-thread sweep assigns them; each block's distinct
 // neighbour ids are listed in order of first appearance, and each of its slots
 // gets (l0 | l1 << 16, l2 | entry << 16) with l_k the local indices of the
@@ -1055,7 +1056,7 @@ int build_staging(vbd_solver *s)
 {
     s->st_ok = false;
     s->st_blk0.assign(s->ncolors, -1);
-    if (!s->dict_shared || s->n_dict > kStageDict || s->n_slots <= 0) return 0;
+    if (!g_staging || !s->dict_shared || s->n_dict > kStageDict || s->n_slots <= 0) return 0;
     hipStream_t st = s->stream;
     const int nsw = s->n_swept;
     std::vector<int32_t> off(nsw + 1);
@@ -1071,7 +1072,7 @@ int build_staging(vbd_solver *s)
     int nblocks = 0;
     for (int c = 0; c < s->ncolors; ++c) {
         const int nv = s->col_size[c];
-        if (nv < s->launch.small_color) continue;
+        if (nv < std::max(s->launch.small_color, kStageMinColor)) continue;
         blk0[c] = nblocks;
         const int c0 = s->col_start[c];
         for (int j = 0; j < nv; j += kStageRows, ++nblocks) {
