@@ -133,14 +133,29 @@ __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int n
     const VertexPre pre = vertex_pre(a, v, valid);
     const bool live = !c->converged;  // the same for every thread of the block
     if (live) {
+        // every id, then every position, requested together: two memory
+        // latencies for the whole list (kStageMax = kStageUnroll x 256)
+        constexpr int kStageUnroll = kStageMax / 256;
         const int n0 = st_off[blk0 + b], cnt = st_off[blk0 + b + 1] - n0;
-        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-            const float4 p = a.u[st_nb[n0 + k]];
-            const float3 lo = disp_lo(p);
-            s_na[k] = make_float4(p.x, p.y, p.z, lo.x);
-            s_nb[k] = make_float2(lo.y, lo.z);
+        int id[kStageUnroll];
+#pragma unroll
+        for (int q = 0; q < kStageUnroll; ++q) {
+            const int k = threadIdx.x + 256 * q;
+            id[q] = k < cnt ? st_nb[n0 + k] : -1;
         }
+        float4 p[kStageUnroll];
+#pragma unroll
+        for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
         for (int k = threadIdx.x; k < 3 * n_dict; k += blockDim.x) s_dict[k] = a.dict[k];
+#pragma unroll
+        for (int q = 0; q < kStageUnroll; ++q) {
+            const int k = threadIdx.x + 256 * q;
+            if (k < cnt) {
+                const float3 lo = disp_lo(p[q]);
+                s_na[k] = make_float4(p[q].x, p[q].y, p[q].z, lo.x);
+                s_nb[k] = make_float2(lo.y, lo.z);
+            }
+        }
         __syncthreads();
     }
     const Staged stg{st_rec, s_na, s_nb, s_dict};
