@@ -31,11 +31,6 @@ namespace vbd {
 
 namespace {
 
-// One colour: a group of L lanes per vertex.  Each block leaves its metric
-// maximum in a.part[blockIdx.x].  close != 0 on the iteration's last launch:
-// the block that finishes last (device counter in Ctrl) folds all np_all
-// partial maxima of the iteration and decides convergence
-// (src/vbd_solver.py:578), so no separate check launch is needed.
 // A launch's end: the block's metric maximum mb (thread 0) into
 // a.part[blockIdx.x]; with close, the iteration's last launch folds every
 // partial maximum and decides convergence.
