@@ -131,6 +131,7 @@ __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int n
         // every id, then every position, requested together: two memory
         // latencies for the whole list (kStageMax = kStageUnroll x 256)
         constexpr int kStageUnroll = kStageMax / 256;
+        static_assert(kStageMax % 256 == 0, "each of the 256 threads stages kStageUnroll neighbours");
         const int n0 = st_off[blk0 + b], cnt = st_off[blk0 + b + 1] - n0;
         int id[kStageUnroll];
 #pragma unroll
