@@ -31,6 +31,43 @@ namespace vbd {
 
 namespace {
 
+// Max over the np per-block partial maxima (this thread's share; entry skip
+// reads as zero).  Eight loads are requested before any is used, so the
+// fold waits for about np / (8 blockDim.x) memory latencies instead of
+// np / blockDim.x.  ATOMIC: relaxed agent-scope loads, for partials that
+// other blocks of the running launch wrote (read after their counter).
+template <bool ATOMIC>
+__device__ __forceinline__ uint32_t load_partial(const uint32_t *p)
+{
+    if constexpr (ATOMIC) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template <bool ATOMIC>
+__device__ __forceinline__ uint32_t fold_partials(const uint32_t *part, int np, int skip)
+{
+    constexpr int U = 8;
+    uint32_t x = 0;
+    int i = threadIdx.x;
+    const int step = blockDim.x;
+    for (; i + (U - 1) * step < np; i += U * step) {
+        uint32_t y[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            y[k] = load_partial<ATOMIC>(&part[i + k * step]);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const uint32_t v = i + k * step == skip ? 0u : y[k];
+            x = x > v ? x : v;
+        }
+    }
+    for (; i < np; i += step) {
+        const uint32_t v = i == skip ? 0u : load_partial<ATOMIC>(&part[i]);
+        x = x > v ? x : v;
+    }
+    return x;
+}
+
 // A launch's end: the block's metric maximum mb (thread 0) into
 // a.part[blockIdx.x]; with close, the iteration's last launch folds every
 // partial maximum and decides convergence.
@@ -47,13 +84,9 @@ __device__ __forceinline__ void finish_sweep(const SweepArgs &a, Ctrl *c, uint32
         // fence or block counter is needed.  Its own slot still holds an older
         // value for the other threads, so it is skipped.
         const int own = (int)(a.part - part_all);
-        uint32_t x = 0;
-        for (int i = threadIdx.x; i < np_all; i += blockDim.x) {
-            const uint32_t y = i == own ? 0u : part_all[i];
-            x = x > y ? x : y;
-        }
+        const uint32_t x0 = fold_partials<false>(part_all, np_all, own);
         __syncthreads();  // block_max_u32's staging is free again
-        x = block_max_u32(x);
+        const uint32_t x = block_max_u32(x0);
         if (threadIdx.x == 0) close_iteration(c, c->it, x > mb ? x : mb, tol);
         return;
     }
@@ -65,12 +98,7 @@ __device__ __forceinline__ void finish_sweep(const SweepArgs &a, Ctrl *c, uint32
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    uint32_t x = 0;
-    for (int i = threadIdx.x; i < np_all; i += blockDim.x) {
-        const uint32_t y = __hip_atomic_load(&part_all[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = x > y ? x : y;
-    }
-    x = block_max_u32(x);
+    const uint32_t x = block_max_u32(fold_partials<true>(part_all, np_all, -1));
     if (threadIdx.x == 0) {
         const int it = c->it;
         c->blocks_done = 0u;
