@@ -45,17 +45,6 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t x)
     return x;
 }
 
-// Max over np per-block partial maxima with the whole (<= 1024-thread) block;
-// thread 0 gets the result.
-__device__ __forceinline__ uint32_t reduce_partials(const uint32_t *part, int np, uint32_t x)
-{
-    for (int i = threadIdx.x; i < np; i += blockDim.x) {
-        const uint32_t y = part[i];
-        x = x > y ? x : y;
-    }
-    return block_max_u32(x);
-}
-
 __device__ __forceinline__ void close_iteration(Ctrl *c, int it, uint32_t bits, float tol)
 {
     c->metric = bits;
