@@ -1,0 +1,219 @@
+"""Parity at the sizes the benchmark reports, each against the FP64 oracle:
+
+* the 256x32x32 beam (279,873 vertices, 1,310,720 tets), SNH and StVK: the
+  big-colour sweep with neighbour staging and the per-iteration fold of
+  the block maxima run here and nowhere smaller;
+* 512 mass-spring bodies (seed 3) and the 1024-body mixed batch (seed 5),
+  with 16 sampled bodies each against their own single-body oracle runs, in
+  both batched forms;
+* the 128x16x16 beam with the warm start's history guess against the oracle
+  started from the same guess, and the history-guess batch against a
+  converged FP64 solve (error no larger than the plain batched solve's).
+
+Gates as in the smaller tests: 1e-6 x bounding-box diagonal at a fixed
+iteration count; at tolerance termination iteration counts within +-1 and
+positions within 1e-5 x diagonal per step (a step whose last metric sits
+within rounding of the tolerance can stop one iteration apart)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from warpvbd_amd import BatchedVBDSolver, VBDSolver
+from warpvbd_amd.scenes import cantilever, mass_spring_batch, mixed_batch
+from warpvbd_amd.warm_start import WarmStart, flatten, init_params
+
+pytestmark = pytest.mark.gpu
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def diag(x):
+    return float(np.linalg.norm(x.max(0) - x.min(0)))
+
+
+@pytest.fixture(scope="module")
+def beam():
+    return cantilever((256, 32, 32))
+
+
+@pytest.mark.parametrize("material", ["snh", "stvk"])
+def test_beam_256x32x32_fixed_iterations(beam, material):
+    """Two steps of five iterations: positions and the last iteration's
+    metric (max |dx_x + dx_y + dx_z| over all colours, folded on the device)."""
+    sc = beam
+    g = VBDSolver(**sc.solver_kwargs(), dx_tol=0.0, max_iter=5, material=material, verbose=False)
+    assert g.staged_colors() == 4
+    r = oracle.OracleSolver(**sc.solver_kwargs(), dx_tol=0.0, max_iter=5, material=material, expanded=False,
+                            threads=THREADS)
+    assert np.array_equal(g.colors(), r.colors())
+    gate = 1e-6 * diag(sc.vertices)
+    for k in range(2):
+        print(f"256x32x32 {material} step {k}", flush=True)
+        x = g.step(g.old_positions, 1e-3).numpy()
+        xr = r.step(1e-3)
+        err = float(np.abs(x - xr).max())
+        rel = abs(g.last_metric - r.last_metric) / r.last_metric
+        print(f"  |x - x_fp64| {err:.3e} m (gate {gate:.3e}); metric {g.last_metric:.6e} vs {r.last_metric:.6e} "
+              f"({rel:.2e})", flush=True)
+        assert g.last_iterations == 5
+        assert err < gate
+        assert rel < 1e-2
+
+
+def test_beam_256x32x32_tolerance_steps(beam):
+    """Ten steps from rest at tol 1e-7 (the headline workload)."""
+    sc = beam
+    g = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
+    r = oracle.OracleSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, expanded=False, threads=THREADS)
+    gate = 1e-5 * diag(sc.vertices)
+    it_g, it_r, errs = [], [], []
+    for k in range(10):
+        x = g.step(g.old_positions, 1e-3).numpy()
+        xr = r.step(1e-3)
+        it_g.append(g.last_iterations)
+        it_r.append(r.last_iters)
+        errs.append(float(np.abs(x - xr).max()))
+        print(f"step {k}: iterations {it_g[-1]} vs {it_r[-1]}, |x - x_fp64| {errs[-1]:.3e} m", flush=True)
+    assert np.all(np.abs(np.array(it_g) - np.array(it_r)) <= 1)
+    assert np.mean(np.array(it_g) == np.array(it_r)) >= 0.8
+    assert max(errs) < gate
+
+
+def _sample(n, k=16, seed=0):
+    return sorted(np.random.default_rng(seed).choice(n, size=k, replace=False).tolist())
+
+
+def _batch_fixed(bodies, mode, lanes, block, graph, steps=3, iters=8):
+    pick = _sample(len(bodies))
+    g = BatchedVBDSolver(bodies, dx_tol=0.0, max_iter=iters, verbose=False)
+    g.set_mode(mode, lanes, block, graph)
+    refs = {i: oracle.OracleSolver(**bodies[i].solver_kwargs(), dx_tol=0.0, max_iter=iters, expanded=False)
+            for i in pick}
+    worst = 0.0
+    for k in range(steps):
+        x = g.step(dt=1e-3).numpy()
+        assert np.all(g.body_iterations == iters)
+        for i in pick:
+            xr = refs[i].step(1e-3)
+            e = float(np.abs(x[g.body_slice(i)] - xr).max()) / diag(bodies[i].vertices)
+            worst = max(worst, e)
+            assert e < 1e-6, (mode, k, i, e)
+    print(f"{mode}: worst |x - x_fp64| / diagonal over {len(pick)} bodies x {steps} steps: {worst:.2e}")
+
+
+def _batch_tolerance(bodies, steps=20):
+    pick = _sample(len(bodies))
+    g = BatchedVBDSolver(bodies, dx_tol=1e-7, max_iter=1000, verbose=False)
+    refs = {i: oracle.OracleSolver(**bodies[i].solver_kwargs(), dx_tol=1e-7, max_iter=1000, expanded=False)
+            for i in pick}
+    same, worst = [], 0.0
+    for k in range(steps):
+        x = g.step(dt=1e-3).numpy()
+        for i in pick:
+            xr = refs[i].step(1e-3)
+            d = abs(int(g.body_iterations[i]) - refs[i].last_iters)
+            same.append(d == 0)
+            assert d <= 1, (k, i)
+            e = float(np.abs(x[g.body_slice(i)] - xr).max()) / diag(bodies[i].vertices)
+            worst = max(worst, e)
+            assert e < 1e-5, (k, i, e)
+    print(f"tolerance: {np.mean(same):.3f} of body-steps at the oracle's iteration count; "
+          f"worst |x - x_fp64| / diagonal {worst:.2e}")
+    assert np.mean(same) >= 0.9
+
+
+@pytest.fixture(scope="module")
+def springs512():
+    return mass_spring_batch(512, seed=3)
+
+
+@pytest.fixture(scope="module")
+def mixed1024():
+    return mixed_batch(1024, seed=5)
+
+
+@pytest.mark.parametrize("mode,lanes,block,graph", [("launches", 4, 256, True), ("resident", 4, 1024, False)])
+def test_mass_spring_512_fixed_iterations(springs512, mode, lanes, block, graph):
+    _batch_fixed(springs512, mode, lanes, block, graph)
+
+
+def test_mass_spring_512_tolerance(springs512):
+    _batch_tolerance(springs512)
+
+
+@pytest.mark.parametrize("mode,lanes,block,graph", [("launches", 4, 256, True), ("resident", 8, 1024, False)])
+def test_mixed_1024_fixed_iterations(mixed1024, mode, lanes, block, graph):
+    _batch_fixed(mixed1024, mode, lanes, block, graph)
+
+
+def test_mixed_1024_tolerance(mixed1024):
+    _batch_tolerance(mixed1024)
+
+
+def test_mixed_1024_history_guess_error_against_converged_solve(mixed1024):
+    """Batched + history guess at tol 1e-7 against a converged FP64 solve
+    (tol 1e-11) from the same state, step by step (the oracle is given the
+    GPU state before each step): the warm-started steps stop nearer to the
+    converged solution than the plain batched steps, body by body on
+    average, and need fewer iterations."""
+    bodies = mixed1024
+    pick = _sample(len(bodies), 8, seed=1)
+    plain = BatchedVBDSolver(bodies, dx_tol=1e-7, max_iter=1000, verbose=False)
+    warm = BatchedVBDSolver(bodies, dx_tol=1e-7, max_iter=1000, verbose=False)
+    ws = WarmStart(warm, capacity=1)
+    ws.set_policy(WarmStart.HISTORY)
+    refs = {i: oracle.OracleSolver(**bodies[i].solver_kwargs(), dx_tol=1e-11, max_iter=100000, expanded=False)
+            for i in pick}
+    for _ in range(5):  # build a correction history on both
+        plain.step(dt=1e-3)
+        warm.step(dt=1e-3, warm_start=ws)
+    e_p, e_w, it_p, it_w = [], [], [], []
+    for k in range(10):
+        for s, errs, its, w in ((plain, e_p, it_p, None), (warm, e_w, it_w, ws)):
+            x0, v0 = s.old_positions.numpy(), s.old_velocities.numpy()
+            x = s.step(dt=1e-3, warm_start=w).numpy()
+            its.append(float(s.body_iterations.mean()))
+            for i in pick:
+                sl = s.body_slice(i)
+                refs[i].set_state(x=x0[sl], v=v0[sl])
+                errs.append(float(np.abs(x[sl] - refs[i].step(1e-3)).max()))
+    e_p, e_w = np.array(e_p), np.array(e_w)
+    print(f"mean body iterations plain {np.mean(it_p):.2f} history {np.mean(it_w):.2f}; step error vs converged "
+          f"plain {e_p.mean():.3e} (max {e_p.max():.3e}) history {e_w.mean():.3e} (max {e_w.max():.3e})")
+    assert np.mean(it_w) < np.mean(it_p)
+    assert e_w.mean() <= 1.5 * e_p.mean() and e_w.max() <= 1.5 * e_p.max()
+
+
+def test_beam_128x16x16_history_guess_matches_oracle():
+    """The history guess (network output zeroed, NETWORK policy) on the
+    128x16x16 warm-start beam, each step from the oracle's state at a fixed
+    iteration count, against the oracle started from the same guess."""
+    sc = cantilever((128, 16, 16))
+    s = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
+    ws = WarmStart(s, capacity=1, seed=3)
+    p = init_params(3)
+    p["W_out"][:] = 0.0
+    p["b_out"][:] = 0.0
+    ws.set_params(flatten(p))
+    ws.set_policy(ws.NETWORK)
+    ref = oracle.OracleSolver(**sc.solver_kwargs(), dx_tol=0.0, max_iter=6, expanded=False, threads=THREADS)
+    dt, K = 1e-3, 6
+    act = sc.active_mask[:, None].astype(np.float64)
+    g = np.asarray(sc.gravity, np.float64)
+    gate = 1e-6 * diag(sc.vertices)
+    c, c2 = np.zeros((sc.n_vertices, 3)), None
+    for k in range(6):
+        x0, v0 = ref.old_positions, ref.old_velocities
+        y = x0 + (dt * v0 + dt * dt * g) * act
+        xr = ref.step(dt, dx_tol=0.0, max_iter=K, guess=y + (c if c2 is None else 2.0 * c - c2))
+        s.old_positions = x0
+        s.old_velocities = v0
+        x = s.step(s.old_positions, dt, dx_tol=0.0, max_iter=K, warm_start=ws).numpy()
+        err = float(np.abs(x - xr).max())
+        print(f"step {k}: |x - x_fp64| {err:.3e} m (gate {gate:.3e})", flush=True)
+        assert s.last_used_network and s.last_iterations == K
+        assert err <= gate
+        if k > 0:
+            c2 = c
+        c = (xr - y) * act

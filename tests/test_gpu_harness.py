@@ -87,3 +87,17 @@ def test_timing_sweeps_quick(tmp_path):
     assert m["num_elements"] == [135, 320]
     for f in ("tol.npz", "mesh.npz", "par.npz", "col.npz"):
         assert (tmp_path / f).exists()
+
+
+def test_run_cantilever_reproduces_the_readme_tip_curve(tmp_path):
+    """The reference's README tip curve (examples/run_cantilever.py:129-177,
+    1.5 s): the GPU run's 19 crests and troughs within 5 ms and 0.3 mm of the
+    ones digitised from asset/imgs/displacement_cantilever.png."""
+    from test_oracle_step_pins import README_T_GATE, README_Z_GATE, readme_anchor_errors, readme_scene
+
+    _, s, anchors = readme_scene()
+    tips = harness.run_cantilever(str(tmp_path), n_seconds=1.5)
+    assert tips.shape == (s["steps"], 3)
+    errs = readme_anchor_errors(tips[:, 2], s["dt"], anchors)
+    print(f"README anchors (GPU): max |dt| {errs[:, 0].max() * 1e3:.2f} ms, max |dz| {errs[:, 1].max():.2e} m")
+    assert errs[:, 0].max() <= README_T_GATE and errs[:, 1].max() <= README_Z_GATE

@@ -315,6 +315,25 @@ def test_returned_arrays_keep_their_values_and_explicit_positions():
     assert np.all(np.isfinite(d))
 
 
+def test_step_from_an_older_array_keeps_the_latest_result():
+    """a = step, b = step, step(a): the step from a's device snapshot must not
+    change what b shows.  b is still the current array when step(a) starts, so
+    it is preserved before the snapshot of a is loaded into x_prev (b read
+    only afterwards, which is the case the earlier test does not reach)."""
+    sc = cantilever((10, 2, 2))
+    g, r = make(sc)
+    a = g.step(g.old_positions, 1e-3)
+    b = g.step(g.old_positions, 1e-3)
+    xr1 = r.step(1e-3)
+    xr2 = r.step(1e-3)
+    c = g.step(a, 1e-3)
+    assert np.abs(b.numpy() - xr2).max() < 1e-8
+    assert np.abs(a.numpy() - xr1).max() < 1e-8
+    r.set_state(x=xr1)
+    xr3 = r.step(1e-3)
+    assert np.abs(c.numpy() - xr3).max() < 1e-8
+
+
 def test_device_tensor_step_matches_host_step():
     """step() on a torch tensor in GPU memory (vbd_step_dev, on torch's current
     stream) returns a GPU tensor equal to the host path's result, and work

@@ -260,14 +260,17 @@ class VBDSolver:
         L = _lib.load()
         pos_ptr = None
         if isinstance(positions, VertexArray) and positions._solver is self and positions._host is None:
+            if positions._slot is None and positions._version != self._version:
+                raise VBDError("stale position array")
+            # the arrays that still show the current positions keep them
+            # before a snapshot load overwrites x_prev
+            self._before_mutation()
             if positions._slot is not None:
                 check(L.vbd_load_snapshot(self._h, positions._slot), "vbd_load_snapshot")
-            elif positions._version != self._version:
-                raise VBDError("stale position array")
         else:
             xs = f64(positions).reshape(self.n_vertices, 3)
             pos_ptr = ptr(xs, ctypes.c_double)
-        self._before_mutation()
+            self._before_mutation()
         it, met = ctypes.c_int32(0), ctypes.c_double(0.0)
         if warm_start is not None:
             if pos_ptr is not None:
