@@ -260,13 +260,16 @@ class VBDSolver:
         L = _lib.load()
         pos_ptr = None
         if isinstance(positions, VertexArray) and positions._solver is self and positions._host is None:
-            if positions._slot is None and positions._version != self._version:
+            # an older array is loaded from its device snapshot; the current
+            # one (no snapshot yet) is x_prev already
+            load_slot = positions._slot
+            if load_slot is None and positions._version != self._version:
                 raise VBDError("stale position array")
             # the arrays that still show the current positions keep them
             # before a snapshot load overwrites x_prev
             self._before_mutation()
-            if positions._slot is not None:
-                check(L.vbd_load_snapshot(self._h, positions._slot), "vbd_load_snapshot")
+            if load_slot is not None:
+                check(L.vbd_load_snapshot(self._h, load_slot), "vbd_load_snapshot")
         else:
             xs = f64(positions).reshape(self.n_vertices, 3)
             pos_ptr = ptr(xs, ctypes.c_double)
