@@ -265,6 +265,22 @@ int vbd_color(const int32_t *tets, int32_t n_tets, int32_t n_vertices, int32_t *
 int vbd_color_graph(const int32_t *adj_off, const int32_t *adj, int32_t n_vertices, int32_t *colors,
                     int32_t *n_colors_out);
 
+/* vbd_color without any allocation.  No counterpart in WarpVBD: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  tets_dev: i32[n_tets*4] and
+ * colors_dev: i32[n_vertices], device memory of the current device; stream:
+ * a hipStream_t (NULL = the null stream).  Same colouring as vbd_color.
+ * Every intermediate array lives in one 64 MiB device array per GPU that the
+ * library holds for the life of the process: 20 bytes per vertex, 16 per tet
+ * and 4 per distinct lower-indexed neighbour of each vertex (at most 24 per
+ * tet) must fit -- the 256x32x32 beam (1.31 M tets) needs about 37 MB; a
+ * larger mesh is rejected with VBD_EINVAL before any of its data is read.
+ * Calls on one GPU share that array, so calls that overlap on one GPU race.
+ * The call waits on the stream (it reads counts back) and returns when the
+ * colours are in colors_dev. */
+int vbd_color_noalloc(const int32_t *tets_dev, int32_t n_tets, int32_t n_vertices, int32_t *colors_dev,
+                      int32_t *n_colors_out, int32_t *max_degree_out, void *stream);
+
 /* Make HIP device `device` current for the calling thread (the reference's
  * device= argument, src/vbd_solver.py:383). */
 int vbd_use_device(int32_t device);

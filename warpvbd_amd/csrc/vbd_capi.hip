@@ -8,8 +8,14 @@
 namespace vbd {
 namespace {
 thread_local std::string g_err;
+thread_local const char *g_err_literal = nullptr;
 }
-void set_error(const std::string &msg) { g_err = msg; }
+void set_error(const std::string &msg)
+{
+    g_err = msg;
+    g_err_literal = nullptr;
+}
+void set_error_literal(const char *msg) { g_err_literal = msg; }
 }  // namespace vbd
 
 using namespace vbd;
@@ -24,7 +30,7 @@ using namespace vbd;
 
 extern "C" {
 
-const char *vbd_last_error(void) { return g_err.c_str(); }
+const char *vbd_last_error(void) { return g_err_literal ? g_err_literal : g_err.c_str(); }
 
 void vbd_destroy(vbd_solver_t *s)
 {
@@ -468,6 +474,22 @@ int vbd_color(const int32_t *tets, int32_t n_tets, int32_t n_vertices, int32_t *
         }
     }
     cleanup();
+    if (rc == 0) {
+        if (n_colors_out) *n_colors_out = nc;
+        if (max_degree_out) *max_degree_out = md;
+    }
+    return rc;
+}
+
+int vbd_color_noalloc(const int32_t *tets_dev, int32_t n_tets, int32_t n_vertices, int32_t *colors_dev,
+                      int32_t *n_colors_out, int32_t *max_degree_out, void *stream)
+{
+    if (!tets_dev || !colors_dev || n_vertices <= 0 || n_tets < 0 || n_tets > (1 << 29)) {
+        set_error_literal("bad arguments");
+        return VBD_EINVAL;
+    }
+    int nc = 0, md = 0;
+    const int rc = color_tets_noalloc(tets_dev, n_tets, n_vertices, colors_dev, &nc, &md, (hipStream_t)stream);
     if (rc == 0) {
         if (n_colors_out) *n_colors_out = nc;
         if (max_degree_out) *max_degree_out = md;

@@ -25,7 +25,14 @@ constexpr int kStageDict = 64;    // dictionary entries staged with them (48 byt
 // 70k 13.05 vs 14.64).
 constexpr int kStageMinColor = 40960;
 
+// Scratch of the allocation-free entry points (a file-scope device array in
+// vbd_setup.hip, one copy per GPU for the life of the process).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+
 void set_error(const std::string &msg);
+// a message with static storage, set without allocating
+void set_error_literal(const char *msg);
 
 #define VBD_HIP(call)                                                                          \
     do {                                                                                       \
@@ -215,6 +222,10 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
                int n_bodies, const double *body_grav);
 int color_tets_device(const int32_t *d_tets, int m, int n, int32_t *d_colors, int32_t *d_inc_off,
                       int32_t *d_inc, int *ncolors, int *max_degree, hipStream_t stream);
+// color_tets_device on caller-owned device buffers, every intermediate array
+// in the scratch array (no allocation); kernels on `stream`
+int color_tets_noalloc(const int32_t *d_tets, int m, int n, int32_t *d_colors, int *ncolors, int *max_degree,
+                       hipStream_t stream);
 int color_graph_device(const int32_t *d_off, const int32_t *d_adj, int n, int32_t *d_colors, int *ncolors,
                        hipStream_t stream);
 int step_device(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric);

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/rocprim.hpp>
 
@@ -1110,6 +1111,197 @@ int build_staging(vbd_solver *s)
     s->st_blk0 = blk0;
     s->st_ids = ids;
     s->st_ok = true;
+    return 0;
+}
+
+
+// ------------------------------------------------------------------ colouring without allocation
+//
+// color_tets_noalloc: color_tets_device on device buffers that the caller
+// owns, with every intermediate array (incidence CSR, counters, lower-neighbour
+// CSR, scan temporaries, round flags) carved out of one file-scope device
+// array.  Each GPU holds its own copy of the array for the life of the
+// process; its address on a device comes from hipGetSymbolAddress, taken
+// once per device with that device current.  All calls on one GPU share the
+// array, so calls that overlap on one GPU race.
+namespace {
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// incidence counts with a range check (the tets are the caller's device data)
+__global__ void k_count_incidence_checked(const int32_t *tets, int m, int n, int32_t *cnt, int32_t *scal)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    int mx = -1;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int v = tets[4 * e + k];
+        if (v < 0 || v >= n) {
+            bad = true;
+            continue;
+        }
+        atomicAdd(&cnt[v], 1);
+        mx = max(mx, v);
+    }
+    atomicMax(&scal[0], mx);
+    if (bad) atomicOr(&scal[4], 1);
+}
+
+// largest colour (scal[5]); colours start at -1 for unreferenced vertices
+__global__ void k_max_color(const int32_t *colors, int n, int32_t *scal)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = v < n ? colors[v] : -1;
+    c = max(c, __shfl_xor(c, 1));
+    c = max(c, __shfl_xor(c, 2));
+    c = max(c, __shfl_xor(c, 4));
+    c = max(c, __shfl_xor(c, 8));
+    c = max(c, __shfl_xor(c, 16));
+    c = max(c, __shfl_xor(c, 32));
+    if ((threadIdx.x & 63) == 0) atomicMax(&scal[5], c);
+}
+
+// Bump allocator over the scratch array: 256-byte aligned pieces.
+struct Carve {
+    char *base;
+    size_t used, cap;
+    template <typename T>
+    T *take(size_t count)
+    {
+        const size_t off = (used + 255) & ~(size_t)255;
+        const size_t bytes = count * sizeof(T);
+        if (off > cap || bytes > cap - off) return nullptr;
+        used = off + bytes;
+        return reinterpret_cast<T *>(base + off);
+    }
+};
+
+}  // namespace
+
+int scratch_address(int device, void **out)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        set_error_literal("device index out of range for the allocation-free entry points");
+        return VBD_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            set_error_literal("hipGetSymbolAddress(scratch) failed");
+            return VBD_EHIP;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
+#define VBD_NA(call)                                                                           \
+    do {                                                                                       \
+        if ((call) != hipSuccess) {                                                            \
+            set_error_literal("a HIP call of vbd_color_noalloc failed: " #call);               \
+            return VBD_EHIP;                                                                   \
+        }                                                                                      \
+    } while (0)
+
+int color_tets_noalloc(const int32_t *d_tets, int m, int n, int32_t *d_colors, int *ncolors, int *max_degree,
+                       hipStream_t st)
+{
+    int dev = 0;
+    VBD_NA(hipGetDevice(&dev));
+    void *base = nullptr;
+    VBD_TRY(scratch_address(dev, &base));
+    Carve cv{static_cast<char *>(base), 0, kNoAllocScratchBytes};
+    // the fixed part: 5 (n + 1) + 4 m counters and lists, 64 scalars and flags,
+    // the scan's temporary storage; the lower-neighbour list comes last
+    if ((int64_t)20 * ((int64_t)n + 1) + (int64_t)16 * m > (int64_t)kNoAllocScratchBytes) {
+        set_error_literal("mesh too large for vbd_color_noalloc's scratch (20 bytes per vertex + 16 per tet + the "
+                          "lower-neighbour lists must fit in 64 MiB)");
+        return VBD_EINVAL;
+    }
+    size_t tb = 0;
+    VBD_NA(rocprim::exclusive_scan(nullptr, tb, (const int32_t *)nullptr, (int32_t *)nullptr, 0, (size_t)n + 1,
+                                   rocprim::plus<int32_t>(), st));
+    int32_t *inc_off = cv.take<int32_t>((size_t)n + 1), *inc = cv.take<int32_t>(std::max<size_t>(1, 4 * (size_t)m));
+    int32_t *cnt = cv.take<int32_t>((size_t)n + 1), *cursor = cv.take<int32_t>((size_t)n + 1);
+    int32_t *lo_cnt = cv.take<int32_t>((size_t)n + 1), *lo_off = cv.take<int32_t>((size_t)n + 1);
+    int32_t *scal = cv.take<int32_t>(8), *changed = cv.take<int32_t>(32);
+    void *tmp = cv.take<char>(std::max<size_t>(tb, 1));
+    if (!inc_off || !inc || !cnt || !cursor || !lo_cnt || !lo_off || !scal || !changed || !tmp) {
+        set_error_literal("mesh too large for vbd_color_noalloc's scratch (64 MiB)");
+        return VBD_EINVAL;
+    }
+    // scal: 0 largest referenced vertex, 1 largest incidence, 2 largest degree,
+    // 3 colour overflow, 4 index out of range, 5 largest colour
+    k_fill_i32<<<nblk(n + 1), kTB, 0, st>>>(cnt, n + 1, 0);
+    k_fill_i32<<<1, kTB, 0, st>>>(scal, 8, 0);
+    k_fill_i32<<<1, 1, 0, st>>>(scal, 1, -1);
+    k_fill_i32<<<1, 1, 0, st>>>(scal + 5, 1, -1);
+    if (m > 0) k_count_incidence_checked<<<nblk(m), kTB, 0, st>>>(d_tets, m, n, cnt, scal);
+    int32_t h[8];
+    VBD_NA(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    VBD_NA(hipStreamSynchronize(st));
+    if (h[4]) {
+        set_error_literal("element index outside [0, n_vertices)");
+        return VBD_EMESH;
+    }
+    const int nref = h[0] + 1;
+    VBD_NA(rocprim::exclusive_scan(tmp, tb, cnt, inc_off, 0, (size_t)n + 1, rocprim::plus<int32_t>(), st));
+    VBD_NA(hipMemcpyAsync(cursor, inc_off, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, st));
+    if (m > 0) k_fill_incidence<<<nblk(m), kTB, 0, st>>>(d_tets, m, cursor, inc);
+    k_sort_incidence<<<nblk(n), kTB, 0, st>>>(inc_off, inc, n, scal + 1);
+    k_degree<<<nblk(n), kTB, 0, st>>>(d_tets, inc_off, inc, n, scal + 2);
+    k_fill_i32<<<nblk(n + 1), kTB, 0, st>>>(lo_cnt, n + 1, 0);
+    k_lower_neighbours<false><<<nblk(n), kTB, 0, st>>>(d_tets, inc_off, inc, n, lo_cnt, nullptr, nullptr);
+    VBD_NA(rocprim::exclusive_scan(tmp, tb, lo_cnt, lo_off, 0, (size_t)n + 1, rocprim::plus<int32_t>(), st));
+    int32_t n_lo = 0;
+    VBD_NA(hipMemcpyAsync(&n_lo, lo_off + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    VBD_NA(hipStreamSynchronize(st));
+    int32_t *lo = cv.take<int32_t>((size_t)std::max(n_lo, 1));
+    if (!lo) {
+        set_error_literal("mesh too large for vbd_color_noalloc's scratch (64 MiB): the lower-neighbour lists "
+                          "do not fit");
+        return VBD_EINVAL;
+    }
+    k_lower_neighbours<true><<<nblk(n), kTB, 0, st>>>(d_tets, inc_off, inc, n, nullptr, lo_off, lo);
+    k_color_init<<<nblk(n), kTB, 0, st>>>(d_colors, n, nref);
+    constexpr int kBatch = 32;
+    int32_t hc[kBatch];
+    const int max_rounds = std::max(64, 4 * nref + 8);
+    bool done = false;
+    for (int r = 0; r < max_rounds && !done; r += kBatch) {
+        k_fill_i32<<<1, kBatch, 0, st>>>(changed, kBatch, 0);
+        for (int b = 0; b < kBatch; ++b)
+            k_color_round_graph<<<nblk(std::max(nref, 1)), kTB, 0, st>>>(lo_off, lo, nref, d_colors, changed + b,
+                                                                          scal + 3);
+        VBD_NA(hipMemcpyAsync(hc, changed, sizeof(hc), hipMemcpyDeviceToHost, st));
+        VBD_NA(hipStreamSynchronize(st));
+        for (int b = 0; b < kBatch; ++b) done |= (hc[b] == 0);
+    }
+    // the overflow flag of the rounds above may come from a transient colour;
+    // one more round at the fixed point decides it
+    k_fill_i32<<<1, 1, 0, st>>>(scal + 3, 1, 0);
+    k_color_round_graph<<<nblk(std::max(nref, 1)), kTB, 0, st>>>(lo_off, lo, nref, d_colors, changed, scal + 3);
+    k_max_color<<<nblk(n), kTB, 0, st>>>(d_colors, n, scal);
+    VBD_NA(hipGetLastError());
+    VBD_NA(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    VBD_NA(hipStreamSynchronize(st));
+    if (!done) {
+        set_error_literal("colouring did not reach its fixed point");
+        return VBD_EHIP;
+    }
+    if (h[3]) {
+        set_error_literal("a vertex needs more than 256 colours");
+        return VBD_ECOLORS;
+    }
+    *max_degree = h[2];
+    *ncolors = h[5] + 1;
     return 0;
 }
 
