@@ -264,6 +264,27 @@ def test_neighbour_staging_is_bit_exact(material):
         assert np.array_equal(xa, xb)
 
 
+def test_small_colour_kernel_is_bit_exact():
+    """Colours below small_color run through the kernel that reads a table of
+    records padded to 32 per vertex (and, as the iteration's last launch,
+    folds the other launches' maxima with its first memory round); with
+    small_color = 0 the same colours run through the 32-lane per-colour kernel
+    that reads records by slot offset.  The arithmetic is the same, so
+    positions, iteration counts and metrics agree bit for bit."""
+    sc = cantilever((96, 12, 12))
+    a = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
+    b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
+    sizes = a.color_sizes()
+    assert sizes.max() > 2048 and sizes.min() < 2048
+    a.set_launch(32, 256, 2048, 0)
+    b.set_launch(32, 256, 0, 0)
+    for _ in range(6):
+        xa = a.step(a.old_positions, 1e-3).numpy()
+        xb = b.step(b.old_positions, 1e-3).numpy()
+        assert a.last_iterations == b.last_iterations and a.last_metric == b.last_metric
+        assert np.array_equal(xa, xb)
+
+
 def test_translated_free_fall_keeps_32_bit_displacements():
     """Free fall of an unpinned body 2.9 m from its rest shape, positions passed
     in each step.  The iterate holds 32 significant bits of the displacement:

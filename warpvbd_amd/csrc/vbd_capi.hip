@@ -41,7 +41,7 @@ void vbd_destroy(vbd_solver_t *s)
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict,
-                   s->st_off, s->st_nb, s->st_rec};
+                   s->st_nb, s->st_rec, s->small_rec};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (double4 *p : s->snap)
@@ -644,6 +644,7 @@ int vbd_set_launch(vbd_solver_t *s, int32_t lanes, int32_t block, int32_t small_
     s->launch.small_color = small_color;
     s->launch.chunk = chunk;
     plan_segments(s);
+    VBD_TRY(build_small_records(s));
     for (auto &g : s->graphs) {  // captured with the previous plan
         (void)hipGraphExecDestroy(g.exec);
         (void)hipGraphDestroy(g.graph);

@@ -9,7 +9,7 @@
 // contiguously in colour-major vertex order as compact 16-byte records
 //     slots  (1 float4 per slot)  c = (o0, o1, o2, entry as int bits)
 // and the constants live in a record dictionary
-//     dict   (3 float4 per entry) d0 = qb, d1 = qc, d2 = (qd.x, qd.y, ka, 0).
+//     dict   (3 float4 per entry) d0 = qb, d1 = qc, d2 = (qd.x, qd.y, ka, |bs|^2).
 // Meshes built from voxels have a handful of distinct rest shapes (5 for the
 // 5-tet split of a uniform grid), so their slots share few entries (20 per
 // material) and a sweep reads 16 bytes per slot from HBM, the dictionary from
@@ -103,8 +103,12 @@ __device__ __forceinline__ void acc_zero(Acc &a)
 // StVK (src/vbd_solver.py:127-144):
 //   g = V F S b, S = lambda tr(E) I + 2 mu E
 //   H = V[(lambda+mu)(Fb)(Fb)^T + mu |b|^2 F F^T + (lambda tr(E)|b|^2 + 2 mu b^T E b) I]
+// |bs|^2 of a slot, as the record dictionary stores it (d2.w): every path
+// that evaluates a slot uses this one rounding
+__device__ __forceinline__ float slot_bb(const float3 bs) { return fmaf(bs.z, bs.z, fmaf(bs.y, bs.y, bs.x * bs.x)); }
+
 template <int MAT>
-__device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const float3 bs, const float3 b0,
+__device__ __forceinline__ void slot_eval(float ka, float kb, float kc, float bb, const float3 bs, const float3 b0,
                                           const float3 b1, const float3 e0, const float3 e1, const float3 d2,
                                           Acc &a)
 {
@@ -120,7 +124,6 @@ __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const fl
     const float D22 = e0.z * b0.z + e1.z * b1.z - d2.z * bs.z;
     const float F00 = 1.f + D00, F11 = 1.f + D11, F22 = 1.f + D22;
     const float F01 = D01, F02 = D02, F10 = D10, F12 = D12, F20 = D20, F21 = D21;
-    const float bb = bs.x * bs.x + bs.y * bs.y + bs.z * bs.z;
     const float Fb0 = F00 * bs.x + F01 * bs.y + F02 * bs.z;
     const float Fb1 = F10 * bs.x + F11 * bs.y + F12 * bs.z;
     const float Fb2 = F20 * bs.x + F21 * bs.y + F22 * bs.z;
@@ -146,16 +149,19 @@ __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const fl
         const float s1 = ka * (Ic * r);        // V mu'(1 - 1/(Ic+1))
         const float s2 = 2.f * ka * r * r;     // V 2mu'/(Ic+1)^2
         const float sj = kb * Jm1 - kc;        // V lambda'(J - alpha)
-        a.g0 += s1 * Fb0 + sj * cb0;
-        a.g1 += s1 * Fb1 + sj * cb1;
-        a.g2 += s1 * Fb2 + sj * cb2;
+        // accumulated term by term, each product fused into the running sum
+        a.g0 = fmaf(sj, cb0, fmaf(s1, Fb0, a.g0));
+        a.g1 = fmaf(sj, cb1, fmaf(s1, Fb1, a.g1));
+        a.g2 = fmaf(sj, cb2, fmaf(s1, Fb2, a.g2));
         const float sd = s1 * bb;
-        a.h00 += s2 * Fb0 * Fb0 + sd + kb * cb0 * cb0;
-        a.h01 += s2 * Fb0 * Fb1 + kb * cb0 * cb1;
-        a.h02 += s2 * Fb0 * Fb2 + kb * cb0 * cb2;
-        a.h11 += s2 * Fb1 * Fb1 + sd + kb * cb1 * cb1;
-        a.h12 += s2 * Fb1 * Fb2 + kb * cb1 * cb2;
-        a.h22 += s2 * Fb2 * Fb2 + sd + kb * cb2 * cb2;
+        const float t0 = s2 * Fb0, t1 = s2 * Fb1, t2 = s2 * Fb2;
+        const float q0 = kb * cb0, q1 = kb * cb1, q2 = kb * cb2;
+        a.h00 = fmaf(q0, cb0, fmaf(t0, Fb0, a.h00 + sd));
+        a.h01 = fmaf(q0, cb1, fmaf(t0, Fb1, a.h01));
+        a.h02 = fmaf(q0, cb2, fmaf(t0, Fb2, a.h02));
+        a.h11 = fmaf(q1, cb1, fmaf(t1, Fb1, a.h11 + sd));
+        a.h12 = fmaf(q1, cb2, fmaf(t1, Fb2, a.h12));
+        a.h22 = fmaf(q2, cb2, fmaf(t2, Fb2, a.h22 + sd));
     } else {
         // E = (D + D^T + D^T D) / 2
         const float E00 = D00 + 0.5f * (D00 * D00 + D10 * D10 + D20 * D20);
@@ -173,17 +179,18 @@ __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, const fl
         const float w0 = kb * trE * bs.x + 2.f * ka * Eb0;
         const float w1 = kb * trE * bs.y + 2.f * ka * Eb1;
         const float w2 = kb * trE * bs.z + 2.f * ka * Eb2;
-        a.g0 += F00 * w0 + F01 * w1 + F02 * w2;
-        a.g1 += F10 * w0 + F11 * w1 + F12 * w2;
-        a.g2 += F20 * w0 + F21 * w1 + F22 * w2;
+        a.g0 = fmaf(F02, w2, fmaf(F01, w1, fmaf(F00, w0, a.g0)));
+        a.g1 = fmaf(F12, w2, fmaf(F11, w1, fmaf(F10, w0, a.g1)));
+        a.g2 = fmaf(F22, w2, fmaf(F21, w1, fmaf(F20, w0, a.g2)));
         const float kab = ka + kb, kbb = ka * bb;
         const float sd = kb * trE * bb + 2.f * ka * bEb;
-        a.h00 += kab * Fb0 * Fb0 + kbb * (F00 * F00 + F01 * F01 + F02 * F02) + sd;
-        a.h01 += kab * Fb0 * Fb1 + kbb * (F00 * F10 + F01 * F11 + F02 * F12);
-        a.h02 += kab * Fb0 * Fb2 + kbb * (F00 * F20 + F01 * F21 + F02 * F22);
-        a.h11 += kab * Fb1 * Fb1 + kbb * (F10 * F10 + F11 * F11 + F12 * F12) + sd;
-        a.h12 += kab * Fb1 * Fb2 + kbb * (F10 * F20 + F11 * F21 + F12 * F22);
-        a.h22 += kab * Fb2 * Fb2 + kbb * (F20 * F20 + F21 * F21 + F22 * F22) + sd;
+        const float u0 = kab * Fb0, u1 = kab * Fb1, u2 = kab * Fb2;
+        a.h00 = fmaf(u0, Fb0, fmaf(kbb, F00 * F00 + F01 * F01 + F02 * F02, a.h00 + sd));
+        a.h01 = fmaf(u0, Fb1, fmaf(kbb, F00 * F10 + F01 * F11 + F02 * F12, a.h01));
+        a.h02 = fmaf(u0, Fb2, fmaf(kbb, F00 * F20 + F01 * F21 + F02 * F22, a.h02));
+        a.h11 = fmaf(u1, Fb1, fmaf(kbb, F10 * F10 + F11 * F11 + F12 * F12, a.h11 + sd));
+        a.h12 = fmaf(u1, Fb2, fmaf(kbb, F10 * F20 + F11 * F21 + F12 * F22, a.h12));
+        a.h22 = fmaf(u2, Fb2, fmaf(kbb, F20 * F20 + F21 * F21 + F22 * F22, a.h22 + sd));
     }
 }
 
@@ -253,7 +260,8 @@ __device__ __forceinline__ float3 solve3_guarded(const Acc &a)
     if (fabsf(det) <= 1e-6f * scale && det == det && fabsf(det) != INFINITY &&
         a.g0 == a.g0 && a.g1 == a.g1 && a.g2 == a.g2)
         return make_float3(0.f, 0.f, 0.f);
-    const float id = -1.f / det;
+    // hardware reciprocal (1 ulp): the solve needs no correctly rounded divide
+    const float id = -__builtin_amdgcn_rcpf(det);
     return make_float3(id * (c00 * a.g0 + c01 * a.g1 + c02 * a.g2), id * (c01 * a.g0 + c11 * a.g1 + c12 * a.g2),
                        id * (c02 * a.g0 + c12 * a.g1 + c22 * a.g2));
 }

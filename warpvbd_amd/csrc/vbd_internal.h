@@ -92,6 +92,7 @@ struct Segment {
     int part_off;  // first per-block metric partial of this launch
     int nparts;    // blocks of this launch
     int lanes;     // lanes per vertex of this launch
+    int block;     // threads per block of this launch
 };
 
 struct DevBuf {
@@ -186,11 +187,15 @@ struct vbd_solver {
     bool st_ok = false;
     std::vector<int32_t> st_blk0;  // per colour: its first block, or -1 (not staged)
     std::vector<int64_t> st_ids;   // per colour: neighbour ids its blocks stage (sum over blocks)
-    int32_t *st_off = nullptr;     // [blocks + 1] start of each block's list in st_nb
-    int32_t *st_nb = nullptr;      // neighbour ids (colour-major)
+    int32_t *st_nb = nullptr;      // [blocks * kStageMax] neighbour ids (colour-major), -1 past each block's list
     uint2 *st_rec = nullptr;       // [n_slots] (l0 | l1 << 16, l2 | entry << 16)
     int32_t n_dict = 0;
     bool dict_shared = false;     // entries shared between slots (else one per slot)
+    // Colours swept with 32 lanes per vertex (below launch.small_color): their
+    // compact records padded to 32 per vertex (entry -1 past the vertex's
+    // slots), so a lane finds its record without the slot offset.
+    float4 *small_rec = nullptr;
+    std::vector<int64_t> small_off;  // per colour: first row of 32 records, or -1
     int32_t *colors = nullptr;    // original order
     int32_t *inc_off = nullptr;   // original order vertex -> incident (tet<<2|local), ascending tet
     int32_t *inc = nullptr;
@@ -239,6 +244,7 @@ int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out);
 int import_positions_dev(vbd_solver *s, const double *dev_in, double4 *dst);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
+int build_small_records(vbd_solver *s);  // padded records of the colours plan_segments gives 32 lanes
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);
 int build_staging(vbd_solver *s);  // neighbour staging tables (st_*), when the mesh allows
 extern bool g_record_dictionary;  // share record dictionary entries in later creates (default on)

@@ -482,7 +482,8 @@ __global__ void k_record_dict(int64_t ns, const int32_t *is, const int32_t *head
         const float2 qd = slots2[s];
         dict[3 * (size_t)k] = slots[3 * s + 1];
         dict[3 * (size_t)k + 1] = slots[3 * s + 2];
-        dict[3 * (size_t)k + 2] = make_float4(qd.x, qd.y, qa.w, 0.f);
+        const float4 qb = slots[3 * s + 1];
+        dict[3 * (size_t)k + 2] = make_float4(qd.x, qd.y, qa.w, slot_bb(make_float3(qb.x, qb.y, qb.z)));
     }
 }
 
@@ -497,7 +498,8 @@ __global__ void k_record_identity(int64_t ns, const float4 *slots, const float2 
     compact[s] = make_float4(qa.x, qa.y, qa.z, __int_as_float((int)s));
     dict[3 * s] = slots[3 * s + 1];
     dict[3 * s + 1] = slots[3 * s + 2];
-    dict[3 * s + 2] = make_float4(qd.x, qd.y, qa.w, 0.f);
+    const float4 qb = slots[3 * s + 1];
+    dict[3 * s + 2] = make_float4(qd.x, qd.y, qa.w, slot_bb(make_float3(qb.x, qb.y, qb.z)));
 }
 
 // Distinct neighbour vertices gathered by one colour's sweep (for the byte model).
@@ -1043,6 +1045,7 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     }
     VBD_HIP(hipStreamSynchronize(st));
     plan_segments(s);
+    VBD_TRY(build_small_records(s));
     return 0;
 }
 
@@ -1065,7 +1068,7 @@ int build_staging(vbd_solver *s)
     VBD_HIP(hipMemcpyAsync(off.data(), s->slot_off, sizeof(int32_t) * (nsw + 1), hipMemcpyDeviceToHost, st));
     VBD_HIP(hipMemcpyAsync(rec.data(), s->slots, sizeof(float4) * s->n_slots, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
-    std::vector<int32_t> blk_off(1, 0), nb;
+    std::vector<int32_t> nb;  // kStageMax ids per block, -1 past the block's list
     std::vector<uint2> srec(s->n_slots, make_uint2(0u, 0u));
     std::vector<int32_t> stamp(s->n, -1), local(s->n, 0);
     std::vector<int32_t> blk0(s->ncolors, -1);
@@ -1078,6 +1081,8 @@ int build_staging(vbd_solver *s)
         const int c0 = s->col_start[c];
         for (int j = 0; j < nv; j += kStageRows, ++nblocks) {
             const int va = c0 + j, vb = c0 + std::min(nv, j + kStageRows);
+            const size_t base = nb.size();
+            nb.resize(base + kStageMax, -1);
             int cnt = 0;
             for (int64_t q = off[va]; q < off[vb]; ++q) {
                 uint32_t w[4];
@@ -1086,26 +1091,26 @@ int build_staging(vbd_solver *s)
                 for (int k = 0; k < 3; ++k) {
                     const int id = (int)w[k];
                     if (stamp[id] != nblocks) {
+                        if (cnt == kStageMax) return 0;
                         stamp[id] = nblocks;
-                        local[id] = cnt++;
-                        nb.push_back(id);
+                        local[id] = cnt;
+                        nb[base + cnt] = id;
+                        ++cnt;
                     }
                     l[k] = (uint32_t)local[id];
                 }
                 srec[q] = make_uint2(l[0] | l[1] << 16, l[2] | w[3] << 16);
             }
-            if (cnt > kStageMax) return 0;
             ids[c] += cnt;
-            blk_off.push_back((int32_t)nb.size());
         }
     }
     if (nblocks == 0) return 0;
-    VBD_TRY(dmalloc(&s->st_off, blk_off.size()));
-    VBD_TRY(dmalloc(&s->st_nb, std::max<size_t>(1, nb.size())));
+    // zero records past the last slot: the sweep loads a lane's records
+    // without a bound (the ones past its vertex's last slot are not used)
+    srec.resize(srec.size() + kMaxElementsPerVertex, make_uint2(0u, 0u));
+    VBD_TRY(dmalloc(&s->st_nb, nb.size()));
     VBD_TRY(dmalloc(&s->st_rec, srec.size()));
-    VBD_HIP(hipMemcpyAsync(s->st_off, blk_off.data(), sizeof(int32_t) * blk_off.size(), hipMemcpyHostToDevice, st));
-    if (!nb.empty())
-        VBD_HIP(hipMemcpyAsync(s->st_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice, st));
+    VBD_HIP(hipMemcpyAsync(s->st_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipMemcpyAsync(s->st_rec, srec.data(), sizeof(uint2) * srec.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipStreamSynchronize(st));
     s->st_blk0 = blk0;

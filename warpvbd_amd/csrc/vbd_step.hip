@@ -132,18 +132,28 @@ __global__ __launch_bounds__(1024) void k_sweep(
 }
 
 // k_sweep<4, 1, MAT> with neighbour staging (256 threads, kStageRows
-// vertices per block): the block first unpacks its distinct neighbours
-// (st_nb[st_off[blk0 + b] ..]) and the record dictionary into LDS, so a slot
-// reads one 8-byte record from memory instead of a 16-byte record, three
-// dictionary rows and three positions.  Neighbours belong to other colours,
-// which this launch does not write, so staging them at the start is exact.
+// vertices per block).  Two memory rounds, then arithmetic only:
+//   1. the vertex's own data, the block's neighbour ids (a list of kStageMax
+//      entries per block, -1 past its end) and the record dictionary;
+//   2. the neighbours' positions and every 8-byte slot record of the lane
+//      (l0 | l1 << 16, l2 | entry << 16: local indices into the block's
+//      neighbour list and the dictionary entry; at most 32 / 4 per lane).
+// The neighbours are unpacked into LDS (FP32 parts and remainders) with the
+// dictionary; the slot loop then reads LDS only.  Neighbours belong to other
+// colours, which this launch does not write, so staging them at the start is
+// exact, and the arithmetic is k_sweep's, so the results are the same bit for
+// bit.
+constexpr int kStagedSlots = kMaxElementsPerVertex / 4;
+
 template <int MAT>
 __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int nv, int close,
                                                       const uint32_t *part_all, int np_all, float tol,
-                                                      const int32_t *st_off, const int32_t *st_nb,
-                                                      const uint2 *st_rec, int blk0, int n_dict)
+                                                      const int32_t *st_nb, const uint2 *st_rec, int blk0, int n_dict)
 {
     constexpr int L = 4;
+    constexpr int kStageUnroll = kStageMax / 256;
+    static_assert(kStageMax % 256 == 0, "each of the 256 threads stages kStageUnroll neighbours");
+    static_assert(3 * kStageDict <= 256, "one dictionary row per thread");
     __shared__ float4 s_na[kStageMax];
     __shared__ float2 s_nb[kStageMax];
     __shared__ float4 s_dict[3 * kStageDict];
@@ -153,39 +163,164 @@ __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int n
     const int vi = t / L, lane = t % L;
     const bool valid = vi < nv;
     const int v = v0 + (valid ? vi : 0);
+    // round 1
     const VertexPre pre = vertex_pre(a, v, valid);
-    const bool live = !c->converged;  // the same for every thread of the block
-    if (live) {
-        // every id, then every position, requested together: two memory
-        // latencies for the whole list (kStageMax = kStageUnroll x 256)
-        constexpr int kStageUnroll = kStageMax / 256;
-        static_assert(kStageMax % 256 == 0, "each of the 256 threads stages kStageUnroll neighbours");
-        const int n0 = st_off[blk0 + b], cnt = st_off[blk0 + b + 1] - n0;
-        int id[kStageUnroll];
+    const int32_t *ids = st_nb + (size_t)(blk0 + b) * kStageMax;
+    int id[kStageUnroll];
 #pragma unroll
-        for (int q = 0; q < kStageUnroll; ++q) {
+    for (int q = 0; q < kStageUnroll; ++q) id[q] = ids[threadIdx.x + 256 * q];
+    const bool has_dict = (int)threadIdx.x < 3 * n_dict;
+    const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // the flag was written by an earlier launch: the same for every thread
+    if (c->converged) return;
+    // round 2
+    float4 p[kStageUnroll];
+#pragma unroll
+    for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // (st_rec holds kStagedSlots * L zero records past the last slot)
+    uint2 rec[kStagedSlots];
+    const uint2 *rl = st_rec + pre.s0 + lane;
+#pragma unroll
+    for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[L * k];
+    if (has_dict) s_dict[threadIdx.x] = drow;
+#pragma unroll
+    for (int q = 0; q < kStageUnroll; ++q) {
+        if (id[q] >= 0) {
             const int k = threadIdx.x + 256 * q;
-            id[q] = k < cnt ? st_nb[n0 + k] : -1;
+            const float3 lo = disp_lo(p[q]);
+            s_na[k] = make_float4(p[q].x, p[q].y, p[q].z, lo.x);
+            s_nb[k] = make_float2(lo.y, lo.z);
         }
-        float4 p[kStageUnroll];
-#pragma unroll
-        for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int k = threadIdx.x; k < 3 * n_dict; k += blockDim.x) s_dict[k] = a.dict[k];
-#pragma unroll
-        for (int q = 0; q < kStageUnroll; ++q) {
-            const int k = threadIdx.x + 256 * q;
-            if (k < cnt) {
-                const float3 lo = disp_lo(p[q]);
-                s_na[k] = make_float4(p[q].x, p[q].y, p[q].z, lo.x);
-                s_nb[k] = make_float2(lo.y, lo.z);
-            }
-        }
-        __syncthreads();
     }
-    const Staged stg{st_rec, s_na, s_nb, s_dict};
-    const uint32_t mb = block_max_u32(update_vertex<L, 1, MAT, true>(a, v, valid && live, lane, pre, &stg));
-    if (!live) return;
+    __syncthreads();
+    Acc acc;
+    acc_zero(acc);
+    VertexLocal ql = vertex_local(pre);
+    const float3 hi = ql.hi, lo = ql.lo;
+    // this lane's slots: pre.s0 + lane, + L, ... below pre.s1
+    const int nk = pre.s1 > pre.s0 + lane ? (pre.s1 - pre.s0 - lane + L - 1) / L : 0;
+#pragma unroll
+    for (int k = 0; k < kStagedSlots; ++k) {
+        if (k < nk) {
+            const uint2 r = rec[k];
+            const uint32_t e = r.y >> 16, l0 = r.x & 0xffffu, l1 = r.x >> 16, l2 = r.y & 0xffffu;
+            const float4 d0 = s_dict[3 * e], d1 = s_dict[3 * e + 1], d2 = s_dict[3 * e + 2];
+            const float4 p0 = s_na[l0], p1 = s_na[l1], p2 = s_na[l2];
+            const float2 q0 = s_nb[l0], q1 = s_nb[l1], q2 = s_nb[l2];
+            // the differences slot_rel forms, from the unpacked parts
+            const float3 m0 = make_float3(p0.w, q0.x, q0.y), m1 = make_float3(p1.w, q1.x, q1.y),
+                         m2 = make_float3(p2.w, q2.x, q2.y);
+            const float3 e0 = diff_packed(p0, m0, p2, m2), e1 = diff_packed(p1, m1, p2, m2);
+            const float3 f2 = make_float3((p2.x - hi.x) + (m2.x - lo.x), (p2.y - hi.y) + (m2.y - lo.y),
+                                          (p2.z - hi.z) + (m2.z - lo.z));
+            slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
+                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), e0, e1, f2, acc);
+        }
+    }
+    double3 us;
+    const float3 dx = vertex_finish<L>(a, v, valid, acc, ql, us);
+    uint32_t m = 0u;
+    if (valid) {
+        if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
+        m = metric_bits(dx);
+    }
+    const uint32_t mb = block_max_u32(m);
     finish_sweep(a, c, mb, close, part_all, np_all, tol);
+}
+
+// A colour below small_color vertices: 32 lanes per vertex, one slot per
+// lane, as k_sweep<32, 1, MAT> (the same arithmetic, so the same results bit
+// for bit), but each lane reads its compact record from a table padded to 32
+// records per vertex (entry -1 past the vertex's slots), so a lane needs no
+// slot offset: two memory rounds (the vertex's data and its record; then the
+// record's constants and the three neighbour positions) instead of three.
+// These colours are latency bound.  In the iteration's last launch with a
+// single block the fold of the other launches' maxima (final: written by
+// launches that have finished) is requested with the first round, so its
+// latency overlaps the vertex's.
+template <int MAT>
+__global__ __launch_bounds__(1024) void k_sweep_small(SweepArgs a, int v0, int nv, int close, const uint32_t *part_all,
+                                                      int np_all, float tol, const float4 *prec)
+{
+    constexpr int L = 32;
+    constexpr int U = 16;  // fold loads per thread requested with the first round
+    Ctrl *c = a.ctrl;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int vi = t / L, lane = t % L;
+    const bool valid = vi < nv;
+    const int vr = valid ? vi : 0;
+    const int v = v0 + vr;
+    const bool early = close && gridDim.x == 1;
+    const int own = (int)(a.part - part_all);
+    uint32_t fl[U];
+    if (early) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int i = threadIdx.x + k * blockDim.x;
+            fl[k] = i < np_all && i != own ? part_all[i] : 0u;
+        }
+    }
+    const float4 uv = a.u[v];
+    const double4 tg = a.tgt[v];
+    const float4 r = prec[(size_t)vr * L + lane];
+    const bool live = !c->converged;
+    VertexPre pre;
+    pre.uv = uv;
+    pre.tg = tg;
+    pre.s0 = pre.s1 = 0;
+    Acc acc;
+    acc_zero(acc);
+    VertexLocal ql = vertex_local(pre);
+    const int e = __float_as_int(r.w);
+    if (valid && live && e >= 0) {
+        const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) + (uint32_t)e * 48u);
+        const float4 d0 = d[0], d1 = d[1], d2 = d[2];
+        const float4 x0 = *at(a.u, __float_as_uint(r.x));
+        const float4 x1 = *at(a.u, __float_as_uint(r.y));
+        const float4 x2 = *at(a.u, __float_as_uint(r.z));
+        const SlotRel sr = slot_rel(x0, x1, x2, ql.hi, ql.lo);
+        slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, acc);
+    }
+    double3 us;
+    const float3 dx = vertex_finish<L>(a, v, valid && live, acc, ql, us);
+    uint32_t m = 0u;
+    if (valid && live) {
+        if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
+        m = metric_bits(dx);
+    }
+    const uint32_t mb = block_max_u32(m);
+    if (!live) return;
+    if (!early) {
+        finish_sweep(a, c, mb, close, part_all, np_all, tol);
+        return;
+    }
+    // the rest of the fold (partials beyond the first U per thread), then the
+    // iteration's close
+    uint32_t x = 0u;
+#pragma unroll
+    for (int k = 0; k < U; ++k) x = x > fl[k] ? x : fl[k];
+    for (int i = threadIdx.x + U * blockDim.x; i < np_all; i += blockDim.x) {
+        const uint32_t y = i == own ? 0u : part_all[i];
+        x = x > y ? x : y;
+    }
+    __syncthreads();  // block_max_u32's staging is free again
+    const uint32_t all = block_max_u32(x);
+    if (threadIdx.x == 0) {
+        a.part[0] = mb;
+        close_iteration(c, c->it, all > mb ? all : mb, tol);
+    }
+}
+
+// The padded record table of one colour: row (v - v0) holds the vertex's
+// compact records in its 32 lanes, entry -1 past its last slot.
+__global__ void k_pad_records(const float4 *slots, const int32_t *slot_off, int v0, int nv, float4 *out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nv * 32) return;
+    const int v = v0 + (int)(t >> 5), lane = (int)(t & 31);
+    const int s0 = slot_off[v], s1 = slot_off[v + 1];
+    out[t] = s0 + lane < s1 ? slots[s0 + lane] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
 }
 
 // Iteration close when no colour has a vertex to sweep.
@@ -271,16 +406,28 @@ void launch_sweep(const SweepArgs &a, int v0, int nv, int block, int close, cons
 }
 
 void launch_staged(int material, const SweepArgs &a, int v0, int nv, int close, const uint32_t *part_all,
-                   int np_all, float tol, const int32_t *st_off, const int32_t *st_nb, const uint2 *st_rec, int blk0,
-                   int n_dict, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
+                   int np_all, float tol, const int32_t *st_nb, const uint2 *st_rec, int blk0, int n_dict,
+                   hipEvent_t e0, hipEvent_t e1, hipStream_t st)
 {
     const dim3 grid(nblk(nv, kStageRows));
     if (material == kSNH)
         hipExtLaunchKernelGGL(k_sweep_staged<kSNH>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, close, part_all, np_all,
-                              tol, st_off, st_nb, st_rec, blk0, n_dict);
+                              tol, st_nb, st_rec, blk0, n_dict);
     else
         hipExtLaunchKernelGGL(k_sweep_staged<kStVK>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, close, part_all,
-                              np_all, tol, st_off, st_nb, st_rec, blk0, n_dict);
+                              np_all, tol, st_nb, st_rec, blk0, n_dict);
+}
+
+void launch_small(int material, const SweepArgs &a, int v0, int nv, int block, int close, const uint32_t *part_all,
+                  int np_all, float tol, const float4 *prec, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
+{
+    const dim3 grid(nblk((int64_t)nv * 32, block));
+    if (material == kSNH)
+        hipExtLaunchKernelGGL(k_sweep_small<kSNH>, grid, dim3(block), 0, st, e0, e1, 0, a, v0, nv, close, part_all,
+                              np_all, tol, prec);
+    else
+        hipExtLaunchKernelGGL(k_sweep_small<kStVK>, grid, dim3(block), 0, st, e0, e1, 0, a, v0, nv, close, part_all,
+                              np_all, tol, prec);
 }
 
 template <int MAT>
@@ -313,7 +460,18 @@ void plan_segments(vbd_solver *s)
         if (nv == 0) continue;
         const int lanes = nv < s->launch.small_color ? 32 : s->launch.lanes;
         const int64_t threads = (int64_t)nv * lanes;
-        s->segments.push_back({c, c + 1, 0, nblk(threads, s->launch.block), lanes});
+        s->segments.push_back({c, c + 1, 0, nblk(threads, s->launch.block), lanes, s->launch.block});
+    }
+    // the iteration's last launch: one block of up to 1024 threads when its
+    // colour fits (the padded small-colour kernel then folds the other
+    // launches' maxima alongside its own vertices)
+    if (!s->segments.empty()) {
+        Segment &g = s->segments.back();
+        const int64_t threads = (int64_t)s->col_size[g.c0] * g.lanes;
+        if (g.lanes == 32 && threads <= 1024) {
+            g.block = 1024;
+            g.nparts = 1;
+        }
     }
     int off = 0;
     for (auto &g : s->segments) {
@@ -338,6 +496,35 @@ static int ensure_partials(vbd_solver *s)
     VBD_HIP(hipMalloc((void **)&s->part, sizeof(uint32_t) * need));
     s->part_cap = need;
     return 0;
+}
+
+int build_small_records(vbd_solver *s)
+{
+    if (s->small_rec) VBD_HIP(hipFree(s->small_rec));
+    s->small_rec = nullptr;
+    s->small_off.assign(s->ncolors, -1);
+#if VBD_TET_RECORDS
+    return 0;
+#else
+    if (s->n_bodies > 1 || !s->slots) return 0;
+    int64_t rows = 0;
+    for (const Segment &g : s->segments) {
+        if (g.lanes != 32) continue;
+        s->small_off[g.c0] = rows;
+        rows += s->col_size[g.c0];
+    }
+    if (rows == 0) return 0;
+    VBD_HIP(hipMalloc((void **)&s->small_rec, sizeof(float4) * 32 * rows));
+    for (const Segment &g : s->segments) {
+        if (g.lanes != 32) continue;
+        const int nv = s->col_size[g.c0];
+        k_pad_records<<<nblk((int64_t)nv * 32, 256), 256, 0, s->stream>>>(
+            s->slots, s->slot_off, s->col_start[g.c0], nv, s->small_rec + 32 * s->small_off[g.c0]);
+    }
+    VBD_HIP(hipGetLastError());
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+#endif
 }
 
 static int record_event(vbd_solver *s, int color, int iter, hipEvent_t *ev)
@@ -381,13 +568,21 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
         const int blk0 = s->st_ok && g.lanes == 4 && block == 256 && g.c0 < (int)s->st_blk0.size()
                              ? s->st_blk0[g.c0]
                              : -1;
+        const float4 *prec = s->small_rec && g.lanes == 32 && s->small_off[g.c0] >= 0
+                                 ? s->small_rec + 32 * s->small_off[g.c0]
+                                 : nullptr;
         if (blk0 >= 0)
-            launch_staged(s->material, a, v0, nv, close, s->part, s->n_parts, tol, s->st_off, s->st_nb,
-                          s->st_rec, blk0, s->n_dict, ev[0], ev[1], s->stream);
+            launch_staged(s->material, a, v0, nv, close, s->part, s->n_parts, tol, s->st_nb, s->st_rec, blk0,
+                          s->n_dict, ev[0], ev[1], s->stream);
+        else if (prec)
+            launch_small(s->material, a, v0, nv, g.block, close, s->part, s->n_parts, tol, prec, ev[0], ev[1],
+                         s->stream);
         else if (s->material == kSNH)
-            dispatch_sweep<kSNH>(g.lanes, a, v0, nv, block, close, s->part, s->n_parts, tol, ev[0], ev[1], s->stream);
+            dispatch_sweep<kSNH>(g.lanes, a, v0, nv, g.block, close, s->part, s->n_parts, tol, ev[0], ev[1],
+                                 s->stream);
         else
-            dispatch_sweep<kStVK>(g.lanes, a, v0, nv, block, close, s->part, s->n_parts, tol, ev[0], ev[1], s->stream);
+            dispatch_sweep<kStVK>(g.lanes, a, v0, nv, g.block, close, s->part, s->n_parts, tol, ev[0], ev[1],
+                                  s->stream);
     }
     if (s->segments.empty()) k_check_empty<<<1, 1, 0, s->stream>>>(s->ctrl, tol);
     VBD_HIP(hipGetLastError());

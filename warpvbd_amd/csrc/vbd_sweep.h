@@ -167,38 +167,81 @@ __device__ __forceinline__ VertexPre vertex_pre(const SweepArgs &a, int v, bool 
     return p;
 }
 
-// A block's staged sweep inputs (the big-colour sweep with neighbour
-// staging): per slot an 8-byte record (l0 | l1 << 16, l2 | entry << 16) in
-// memory, and in LDS the block's distinct neighbours unpacked (FP32 parts
-// and remainders) and the record dictionary.
-struct Staged {
-    const uint2 *rec;
-    const float4 *na;    // (x, y, z, remainder x) per neighbour
-    const float2 *nb;    // (remainder y, remainder z)
-    const float4 *dict;  // 3 rows per entry
+// The swept vertex's own terms, formed before its slot loop: its packed
+// displacement split into FP32 parts (hi) and remainders (lo), and the
+// inertia term's M/dt^2 (w) and x - t (t0..t2), formed in FP64 and rounded
+// once, so the loop carries four FP32 values instead of the FP64 target.
+struct VertexLocal {
+    float3 hi, lo;
+    float w, t0, t1, t2;
 };
+
+__device__ __forceinline__ VertexLocal vertex_local(const VertexPre &pre)
+{
+    VertexLocal q;
+    const float4 uv4 = pre.uv;
+    q.hi = make_float3(uv4.x, uv4.y, uv4.z);
+    q.lo = disp_lo(uv4);
+    const double4 tg = pre.tg;
+    const double3 u0 = unpack_disp(uv4);
+    q.w = (float)tg.w;
+    q.t0 = (float)(u0.x - tg.x);
+    q.t1 = (float)(u0.y - tg.y);
+    q.t2 = (float)(u0.z - tg.z);
+    // an empty asm that reads and writes them keeps the compiler from sinking
+    // the subtraction (and the FP64 target with it) past the loop
+    asm volatile("" : "+v"(q.t0), "+v"(q.t1), "+v"(q.t2), "+v"(q.w));
+    return q;
+}
+
+// After the slot loop: the group sum of gradient and Hessian, damping and
+// inertia, and the guarded 3x3 solve.  us receives the vertex's current
+// displacement in FP64 (from the parts the loop holds).  Lanes without a
+// vertex get dx = 0.
+template <int L>
+__device__ __forceinline__ float3 vertex_finish(const SweepArgs &a, int v, bool valid, Acc &acc, VertexLocal &q,
+                                                double3 &us)
+{
+    // the empty asm keeps the compiler from carrying the FP64 displacement
+    // through the loop instead of rebuilding it from the parts here
+    asm volatile("" : "+v"(q.hi.x), "+v"(q.hi.y), "+v"(q.hi.z), "+v"(q.lo.x), "+v"(q.lo.y), "+v"(q.lo.z));
+    us = make_double3((double)q.hi.x + (double)q.lo.x, (double)q.hi.y + (double)q.lo.y,
+                      (double)q.hi.z + (double)q.lo.z);
+    acc_group_sum<L>(acc);
+    if (!valid) return make_float3(0.f, 0.f, 0.f);
+    if (a.kd_dt != 0.f) {
+        // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
+        const double4 up = a.uprev[v];
+        const float e0 = (float)(us.x - up.x), e1 = (float)(us.y - up.y), e2 = (float)(us.z - up.z);
+        const float k = a.kd_dt;
+        acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
+        acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
+        acc.g2 += k * (acc.h02 * e0 + acc.h12 * e1 + acc.h22 * e2);
+        const float f = 1.f + k;
+        acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
+    }
+    // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
+    acc.g0 += q.w * q.t0;
+    acc.g1 += q.w * q.t1;
+    acc.g2 += q.w * q.t2;
+    acc.h00 += q.w;
+    acc.h11 += q.w;
+    acc.h22 += q.w;
+    return solve3_guarded(acc);
+}
 
 // Gradient and Hessian of one vertex's local energy (all incident tets plus
 // inertia and damping) accumulated by a group of L lanes, and the block solve:
 // the update dx (zero for lanes without a vertex).  us receives the vertex's
-// current displacement.  STAGED: slots read through stg (B == 1).
-template <int L, int B, int MAT, bool STAGED = false>
+// current displacement.
+template <int L, int B, int MAT>
 __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool valid, int lane, double3 &us,
-                                            const VertexPre &pre, const Staged *stg = nullptr)
+                                            const VertexPre &pre)
 {
     Acc acc;
     acc_zero(acc);
-    const float4 uv4 = pre.uv;
-    float3 hi = make_float3(uv4.x, uv4.y, uv4.z), lo = disp_lo(uv4);
-    // the inertia term needs only x - t and M/dt^2: formed here in FP64, so
-    // the slot loop carries four FP32 values instead of the FP64 target
-    const double4 tg = pre.tg;
-    const double3 u0 = unpack_disp(uv4);
-    float w = (float)tg.w;
-    float t0 = (float)(u0.x - tg.x), t1 = (float)(u0.y - tg.y), t2 = (float)(u0.z - tg.z);
-    // an empty asm that reads and writes them keeps the compiler from sinking
-    // the subtraction (and the FP64 target with it) past the loop
-    asm volatile("" : "+v"(t0), "+v"(t1), "+v"(t2), "+v"(w));
+    VertexLocal ql = vertex_local(pre);
+    const float3 hi = ql.hi, lo = ql.lo;
     if (valid) {
         const int s0 = pre.s0, s1 = pre.s1;
 #if VBD_TET_RECORDS
@@ -236,34 +279,12 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
                     const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
-                    slot_eval<MAT>(ka[j], kb[j], kc[j], bs[j], b0[j], b1[j], r.e0, r.e1, r.d2, acc);
+                    slot_eval<MAT>(ka[j], kb[j], kc[j], slot_bb(bs[j]), bs[j], b0[j], b1[j], r.e0, r.e1, r.d2, acc);
                 }
             }
         }
 #else
-        if constexpr (STAGED) {
-            // One slot per lane per batch from the staged neighbours and
-            // dictionary; only the 8-byte record comes from memory, and the
-            // next one is requested before this slot's LDS reads.  The
-            // differences are formed exactly as slot_rel forms them.
-            int s = s0 + lane;
-            uint2 r = s < s1 ? stg->rec[s] : make_uint2(0u, 0u);
-            for (; s < s1; s += L) {
-                const uint2 rn = stg->rec[min(s + L, s1 - 1)];  // the last one reloads a record in cache
-                const uint32_t e = r.y >> 16, l0 = r.x & 0xffffu, l1 = r.x >> 16, l2 = r.y & 0xffffu;
-                const float4 d0 = stg->dict[3 * e], d1 = stg->dict[3 * e + 1], d2 = stg->dict[3 * e + 2];
-                const float4 p0 = stg->na[l0], p1 = stg->na[l1], p2 = stg->na[l2];
-                const float2 q0 = stg->nb[l0], q1 = stg->nb[l1], q2 = stg->nb[l2];
-                const float3 m0 = make_float3(p0.w, q0.x, q0.y), m1 = make_float3(p1.w, q1.x, q1.y),
-                             m2 = make_float3(p2.w, q2.x, q2.y);
-                const float3 e0 = diff_packed(p0, m0, p2, m2), e1 = diff_packed(p1, m1, p2, m2);
-                const float3 f2 = make_float3((p2.x - hi.x) + (m2.x - lo.x), (p2.y - hi.y) + (m2.y - lo.y),
-                                              (p2.z - hi.z) + (m2.z - lo.z));
-                slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
-                               make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), e0, e1, f2, acc);
-                r = rn;
-            }
-        } else if constexpr (B == 1) {
+        if constexpr (B == 1) {
             // One slot per lane per batch, software-pipelined: the next compact
             // record is requested after the current slot's constants and
             // gathers, so each batch waits for one memory latency, not two.
@@ -284,7 +305,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                 const float4 cn = *at(a.slots, (uint32_t)min(s + L, s1 - 1));  // the last one reloads a record in cache
                 __builtin_amdgcn_sched_barrier(0);
                 const SlotRel sr = slot_rel(x0, x1, x2, hi, lo);
-                slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), make_float3(d0.x, d0.y, d0.z),
+                slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
                                make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2,
                                acc);
                 c = cn;
@@ -295,6 +316,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         for (int s = s0 + lane; s < s1; s += B * L) {
             float4 q[B][3];
             float2 qd[B];
+            float qbb[B];
             float4 x[B][3];
             // compact record (ids, entry), then the entry's constants
 #pragma unroll
@@ -308,6 +330,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                     q[j][1] = d[0];
                     q[j][2] = d[1];
                     qd[j] = make_float2(d2.x, d2.y);
+                    qbb[j] = d2.w;
                 }
             }
 #pragma unroll
@@ -323,7 +346,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                 if (s + j * L < s1) {
                     const float4 *qq = q[j];
                     const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
-                    slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), make_float3(qq[1].x, qq[1].y, qq[1].z),
+                    slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), qbb[j], make_float3(qq[1].x, qq[1].y, qq[1].z),
                                    make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[2].w, qd[j].x, qd[j].y),
                                    r.e0, r.e1, r.d2, acc);
                 }
@@ -331,31 +354,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         }
 #endif
     }
-    // the FP64 displacement again from its parts, which the loop holds anyway
-    // (the empty asm keeps the compiler from carrying u0 through the loop)
-    asm volatile("" : "+v"(hi.x), "+v"(hi.y), "+v"(hi.z), "+v"(lo.x), "+v"(lo.y), "+v"(lo.z));
-    us = make_double3((double)hi.x + (double)lo.x, (double)hi.y + (double)lo.y, (double)hi.z + (double)lo.z);
-    acc_group_sum<L>(acc);
-    if (!valid) return make_float3(0.f, 0.f, 0.f);
-    if (a.kd_dt != 0.f) {
-        // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
-        const double4 up = a.uprev[v];
-        const float e0 = (float)(us.x - up.x), e1 = (float)(us.y - up.y), e2 = (float)(us.z - up.z);
-        const float k = a.kd_dt;
-        acc.g0 += k * (acc.h00 * e0 + acc.h01 * e1 + acc.h02 * e2);
-        acc.g1 += k * (acc.h01 * e0 + acc.h11 * e1 + acc.h12 * e2);
-        acc.g2 += k * (acc.h02 * e0 + acc.h12 * e1 + acc.h22 * e2);
-        const float f = 1.f + k;
-        acc.h00 *= f; acc.h01 *= f; acc.h02 *= f; acc.h11 *= f; acc.h12 *= f; acc.h22 *= f;
-    }
-    // inertia + gravity (src/vbd_solver.py:15-47, summed over slots): M/dt^2 (x - t)
-    acc.g0 += w * t0;
-    acc.g1 += w * t1;
-    acc.g2 += w * t2;
-    acc.h00 += w;
-    acc.h11 += w;
-    acc.h22 += w;
-    return solve3_guarded(acc);
+    return vertex_finish<L>(a, v, valid, acc, ql, us);
 }
 
 
@@ -367,12 +366,12 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
     return vertex_dx<L, B, MAT>(a, v, valid, lane, us, vertex_pre(a, v, valid));
 }
 
-template <int L, int B, int MAT, bool STAGED = false>
+template <int L, int B, int MAT>
 __device__ __forceinline__ uint32_t update_vertex(const SweepArgs &a, int v, bool valid, int lane,
-                                                  const VertexPre &pre, const Staged *stg = nullptr)
+                                                  const VertexPre &pre)
 {
     double3 us;
-    const float3 dx = vertex_dx<L, B, MAT, STAGED>(a, v, valid, lane, us, pre, stg);
+    const float3 dx = vertex_dx<L, B, MAT>(a, v, valid, lane, us, pre);
     if (!valid) return 0u;
     if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
     return metric_bits(dx);
