@@ -50,6 +50,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "sim-s/wall-s at tol 1e-7 (warm-start+batched vs plain VBD); HBM GB/s"
+CONVERGED_TOL = 1e-11  # the checker's converged FP64 solve
 HBM_PEAK_GBS = 8000.0
 
 
@@ -73,6 +74,8 @@ def parse():
     p.add_argument("--warm-epochs", type=int, default=12)
     p.add_argument("--mixed-samples", type=int, default=24)
     p.add_argument("--roofline-steps", type=int, default=100)
+    p.add_argument("--check-steps", type=int, default=3,
+                   help="steps per workload checked against the FP64 oracle by the CPU-baseline leg")
     return p.parse_args()
 
 
@@ -89,6 +92,19 @@ def time_steps(solver, steps, dt, **kw):
         ms.append((time.perf_counter() - a) * 1e3)
         iters.append(solver.last_iterations)
     return time.perf_counter() - t0, np.array(iters), np.array(ms)
+
+
+def check_window(step, state, steps):
+    """A short window for the oracle check (done by the CPU-baseline leg, the
+    checker): per step the state it starts from (x_prev, v_prev), then the
+    GPU's positions after it.  step() advances the solver; state() returns
+    (x, v) as host arrays."""
+    out = []
+    for _ in range(steps):
+        x0, v0 = state()
+        x1 = step()
+        out.append((x0, v0, x1))
+    return out
 
 
 def headline(args, material):
@@ -114,6 +130,10 @@ def headline(args, material):
     alg, lay = solver.color_bytes()[big].sum(), solver.color_layout_bytes()[big].sum()
     prof["layout_ratio"] = float(lay / alg) if alg > 0 else 1.0
     prof["records"] = solver.record_layout()
+    prof["check"] = {"kwargs": sc.solver_kwargs(), "material": material, "tol": args.tol, "dt": args.dt,
+                     "window": check_window(lambda: solver.step(solver.old_positions, args.dt).numpy().copy(),
+                                            lambda: (solver.old_positions.numpy().copy(),
+                                                     solver.old_velocities.numpy().copy()), args.check_steps)}
     return sc, solver, wall, iters, ms, prof
 
 
@@ -171,6 +191,42 @@ def cpu_baseline(scene, args, gpu_iters_per_step):
     }
 
 
+def oracle_check(check, threads, max_iter):
+    """The checker of a check window: before each step the FP64 oracle
+    (oracle/vbd_oracle.c, the reference formula) is given the state the GPU
+    step started from and steps to the same tolerance (for a warm-started GPU
+    step: the oracle's plain step), and once more to CONVERGED_TOL.  Returns
+    the largest |x_gpu - x_oracle| of each (a step stopped at a tolerance is
+    within a tolerance-sized distance of the converged one, from either side,
+    so a warm start and a plain start stop at different points)."""
+    import oracle
+
+    def run(kw, items):
+        r = oracle.OracleSolver(**kw, dx_tol=check["tol"], max_iter=max_iter, material=check["material"],
+                                expanded=False, threads=threads)
+        err = err_c = 0.0
+        for x0, v0, x1 in items:
+            r.set_state(x=x0, v=v0)
+            err = max(err, float(np.abs(x1 - r.step(check["dt"])).max()))
+            r.set_state(x=x0, v=v0)
+            xc = r.step(check["dt"], dx_tol=CONVERGED_TOL, max_iter=100000)
+            err_c = max(err_c, float(np.abs(x1 - xc).max()))
+        X = np.asarray(kw["initial_positions"])
+        return err, err_c, float(np.linalg.norm(X.max(0) - X.min(0)))
+
+    items = ([(check["kwargs"], check["window"])] if "kwargs" in check else
+             [(kw, [(x0[b], v0[b], x1[b]) for x0, v0, x1 in check["window"]]) for b, kw in enumerate(check["bodies"])])
+    worst = worst_c = rel = 0.0
+    for kw, win in items:
+        e, ec, d = run(kw, win)
+        worst, worst_c, rel = max(worst, e), max(worst_c, ec), max(rel, e / d)
+    out = {"oracle_err_m": worst, "rel_to_bbox_diagonal": rel, "err_vs_converged_m": worst_c,
+           "steps": len(check["window"])}
+    if "bodies" in check:
+        out["bodies"] = len(check["bodies"])
+    return out
+
+
 def plain_per_body(bodies, args, n_sample=32, steps=10, seed=0):
     """One plain VBDSolver per body, stepped one body after another (a
     WarpVBD user without the concatenation trick): mean wall seconds per step
@@ -190,6 +246,18 @@ def plain_per_body(bodies, args, n_sample=32, steps=10, seed=0):
     return {"aggregate_sim_s_per_wall_s": len(bodies) * args.dt / wall_per_step_all,
             "ms_per_step_all_bodies": 1e3 * wall_per_step_all,
             "sample": f"{len(pick)} bodies drawn with seed {seed}, {steps} timed steps each after 3 untimed"}
+
+
+def body_check(bat, bodies, args, step, n_sample=8, seed=0):
+    """A check window of a batched solver: the sampled bodies' slices of the
+    state before each step and of the positions after it."""
+    rng = np.random.default_rng(seed)
+    pick = sorted(rng.choice(len(bodies), size=min(n_sample, len(bodies)), replace=False).tolist())
+    win = check_window(step, lambda: (bat.old_positions.numpy().copy(), bat.old_velocities.numpy().copy()),
+                       args.check_steps)
+    sl = [bat.body_slice(i) for i in pick]
+    return {"bodies": [bodies[i].solver_kwargs() for i in pick], "tol": args.tol, "dt": args.dt, "material": "snh",
+            "window": [([x0[q] for q in sl], [v0[q] for q in sl], [x1[q] for q in sl]) for x0, v0, x1 in win]}
 
 
 def batched_section(args):
@@ -213,6 +281,7 @@ def batched_section(args):
     tune = bat.autotune(args.dt, trial_steps=30, cache=False)
     w_bat, it_bat, ms_bat = time_steps(bat, args.batch_steps, args.dt)
     nb = len(bodies)
+    check = body_check(bat, bodies, args, lambda: bat.step(dt=args.dt).numpy().copy())
     sim = args.batch_steps * args.dt
     per_body = plain_per_body(bodies, args)
     return {
@@ -228,6 +297,7 @@ def batched_section(args):
                     "mean_body_iterations_per_step": float(np.mean(bat.body_iterations)),
                     "autotuned": {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}},
         "speedup": w_plain / w_bat,
+        "_check": {"batched": check},
     }
 
 
@@ -244,7 +314,7 @@ def warm_big_section(args):
     sc = cantilever((256, 32, 32))
     out = {"workload": "cantilever 256x32x32 SNH, E=1e5, dt=1e-3, tol 1e-7, from rest: "
                        f"{args.warmup} untimed + {args.steps} timed steps per form"}
-    xs = {}
+    xs, checks = {}, {}
     for form in ("plain", "history"):
         s = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
         ws = None
@@ -255,6 +325,10 @@ def warm_big_section(args):
             s.step(s.old_positions, args.dt, warm_start=ws)
         w, it, ms = time_steps(s, args.steps, args.dt, warm_start=ws)
         xs[form] = s.old_positions.numpy()
+        checks[form] = {"kwargs": sc.solver_kwargs(), "material": "snh", "tol": args.tol, "dt": args.dt,
+                        "window": check_window(lambda: s.step(s.old_positions, args.dt, warm_start=ws).numpy().copy(),
+                                               lambda: (s.old_positions.numpy().copy(),
+                                                        s.old_velocities.numpy().copy()), args.check_steps)}
         out[form] = {"sim_s_per_wall_s": args.steps * args.dt / w, "ms_per_step": 1e3 * w / args.steps,
                      "iterations_per_step": float(it.mean()),
                      "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)]}
@@ -263,6 +337,7 @@ def warm_big_section(args):
         del s
     out["speedup_history_vs_plain"] = out["history"]["sim_s_per_wall_s"] / out["plain"]["sim_s_per_wall_s"]
     out["max_abs_diff_history_vs_plain_m"] = float(np.abs(xs["history"] - xs["plain"]).max())
+    out["_check"] = checks
     return out
 
 
@@ -321,7 +396,15 @@ def warm_section(args):
         err.append(float(np.abs(xw.numpy() - xp.numpy()).max()))
         err_h.append(float(np.abs(xh.numpy() - xp.numpy()).max()))
     sim = n_eval * eval_steps * args.dt
+    kw = dict(sc.solver_kwargs(), youngs_modulus=E, poissons_ratio=nu, gravity=tuple(g))
+    checks = {}
+    for form, sol, w in (("plain", ref, None), ("warm", s, ws), ("history_only", sh, wh)):
+        checks[form] = {"kwargs": kw, "material": "snh", "tol": args.tol, "dt": args.dt,
+                        "window": check_window(lambda: sol.step(sol.old_positions, args.dt, warm_start=w).numpy().copy(),
+                                               lambda: (sol.old_positions.numpy().copy(),
+                                                        sol.old_velocities.numpy().copy()), args.check_steps)}
     return {
+        "_check": checks,
         "workload": f"cantilever 128x16x16 ({sc.n_vertices} vertices); training: {args.warm_trajectories} seeded "
                     f"trajectories x {args.warm_steps} steps (every 2nd step stored, {ws.n_samples} samples; "
                     f"reduced from 200 x 300 to fit the benchmark budget), E log-U[1e4,1e6], nu U[0.3,0.45], gravity "
@@ -381,6 +464,7 @@ def mixed_section(args):
     tune = bat.autotune(args.dt, trial_steps=30, cache=False)
     # every timed form starts from this state, after one untimed step of its own
     x0, v0 = bat.old_positions.numpy(), bat.old_velocities.numpy()
+    checks = {}
 
     def timed_leg(policy):
         bat.reset()
@@ -408,6 +492,7 @@ def mixed_section(args):
         if warm is not None:
             r["bodies_per_guess"] = {k: float(np.mean([g[k] for g in guess_l])) / nb
                                      for k in ("plain", "network", "history")}
+        checks[policy] = body_check(bat, bodies, args, lambda: bat.step(dt=args.dt, warm_start=warm).numpy().copy())
         return r
 
     out["batched"] = timed_leg(None)
@@ -425,6 +510,8 @@ def mixed_section(args):
     out["batched_warm_adaptive"] = timed_leg(WarmStart.ADAPTIVE)
     out["batched_warm_adaptive"]["policy"] = ("adaptive: choose, with the network skipped 1..16 steps (history "
                                               "or plain guess) after a step where fewer than 2% of bodies took it")
+    out["_check"] = {"batched": checks[None], "batched_history": checks[WarmStart.HISTORY],
+                     "batched_warm": checks[WarmStart.CHOOSE], "batched_warm_adaptive": checks[WarmStart.ADAPTIVE]}
     out["plain_per_body"] = plain_per_body(bodies, args)
     base = out["plain_concatenated"]["aggregate_sim_s_per_wall_s"]
     per_body = out["plain_per_body"]["aggregate_sim_s_per_wall_s"]
@@ -447,11 +534,14 @@ def main():
     sc, solver, wall, iters, ms, prof = headline(args, args.material)
     n_active = int(sc.active_mask.sum())
     achieved = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
-    traffic = None
+    traffic, sq = None, {}
     tf = os.path.join(ROOT, "profiles", f"k_sweep_cantilever256x32x32_{args.material}_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof_doc = json.load(f)
+        traffic, sq = prof_doc.get("hbm_bytes_per_launch"), prof_doc.get("sq", {})
+    avg_us = prof["ms"] * 1e3 / max(1, prof["launches"])
+    hbm_meas = traffic / (avg_us * 1e-6) / 1e9 if traffic and avg_us > 0 else None
     out = {
         "metric": METRIC,
         "value": args.steps * args.dt / wall,
@@ -482,16 +572,24 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_sweep (one colour sweep, colours with >= 4096 vertices)",
+            "kernel": "k_sweep_staged (one colour sweep, colours with >= 4096 vertices)",
             "launches": prof["launches"],
-            "avg_launch_us": prof["ms"] * 1e3 / max(1, prof["launches"]),
+            "avg_launch_us": avg_us,
             "algorithmic_bytes_per_launch": prof["bytes"] / max(1, prof["launches"]),
+            "hbm_measured_GBps": hbm_meas,
+            "hbm_measured_frac": hbm_meas / HBM_PEAK_GBS if hbm_meas else None,
             "layout_bytes_per_launch": prof["layout_ratio"] * prof["bytes"] / max(1, prof["launches"]),
             "layout_GBps": prof["layout_ratio"] * achieved,
+            "limiter": "valu" if sq else None,
+            "valu_busy_frac": sq.get("valu_busy_frac"),
+            "valu_insts_per_wave": sq.get("valu_insts_per_wave"),
             "records": prof["records"],
-            "note": ("achieved counts the reference's 68 bytes per incident tet (algorithmic model); the "
-                     "sweep reads 16-byte slot records plus shared constants from cache (layout bytes; "
-                     "traffic = measured HBM bytes)"),
+            "note": ("achieved / frac: the algorithmic model (the reference's 68 bytes per incident tet, 40 per "
+                     "vertex, 12 per gathered neighbour) over the measured launch time.  The kernel moves far "
+                     "fewer bytes (16- and 8-byte records, a shared dictionary, neighbours staged in LDS): "
+                     "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, hbm_measured_frac = "
+                     "that traffic over the launch time.  The SQ counters show the limiter: VALU issue "
+                     "(valu_busy_frac of the kernel's busy cycles; profiles/k_sweep_cantilever256x32x32_snh.md)"),
         },
     }
     del solver
@@ -500,7 +598,8 @@ def main():
         log("stvk")
         sc2, s2, w2, it2, ms2, pr2 = headline(argparse.Namespace(**{**vars(args), "steps": 100}), "stvk")
         out["stvk"] = {"sim_s_per_wall_s": 100 * args.dt / w2, "iterations_per_step": float(it2.mean()),
-                       "sweep_GBps": pr2["bytes"] / (pr2["ms"] * 1e-3) / 1e9 if pr2["ms"] else None}
+                       "sweep_GBps": pr2["bytes"] / (pr2["ms"] * 1e-3) / 1e9 if pr2["ms"] else None,
+                       "_check": {"plain": pr2.pop("check")}}
         del s2
     if "batched" in sections:
         log("batched")
@@ -514,9 +613,37 @@ def main():
     if "mixed" in sections:
         log("mixed")
         out["mixed"] = mixed_section(args)
+    if "mixed" in out:
+        m = out["mixed"]
+        out["warm_batched_ratio"] = {
+            "workload": "1024 mixed bodies (512 cantilevers, 512 mass-springs, seed 5), SNH, dt=1e-3, tol 1e-7",
+            "warm_start_batched_vs_plain_batched": m["batched_warm_adaptive"]["vs_plain_batched"],
+            "warm_start_batched_vs_plain_concatenated": m["batched_warm_adaptive"]["vs_plain_concatenated"],
+            "history_batched_vs_plain_batched": m["batched_history"]["vs_plain_batched"],
+            "history_batched_vs_plain_concatenated": m["batched_history"]["vs_plain_concatenated"],
+            "note": ("warm start = the ADAPTIVE policy (learned network, history guess and plain guess, chosen "
+                     "per body by first-sweep residual); history = the same without the network; plain "
+                     "batched = the batched solver from the inertia guess; plain concatenated = one solver on "
+                     "the concatenated mesh (the reference's parallel-environment form)"),
+        }
+    checks = {"headline": prof.pop("check")}
+    for name in ("stvk", "batched", "warm_big", "warm_start", "mixed"):
+        if name in out:
+            for form, c in out[name].pop("_check").items():
+                checks[f"{name}.{form}"] = c
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(sc, args, float(iters.mean()))
+        log("oracle check")
+        threads = len(os.sched_getaffinity(0))
+        for key, c in checks.items():
+            r = oracle_check(c, threads, args.max_iter)
+            if key == "headline":
+                out["oracle_err_m"] = r["oracle_err_m"]
+                out["oracle_check"] = r
+            else:
+                sec, form = key.split(".")
+                out[sec].setdefault(form, {})["oracle_check"] = r
     print(json.dumps(out), flush=True)
 
 
