@@ -217,3 +217,23 @@ def test_beam_128x16x16_history_guess_matches_oracle():
         if k > 0:
             c2 = c
         c = (xr - y) * act
+
+
+def test_mass_spring_512_live_block_lists_equal_full_launches(springs512):
+    """The batch's big colours run the staged sweep; once most bodies have
+    converged, only the staging blocks that hold a live body's vertex are
+    launched.  Same per-vertex work as launching every block: same iteration
+    counts and bit-identical positions."""
+    a = BatchedVBDSolver(springs512, dx_tol=1e-7, max_iter=1000, verbose=False)
+    b = BatchedVBDSolver(springs512, dx_tol=1e-7, max_iter=1000, verbose=False)
+    assert a.staged_colors() == 4
+    a.set_mode("launches", 4, 256, True)
+    b.set_mode("launches_full", 4, 256, True)
+    spread = 0
+    for _ in range(8):
+        xa = a.step(dt=1e-3).numpy()
+        xb = b.step(dt=1e-3).numpy()
+        assert np.array_equal(a.body_iterations, b.body_iterations)
+        assert np.array_equal(xa, xb)
+        spread = max(spread, int(a.body_iterations.max() - a.body_iterations.min()))
+    assert spread >= 2

@@ -32,7 +32,7 @@ namespace {
 
 template <int L, int B, int MAT>
 __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_t *body_new, const BodyState *bs,
-                                                       uint32_t *dxm, int v0, int nv)
+                                                       uint32_t *bmet, int v0, int nv)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -41,9 +41,10 @@ __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_
     const int v = v0 + (in ? vi : 0);
     // the vertex's own loads go out with the body lookup, not after it
     const VertexPre pre = vertex_pre(a, v, in);
-    const bool valid = in && !bs[body_new[v]].done;
+    const int body = body_new[v];
+    const bool valid = in && !bs[body].done;
     const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane, pre);
-    if (valid && lane == 0) dxm[v] = m;
+    body_metric_max(valid ? body : -1, m, bmet);
 }
 
 // Colour sweep over a task list: block b updates the vertices
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_
 // covers only their ranges.
 template <int L, int B, int MAT>
 __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t *body_new, const BodyState *bs,
-                                                      uint32_t *dxm, const int2 *tasks)
+                                                      uint32_t *bmet, const int2 *tasks)
 {
     const int2 t = tasks[xcd_remap(blockIdx.x, gridDim.x)];
     const int vi = threadIdx.x / L, lane = threadIdx.x % L;
@@ -60,60 +61,59 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t
     const int v = t.x + (in ? vi : 0);
     // the vertex's own loads go out with the body lookup, not after it
     const VertexPre pre = vertex_pre(a, v, in);
-    const bool valid = in && !bs[body_new[v]].done;
+    const int body = body_new[v];
+    const bool valid = in && !bs[body].done;
     const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane, pre);
-    if (valid && lane == 0) dxm[v] = m;
+    body_metric_max(valid ? body : -1, m, bmet);
 }
 
-// One workgroup per body: fold the body's per-vertex metrics of this iteration
-// and retire the body when it converged or used its iterations.  Eight
-// colours at a time: their ranges, then each thread's first metric of each,
-// are requested before any is used (a body's colour range is mostly shorter
-// than the block), so the fold waits for two memory latencies per eight
-// colours instead of two per colour.
-__global__ __launch_bounds__(256) void k_check_bodies(const int2 *ranges, int C, const uint32_t *dxm, BodyState *bs,
-                                                      int max_iter, float tol, int32_t *n_live)
+// A staged colour (staged_sweep in vbd_sweep.h) across every body: 256
+// threads per block of kStageRows consecutive vertices of the colour; block
+// blist[i].x for the i-th block when a list is given (the blocks that still
+// hold a live body's vertex), else all of them.
+template <int MAT>
+__global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0, int nv, const int32_t *st_nb,
+                                                             const uint2 *st_rec, int blk0, int n_dict,
+                                                             const int32_t *body_new, const BodyState *bs,
+                                                             uint32_t *bmet, const int2 *blist)
 {
-    constexpr int U = 8;
-    BodyState *st = bs + blockIdx.x;
+    const int b = blist ? blist[blockIdx.x].x : xcd_remap(blockIdx.x, gridDim.x);
+    uint32_t m;
+    int body;
+    (void)staged_sweep<MAT, true>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, body_new, bs, body, m);
+    body_metric_max(body, m, bmet);
+}
+
+// After an iteration's colour launches: per body (one thread each) the
+// iteration's max |dx_x + dx_y + dx_z| (bmet, folded by the launches with
+// body_metric_max), its iteration count, and the body retires when it
+// converged or used its iterations.  bmet is cleared for the next iteration.
+__global__ void k_check_bodies(int nb, uint32_t *bmet, BodyState *bs, int max_iter, float tol, int32_t *n_live)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    BodyState *st = bs + b;
     if (st->done) return;
-    const int2 *r = ranges + (size_t)blockIdx.x * (C + 1);
-    const int tid = threadIdx.x, step = blockDim.x;
-    uint32_t mx = 0;
-    for (int c0 = 0; c0 < C; c0 += U) {
-        int2 rc[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) rc[k] = c0 + k < C ? r[c0 + k] : make_int2(0, 0);
-        uint32_t y[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) y[k] = tid < rc[k].y ? dxm[rc[k].x + tid] : 0u;
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            mx = mx > y[k] ? mx : y[k];
-            for (int i = rc[k].x + step + tid; i < rc[k].x + rc[k].y; i += step) {
-                const uint32_t z = dxm[i];
-                mx = mx > z ? mx : z;
-            }
-        }
-    }
-    mx = block_max_u32(mx);
-    if (threadIdx.x == 0) {
-        const int it = st->iters + 1;
-        st->iters = it;
-        st->metric = mx;
-        if (__uint_as_float(mx) < tol || it >= max_iter) {
-            st->done = 1;
-            atomicSub(n_live, 1);
-        }
+    const uint32_t mx = bmet[b];
+    bmet[b] = 0u;
+    const int it = st->iters + 1;
+    st->iters = it;
+    st->metric = mx;
+    if (__uint_as_float(mx) < tol || it >= max_iter) {
+        st->done = 1;
+        atomicSub(n_live, 1);
     }
 }
 
 __global__ void k_init_bodies(int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
                               const uint8_t *active, const int32_t *body_new, const float4 *grav, float dt, int nb,
-                              BodyState *bs, int32_t *n_live)
+                              BodyState *bs, uint32_t *bmet, int32_t *n_live)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) bs[i] = BodyState{0, 0, 0u, 0};
+    if (i < nb) {
+        bs[i] = BodyState{0, 0, 0u, 0};
+        bmet[i] = 0u;
+    }
     if (i == 0) *n_live = nb;
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body_new[i]]);
 }
@@ -128,46 +128,62 @@ __global__ void k_finish_all(int n, const float4 *u, double4 *uprev, float4 *vel
 }
 
 template <int L, int MAT>
-void launch_sweep_bodies(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm, int v0,
+void launch_sweep_bodies(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet, int v0,
                          int nv, int block, hipStream_t st)
 {
-    k_sweep_bodies<L, Batch<L>::value, MAT><<<nblk((int64_t)nv * L, block), block, 0, st>>>(a, body_new, bs, dxm,
+    k_sweep_bodies<L, Batch<L>::value, MAT><<<nblk((int64_t)nv * L, block), block, 0, st>>>(a, body_new, bs, bmet,
                                                                                             v0, nv);
 }
 
 template <int MAT>
-void dispatch_sweep_bodies(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm,
+void dispatch_sweep_bodies(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
                            int v0, int nv, int block, hipStream_t st)
 {
     switch (L) {
-    case 1: launch_sweep_bodies<1, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
-    case 2: launch_sweep_bodies<2, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
-    case 8: launch_sweep_bodies<8, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
-    case 16: launch_sweep_bodies<16, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
-    case 32: launch_sweep_bodies<32, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
-    default: launch_sweep_bodies<4, MAT>(a, body_new, bs, dxm, v0, nv, block, st); break;
+    case 1: launch_sweep_bodies<1, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
+    case 2: launch_sweep_bodies<2, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
+    case 8: launch_sweep_bodies<8, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
+    case 16: launch_sweep_bodies<16, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
+    case 32: launch_sweep_bodies<32, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
+    default: launch_sweep_bodies<4, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
     }
 }
 
 template <int L, int MAT>
-void launch_sweep_tasks(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm,
+void launch_sweep_tasks(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
                         const int2 *tasks, int ntasks, int block, hipStream_t st)
 {
-    k_sweep_tasks<L, Batch<L>::value, MAT><<<ntasks, block, 0, st>>>(a, body_new, bs, dxm, tasks);
+    k_sweep_tasks<L, Batch<L>::value, MAT><<<ntasks, block, 0, st>>>(a, body_new, bs, bmet, tasks);
 }
 
 template <int MAT>
-void dispatch_sweep_tasks(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *dxm,
+void dispatch_sweep_tasks(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
                           const int2 *tasks, int ntasks, int block, hipStream_t st)
 {
     switch (L) {
-    case 1: launch_sweep_tasks<1, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
-    case 2: launch_sweep_tasks<2, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
-    case 8: launch_sweep_tasks<8, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
-    case 16: launch_sweep_tasks<16, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
-    case 32: launch_sweep_tasks<32, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
-    default: launch_sweep_tasks<4, MAT>(a, body_new, bs, dxm, tasks, ntasks, block, st); break;
+    case 1: launch_sweep_tasks<1, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
+    case 2: launch_sweep_tasks<2, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
+    case 8: launch_sweep_tasks<8, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
+    case 16: launch_sweep_tasks<16, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
+    case 32: launch_sweep_tasks<32, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
+    default: launch_sweep_tasks<4, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
     }
+}
+
+// The staged kernel for colour c when the solver has staging tables for it.
+bool staged_color(const vbd_solver *s, int c) { return s->st_ok && c < (int)s->st_blk0.size() && s->st_blk0[c] >= 0; }
+
+void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *blist, int nblocks)
+{
+    const int v0 = s->col_start[c], nv = s->col_size[c];
+    const dim3 grid(blist ? nblocks : nblk(nv, kStageRows));
+    if (grid.x == 0) return;
+    if (s->material == kSNH)
+        k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
+                                                                s->n_dict, s->body_new, s->bstate, s->dxm, blist);
+    else
+        k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
+                                                                 s->n_dict, s->body_new, s->bstate, s->dxm, blist);
 }
 
 SweepArgs sweep_args(vbd_solver *s, float kd_dt)
@@ -192,7 +208,9 @@ int ensure_batch_buffers(vbd_solver *s)
 {
     if (s->bstate) return 0;
     VBD_HIP(hipMalloc((void **)&s->bstate, sizeof(BodyState) * s->n_bodies));
-    VBD_HIP(hipMalloc((void **)&s->dxm, sizeof(uint32_t) * std::max(1, s->n)));
+    VBD_HIP(hipMalloc((void **)&s->dxm, sizeof(uint32_t) * s->n_bodies));
+    VBD_HIP(hipMemset(s->dxm, 0, sizeof(uint32_t) * s->n_bodies));
+
     VBD_HIP(hipMalloc((void **)&s->n_live, sizeof(int32_t)));
     VBD_HIP(hipHostMalloc((void **)&s->n_live_host, sizeof(int32_t), hipHostMallocDefault));
     VBD_HIP(hipHostMalloc((void **)&s->bstate_host, sizeof(BodyState) * s->n_bodies, hipHostMallocDefault));
@@ -207,13 +225,15 @@ int enqueue_body_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
     for (int c = 0; c < s->ncolors; ++c) {
         const int nv = s->col_size[c];
         if (!nv) continue;
-        if (s->material == kSNH)
+        if (staged_color(s, c))
+            launch_staged_bodies(s, a, c, nullptr, 0);
+        else if (s->material == kSNH)
             dispatch_sweep_bodies<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
         else
             dispatch_sweep_bodies<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
     }
-    k_check_bodies<<<s->n_bodies, 256, 0, s->stream>>>(s->d_ranges, s->ncolors, s->dxm, s->bstate, max_iter, tol,
-                                                        s->n_live);
+    k_check_bodies<<<nblk(s->n_bodies, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
+                                                                   s->n_live);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -230,6 +250,23 @@ int build_tasks(vbd_solver *s)
     s->task_vertices = 0;
     for (int c = 0; c < C; ++c) {
         s->task_off[c] = (int)s->tasks_host.size();
+        if (staged_color(s, c)) {
+            // the colour's staging blocks that hold a live body's vertex, as
+            // (block index, 0) entries
+            const int c0 = s->col_start[c], nblocks = nblk(s->col_size[c], kStageRows);
+            int last = -1;
+            for (int b = 0; b < nb; ++b) {
+                if (s->bstate_host[b].done) continue;
+                const int2 r = s->ranges_host[(size_t)b * (C + 1) + c];
+                if (r.y <= 0) continue;
+                const int j0 = std::max(last + 1, (r.x - c0) / kStageRows);
+                const int j1 = std::min(nblocks - 1, (r.x + r.y - 1 - c0) / kStageRows);
+                for (int j = j0; j <= j1; ++j) s->tasks_host.push_back(make_int2(j, 0));
+                last = std::max(last, j1);
+                s->task_vertices += r.y;
+            }
+            continue;
+        }
         int run0 = -1, run1 = -1;
         auto flush = [&]() {
             for (int v = run0; v < run1; v += vpb) s->tasks_host.push_back(make_int2(v, std::min(vpb, run1 - v)));
@@ -250,8 +287,14 @@ int build_tasks(vbd_solver *s)
         if (run1 > run0) flush();
     }
     s->task_off[C] = (int)s->tasks_host.size();
-    const size_t nt = std::max<size_t>(1, s->tasks_host.size());
+    // sized once for any list (a reallocation in the middle of a run stalls
+    // it: hipFree waits for the device): a colour's list has at most one
+    // entry per vpb of its vertices plus one per body, or one per staging block
+    size_t nt = std::max<size_t>(1, s->tasks_host.size());
     if (nt > s->task_cap) {
+        size_t cap = 16;
+        for (int c = 0; c < C; ++c) cap += (size_t)s->col_size[c] / vpb + nb + 1;
+        nt = std::max(nt, cap);
         if (s->d_tasks) VBD_HIP(hipFree(s->d_tasks));
         s->d_tasks = nullptr;
         VBD_HIP(hipMalloc((void **)&s->d_tasks, sizeof(int2) * nt));
@@ -271,14 +314,16 @@ int enqueue_task_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
     for (int c = 0; c < s->ncolors; ++c) {
         const int t0 = s->task_off[c], nt = s->task_off[c + 1] - t0;
         if (!nt) continue;
-        if (s->material == kSNH)
+        if (staged_color(s, c))
+            launch_staged_bodies(s, a, c, s->d_tasks + t0, nt);
+        else if (s->material == kSNH)
             dispatch_sweep_tasks<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->d_tasks + t0, nt, block, s->stream);
         else
             dispatch_sweep_tasks<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->d_tasks + t0, nt, block,
                                         s->stream);
     }
-    k_check_bodies<<<s->n_bodies, 256, 0, s->stream>>>(s->d_ranges, s->ncolors, s->dxm, s->bstate, max_iter, tol,
-                                                        s->n_live);
+    k_check_bodies<<<nblk(s->n_bodies, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
+                                                                   s->n_live);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -333,7 +378,7 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
     const float ftol = (float)tol;
     k_init_bodies<<<nblk(std::max(s->n, s->n_bodies), 256), 256, 0, s->stream>>>(
         s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt, s->n_bodies,
-        s->bstate, s->n_live);
+        s->bstate, s->dxm, s->n_live);
     VBD_HIP(hipGetLastError());
     if (s->warm) VBD_TRY(mpnn_apply_guess(s->warm, dt));
     // First chunk: every body, replayed from a graph; its length is the
@@ -342,6 +387,7 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
     // chunks launch only the live bodies' ranges (lists rebuilt whenever the
     // live vertices fall below half of what the current list covers).
     const bool compact = s->launch.compact && s->n_bodies > 1;
+    const int prev_max = s->last_iters;
     int launched = 0;
     int chunk = std::max(1, compact ? s->median_iters : s->last_iters);
     bool tasks = false;
@@ -373,7 +419,9 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
                 tasks = true;
             }
         }
-        chunk = s->launch.chunk > 0 ? s->launch.chunk : 2;
+        // the rest of the step in one more chunk if it needs as many
+        // iterations as the last step's slowest body did
+        chunk = s->launch.chunk > 0 ? s->launch.chunk : tasks ? std::max(2, prev_max - launched) : 2;
     }
     k_finish_all<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->n, s->u, s->uprev, s->vel, (float)dt);
     VBD_HIP(hipGetLastError());
@@ -410,7 +458,7 @@ int step_bodies_resident(vbd_solver *s, double dt, double tol, int max_iter)
         VBD_TRY(ensure_batch_buffers(s));
         k_init_bodies<<<nblk(std::max(s->n, s->n_bodies), 256), 256, 0, s->stream>>>(
             s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt,
-            s->n_bodies, s->bstate, s->n_live);
+            s->n_bodies, s->bstate, s->dxm, s->n_live);
         VBD_HIP(hipGetLastError());
         VBD_TRY(mpnn_apply_guess(s->warm, dt));
         p.skip_init = 1;
@@ -453,6 +501,7 @@ void release_batch(vbd_solver *s)
     s->task_cap = 0;
     if (s->bstate) (void)hipFree(s->bstate);
     if (s->dxm) (void)hipFree(s->dxm);
+
     if (s->n_live) (void)hipFree(s->n_live);
     if (s->n_live_host) (void)hipHostFree(s->n_live_host);
     if (s->bstate_host) (void)hipHostFree(s->bstate_host);

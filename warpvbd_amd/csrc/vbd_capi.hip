@@ -151,6 +151,7 @@ int vbd_batch_create(const double *rest_xyz, int32_t n_vertices, const int32_t *
         return VBD_EHIP;
     }
     int r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active, body_vertex_offsets, n_bodies, gravity);
+    if (r == 0) r = build_staging(s);
     if (r != 0) {
         vbd_destroy(s);
         return r;
@@ -594,10 +595,12 @@ int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
         // packed position 16 + FP64 target 32 + slot offset 4 + write-back 16
         bytes[c] = 68.0 * nv + per_slot * s->col_slots[c] + 16.0 * s->col_boundary[c];
         // staged sweep (4 lanes, 256 threads, the default launch): 8-byte
-        // slot records, plus each block's list of neighbour ids
+        // slot records, plus each block's list of kStageMax neighbour ids
         if (s->st_ok && c < (int)s->st_blk0.size() && s->st_blk0[c] >= 0 && s->launch.lanes == 4 &&
-            s->launch.block == 256)
-            bytes[c] = 68.0 * nv + 8.0 * s->col_slots[c] + 16.0 * s->col_boundary[c] + 4.0 * (double)s->st_ids[c];
+            s->launch.block == 256) {
+            const double nblocks = (double)((s->col_size[c] + kStageRows - 1) / kStageRows);
+            bytes[c] = 68.0 * nv + 8.0 * s->col_slots[c] + 16.0 * s->col_boundary[c] + 4.0 * kStageMax * nblocks;
+        }
     }
     return 0;
 }

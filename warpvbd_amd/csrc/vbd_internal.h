@@ -129,7 +129,7 @@ struct vbd_solver {
     bool batched = false;            // built by vbd_batch_create
     int batch_mode = vbd::kBatchLaunches;
     vbd::BodyState *bstate = nullptr, *bstate_host = nullptr;
-    uint32_t *dxm = nullptr;         // per-vertex metric bits of the current iteration
+    uint32_t *dxm = nullptr;         // per body: metric bits of the current iteration (max over its vertices)
     int32_t *n_live = nullptr, *n_live_host = nullptr;
     // live-range tasks of the batched colour launches: per colour, blocks of
     // (first vertex, count) covering the bodies live when the list was built

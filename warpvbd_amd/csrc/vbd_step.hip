@@ -131,101 +131,20 @@ __global__ __launch_bounds__(1024) void k_sweep(
     finish_sweep(a, c, mb, close, part_all, np_all, tol);
 }
 
-// k_sweep<4, 1, MAT> with neighbour staging (256 threads, kStageRows
-// vertices per block).  Two memory rounds, then arithmetic only:
-//   1. the vertex's own data, the block's neighbour ids (a list of kStageMax
-//      entries per block, -1 past its end) and the record dictionary;
-//   2. the neighbours' positions and every 8-byte slot record of the lane
-//      (l0 | l1 << 16, l2 | entry << 16: local indices into the block's
-//      neighbour list and the dictionary entry; at most 32 / 4 per lane).
-// The neighbours are unpacked into LDS (FP32 parts and remainders) with the
-// dictionary; the slot loop then reads LDS only.  Neighbours belong to other
-// colours, which this launch does not write, so staging them at the start is
-// exact, and the arithmetic is k_sweep's, so the results are the same bit for
-// bit.
-constexpr int kStagedSlots = kMaxElementsPerVertex / 4;
-
+// k_sweep<4, 1, MAT> with neighbour staging (staged_sweep in vbd_sweep.h):
+// 256 threads per block of kStageRows vertices; the block's metric maximum
+// goes to a.part, and the iteration's last launch folds them all.
 template <int MAT>
 __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int nv, int close,
                                                       const uint32_t *part_all, int np_all, float tol,
                                                       const int32_t *st_nb, const uint2 *st_rec, int blk0, int n_dict)
 {
-    constexpr int L = 4;
-    constexpr int kStageUnroll = kStageMax / 256;
-    static_assert(kStageMax % 256 == 0, "each of the 256 threads stages kStageUnroll neighbours");
-    static_assert(3 * kStageDict <= 256, "one dictionary row per thread");
-    __shared__ float4 s_na[kStageMax];
-    __shared__ float2 s_nb[kStageMax];
-    __shared__ float4 s_dict[3 * kStageDict];
-    Ctrl *c = a.ctrl;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int t = b * blockDim.x + threadIdx.x;
-    const int vi = t / L, lane = t % L;
-    const bool valid = vi < nv;
-    const int v = v0 + (valid ? vi : 0);
-    // round 1
-    const VertexPre pre = vertex_pre(a, v, valid);
-    const int32_t *ids = st_nb + (size_t)(blk0 + b) * kStageMax;
-    int id[kStageUnroll];
-#pragma unroll
-    for (int q = 0; q < kStageUnroll; ++q) id[q] = ids[threadIdx.x + 256 * q];
-    const bool has_dict = (int)threadIdx.x < 3 * n_dict;
-    const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
-    // the flag was written by an earlier launch: the same for every thread
-    if (c->converged) return;
-    // round 2
-    float4 p[kStageUnroll];
-#pragma unroll
-    for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
-    // (st_rec holds kStagedSlots * L zero records past the last slot)
-    uint2 rec[kStagedSlots];
-    const uint2 *rl = st_rec + pre.s0 + lane;
-#pragma unroll
-    for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[L * k];
-    if (has_dict) s_dict[threadIdx.x] = drow;
-#pragma unroll
-    for (int q = 0; q < kStageUnroll; ++q) {
-        if (id[q] >= 0) {
-            const int k = threadIdx.x + 256 * q;
-            const float3 lo = disp_lo(p[q]);
-            s_na[k] = make_float4(p[q].x, p[q].y, p[q].z, lo.x);
-            s_nb[k] = make_float2(lo.y, lo.z);
-        }
-    }
-    __syncthreads();
-    Acc acc;
-    acc_zero(acc);
-    VertexLocal ql = vertex_local(pre);
-    const float3 hi = ql.hi, lo = ql.lo;
-    // this lane's slots: pre.s0 + lane, + L, ... below pre.s1
-    const int nk = pre.s1 > pre.s0 + lane ? (pre.s1 - pre.s0 - lane + L - 1) / L : 0;
-#pragma unroll
-    for (int k = 0; k < kStagedSlots; ++k) {
-        if (k < nk) {
-            const uint2 r = rec[k];
-            const uint32_t e = r.y >> 16, l0 = r.x & 0xffffu, l1 = r.x >> 16, l2 = r.y & 0xffffu;
-            const float4 d0 = s_dict[3 * e], d1 = s_dict[3 * e + 1], d2 = s_dict[3 * e + 2];
-            const float4 p0 = s_na[l0], p1 = s_na[l1], p2 = s_na[l2];
-            const float2 q0 = s_nb[l0], q1 = s_nb[l1], q2 = s_nb[l2];
-            // the differences slot_rel forms, from the unpacked parts
-            const float3 m0 = make_float3(p0.w, q0.x, q0.y), m1 = make_float3(p1.w, q1.x, q1.y),
-                         m2 = make_float3(p2.w, q2.x, q2.y);
-            const float3 e0 = diff_packed(p0, m0, p2, m2), e1 = diff_packed(p1, m1, p2, m2);
-            const float3 f2 = make_float3((p2.x - hi.x) + (m2.x - lo.x), (p2.y - hi.y) + (m2.y - lo.y),
-                                          (p2.z - hi.z) + (m2.z - lo.z));
-            slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), e0, e1, f2, acc);
-        }
-    }
-    double3 us;
-    const float3 dx = vertex_finish<L>(a, v, valid, acc, ql, us);
-    uint32_t m = 0u;
-    if (valid) {
-        if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
-        m = metric_bits(dx);
-    }
+    uint32_t m;
+    int body;
+    if (!staged_sweep<MAT, false>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, nullptr, nullptr, body, m)) return;
     const uint32_t mb = block_max_u32(m);
-    finish_sweep(a, c, mb, close, part_all, np_all, tol);
+    finish_sweep(a, a.ctrl, mb, close, part_all, np_all, tol);
 }
 
 // A colour below small_color vertices: 32 lanes per vertex, one slot per

@@ -395,6 +395,138 @@ __device__ __forceinline__ float3 eval_vertex(const SweepArgs &a, int v, bool va
     return vertex_dx<L, B, MAT>(a, v, valid, lane, us);
 }
 
+// Slot records a lane of the staged sweep holds: 32 incident tets / 4 lanes.
+constexpr int kStagedSlots = kMaxElementsPerVertex / 4;
+
+// One block of the staged colour sweep: 256 threads, 4 lanes per vertex, the
+// vertices [v0 + 64 b, v0 + 64 b + 64) of a colour of nv.  Two memory rounds,
+// then arithmetic only:
+//   1. the vertex's own data, the block's neighbour ids (st_nb: kStageMax per
+//      block, -1 past its list) and the record dictionary;
+//   2. the neighbours' positions and every 8-byte slot record of the lane
+//      (st_rec: (l0 | l1 << 16, l2 | entry << 16), local indices into the
+//      block's list and the dictionary entry, with kStagedSlots * 4 zero
+//      records past the last slot, so a lane reads its 8 without a bound).
+// The neighbours are unpacked into LDS (FP32 parts and remainders) with the
+// dictionary, and the slot loop reads LDS only.  Neighbours belong to other
+// colours, which the launch does not write, so staging them first is exact;
+// the arithmetic is k_sweep's, so the results are the same bit for bit.
+// BODIES: a batched solver's launch -- a vertex of a body that has converged
+// is not updated (its lanes get body -1), instead of the step's convergence
+// flag being tested.  Returns false when the (single-body) step has converged
+// already (the whole block then does nothing); m receives the lane's metric
+// bits (0 for lanes without an updated vertex), body the vertex's body.
+template <int MAT, bool BODIES>
+__device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, int b, const int32_t *st_nb,
+                                             const uint2 *st_rec, int blk0, int n_dict, const int32_t *body_new,
+                                             const BodyState *bst, int &body_out, uint32_t &m)
+{
+    constexpr int L = 4;
+    constexpr int kStageUnroll = kStageMax / 256;
+    static_assert(kStageMax % 256 == 0, "each of the 256 threads stages kStageUnroll neighbours");
+    static_assert(3 * kStageDict <= 256, "one dictionary row per thread");
+    __shared__ float4 s_na[kStageMax];
+    __shared__ float2 s_nb[kStageMax];
+    __shared__ float4 s_dict[3 * kStageDict];
+    const int t = b * 256 + threadIdx.x;
+    const int vi = t / L, lane = t % L;
+    const bool in = vi < nv;
+    const int v = v0 + (in ? vi : 0);
+    // round 1
+    const VertexPre pre = vertex_pre(a, v, in);
+    const int32_t *ids = st_nb + (size_t)(blk0 + b) * kStageMax;
+    int id[kStageUnroll];
+#pragma unroll
+    for (int q = 0; q < kStageUnroll; ++q) id[q] = ids[threadIdx.x + 256 * q];
+    const bool has_dict = (int)threadIdx.x < 3 * n_dict;
+    const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    int body = 0;
+    if constexpr (BODIES) {
+        body = body_new[v];
+    } else {
+        // the flag was written by an earlier launch: the same for every thread
+        if (a.ctrl->converged) {
+            m = 0u;
+            return false;
+        }
+    }
+    // round 2
+    bool valid = in;
+    if constexpr (BODIES) valid = in && !bst[body].done;
+    float4 p[kStageUnroll];
+#pragma unroll
+    for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint2 rec[kStagedSlots];
+    const uint2 *rl = st_rec + pre.s0 + lane;
+#pragma unroll
+    for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[L * k];
+    if (has_dict) s_dict[threadIdx.x] = drow;
+#pragma unroll
+    for (int q = 0; q < kStageUnroll; ++q) {
+        if (id[q] >= 0) {
+            const int k = threadIdx.x + 256 * q;
+            const float3 lo = disp_lo(p[q]);
+            s_na[k] = make_float4(p[q].x, p[q].y, p[q].z, lo.x);
+            s_nb[k] = make_float2(lo.y, lo.z);
+        }
+    }
+    __syncthreads();
+    Acc acc;
+    acc_zero(acc);
+    VertexLocal ql = vertex_local(pre);
+    const float3 hi = ql.hi, lo = ql.lo;
+    // this lane's slots: pre.s0 + lane, + L, ... below pre.s1
+    const int nk = valid && pre.s1 > pre.s0 + lane ? (pre.s1 - pre.s0 - lane + L - 1) / L : 0;
+#pragma unroll
+    for (int k = 0; k < kStagedSlots; ++k) {
+        if (k < nk) {
+            const uint2 r = rec[k];
+            const uint32_t e = r.y >> 16, l0 = r.x & 0xffffu, l1 = r.x >> 16, l2 = r.y & 0xffffu;
+            const float4 d0 = s_dict[3 * e], d1 = s_dict[3 * e + 1], d2 = s_dict[3 * e + 2];
+            const float4 p0 = s_na[l0], p1 = s_na[l1], p2 = s_na[l2];
+            const float2 q0 = s_nb[l0], q1 = s_nb[l1], q2 = s_nb[l2];
+            // the differences slot_rel forms, from the unpacked parts
+            const float3 m0 = make_float3(p0.w, q0.x, q0.y), m1 = make_float3(p1.w, q1.x, q1.y),
+                         m2 = make_float3(p2.w, q2.x, q2.y);
+            const float3 e0 = diff_packed(p0, m0, p2, m2), e1 = diff_packed(p1, m1, p2, m2);
+            const float3 f2 = make_float3((p2.x - hi.x) + (m2.x - lo.x), (p2.y - hi.y) + (m2.y - lo.y),
+                                          (p2.z - hi.z) + (m2.z - lo.z));
+            slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
+                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), e0, e1, f2, acc);
+        }
+    }
+    double3 us;
+    const float3 dx = vertex_finish<L>(a, v, valid, acc, ql, us);
+    m = 0u;
+    body_out = valid ? body : -1;
+    if (valid) {
+        if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
+        m = metric_bits(dx);
+    }
+    return true;
+}
+
+// A batched launch's metrics into the per-body maxima bmet (bits of
+// max |dx_x + dx_y + dx_z| of the iteration so far): per wave, one
+// device-scope atomic max per body among its lanes (a wave's vertices are
+// consecutive in colour-major, body-minor order, so mostly one body).  Lanes
+// with body < 0 take no part.
+__device__ __forceinline__ void body_metric_max(int body, uint32_t m, uint32_t *bmet)
+{
+    const int lid = (int)(threadIdx.x & 63);
+    bool pending = body >= 0;
+    while (true) {
+        const uint64_t mask = __ballot(pending);
+        if (!mask) break;
+        const int lead = __builtin_ctzll(mask);
+        const int lb = __builtin_amdgcn_readlane(body, lead);
+        const bool mine = pending && body == lb;
+        const uint32_t x = wave_max_u32(mine ? m : 0u);
+        if (lid == lead) atomicMax(&bmet[lb], x);
+        pending = pending && !mine;
+    }
+}
+
 // slots per lane per load batch, by lanes per vertex
 template <int L>
 struct Batch {
