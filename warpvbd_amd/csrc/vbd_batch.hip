@@ -8,9 +8,12 @@
 // out of the remaining sweeps of the step.  Two execution forms:
 //
 //   colour launches  one launch per colour across every body, then a check
-//                    launch (one workgroup per body); the iterations are
-//                    captured in a hipGraph and replayed in chunks, the host
-//                    waiting once per chunk on a device count of live bodies
+//                    launch (one wave per body); the iterations are captured
+//                    in a hipGraph and replayed in chunks, the host waiting
+//                    once per chunk on a device count of live bodies.  Rows
+//                    of kStageRows vertices of a colour whose bodies have all
+//                    converged are skipped by the launches (a live count per
+//                    row, kept by the check launch).
 //   resident         one workgroup per body runs its whole step
 //                    (k_step_resident), one launch per step
 //
@@ -30,9 +33,18 @@ namespace vbd {
 
 namespace {
 
+#ifndef VBD_FULL_CHUNK_MARGIN
+#define VBD_FULL_CHUNK_MARGIN 1
+#endif
+// Iterations the first chunk of an uncompacted step adds to the last step's
+// largest body count: an iteration after every body has converged costs only
+// launches whose blocks leave at once, a chunk too short costs a wait on the
+// host and another submission.
+constexpr int kFullChunkMargin = VBD_FULL_CHUNK_MARGIN;
+
 template <int L, int B, int MAT>
 __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_t *body_new, const BodyState *bs,
-                                                       uint32_t *bmet, int v0, int nv)
+                                                       uint32_t *bmet, const int32_t *rows, int v0, int nv)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -42,7 +54,16 @@ __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_
     // the vertex's own loads go out with the body lookup, not after it
     const VertexPre pre = vertex_pre(a, v, in);
     const int body = body_new[v];
-    const bool valid = in && !bs[body].done;
+    // A wave's 64 / L vertices lie in one row of kStageRows (rows start at
+    // multiples of kStageRows in the colour), so its live count is a scalar,
+    // and a wave of a row without live bodies leaves here.  The loads above
+    // stay ahead of the test (sunk past it, they would wait for the count).
+    const int live = rows[__builtin_amdgcn_readfirstlane(min(vi, nv - 1) / kStageRows)];
+    asm volatile("" ::"v"(body), "s"(live) : "memory");
+    if (live == 0) return;
+    // read by every lane (v is a swept vertex for all), so no branch around the load
+    const int done = bs[body].done;
+    const bool valid = in && !done;
     const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane, pre);
     body_metric_max(valid ? body : -1, m, bmet);
 }
@@ -75,45 +96,69 @@ template <int MAT>
 __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0, int nv, const int32_t *st_nb,
                                                              const uint2 *st_rec, int blk0, int n_dict,
                                                              const int32_t *body_new, const BodyState *bs,
-                                                             uint32_t *bmet, const int2 *blist)
+                                                             const int32_t *rows, uint32_t *bmet, const int2 *blist)
 {
     const int b = blist ? blist[blockIdx.x].x : xcd_remap(blockIdx.x, gridDim.x);
     uint32_t m;
     int body;
-    (void)staged_sweep<MAT, true>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, body_new, bs, body, m);
-    body_metric_max(body, m, bmet);
+    // The block's maxima of its first kFoldBodies bodies gather in LDS and go
+    // out as one device atomic each at the end: a body's vertices fill
+    // consecutive blocks that finish together, and one atomic per wave on
+    // the same word costs the launch about 4% (cleared here, before
+    // staged_sweep's barrier).
+    constexpr int kFoldBodies = 8;
+    __shared__ uint32_t s_bm[kFoldBodies];
+    if (threadIdx.x < kFoldBodies) s_bm[threadIdx.x] = 0u;
+    const int b0 = body_new[v0 + b * kStageRows];
+    // (the whole block returns together)
+    if (!staged_sweep<MAT, true>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, body_new, bs, rows, body, m)) return;
+    body_metric_max(body, m, bmet, s_bm, b0, kFoldBodies);
+    __syncthreads();
+    if (threadIdx.x < kFoldBodies && s_bm[threadIdx.x]) atomicMax(&bmet[b0 + threadIdx.x], s_bm[threadIdx.x]);
 }
 
-// After an iteration's colour launches: per body (one thread each) the
+// After an iteration's colour launches: per body (one wave each) the
 // iteration's max |dx_x + dx_y + dx_z| (bmet, folded by the launches with
 // body_metric_max), its iteration count, and the body retires when it
-// converged or used its iterations.  bmet is cleared for the next iteration.
-__global__ void k_check_bodies(int nb, uint32_t *bmet, BodyState *bs, int max_iter, float tol, int32_t *n_live)
+// converged or used its iterations; a body that retires leaves the live
+// counts of its rows (the wave's lanes share them out).  bmet is cleared for
+// the next iteration.
+__global__ __launch_bounds__(256) void k_check_bodies(int nb, uint32_t *bmet, BodyState *bs, int max_iter, float tol,
+                                                      int32_t *n_live, const int32_t *body_row_off,
+                                                      const int32_t *body_rows, int32_t *row_live)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
     if (b >= nb) return;
     BodyState *st = bs + b;
+    // every lane reads the same state; lane 0 writes it back
     if (st->done) return;
     const uint32_t mx = bmet[b];
-    bmet[b] = 0u;
     const int it = st->iters + 1;
-    st->iters = it;
-    st->metric = mx;
-    if (__uint_as_float(mx) < tol || it >= max_iter) {
-        st->done = 1;
-        atomicSub(n_live, 1);
+    const bool retire = __uint_as_float(mx) < tol || it >= max_iter;
+    if (lane == 0) {
+        bmet[b] = 0u;
+        st->iters = it;
+        st->metric = mx;
+        if (retire) {
+            st->done = 1;
+            atomicSub(n_live, 1);
+        }
     }
+    if (retire)
+        for (int i = body_row_off[b] + lane; i < body_row_off[b + 1]; i += 64) atomicSub(&row_live[body_rows[i]], 1);
 }
 
 __global__ void k_init_bodies(int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
                               const uint8_t *active, const int32_t *body_new, const float4 *grav, float dt, int nb,
-                              BodyState *bs, uint32_t *bmet, int32_t *n_live)
+                              BodyState *bs, uint32_t *bmet, int32_t *n_live, int nrows, const int32_t *row_live0,
+                              int32_t *row_live)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nb) {
         bs[i] = BodyState{0, 0, 0u, 0};
         bmet[i] = 0u;
     }
+    if (i < nrows) row_live[i] = row_live0[i];
     if (i == 0) *n_live = nb;
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body_new[i]]);
 }
@@ -128,24 +173,24 @@ __global__ void k_finish_all(int n, const float4 *u, double4 *uprev, float4 *vel
 }
 
 template <int L, int MAT>
-void launch_sweep_bodies(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet, int v0,
-                         int nv, int block, hipStream_t st)
+void launch_sweep_bodies(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
+                         const int32_t *rows, int v0, int nv, int block, hipStream_t st)
 {
     k_sweep_bodies<L, Batch<L>::value, MAT><<<nblk((int64_t)nv * L, block), block, 0, st>>>(a, body_new, bs, bmet,
-                                                                                            v0, nv);
+                                                                                            rows, v0, nv);
 }
 
 template <int MAT>
 void dispatch_sweep_bodies(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
-                           int v0, int nv, int block, hipStream_t st)
+                           const int32_t *rows, int v0, int nv, int block, hipStream_t st)
 {
     switch (L) {
-    case 1: launch_sweep_bodies<1, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
-    case 2: launch_sweep_bodies<2, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
-    case 8: launch_sweep_bodies<8, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
-    case 16: launch_sweep_bodies<16, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
-    case 32: launch_sweep_bodies<32, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
-    default: launch_sweep_bodies<4, MAT>(a, body_new, bs, bmet, v0, nv, block, st); break;
+    case 1: launch_sweep_bodies<1, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
+    case 2: launch_sweep_bodies<2, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
+    case 8: launch_sweep_bodies<8, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
+    case 16: launch_sweep_bodies<16, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
+    case 32: launch_sweep_bodies<32, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
+    default: launch_sweep_bodies<4, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
     }
 }
 
@@ -180,10 +225,12 @@ void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *
     if (grid.x == 0) return;
     if (s->material == kSNH)
         k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
-                                                                s->n_dict, s->body_new, s->bstate, s->dxm, blist);
+                                                                s->n_dict, s->body_new, s->bstate,
+                                                                s->row_live + s->row_base[c], s->dxm, blist);
     else
         k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
-                                                                 s->n_dict, s->body_new, s->bstate, s->dxm, blist);
+                                                                 s->n_dict, s->body_new, s->bstate,
+                                                                 s->row_live + s->row_base[c], s->dxm, blist);
 }
 
 SweepArgs sweep_args(vbd_solver *s, float kd_dt)
@@ -210,6 +257,34 @@ int ensure_batch_buffers(vbd_solver *s)
     VBD_HIP(hipMalloc((void **)&s->bstate, sizeof(BodyState) * s->n_bodies));
     VBD_HIP(hipMalloc((void **)&s->dxm, sizeof(uint32_t) * s->n_bodies));
     VBD_HIP(hipMemset(s->dxm, 0, sizeof(uint32_t) * s->n_bodies));
+    // rows of kStageRows vertices per colour, and per body the rows that hold
+    // its vertices (a body's colour range is contiguous)
+    const int C = s->ncolors, nb = s->n_bodies;
+    s->row_base.assign(C + 1, 0);
+    for (int c = 0; c < C; ++c) s->row_base[c + 1] = s->row_base[c] + nblk(s->col_size[c], kStageRows);
+    s->n_rows = s->row_base[C];
+    std::vector<int32_t> live0(std::max(s->n_rows, 1), 0), off(nb + 1, 0), rows;
+    for (int b = 0; b < nb; ++b) {
+        for (int c = 0; c < C; ++c) {
+            const int2 r = s->ranges_host[(size_t)b * (C + 1) + c];
+            if (r.y <= 0) continue;
+            const int j0 = (r.x - s->col_start[c]) / kStageRows, j1 = (r.x + r.y - 1 - s->col_start[c]) / kStageRows;
+            for (int j = j0; j <= j1; ++j) {
+                rows.push_back(s->row_base[c] + j);
+                ++live0[s->row_base[c] + j];
+            }
+        }
+        off[b + 1] = (int32_t)rows.size();
+    }
+    rows.resize(std::max<size_t>(rows.size(), 1), 0);
+    VBD_HIP(hipMalloc((void **)&s->row_live, sizeof(int32_t) * live0.size()));
+    VBD_HIP(hipMalloc((void **)&s->row_live0, sizeof(int32_t) * live0.size()));
+    VBD_HIP(hipMalloc((void **)&s->body_row_off, sizeof(int32_t) * off.size()));
+    VBD_HIP(hipMalloc((void **)&s->body_rows, sizeof(int32_t) * rows.size()));
+    VBD_HIP(hipMemcpy(s->row_live0, live0.data(), sizeof(int32_t) * live0.size(), hipMemcpyHostToDevice));
+    VBD_HIP(hipMemcpy(s->row_live, live0.data(), sizeof(int32_t) * live0.size(), hipMemcpyHostToDevice));
+    VBD_HIP(hipMemcpy(s->body_row_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
+    VBD_HIP(hipMemcpy(s->body_rows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice));
 
     VBD_HIP(hipMalloc((void **)&s->n_live, sizeof(int32_t)));
     VBD_HIP(hipHostMalloc((void **)&s->n_live_host, sizeof(int32_t), hipHostMallocDefault));
@@ -228,12 +303,15 @@ int enqueue_body_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
         if (staged_color(s, c))
             launch_staged_bodies(s, a, c, nullptr, 0);
         else if (s->material == kSNH)
-            dispatch_sweep_bodies<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
+            dispatch_sweep_bodies<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->row_live + s->row_base[c],
+                                        s->col_start[c], nv, block, s->stream);
         else
-            dispatch_sweep_bodies<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->col_start[c], nv, block, s->stream);
+            dispatch_sweep_bodies<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->row_live + s->row_base[c],
+                                         s->col_start[c], nv, block, s->stream);
     }
-    k_check_bodies<<<nblk(s->n_bodies, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
-                                                                   s->n_live);
+    k_check_bodies<<<nblk((int64_t)s->n_bodies * 64, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
+                                                                   s->n_live, s->body_row_off, s->body_rows,
+                                                                   s->row_live);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -322,8 +400,9 @@ int enqueue_task_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
             dispatch_sweep_tasks<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->d_tasks + t0, nt, block,
                                         s->stream);
     }
-    k_check_bodies<<<nblk(s->n_bodies, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
-                                                                   s->n_live);
+    k_check_bodies<<<nblk((int64_t)s->n_bodies * 64, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
+                                                                   s->n_live, s->body_row_off, s->body_rows,
+                                                                   s->row_live);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -343,10 +422,11 @@ int get_graph(vbd_solver *s, int k, float kd_dt, float tol, int max_iter, hipGra
     for (auto &g : s->graphs)
         if (g.k == k && g.tol == tol && g.max_iter == max_iter && g.kd_dt == kd_dt && g.lanes == s->launch.lanes &&
             g.block == s->launch.block) {
+            graph_touch(s, g);
             *out = g.exec;
             return 0;
         }
-    if (s->graphs.size() > 16) drop_graphs(s);
+    graph_make_room(s);
     CachedGraph g;
     g.k = k;
     g.tol = tol;
@@ -366,6 +446,7 @@ int get_graph(vbd_solver *s, int k, float kd_dt, float tol, int max_iter, hipGra
     }
     g.graph = graph;
     VBD_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    graph_touch(s, g);
     s->graphs.push_back(g);
     *out = g.exec;
     return 0;
@@ -376,20 +457,23 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
     VBD_TRY(ensure_batch_buffers(s));
     const float kd_dt = (float)(s->damping / dt);
     const float ftol = (float)tol;
-    k_init_bodies<<<nblk(std::max(s->n, s->n_bodies), 256), 256, 0, s->stream>>>(
+    k_init_bodies<<<nblk(std::max({s->n, s->n_bodies, s->n_rows}), 256), 256, 0, s->stream>>>(
         s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt, s->n_bodies,
-        s->bstate, s->dxm, s->n_live);
+        s->bstate, s->dxm, s->n_live, s->n_rows, s->row_live0, s->row_live);
     VBD_HIP(hipGetLastError());
     if (s->warm) VBD_TRY(mpnn_apply_guess(s->warm, dt));
-    // First chunk: every body, replayed from a graph; its length is the
-    // previous step's largest body iteration count, or with compaction its
-    // median, after which about half the bodies are done and the remaining
-    // chunks launch only the live bodies' ranges (lists rebuilt whenever the
-    // live vertices fall below half of what the current list covers).
+    // First chunk: every body, replayed from a graph (rows whose bodies have
+    // all converged are skipped on the device).  Without compaction its length
+    // is the previous step's largest body iteration count plus
+    // kFullChunkMargin, so that a step seldom waits on the host more than
+    // once; with compaction it is the median count, after which about half the
+    // bodies are done and the remaining chunks launch only the live bodies'
+    // ranges (lists rebuilt whenever the live vertices fall below half of what
+    // the current list covers).
     const bool compact = s->launch.compact && s->n_bodies > 1;
     const int prev_max = s->last_iters;
     int launched = 0;
-    int chunk = std::max(1, compact ? s->median_iters : s->last_iters);
+    int chunk = std::max(1, compact ? s->median_iters : s->last_iters + kFullChunkMargin);
     bool tasks = false;
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);
@@ -456,9 +540,9 @@ int step_bodies_resident(vbd_solver *s, double dt, double tol, int max_iter)
     if (s->warm) {
         // initial guess outside the resident kernel: plain guess, then the network's where it wins
         VBD_TRY(ensure_batch_buffers(s));
-        k_init_bodies<<<nblk(std::max(s->n, s->n_bodies), 256), 256, 0, s->stream>>>(
+        k_init_bodies<<<nblk(std::max({s->n, s->n_bodies, s->n_rows}), 256), 256, 0, s->stream>>>(
             s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt,
-            s->n_bodies, s->bstate, s->dxm, s->n_live);
+            s->n_bodies, s->bstate, s->dxm, s->n_live, s->n_rows, s->row_live0, s->row_live);
         VBD_HIP(hipGetLastError());
         VBD_TRY(mpnn_apply_guess(s->warm, dt));
         p.skip_init = 1;
@@ -502,6 +586,11 @@ void release_batch(vbd_solver *s)
     if (s->bstate) (void)hipFree(s->bstate);
     if (s->dxm) (void)hipFree(s->dxm);
 
+    for (int32_t **p : {&s->row_live, &s->row_live0, &s->body_row_off, &s->body_rows}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    s->n_rows = 0;
     if (s->n_live) (void)hipFree(s->n_live);
     if (s->n_live_host) (void)hipHostFree(s->n_live_host);
     if (s->bstate_host) (void)hipHostFree(s->bstate_host);
@@ -537,21 +626,24 @@ int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int tria
     const int keep_last = s->last_iters, keep_median = s->median_iters;
     struct Cand {
         int mode, lanes, block;
+        bool compact;
     };
     std::vector<Cand> cands;
     // (1 and 16 lanes per vertex never won on the voxel bodies measured)
+    for (bool compact : {true, false})
+        for (int L : {2, 4, 8})
+            for (int blk : {128, 256, 512}) cands.push_back({kBatchLaunches, L, blk, compact});
     for (int L : {2, 4, 8})
-        for (int blk : {128, 256, 512}) cands.push_back({kBatchLaunches, L, blk});
-    for (int L : {2, 4, 8})
-        for (int blk : {512, 1024}) cands.push_back({kBatchResident, L, blk});
+        for (int blk : {512, 1024}) cands.push_back({kBatchResident, L, blk, true});
     double best = 1e300;
-    Cand bc = {s->batch_mode, s->launch.lanes, s->launch.block};
+    Cand bc = {s->batch_mode, s->launch.lanes, s->launch.block, s->launch.compact};
     const Launch keep = s->launch;
     const int keep_mode = s->batch_mode;
     int rc = 0;
     for (const Cand &c : cands) {
         s->batch_mode = c.mode;
         s->launch.lanes = c.lanes;
+        s->launch.compact = c.compact;
         if (c.mode == kBatchResident) s->launch.resident_block = c.block;
         else s->launch.block = c.block;
         // restore the start state and run the trial steps twice, timing the
@@ -574,9 +666,20 @@ int autotune_bodies(vbd_solver *s, double dt, double tol, int max_iter, int tria
         }
         if (rc != 0) break;
     }
+    // the trials' graphs of the other forms would crowd the cache
+    for (auto it = s->graphs.begin(); it != s->graphs.end();) {
+        if (bc.mode == kBatchLaunches && it->lanes == bc.lanes && it->block == bc.block) {
+            ++it;
+            continue;
+        }
+        (void)hipGraphExecDestroy(it->exec);
+        (void)hipGraphDestroy(it->graph);
+        it = s->graphs.erase(it);
+    }
     s->launch = keep;
     s->batch_mode = bc.mode;
     s->launch.lanes = bc.lanes;
+    s->launch.compact = bc.compact;
     if (bc.mode == kBatchResident) s->launch.resident_block = bc.block;
     else s->launch.block = bc.block;
     if (best >= 1e300) {

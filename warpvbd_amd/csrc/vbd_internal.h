@@ -73,7 +73,9 @@ struct CachedGraph {
     float tol = 0.f, kd_dt = 0.f;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;  // graph_clock at the last replay
 };
+constexpr size_t kGraphCacheSize = 32;  // captured graphs a solver keeps (least recently used evicted)
 
 struct Launch {
     int lanes = 4;        // lanes per vertex
@@ -131,6 +133,17 @@ struct vbd_solver {
     vbd::BodyState *bstate = nullptr, *bstate_host = nullptr;
     uint32_t *dxm = nullptr;         // per body: metric bits of the current iteration (max over its vertices)
     int32_t *n_live = nullptr, *n_live_host = nullptr;
+    // Rows of kStageRows consecutive vertices of a colour, numbered colour by
+    // colour from row_base[c]: per row the number of live bodies with a
+    // vertex in it.  A launch skips the rows whose count is zero, so a body's
+    // convergence removes its rows from the remaining launches of the step
+    // without the host building lists.
+    std::vector<int32_t> row_base;     // per colour: first row (size ncolors+1)
+    int n_rows = 0;
+    int32_t *row_live = nullptr;       // [n_rows] during a step
+    int32_t *row_live0 = nullptr;      // [n_rows] at the start of a step (every body live)
+    int32_t *body_row_off = nullptr;   // [n_bodies+1] per body: its rows, in body_rows
+    int32_t *body_rows = nullptr;
     // live-range tasks of the batched colour launches: per colour, blocks of
     // (first vertex, count) covering the bodies live when the list was built
     std::vector<int2> tasks_host;
@@ -140,6 +153,7 @@ struct vbd_solver {
     int64_t task_vertices = 0;       // vertices the current list covers
     int median_iters = 0;            // median body iteration count of the last step
     std::vector<vbd::CachedGraph> graphs;
+    uint64_t graph_clock = 0;
     int32_t *tets_dev = nullptr;     // elements (original ids), kept for the warm-start graph
     double edge_length = 0.01;       // cube root of 6 x mean tet volume
     vbd_mpnn *warm = nullptr;        // warm-start network used by the running step
@@ -222,6 +236,19 @@ struct vbd_solver {
 };
 
 namespace vbd {
+// The graph cache: a replayed entry is marked used; before a capture, the
+// least recently used entry goes once kGraphCacheSize are held.
+inline void graph_touch(vbd_solver *s, CachedGraph &g) { g.used = ++s->graph_clock; }
+inline void graph_make_room(vbd_solver *s)
+{
+    if (s->graphs.size() < kGraphCacheSize) return;
+    auto old = s->graphs.begin();
+    for (auto it = s->graphs.begin(); it != s->graphs.end(); ++it)
+        if (it->used < old->used) old = it;
+    (void)hipGraphExecDestroy(old->exec);
+    (void)hipGraphDestroy(old->graph);
+    s->graphs.erase(old);
+}
 int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const double *density,
                const double *youngs, const double *poisson, const uint8_t *active, const int32_t *body_voff,
                int n_bodies, const double *body_grav);

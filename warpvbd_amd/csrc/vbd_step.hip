@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int n
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     uint32_t m;
     int body;
-    if (!staged_sweep<MAT, false>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, nullptr, nullptr, body, m)) return;
+    if (!staged_sweep<MAT, false>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, nullptr, nullptr, nullptr, body, m)) return;
     const uint32_t mb = block_max_u32(m);
     finish_sweep(a, a.ctrl, mb, close, part_all, np_all, tol);
 }
@@ -516,16 +516,11 @@ static int single_graph(vbd_solver *s, int k, float tol, hipGraphExec_t *out)
     for (auto &g : s->graphs)
         if (g.k == k && g.tol == tol && g.kd_dt == s->kd_dt && g.lanes == s->launch.lanes &&
             g.block == s->launch.block && g.max_iter == s->launch.small_color) {
+            graph_touch(s, g);
             *out = g.exec;
             return 0;
         }
-    if (s->graphs.size() > 24) {
-        for (auto &g : s->graphs) {
-            (void)hipGraphExecDestroy(g.exec);
-            (void)hipGraphDestroy(g.graph);
-        }
-        s->graphs.clear();
-    }
+    graph_make_room(s);
     CachedGraph g;
     g.k = k;
     g.tol = tol;
@@ -545,6 +540,7 @@ static int single_graph(vbd_solver *s, int k, float tol, hipGraphExec_t *out)
     }
     g.graph = graph;
     VBD_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    graph_touch(s, g);
     s->graphs.push_back(g);
     *out = g.exec;
     return 0;

@@ -413,13 +413,15 @@ constexpr int kStagedSlots = kMaxElementsPerVertex / 4;
 // the arithmetic is k_sweep's, so the results are the same bit for bit.
 // BODIES: a batched solver's launch -- a vertex of a body that has converged
 // is not updated (its lanes get body -1), instead of the step's convergence
-// flag being tested.  Returns false when the (single-body) step has converged
-// already (the whole block then does nothing); m receives the lane's metric
-// bits (0 for lanes without an updated vertex), body the vertex's body.
+// flag being tested, and rows[b] counts the live bodies with a vertex in the
+// block.  Returns false when the (single-body) step has converged already, or
+// no body of the block is live (the whole block then does nothing); m
+// receives the lane's metric bits (0 for lanes without an updated vertex),
+// body the vertex's body.
 template <int MAT, bool BODIES>
 __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, int b, const int32_t *st_nb,
                                              const uint2 *st_rec, int blk0, int n_dict, const int32_t *body_new,
-                                             const BodyState *bst, int &body_out, uint32_t &m)
+                                             const BodyState *bst, const int32_t *rows, int &body_out, uint32_t &m)
 {
     constexpr int L = 4;
     constexpr int kStageUnroll = kStageMax / 256;
@@ -443,6 +445,17 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     int body = 0;
     if constexpr (BODIES) {
         body = body_new[v];
+        // this block's row: the live bodies with a vertex in it (same for every thread)
+        const int live = rows[b];
+        // Keep the round's loads (the body's among them) ahead of the test
+        // below: sunk past it, they would start only after the row's count
+        // arrived, a memory latency later.
+        asm volatile("" ::"v"(body), "s"(live) : "memory");
+        if (live == 0) {
+            m = 0u;
+            body_out = -1;
+            return false;
+        }
     } else {
         // the flag was written by an earlier launch: the same for every thread
         if (a.ctrl->converged) {
@@ -450,9 +463,10 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
             return false;
         }
     }
-    // round 2
-    bool valid = in;
-    if constexpr (BODIES) valid = in && !bst[body].done;
+    // round 2 (the body's flag is read by every thread, so that no branch
+    // makes the neighbour loads wait for it)
+    int done = 0;
+    if constexpr (BODIES) done = bst[body].done;
     float4 p[kStageUnroll];
 #pragma unroll
     for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -471,6 +485,7 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
         }
     }
     __syncthreads();
+    const bool valid = in && !done;
     Acc acc;
     acc_zero(acc);
     VertexLocal ql = vertex_local(pre);
@@ -511,7 +526,8 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
 // device-scope atomic max per body among its lanes (a wave's vertices are
 // consecutive in colour-major, body-minor order, so mostly one body).  Lanes
 // with body < 0 take no part.
-__device__ __forceinline__ void body_metric_max(int body, uint32_t m, uint32_t *bmet)
+__device__ __forceinline__ void body_metric_max(int body, uint32_t m, uint32_t *bmet, uint32_t *lds = nullptr,
+                                                int b0 = 0, int nlds = 0)
 {
     const int lid = (int)(threadIdx.x & 63);
     bool pending = body >= 0;
@@ -522,7 +538,12 @@ __device__ __forceinline__ void body_metric_max(int body, uint32_t m, uint32_t *
         const int lb = __builtin_amdgcn_readlane(body, lead);
         const bool mine = pending && body == lb;
         const uint32_t x = wave_max_u32(mine ? m : 0u);
-        if (lid == lead) atomicMax(&bmet[lb], x);
+        if (lid == lead) {
+            // bodies b0 .. b0 + nlds - 1 into the block's LDS maxima instead
+            const unsigned r = (unsigned)(lb - b0);
+            if (r < (unsigned)nlds) atomicMax(&lds[r], x);
+            else atomicMax(&bmet[lb], x);
+        }
         pending = pending && !mine;
     }
 }
