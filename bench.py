@@ -25,7 +25,8 @@ Extra entries (each bounded to seconds):
               extrapolated linearly over the last two)
   batched     512 mass-spring bodies (seed 3): one BatchedVBDSolver (per-body
               convergence, autotuned form, hipGraph) vs the reference's form,
-              one VBDSolver on the concatenated mesh
+              one VBDSolver on the concatenated mesh (the faster of its graph
+              and direct launches)
   warm_start  128x16x16 cantilever: the network trained on solver-made
               trajectories (reduced budget, stated), then held-out seeds
               plain, with the warm start, and with the history guess only
@@ -39,6 +40,7 @@ cpu_baseline: the FP64 CPU oracle (C restatement of the reference algorithm,
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -83,15 +85,33 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+class quiet_gc:
+    """A timed region without the interpreter's cyclic garbage collector (as
+    timeit does): collected before, re-enabled after, so that a collection of
+    objects made elsewhere (and any device memory they free) is not charged to
+    the steps being timed."""
+
+    def __enter__(self):
+        gc.collect()
+        self.was = gc.isenabled()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        if self.was:
+            gc.enable()
+
+
 def time_steps(solver, steps, dt, **kw):
     iters, ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        a = time.perf_counter()
-        solver.step(solver.old_positions, dt, **kw)
-        ms.append((time.perf_counter() - a) * 1e3)
-        iters.append(solver.last_iterations)
-    return time.perf_counter() - t0, np.array(iters), np.array(ms)
+    with quiet_gc():
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            solver.step(solver.old_positions, dt, **kw)
+            ms.append((time.perf_counter() - a) * 1e3)
+            iters.append(solver.last_iterations)
+        w = time.perf_counter() - t0
+    return w, np.array(iters), np.array(ms)
 
 
 def check_window(step, state, steps):
@@ -260,27 +280,46 @@ def body_check(bat, bodies, args, step, n_sample=8, seed=0):
             "window": [([x0[q] for q in sl], [v0[q] for q in sl], [x1[q] for q in sl]) for x0, v0, x1 in win]}
 
 
+def concatenated_leg(cb, nb, args):
+    """The reference's form of a batch: one VBDSolver on the concatenated mesh
+    with one global convergence test, timed with its iterations replayed from
+    hipGraphs and with direct launches (the faster is the leg's figure)."""
+    from warpvbd_amd import VBDSolver
+
+    steps, sim = args.batch_steps, args.batch_steps * args.dt
+    forms = {}
+    for graph in (True, False):
+        plain = VBDSolver(cb["initial_positions"], cb["elements"], cb["densities"], cb["youngs_modulus"],
+                          cb["poissons_ratio"], damping_coefficient=0.0, active_mask=cb["active_mask"],
+                          gravity=(0.0, 0.0, -9.81), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+        plain.set_graph(graph)
+        for _ in range(3):
+            plain.step(plain.old_positions, args.dt)
+        w, it, ms = time_steps(plain, steps, args.dt)
+        forms["graph" if graph else "direct"] = {
+            "aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
+            "iterations_per_step": float(it.mean()),
+            "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)]}
+        del plain
+    best = max(forms, key=lambda k: forms[k]["aggregate_sim_s_per_wall_s"])
+    return dict(forms[best], launches=best, forms=forms)
+
+
 def batched_section(args):
-    from warpvbd_amd import BatchedVBDSolver, VBDSolver
+    from warpvbd_amd import BatchedVBDSolver
     from warpvbd_amd.batched import concat_bodies
     from warpvbd_amd.scenes import mass_spring_batch
 
     bodies = mass_spring_batch(512, seed=3)
     cb = concat_bodies(bodies)
-    plain = VBDSolver(cb["initial_positions"], cb["elements"], cb["densities"], cb["youngs_modulus"],
-                      cb["poissons_ratio"], damping_coefficient=0.0, active_mask=cb["active_mask"],
-                      gravity=(0.0, 0.0, -9.81), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
-    for _ in range(3):
-        plain.step(plain.old_positions, args.dt)
-    w_plain, it_plain, ms_plain = time_steps(plain, args.batch_steps, args.dt)
-    del plain
+    nb = len(bodies)
+    concat = concatenated_leg(cb, nb, args)
     bat = BatchedVBDSolver(bodies, dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
     for _ in range(3):
         bat.step(dt=args.dt)
     # tuned on steps like the timed ones (trials start from this state, which is restored)
     tune = bat.autotune(args.dt, trial_steps=30, cache=False)
     w_bat, it_bat, ms_bat = time_steps(bat, args.batch_steps, args.dt)
-    nb = len(bodies)
     check = body_check(bat, bodies, args, lambda: bat.step(dt=args.dt).numpy().copy())
     sim = args.batch_steps * args.dt
     per_body = plain_per_body(bodies, args)
@@ -290,13 +329,13 @@ def batched_section(args):
         "workload": f"{nb} mass-spring bodies (cube 6..14, column 3..12 voxels, seed 3): "
                     f"{len(cb['initial_positions'])} vertices, {len(cb['elements'])} tets, SNH E=1e5 nu=0.4, "
                     f"dt=1e-3, tol 1e-7, {args.batch_steps} timed steps",
-        "plain_concatenated": {"aggregate_sim_s_per_wall_s": nb * sim / w_plain, "ms_per_step": 1e3 * w_plain /
-                               args.batch_steps, "iterations_per_step": float(it_plain.mean())},
+        "plain_concatenated": concat,
         "batched": {"aggregate_sim_s_per_wall_s": nb * sim / w_bat, "ms_per_step": 1e3 * w_bat / args.batch_steps,
                     "max_body_iterations_per_step": float(it_bat.mean()),
                     "mean_body_iterations_per_step": float(np.mean(bat.body_iterations)),
+                    "step_ms_p50_p90_p99": [float(np.percentile(ms_bat, q)) for q in (50, 90, 99)],
                     "autotuned": {k: tune[k] for k in ("mode", "lanes_per_vertex", "block_size", "use_graph")}},
-        "speedup": w_plain / w_bat,
+        "speedup": (nb * sim / w_bat) / concat["aggregate_sim_s_per_wall_s"],
         "_check": {"batched": check},
     }
 
@@ -377,22 +416,23 @@ def warm_section(args):
             sol.set_gravity(g)
             sol.old_velocities = v0
         ws.set_material(E, nu)
-        for _ in range(eval_steps):
-            a = time.perf_counter()
-            xw = s.step(s.old_positions, args.dt, warm_start=ws)
-            b = time.perf_counter()
-            xp = ref.step(ref.old_positions, args.dt)
-            c = time.perf_counter()
-            xh = sh.step(sh.old_positions, args.dt, warm_start=wh)
-            d = time.perf_counter()
-            w_w += b - a
-            w_p += c - b
-            w_h += d - c
-            it_w.append(s.last_iterations)
-            it_p.append(ref.last_iterations)
-            it_h.append(sh.last_iterations)
-            used.append(s.last_used_network)
-            used_h.append(ws.last_guess_counts()["history"])
+        with quiet_gc():
+            for _ in range(eval_steps):
+                a = time.perf_counter()
+                xw = s.step(s.old_positions, args.dt, warm_start=ws)
+                b = time.perf_counter()
+                xp = ref.step(ref.old_positions, args.dt)
+                c = time.perf_counter()
+                xh = sh.step(sh.old_positions, args.dt, warm_start=wh)
+                d = time.perf_counter()
+                w_w += b - a
+                w_p += c - b
+                w_h += d - c
+                it_w.append(s.last_iterations)
+                it_p.append(ref.last_iterations)
+                it_h.append(sh.last_iterations)
+                used.append(s.last_used_network)
+                used_h.append(ws.last_guess_counts()["history"])
         err.append(float(np.abs(xw.numpy() - xp.numpy()).max()))
         err_h.append(float(np.abs(xh.numpy() - xp.numpy()).max()))
     sim = n_eval * eval_steps * args.dt
@@ -426,7 +466,7 @@ def mixed_section(args):
     form (one VBDSolver on the concatenated mesh, global convergence), the
     batched solver, and the batched solver with the warm start (trained on
     samples the batched solver itself produced; reduced budget, stated)."""
-    from warpvbd_amd import BatchedVBDSolver, VBDSolver
+    from warpvbd_amd import BatchedVBDSolver
     from warpvbd_amd.batched import concat_bodies
     from warpvbd_amd.scenes import mixed_batch
     from warpvbd_amd.warm_start import WarmStart
@@ -439,16 +479,7 @@ def mixed_section(args):
     out = {"workload": f"{nb} bodies (512 cantilevers L 16..64 x c 4..8 voxels, 512 mass-springs), seed 5: "
                        f"{len(cb['initial_positions'])} vertices, {len(cb['elements'])} tets, E log-U[1e4,1e6], "
                        f"nu U[0.3,0.45], SNH, dt=1e-3, tol 1e-7, {steps} timed steps"}
-    plain = VBDSolver(cb["initial_positions"], cb["elements"], cb["densities"], cb["youngs_modulus"],
-                      cb["poissons_ratio"], damping_coefficient=0.0, active_mask=cb["active_mask"],
-                      gravity=(0.0, 0.0, -9.81), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
-    for _ in range(3):
-        plain.step(plain.old_positions, args.dt)
-    w, it, ms = time_steps(plain, steps, args.dt)
-    out["plain_concatenated"] = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
-                                 "iterations_per_step": float(it.mean()),
-                                 "step_ms_p50_p90_p99": [float(np.percentile(ms, q)) for q in (50, 90, 99)]}
-    del plain
+    out["plain_concatenated"] = concatenated_leg(cb, nb, args)
     bat = BatchedVBDSolver(bodies, dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
     ws = WarmStart(bat, capacity=args.mixed_samples, seed=0)
     t0 = time.perf_counter()
@@ -475,16 +506,17 @@ def mixed_section(args):
         warm = None if policy is None else ws
         bat.step(dt=args.dt, warm_start=warm)
         ms_l, it_l, mean_l, guess_l = [], [], [], []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            a = time.perf_counter()
-            bat.step(dt=args.dt, warm_start=warm)
-            ms_l.append((time.perf_counter() - a) * 1e3)
-            it_l.append(bat.last_iterations)
-            mean_l.append(bat.body_iterations.mean())
-            if warm is not None:
-                guess_l.append(ws.last_guess_counts())
-        w = time.perf_counter() - t0
+        with quiet_gc():
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                a = time.perf_counter()
+                bat.step(dt=args.dt, warm_start=warm)
+                ms_l.append((time.perf_counter() - a) * 1e3)
+                it_l.append(bat.last_iterations)
+                mean_l.append(bat.body_iterations.mean())
+                if warm is not None:
+                    guess_l.append(ws.last_guess_counts())
+            w = time.perf_counter() - t0
         r = {"aggregate_sim_s_per_wall_s": nb * sim / w, "ms_per_step": 1e3 * w / steps,
              "max_body_iterations_per_step": float(np.mean(it_l)),
              "mean_body_iterations_per_step": float(np.mean(mean_l)),

@@ -41,28 +41,49 @@ def test_batch_fixed_iterations_match_single_body_oracle(mode, lanes, block, gra
 
 @pytest.mark.parametrize("mode", ["launches", "resident"])
 def test_batch_tolerance_per_body_convergence(mode):
+    """Tolerance termination, body by body, against two FP64 runs: the oracle
+    at the same tolerance (iteration counts within one, 90% equal; positions
+    within 1e-5 x diagonal; the tip curve within 1e-4 relative for a body
+    whose counts all matched), and the oracle replaying the GPU's iteration
+    count of each step (the tip curve within 1e-4 relative for every body).
+    A count one apart moves a tolerance-terminated trajectory by as much as
+    the tolerance allows (the oracle at tol 1e-7 against itself at 1e-10
+    differs by 6e-3 to 5e-2 relative on these bodies), so the replay is what
+    separates the solver's arithmetic from the termination decision."""
     bs = bodies()
     g = BatchedVBDSolver(bs, dx_tol=1e-7, max_iter=1000, verbose=False)
     g.set_mode(mode, 4, 256 if mode == "launches" else 1024, True)
     refs = [oracle.OracleSolver(**b.solver_kwargs(), dx_tol=1e-7, max_iter=1000) for b in bs]
+    reps = [oracle.OracleSolver(**b.solver_kwargs(), dx_tol=0.0, max_iter=1000) for b in bs]
     diffs = []
-    tip_g = [[] for _ in bs]
-    tip_r = [[] for _ in bs]
+    flips = [0] * len(bs)
+    tip_g, tip_r, tip_p = ([[] for _ in bs] for _ in range(3))
     for k in range(30):
         x = g.step(dt=1e-3).numpy()
-        for i, (b, r) in enumerate(zip(bs, refs)):
+        for i, (b, r, q) in enumerate(zip(bs, refs, reps)):
             xr = r.step(1e-3)
-            diffs.append(abs(int(g.body_iterations[i]) - r.last_iters))
-            assert abs(int(g.body_iterations[i]) - r.last_iters) <= 1
+            xq = q.step(1e-3, max_iter=int(g.body_iterations[i]))
+            d = abs(int(g.body_iterations[i]) - r.last_iters)
+            diffs.append(d)
+            flips[i] += d != 0
+            assert d <= 1
+            xb = x[g.body_slice(i)]
             # tolerance termination: each step may differ by O(tol) per iteration-count flip
-            assert np.abs(x[g.body_slice(i)] - xr).max() < 1e-5 * diag(b)
-            tip_g[i].append(x[g.body_slice(i)][b.tip_idx, 2].mean())
+            assert np.abs(xb - xr).max() < 1e-5 * diag(b)
+            tip_g[i].append(xb[b.tip_idx, 2].mean())
             tip_r[i].append(xr[b.tip_idx, 2].mean())
+            tip_p[i].append(xq[b.tip_idx, 2].mean())
     assert np.mean(np.array(diffs) == 0) > 0.9
     for i, b in enumerate(bs):
-        zr = np.array(tip_r[i]) - b.vertices[b.tip_idx, 2].mean()
-        zg = np.array(tip_g[i]) - b.vertices[b.tip_idx, 2].mean()
-        assert np.abs(zg - zr).max() <= 1e-3 * np.abs(zr).max()
+        z0 = b.vertices[b.tip_idx, 2].mean()
+        zg, zr, zp = np.array(tip_g[i]) - z0, np.array(tip_r[i]) - z0, np.array(tip_p[i]) - z0
+        rel = np.abs(zg - zr).max() / np.abs(zr).max()
+        rel_p = np.abs(zg - zp).max() / np.abs(zp).max()
+        print(f"{mode} body {i}: {flips[i]} count flips; tip curve max rel err {rel:.2e} (same tolerance), "
+              f"{rel_p:.2e} (same counts)")
+        assert rel_p < 1e-4
+        if flips[i] == 0:
+            assert rel < 1e-4
 
 
 def test_live_range_launches_equal_full_launches():
@@ -92,7 +113,7 @@ def test_batch_autotune_restores_state_and_keeps_parity(tmp_path, monkeypatch):
     g.step(dt=1e-3)
     x_before = g.old_positions.numpy().copy()
     res = g.autotune(1e-3, trial_steps=2)
-    assert res["mode"] in ("launches", "resident") and not res["cached"]
+    assert res["mode"] in ("launches", "launches_full", "resident") and not res["cached"]
     assert np.array_equal(g.old_positions.numpy(), x_before)
     res2 = g.autotune(1e-3, trial_steps=2)
     assert res2["cached"] and res2["mode"] == res["mode"]

@@ -107,20 +107,43 @@ def _batch_tolerance(bodies, steps=20):
     g = BatchedVBDSolver(bodies, dx_tol=1e-7, max_iter=1000, verbose=False)
     refs = {i: oracle.OracleSolver(**bodies[i].solver_kwargs(), dx_tol=1e-7, max_iter=1000, expanded=False)
             for i in pick}
+    # the oracle replaying the GPU's iteration count of each step
+    reps = {i: oracle.OracleSolver(**bodies[i].solver_kwargs(), dx_tol=0.0, max_iter=1000, expanded=False)
+            for i in pick}
     same, worst = [], 0.0
+    flips = {i: 0 for i in pick}
+    tips = {i: ([], [], []) for i in pick}
     for k in range(steps):
         x = g.step(dt=1e-3).numpy()
         for i in pick:
             xr = refs[i].step(1e-3)
+            xq = reps[i].step(1e-3, max_iter=int(g.body_iterations[i]))
             d = abs(int(g.body_iterations[i]) - refs[i].last_iters)
             same.append(d == 0)
+            flips[i] += d != 0
             assert d <= 1, (k, i)
-            e = float(np.abs(x[g.body_slice(i)] - xr).max()) / diag(bodies[i].vertices)
+            xb = x[g.body_slice(i)]
+            e = float(np.abs(xb - xr).max()) / diag(bodies[i].vertices)
             worst = max(worst, e)
             assert e < 1e-5, (k, i, e)
+            t = bodies[i].tip_idx
+            for j, y in enumerate((xb, xr, xq)):
+                tips[i][j].append(y[t, 2].mean())
+    # tip-displacement curves within 1e-4 relative, body by body: against the
+    # replay for every body, against the same-tolerance run where no count flipped
+    rel_same, rel_rep = [], []
+    for i in pick:
+        z0 = bodies[i].vertices[bodies[i].tip_idx, 2].mean()
+        zg, zr, zq = (np.array(tips[i][j]) - z0 for j in range(3))
+        rel_rep.append(np.abs(zg - zq).max() / np.abs(zq).max())
+        if flips[i] == 0:
+            rel_same.append(np.abs(zg - zr).max() / np.abs(zr).max())
     print(f"tolerance: {np.mean(same):.3f} of body-steps at the oracle's iteration count; "
-          f"worst |x - x_fp64| / diagonal {worst:.2e}")
+          f"worst |x - x_fp64| / diagonal {worst:.2e}; worst tip curve rel err {max(rel_rep):.2e} (same counts), "
+          f"{max(rel_same, default=0.0):.2e} (same tolerance, {len(rel_same)} bodies without a count flip)")
     assert np.mean(same) >= 0.9
+    assert max(rel_rep) < 1e-4
+    assert max(rel_same, default=0.0) < 1e-4
 
 
 @pytest.fixture(scope="module")

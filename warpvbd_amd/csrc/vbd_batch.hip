@@ -480,9 +480,14 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
         if (tasks) {
             for (int j = 0; j < k; ++j) VBD_TRY(enqueue_task_iteration(s, kd_dt, ftol, max_iter));
         } else if (s->launch.graph) {
-            hipGraphExec_t ex;
-            VBD_TRY(get_graph(s, k, kd_dt, ftol, max_iter, &ex));
-            VBD_HIP(hipGraphLaunch(ex, s->stream));
+            // as graphs of 2^j iterations, largest first (no capture for a new count)
+            for (int rest = k; rest > 0;) {
+                const int p = graph_piece(rest);
+                hipGraphExec_t ex;
+                VBD_TRY(get_graph(s, p, kd_dt, ftol, max_iter, &ex));
+                VBD_HIP(hipGraphLaunch(ex, s->stream));
+                rest -= p;
+            }
         } else {
             for (int j = 0; j < k; ++j) VBD_TRY(enqueue_body_iteration(s, kd_dt, ftol, max_iter));
         }

@@ -76,6 +76,15 @@ struct CachedGraph {
     uint64_t used = 0;  // graph_clock at the last replay
 };
 constexpr size_t kGraphCacheSize = 32;  // captured graphs a solver keeps (least recently used evicted)
+constexpr int kMaxGraphIters = 32;       // iterations in the largest captured graph
+// The largest power of two not above min(n, kMaxGraphIters) (n >= 1): a run of
+// n iterations is replayed from graphs of 2^j iterations.
+inline int graph_piece(int n)
+{
+    int p = 1;
+    while (2 * p <= n && 2 * p <= kMaxGraphIters) p *= 2;
+    return p;
+}
 
 struct Launch {
     int lanes = 4;        // lanes per vertex

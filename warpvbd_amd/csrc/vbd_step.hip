@@ -612,9 +612,16 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
     while (launched < max_iter && !conv) {
         const int k = std::min(chunk, max_iter - launched);
         if (s->launch.graph && !s->prof) {
-            hipGraphExec_t ex;
-            VBD_TRY(single_graph(s, k, ftol, &ex));
-            VBD_HIP(hipGraphLaunch(ex, s->stream));
+            // k iterations as graphs of 2^j iterations, largest first: a few
+            // graphs per launch plan serve every chunk length, so a step whose
+            // count has not been seen before captures nothing
+            for (int rest = k; rest > 0;) {
+                const int p = graph_piece(rest);
+                hipGraphExec_t ex;
+                VBD_TRY(single_graph(s, p, ftol, &ex));
+                VBD_HIP(hipGraphLaunch(ex, s->stream));
+                rest -= p;
+            }
         } else {
             for (int j = 0; j < k; ++j) VBD_TRY(enqueue_iteration(s, launched + j, ftol, s->prof));
         }
