@@ -27,9 +27,10 @@ Extra entries (each bounded to seconds):
               convergence, autotuned form, hipGraph) vs the reference's form,
               one VBDSolver on the concatenated mesh (the faster of its graph
               and direct launches)
-  warm_start  128x16x16 cantilever: the network trained on solver-made
-              trajectories (reduced budget, stated), then held-out seeds
-              plain, with the warm start, and with the history guess only
+  warm_start  128x16x16 cantilever, the committed network parameters
+              (python -m warpvbd_amd.warm_train, 200 x 300 trajectories):
+              held-out seeds plain, history guess only, CHOOSE and ADAPTIVE,
+              and the network's cost per step against the history guess
   mixed       1024 bodies (512 cantilevers + 512 mass-springs, seed 5): one
               concatenated solver, one solver per body, batched, and batched
               with the history guess / warm start (CHOOSE, ADAPTIVE), each
@@ -71,8 +72,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--batch-steps", type=int, default=100)
-    p.add_argument("--warm-trajectories", type=int, default=16)
-    p.add_argument("--warm-steps", type=int, default=60)
+    p.add_argument("--warm-seeds", type=int, default=5, help="held-out seeds of the warm_start section")
+    p.add_argument("--warm-steps", type=int, default=60, help="steps per held-out seed")
     p.add_argument("--warm-epochs", type=int, default=12)
     p.add_argument("--mixed-samples", type=int, default=24)
     p.add_argument("--roofline-steps", type=int, default=100)
@@ -381,84 +382,64 @@ def warm_big_section(args):
 
 
 def warm_section(args):
+    """The learned warm start on the 128x16x16 cantilever with the committed
+    parameters (trained by warpvbd_amd.warm_train on 200 x 300 trajectories,
+    seeds 0..199), on held-out seeds from 1000: plain, history guess only,
+    CHOOSE and ADAPTIVE, each from the same start, with the network's cost
+    against the history guess."""
     from warpvbd_amd import VBDSolver
     from warpvbd_amd.scenes import cantilever
-    from warpvbd_amd.warm_start import WarmStart, gravity_in_cone
+    from warpvbd_amd.warm_start import WarmStart, pretrained
+    from warpvbd_amd.warm_train import evaluate, scene_of_seed
 
     sc = cantilever((128, 16, 16))
-    s = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
-    cap = args.warm_trajectories * args.warm_steps // 2
-    ws = WarmStart(s, capacity=cap, seed=0)
-    t0 = time.perf_counter()
-    ws.generate(trajectories=args.warm_trajectories, steps=args.warm_steps, dt=args.dt, tol=args.tol, seed=0,
-                sample_every=2)
-    t_gen = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    hist = ws.train(epochs=args.warm_epochs, batch=8, lr=2e-3, seed=0)
-    t_train = time.perf_counter() - t0
-    ref = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
-    sh = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
-    wh = WarmStart(sh, capacity=1)
-    wh.set_policy(WarmStart.HISTORY)
-    rng = np.random.default_rng(1000)
-    it_p, it_w, it_h, used, used_h, err, err_h = [], [], [], [], [], [], []
-    w_p = w_w = w_h = 0.0
-    n_eval, eval_steps = 3, 40
-    diag = float(np.linalg.norm(sc.vertices.max(0) - sc.vertices.min(0)))
-    for _ in range(n_eval):
-        E = float(10 ** rng.uniform(4, 6))
-        nu = float(rng.uniform(0.3, 0.45))
-        g = gravity_in_cone(rng, 30.0)
-        v0 = rng.normal(0, 0.05, size=(sc.n_vertices, 3)) * sc.active_mask[:, None]
-        for sol in (s, ref, sh):
-            sol.reset()
-            sol.set_material(E, nu)
-            sol.set_gravity(g)
-            sol.old_velocities = v0
-        ws.set_material(E, nu)
-        with quiet_gc():
-            for _ in range(eval_steps):
-                a = time.perf_counter()
-                xw = s.step(s.old_positions, args.dt, warm_start=ws)
-                b = time.perf_counter()
-                xp = ref.step(ref.old_positions, args.dt)
-                c = time.perf_counter()
-                xh = sh.step(sh.old_positions, args.dt, warm_start=wh)
-                d = time.perf_counter()
-                w_w += b - a
-                w_p += c - b
-                w_h += d - c
-                it_w.append(s.last_iterations)
-                it_p.append(ref.last_iterations)
-                it_h.append(sh.last_iterations)
-                used.append(s.last_used_network)
-                used_h.append(ws.last_guess_counts()["history"])
-        err.append(float(np.abs(xw.numpy() - xp.numpy()).max()))
-        err_h.append(float(np.abs(xh.numpy() - xp.numpy()).max()))
-    sim = n_eval * eval_steps * args.dt
+    params = np.load(pretrained("cantilever128x16x16"))
+    seeds = list(range(1000, 1000 + args.warm_seeds))
+    ev = evaluate(sc, params, seeds, args.warm_steps, policies=("history", "choose", "adaptive"), dt=args.dt,
+                  tol=args.tol, max_iter=args.max_iter, log=log)
+    P = ev["policies"]
+    # oracle-check windows: each form 10 steps into seed 1000, then check_steps steps
+    E, nu, g, v0 = scene_of_seed(sc.n_vertices, sc.active_mask, seeds[0])
     kw = dict(sc.solver_kwargs(), youngs_modulus=E, poissons_ratio=nu, gravity=tuple(g))
     checks = {}
-    for form, sol, w in (("plain", ref, None), ("warm", s, ws), ("history_only", sh, wh)):
+    for form, pol in (("plain", None), ("warm", WarmStart.CHOOSE), ("history_only", WarmStart.HISTORY)):
+        sol = VBDSolver(**sc.solver_kwargs(), dx_tol=args.tol, max_iter=args.max_iter, verbose=False)
+        sol.set_material(E, nu)
+        sol.set_gravity(g)
+        sol.old_velocities = v0
+        w = None
+        if pol is not None:
+            w = WarmStart(sol, capacity=1)
+            w.set_params(params)
+            w.set_policy(pol)
+            w.set_material(E, nu)
+        for _ in range(10):
+            sol.step(sol.old_positions, args.dt, warm_start=w)
         checks[form] = {"kwargs": kw, "material": "snh", "tol": args.tol, "dt": args.dt,
                         "window": check_window(lambda: sol.step(sol.old_positions, args.dt, warm_start=w).numpy().copy(),
                                                lambda: (sol.old_positions.numpy().copy(),
                                                         sol.old_velocities.numpy().copy()), args.check_steps)}
-    return {
+        del w, sol
+    it_h, ms_h = P["history"]["iterations_per_step"], P["history"]["ms_per_step"]
+    it_p, ms_p = P["plain"]["iterations_per_step"], P["plain"]["ms_per_step"]
+    t_it = (ms_p - ms_h) / (it_p - it_h) if it_p > it_h else None
+    out = {
         "_check": checks,
-        "workload": f"cantilever 128x16x16 ({sc.n_vertices} vertices); training: {args.warm_trajectories} seeded "
-                    f"trajectories x {args.warm_steps} steps (every 2nd step stored, {ws.n_samples} samples; "
-                    f"reduced from 200 x 300 to fit the benchmark budget), E log-U[1e4,1e6], nu U[0.3,0.45], gravity "
-                    f"in a 30 deg cone, v0 ~ N(0, 0.05); {args.warm_epochs} epochs; evaluation: {n_eval} held-out "
-                    f"seeds x {eval_steps} steps",
-        "generate_s": t_gen, "train_s": t_train, "train_loss_first_last": [float(hist[0]), float(hist[-1])],
-        "plain": {"iterations_per_step": float(np.mean(it_p)), "sim_s_per_wall_s": sim / w_p},
-        "warm": {"iterations_per_step": float(np.mean(it_w)), "sim_s_per_wall_s": sim / w_w,
-                 "network_used_fraction": float(np.mean(used)), "history_used_fraction": float(np.mean(used_h))},
-        "history_only": {"iterations_per_step": float(np.mean(it_h)), "sim_s_per_wall_s": sim / w_h,
-                         "max_abs_diff_vs_plain_m": max(err_h)},
-        "max_abs_diff_vs_plain_m": max(err), "bbox_diagonal_m": diag,
-        "iteration_reduction": 1.0 - float(np.mean(it_w)) / float(np.mean(it_p)),
+        "workload": f"cantilever 128x16x16 ({sc.n_vertices} vertices), SNH, dt=1e-3, tol 1e-7; committed parameters "
+                    f"warpvbd_amd/weights/warm_start_cantilever128x16x16.npy (python -m warpvbd_amd.warm_train: "
+                    f"200 trajectories x 300 steps, seeds 0..199, 30000 samples, 18 epochs); evaluation: held-out "
+                    f"seeds {seeds[0]}..{seeds[-1]} x {args.warm_steps} steps, every form from the same start",
+        "plain": P["plain"], "history_only": P["history"], "warm": P["choose"], "warm_adaptive": P["adaptive"],
     }
+    if t_it:
+        net_us = 1e3 * (P["choose"]["ms_per_step"] - ms_h + (it_h - P["choose"]["iterations_per_step"]) * t_it)
+        out["network_cost_vs_history"] = {
+            "iteration_us": 1e3 * t_it, "network_us_per_step": net_us,
+            "max_saving_us_per_step": 1e3 * (it_h - 1.0) * t_it,
+            "note": "iteration cost from the plain and history rows; the network's part of a CHOOSE step is CHOOSE "
+                    "minus history corrected for the iterations it saves; no guess can save more than the history "
+                    "guess's iterations beyond the one every step needs"}
+    return out
 
 
 def mixed_section(args):

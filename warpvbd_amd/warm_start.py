@@ -260,6 +260,18 @@ class WarmStart:
         return out
 
 
+def pretrained(name: str = "cantilever128x16x16") -> str:
+    """Path of committed parameters: ``cantilever128x16x16`` was trained by
+    ``python -m warpvbd_amd.warm_train`` on 200 trajectories x 300 steps of the
+    128x16x16 cantilever (seeds 0..199); load with ``WarmStart.load``."""
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", f"warm_start_{name}.npy")
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    return path
+
+
 def gravity_in_cone(rng, cone_deg, g_norm=9.81):
     """Gravity of magnitude g_norm with direction uniform (in solid angle) in a
     cone of half-angle cone_deg about -z."""
