@@ -87,6 +87,13 @@ int vbd_step(vbd_solver_t *h, const double *positions, double dt, double tol, in
 int vbd_step_dev(vbd_solver_t *h, const double *positions_dev, double dt, double tol, int32_t max_iter,
                  double *out_dev, void *stream, int32_t *iters_out, double *metric_out);
 
+/* After a step that ran its last allowed iteration (iters == max_iter): the
+ * largest |dx| component of that iteration over the swept vertices, the value
+ * the reference prints in its non-convergence warning ("Final dx max:
+ * abs(dxs).max()", src/vbd_solver.py:581-582); NaN after a step that stopped
+ * earlier.  Single-body solvers (vbd_step, vbd_step_dev, vbd_step_warm). */
+int vbd_last_dx_max(vbd_solver_t *h, double *dx_max);
+
 /* Change the material of a solver whose elements share one (E, nu): the
  * slot records are rescaled in place (no rebuild).  Replaces constructing a
  * new VBDSolver with other youngs_modulus / poissons_ratio arrays. */

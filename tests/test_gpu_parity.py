@@ -447,6 +447,22 @@ def test_errors_are_reported():
         g.step(g.old_positions, 0.0)
 
 
+def test_material_change_from_a_zero_lame_parameter():
+    """StVK created with nu = 0 has lambda = 0: its constants cannot be
+    rescaled to a nonzero lambda (an error, the material is kept), but a
+    change that keeps lambda = 0 works and matches a solver created with it."""
+    sc = cantilever((6, 2, 2))
+    g, _ = make(sc, material="stvk", poissons_ratio=0.0, tol=0.0, max_iter=4)
+    with pytest.raises(VBDError, match="zero Lame parameter"):
+        g.set_material(1e5, 0.3)
+    g.set_material(2e5, 0.0)
+    h, r = make(sc, material="stvk", youngs_modulus=2e5, poissons_ratio=0.0, tol=0.0, max_iter=4)
+    xg = g.step(g.old_positions, 1e-3).numpy()
+    xh = h.step(h.old_positions, 1e-3).numpy()
+    assert np.abs(xg - xh).max() <= 1e-6 * np.abs(xh).max()
+    assert np.abs(xg - r.step(1e-3)).max() < 1e-6 * float(np.linalg.norm(np.ptp(sc.vertices, axis=0)))
+
+
 def test_more_than_256_colours_is_an_error():
     """The reference leaves vertices uncoloured (-1) past 256 colours
     (src/coloring.py:62-68); here the colouring reports ECOLORS.  A complete
@@ -487,6 +503,33 @@ def test_iteration_cap_warns_like_reference(capsys):
     g.step(g.old_positions, 1e-3)
     assert g.last_iterations == 1
     assert "did not converge" in capsys.readouterr().out
+
+
+@pytest.mark.parametrize("shape", [(10, 2, 2), (64, 8, 8)])
+def test_final_dx_max_is_the_last_iterations_largest_component(shape, capsys):
+    """The warning's "Final dx max" is max |dx| over the components of the last
+    allowed iteration (abs(dxs).max(), src/vbd_solver.py:582): the oracle's
+    positions after 3 iterations minus those after 2, from the same start,
+    within 1e-3 relative (it is a difference of two iterates, each within the
+    FP32 arithmetic's distance of the oracle's; that distance is printed).
+    (10, 2, 2) runs the one-workgroup step, (64, 8, 8) the per-colour
+    launches; a converged step reports NaN."""
+    sc = cantilever(shape)
+    g, _ = make(sc, max_iter=3)
+    x = g.step(g.old_positions, 1e-3, dx_tol=1e-30).numpy()
+    assert g.last_iterations == 3
+    out = capsys.readouterr().out
+    assert "Final dx max" in out
+    r2 = oracle.OracleSolver(**sc.solver_kwargs(), dx_tol=0.0, max_iter=2, expanded=False)
+    r3 = oracle.OracleSolver(**sc.solver_kwargs(), dx_tol=0.0, max_iter=3, expanded=False)
+    x3 = r3.step(1e-3)
+    want = float(np.abs(x3 - r2.step(1e-3)).max())
+    print(f"{shape}: final dx max {g.last_dx_max:.6e} (oracle {want:.6e}); positions after the step within "
+          f"{np.abs(x - x3).max():.2e} m of the oracle's")
+    assert abs(g.last_dx_max - want) <= 1e-3 * want
+    assert f"{g.last_dx_max}" in out
+    g.step(g.old_positions, 1e-3, dx_tol=1.0, max_iter=50)
+    assert g.last_iterations < 50 and np.isnan(g.last_dx_max)
 
 
 def test_inf_velocity_runs_to_max_iter():

@@ -42,6 +42,7 @@ SIGNATURES = {
     "vbd_color": (ctypes.c_int, [_P(_i32), _i32, _i32, _P(_i32), _P(_i32), _P(_i32)]),
     "vbd_color_graph": (ctypes.c_int, [_P(_i32), _P(_i32), _i32, _P(_i32), _P(_i32)]),
     "vbd_color_noalloc": (ctypes.c_int, [_vp, _i32, _i32, _vp, _P(_i32), _P(_i32), _vp]),
+    "vbd_last_dx_max": (ctypes.c_int, [_vp, _P(_d)]),
     "vbd_profile_enable": (ctypes.c_int, [_vp, _i32, _i32]),
     "vbd_profile_read": (ctypes.c_int, [_vp, _P(_d), _P(_i64), _P(_d)]),
     "vbd_color_bytes": (ctypes.c_int, [_vp, _P(_d)]),

@@ -41,7 +41,7 @@ void vbd_destroy(vbd_solver_t *s)
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict,
-                   s->st_nb, s->st_rec, s->small_rec};
+                   s->st_nb, s->st_rec, s->small_rec, s->u_last, s->d_dx_max};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (double4 *p : s->snap)
@@ -565,6 +565,13 @@ int vbd_profile_enable(vbd_solver_t *s, int32_t enable, int32_t min_vertices)
     s->prof_min = min_vertices;
     s->prof_ms = s->prof_bytes = 0.0;
     s->prof_launches = 0;
+    return 0;
+}
+
+int vbd_last_dx_max(vbd_solver_t *s, double *dx_max)
+{
+    CHECK_ARG(s && dx_max, "null argument");
+    *dx_max = s->last_dx_max;
     return 0;
 }
 

@@ -231,6 +231,12 @@ struct vbd_solver {
     double4 *snap[VBD_MAX_SNAPSHOTS] = {};
 
     vbd::Launch launch;
+    // a step that runs its last allowed iteration keeps the iterate from
+    // before it (u_last), and the largest |dx| component of that iteration is
+    // reported (last_dx_max, NaN when the last step stopped earlier)
+    float4 *u_last = nullptr;
+    double *d_dx_max = nullptr;
+    double last_dx_max = 0.0;
     int last_iters = 0;
     int prev_iters = 0;  // iterations of the step before the last (single-body launches)
 

@@ -729,13 +729,23 @@ int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1
     double mu0, lam0, mu1, lam1;
     lame(E0, nu0, mu0, lam0);
     lame(E1, nu1, mu1, lam1);
+    // new / old, 1 when both are zero (the constants stay zero)
+    auto ratio = [](double a1, double a0) { return a0 == 0.0 && a1 == 0.0 ? 1.0 : a1 / a0; };
     double fa, fb;
     if (s->material == kSNH) {
-        fa = mu1 / mu0;  // mu' = 4mu/3
-        fb = (lam1 + 5.0 * mu1 / 6.0) / (lam0 + 5.0 * mu0 / 6.0);
+        fa = ratio(mu1, mu0);  // mu' = 4mu/3
+        fb = ratio(lam1 + 5.0 * mu1 / 6.0, lam0 + 5.0 * mu0 / 6.0);
     } else {
-        fa = mu1 / mu0;
-        fb = lam1 / lam0;
+        fa = ratio(mu1, mu0);
+        fb = ratio(lam1, lam0);
+    }
+    // The constants are rescaled in place, which needs the old parameters to
+    // be nonzero (a solver created with nu = 0 has lambda = 0, and StVK scales
+    // by lambda1 / lambda0); such a solver keeps its material.
+    if (!std::isfinite(fa) || !std::isfinite(fb)) {
+        set_error("set_material: the solver's current material has a zero Lame parameter (E = 0, or nu = 0 "
+                  "with StVK), so its constants cannot be rescaled; create a solver with the new material");
+        return VBD_EINVAL;
     }
 #if VBD_TET_RECORDS
     const int64_t nrec = s->m;

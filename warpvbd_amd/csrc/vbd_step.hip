@@ -29,6 +29,15 @@
 
 namespace vbd {
 
+// The one-workgroup step serves a body whose largest colour has at most this
+// many vertices: one workgroup sweeps a colour in passes of 1024 threads, so
+// its iteration grows with the colour while a launch per colour stays near
+// the launch latency.  Measured on the mesh-size sweep's cantilevers (wall s
+// per simulated s, tol 1e-6, resident against launches): 176 vertices 0.21
+// resident; 1,029: 0.57 against 0.43; 3,100: 1.96 against 0.47; 6,929: 3.5
+// against 0.53.
+constexpr int kResidentMaxColor = 128;
+
 namespace {
 
 // Max over the np per-block partial maxima (this thread's share; entry skip
@@ -263,6 +272,26 @@ __global__ void k_step_init(int n, float4 *u, const double4 *uprev, const float4
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, g);
 }
 
+// The largest |dx| component of the last iteration over the swept vertices:
+// u against u_last, the iterate before it (one workgroup; used only by steps
+// that ran their last allowed iteration).
+__global__ __launch_bounds__(1024) void k_dx_max(int n, const float4 *u, const float4 *u_last, double *out)
+{
+    __shared__ double red[16];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double4 a = unpack_disp4(u[i]), b = unpack_disp4(u_last[i]);
+        m = fmax(m, fmax(fabs(a.x - b.x), fmax(fabs(a.y - b.y), fabs(a.z - b.z))));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+        *out = m;
+    }
+}
+
 __global__ void k_step_finish(int n, const float4 *u, double4 *uprev, float4 *vel, float dt)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -398,7 +427,7 @@ void plan_segments(vbd_solver *s)
         off += g.nparts;
     }
     s->n_parts = off;
-    s->resident = s->launch.resident && mx < s->launch.small_color;
+    s->resident = s->launch.resident && mx < s->launch.small_color && mx <= kResidentMaxColor;
     // lanes of the resident form: as many as keep the largest colour within one
     // pass of the 1024-thread workgroup
     int lanes = 1;
@@ -546,6 +575,28 @@ static int single_graph(vbd_solver *s, int k, float tol, hipGraphExec_t *out)
     return 0;
 }
 
+static int ensure_u_last(vbd_solver *s)
+{
+    if (s->u_last) return 0;
+    VBD_HIP(hipMalloc((void **)&s->u_last, sizeof(float4) * std::max(s->n_swept, 1)));
+    VBD_HIP(hipMalloc((void **)&s->d_dx_max, sizeof(double)));
+    return 0;
+}
+
+// After a step of `it` iterations: the last iteration's largest |dx| component
+// when it was the last allowed one (the reference prints it then,
+// src/vbd_solver.py:581-582), else NaN.
+static int report_dx_max(vbd_solver *s, int it, int max_iter)
+{
+    s->last_dx_max = std::nan("");
+    if (it != max_iter || !s->u_last || s->n_swept == 0) return 0;
+    k_dx_max<<<1, 1024, 0, s->stream>>>(s->n_swept, s->u, s->u_last, s->d_dx_max);
+    VBD_HIP(hipGetLastError());
+    VBD_HIP(hipMemcpyAsync(&s->last_dx_max, s->d_dx_max, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int *iters, double *metric)
 {
     SweepArgs a;
@@ -568,6 +619,8 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     p.kd_dt = a.kd_dt;
     p.max_iter = max_iter;
     p.skip_init = 0;
+    VBD_TRY(ensure_u_last(s));
+    p.u_last = s->u_last;
     if (s->material == kSNH)
         dispatch_resident<kSNH>(s->resident_lanes, a, s->d_ranges, s->ncolors, s->d_grav, s->mass, s->active,
                                 s->uprev, s->vel, p, s->d_body_out, 1, 1024, s->stream);
@@ -582,7 +635,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     s->last_iters = it;
     if (iters) *iters = it;
     if (metric) *metric = (double)__builtin_bit_cast(float, (uint32_t)s->body_out_host[0].y);
-    return 0;
+    return report_dx_max(s, it, max_iter);
 }
 
 namespace {
@@ -609,12 +662,11 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
     // step one iteration longer than the last needs no second host round trip
     int chunk = s->launch.chunk > 0 ? s->launch.chunk : std::max(1, std::max(s->last_iters, s->prev_iters));
     bool conv = false;
-    while (launched < max_iter && !conv) {
-        const int k = std::min(chunk, max_iter - launched);
+    // k iterations from iteration `from`: as graphs of 2^j iterations, largest
+    // first (a few graphs per launch plan serve every chunk length, so a step
+    // whose count has not been seen before captures nothing), or launch by launch
+    auto enqueue = [&](int from, int k) -> int {
         if (s->launch.graph && !s->prof) {
-            // k iterations as graphs of 2^j iterations, largest first: a few
-            // graphs per launch plan serve every chunk length, so a step whose
-            // count has not been seen before captures nothing
             for (int rest = k; rest > 0;) {
                 const int p = graph_piece(rest);
                 hipGraphExec_t ex;
@@ -623,7 +675,23 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
                 rest -= p;
             }
         } else {
-            for (int j = 0; j < k; ++j) VBD_TRY(enqueue_iteration(s, launched + j, ftol, s->prof));
+            for (int j = 0; j < k; ++j) VBD_TRY(enqueue_iteration(s, from + j, ftol, s->prof));
+        }
+        return 0;
+    };
+    while (launched < max_iter && !conv) {
+        const int k = std::min(chunk, max_iter - launched);
+        if (launched + k == max_iter) {
+            // the chunk runs the last allowed iteration: keep the iterate from
+            // before it, for the largest |dx| of that iteration
+            VBD_TRY(ensure_u_last(s));
+            VBD_TRY(enqueue(launched, k - 1));
+            if (s->n_swept > 0)
+                VBD_HIP(hipMemcpyAsync(s->u_last, s->u, sizeof(float4) * s->n_swept, hipMemcpyDeviceToDevice,
+                                       s->stream));
+            VBD_TRY(enqueue(launched + k - 1, 1));
+        } else {
+            VBD_TRY(enqueue(launched, k));
         }
         launched += k;
         VBD_HIP(hipMemcpyAsync(s->ctrl_host, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
@@ -651,7 +719,7 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
         }
         s->ev_pending.clear();
     }
-    return 0;
+    return report_dx_max(s, it, max_iter);
 }
 }  // namespace
 

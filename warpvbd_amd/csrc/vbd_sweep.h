@@ -557,7 +557,8 @@ struct Batch {
 struct StepParams {
     float dt, inv_dt, tol, kd_dt;
     int max_iter;
-    int skip_init;  // the initial guess is already in u (warm start)
+    int skip_init;            // the initial guess is already in u (warm start)
+    float4 *u_last = nullptr;  // if set: u before the last allowed iteration
 };
 
 // inertia target x_prev + dt v + dt^2 g of an active vertex (xyz) and M/dt^2 (w)
@@ -617,6 +618,11 @@ __global__ __launch_bounds__(1024) void k_step_resident(SweepArgs a, const int2 
     int it = 0;
     uint32_t bits = 0;
     for (; it < p.max_iter; ++it) {
+        if (p.u_last && it == p.max_iter - 1) {
+            for (int c = 0; c < C; ++c)
+                for (int i = r[c].x + threadIdx.x; i < r[c].x + r[c].y; i += blockDim.x) p.u_last[i] = a.u[i];
+            __syncthreads();
+        }
         uint32_t mb = 0;
         for (int c = 0; c < C; ++c) {
             const int2 rc = r[c];

@@ -104,16 +104,43 @@ def _mass_spring_env(nx, x0, y0):
     return vertices, hex2tets(hexes)
 
 
+def _mass_spring_grid(nsim, nx):
+    """The parallel scene as the reference builds it
+    (examples/run_mass_spring_parallel.py:35-58,66-95): one voxel grid with
+    nx voxels of padding between the environments' cubes and columns,
+    converted and split as a whole (so vertex numbering runs across the
+    environments), each element assigned to the environment its centroid
+    lies in.  Returns vertices, tets and each element's environment."""
+    h = 1.0 / nx
+    nsimy = int(np.ceil(np.sqrt(nsim)))
+    nsimx = int(np.ceil(nsim / nsimy))
+    pad = int(1.0 * nx)
+    vox = np.zeros((nsimx * (nx + pad) - pad, nsimy * (nx + pad) - pad, 2 * nx), dtype=bool)
+    for k in range(nsim):
+        x0, y0 = (k // nsimy) * (nx + pad), (k % nsimy) * (nx + pad)
+        vox[x0:x0 + nx, y0:y0 + nx, :nx] = True
+        vox[x0 + nx // 2, y0 + nx // 2, nx:] = True
+    vertices, hexes = voxel2hex(vox, h, h, h)
+    tets = hex2tets(hexes)
+    com = vertices[tets].mean(axis=1)
+    cell = (nx + pad) * h
+    env = (com[:, 0] // cell).astype(np.int64) * nsimy + (com[:, 1] // cell).astype(np.int64)
+    assert env.max() < nsim, "element outside every environment"
+    return vertices, tets, env
+
+
 class MassSpringParallelSim:
     """nsim mass-spring environments on an xy grid with nx voxels of padding
     between them, Young's modulus E (1 + U(-0.5, 2.5)) per environment
     (examples/run_mass_spring_parallel.py:27-127).  tip_idx maps each
-    environment to its probe vertices.
+    environment to its probe vertices (bottom face, z < 1e-3).
 
-    batched=False concatenates the environments into one mesh and solver with a
-    single convergence test, as the reference does; batched=True gives each
-    environment its own convergence test in a BatchedVBDSolver.  seed fixes the
-    per-environment moduli (the reference draws them unseeded)."""
+    batched=False builds the reference's scene: one padded voxel grid, one
+    mesh, one solver with a single convergence test (same vertex numbering,
+    so the same colouring and sweep order as the reference).  batched=True
+    gives each environment its own mesh and convergence test in a
+    BatchedVBDSolver.  seed fixes the per-environment moduli (the reference
+    draws them unseeded)."""
 
     def __init__(self, nsim=4, nx=7, density=1, youngs_modulus=2e3, poissons_ratio=0.45, dx_tol=1e-9,
                  max_iter=1000, device="cuda", batched=False, seed=0, material="snh", verbose=True):
@@ -125,34 +152,41 @@ class MassSpringParallelSim:
         rng = np.random.default_rng(seed)
         random_ym = rng.uniform(-0.5, 2.5, size=(nsim,))
         h = 1.0 / nx
-        bodies = []
-        for k in range(nsim):
-            i, j = k // nsimy, k % nsimy
-            V, T = _mass_spring_env(nx, i * (nx + pad), j * (nx + pad))
-            m = len(T)
-            active = ~(V[:, 2] > 2 * nx * h - 1e-3)
-            bodies.append(dict(initial_positions=V, elements=T, densities=density * np.ones(m),
-                               youngs_modulus=youngs_modulus * (1.0 + random_ym[k]) * np.ones(m),
-                               poissons_ratio=poissons_ratio * np.ones(m), active_mask=active,
-                               gravity=(0.0, 0.0, -9.81)))
-        cb = concat_bodies(bodies)
-        self.initial_positions, self.elements = cb["initial_positions"], cb["elements"]
-        self.youngs_moduli = cb["youngs_modulus"]
-        voff = cb["body_vertex_offsets"]
-        self.vertex_sim_env = np.repeat(np.arange(nsim), np.diff(voff))
-        Z = self.initial_positions[:, 2]
-        act = cb["active_mask"]
-        self.tip_idx = {k: list(voff[k] + np.nonzero(act[voff[k]:voff[k + 1]] & (Z[voff[k]:voff[k + 1]] < 1e-3))[0])
-                        for k in range(nsim)}
         self.batched = batched
         if batched:
+            bodies = []
+            for k in range(nsim):
+                i, j = k // nsimy, k % nsimy
+                V, T = _mass_spring_env(nx, i * (nx + pad), j * (nx + pad))
+                m = len(T)
+                active = ~(V[:, 2] > 2 * nx * h - 1e-3)
+                bodies.append(dict(initial_positions=V, elements=T, densities=density * np.ones(m),
+                                   youngs_modulus=youngs_modulus * (1.0 + random_ym[k]) * np.ones(m),
+                                   poissons_ratio=poissons_ratio * np.ones(m), active_mask=active,
+                                   gravity=(0.0, 0.0, -9.81)))
+            cb = concat_bodies(bodies)
+            self.initial_positions, self.elements = cb["initial_positions"], cb["elements"]
+            self.youngs_moduli = cb["youngs_modulus"]
+            voff = cb["body_vertex_offsets"]
+            self.vertex_sim_env = np.repeat(np.arange(nsim), np.diff(voff))
+            act = cb["active_mask"]
             self.solver = BatchedVBDSolver(bodies, damping_coefficient=0.0, dx_tol=dx_tol, max_iter=max_iter,
                                            device=device, material=material, verbose=verbose)
         else:
-            self.solver = VBDSolver(cb["initial_positions"], cb["elements"], cb["densities"], cb["youngs_modulus"],
-                                    cb["poissons_ratio"], damping_coefficient=0.0, active_mask=act,
-                                    gravity=(0.0, 0.0, -9.81), dx_tol=dx_tol, max_iter=max_iter, device=device,
-                                    material=material, verbose=verbose)
+            V, T, env = _mass_spring_grid(nsim, nx)
+            m = len(T)
+            self.initial_positions, self.elements = V, T
+            self.element_sim_env = env
+            self.vertex_sim_env = -np.ones(len(V), dtype=np.int64)
+            self.vertex_sim_env[T.reshape(-1)] = np.repeat(env, 4)
+            self.youngs_moduli = youngs_modulus * (1.0 + random_ym[env])
+            act = ~(V[:, 2] > 2 * nx * h - 1e-3)
+            self.solver = VBDSolver(V, T, density * np.ones(m), self.youngs_moduli, poissons_ratio * np.ones(m),
+                                    damping_coefficient=0.0, active_mask=act, gravity=(0.0, 0.0, -9.81),
+                                    dx_tol=dx_tol, max_iter=max_iter, device=device, material=material,
+                                    verbose=verbose)
+        Z = self.initial_positions[:, 2]
+        self.tip_idx = {k: list(np.nonzero(act & (Z < 1e-3) & (self.vertex_sim_env == k))[0]) for k in range(nsim)}
 
     def step(self, dt):
         return self.solver.step(self.solver.old_positions, float(dt))
@@ -270,6 +304,8 @@ def benchmark_mesh_size(reduction_factors=tuple(range(1, 11)), num_samples=1, n_
         metrics["num_elements"].append(len(sim.elements))
         metrics["times"]["vbd_solve"].append(times["vbd_solve"])
         metrics["tip_positions"].append(np.array(trace))
+        print(f"  rf {rf}: {len(sim.initial_positions)} vertices, "
+              f"{float(np.mean(times['vbd_solve'])) / (timestep * n_timesteps):.3f} wall s per sim s", flush=True)
     metrics["time_per_sim_second"] = [float(np.mean(t)) / (timestep * n_timesteps)
                                       for t in metrics["times"]["vbd_solve"]]
     if out:
@@ -306,19 +342,28 @@ def benchmark_parallel(nsims=tuple(2 ** i for i in range(13)), num_samples=3, n_
 
 
 def benchmark_coloring(num_voxels=(5, 8, 12, 16, 20), num_samples=5, out=None):
-    """Time of the host first-fit colouring (graph_coloring) and of the GPU
-    colouring (graph_coloring_wp) on nx^3-voxel cubes
-    (examples/benchmark_graph_coloring.py:24-45).  Both return the same
-    colouring."""
-    from .coloring import graph_coloring, graph_coloring_wp
+    """Time of the reference's two colouring interfaces on nx^3-voxel cubes,
+    both from the element array (examples/benchmark_graph_coloring.py:24-45):
+    "numpy" is graph_coloring on the adjacency dict compute_adjacency_dict
+    builds (graph_coloring takes that dict, src/coloring.py:40; the dict is
+    built and flattened on the host, the first-fit rounds run on the GPU),
+    "parallel" is graph_coloring_wp on the elements (adjacency and colouring
+    on the GPU).  The two colourings are checked equal at every size."""
+    from .coloring import compute_adjacency_dict, graph_coloring, graph_coloring_wp
+
+    def graph_coloring_host(elements):
+        return graph_coloring(compute_adjacency_dict(elements)[0], verbose=False)[0]
+
     metrics = {"num_vertices": [], "num_elements": [], "times": {"numpy": [], "parallel": []}}
     for nx in num_voxels:
         V, H = voxel2hex(np.ones((nx, nx, nx), dtype=bool), 1.0, 1.0, 1.0)
         T = hex2tets(H)
-        times = benchmark_functions([graph_coloring, graph_coloring_wp], T, num_samples=num_samples)
+        if not np.array_equal(graph_coloring_host(T), graph_coloring_wp(T)):
+            raise AssertionError(f"host and GPU colourings differ on the {nx}^3 cube")
+        times = benchmark_functions([graph_coloring_host, graph_coloring_wp], T, num_samples=num_samples)
         metrics["num_vertices"].append(len(V))
         metrics["num_elements"].append(len(T))
-        metrics["times"]["numpy"].append(times["graph_coloring"])
+        metrics["times"]["numpy"].append(times["graph_coloring_host"])
         metrics["times"]["parallel"].append(times["graph_coloring_wp"])
     if out:
         np.savez_compressed(out, num_vertices=np.array(metrics["num_vertices"]),

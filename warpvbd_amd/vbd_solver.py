@@ -140,6 +140,7 @@ class VBDSolver:
         self.active_mask = np.ones(n, dtype=bool) if act is None else act.astype(bool)
         self.n_vertices, self.n_elements = n, m
         self.last_iterations, self.last_metric = 0, 0.0
+        self.last_dx_max = float("nan")  # set by a step that ran its last allowed iteration
         self.last_used_network = False
         self._version = 0
         self._views = weakref.WeakSet()
@@ -287,10 +288,20 @@ class VBDSolver:
                              ctypes.byref(met)), "vbd_step")
             self.last_used_network = False
         self.last_iterations, self.last_metric = it.value, met.value
-        if it.value == max_iter:  # the reference warns when the last allowed iteration ran (:581-582)
-            print(f"Warning: VBD solver did not converge within the maximum number of iterations. "
-                  f"Final dx metric: {met.value}")
+        if it.value == max_iter:
+            self._warn_not_converged()
+        else:
+            self.last_dx_max = float("nan")
         return self._view()
+
+    def _warn_not_converged(self):
+        """The reference's warning when the last allowed iteration ran, with
+        the largest |dx| component of that iteration (src/vbd_solver.py:581-582)."""
+        d = ctypes.c_double()
+        check(_lib.load().vbd_last_dx_max(self._h, ctypes.byref(d)), "vbd_last_dx_max")
+        self.last_dx_max = d.value
+        print(f"Warning: VBD solver did not converge within the maximum number of iterations. "
+              f"Final dx max: {d.value}")
 
     def _step_tensor(self, positions, dt, dx_tol, max_iter):
         """step() for an [N, 3] tensor on the solver's GPU: positions in and the
@@ -310,8 +321,9 @@ class VBDSolver:
         self.last_iterations, self.last_metric = it.value, met.value
         self.last_used_network = False
         if it.value == max_iter:
-            print(f"Warning: VBD solver did not converge within the maximum number of iterations. "
-                  f"Final dx metric: {met.value}")
+            self._warn_not_converged()
+        else:
+            self.last_dx_max = float("nan")
         return out
 
     # ------------------------------------------------------------------ material / gravity
