@@ -328,6 +328,9 @@ def benchmark_parallel(nsims=tuple(2 ** i for i in range(13)), num_samples=3, n_
         metrics["n_vertices"].append(len(sim.initial_positions))
         metrics["n_simulations"].append(nsim)
         metrics["times"]["vbd_solve"].append(times["vbd_solve"])
+        print(f"  {nsim} environments ({len(sim.initial_positions)} vertices): "
+              f"{float(np.mean(times['vbd_solve'])) / (timestep * n_timesteps):.3f} wall s per sim s", flush=True)
+        del sim
     sim_s = timestep * n_timesteps
     metrics["time_per_sim_second"] = [float(np.mean(t)) / sim_s for t in metrics["times"]["vbd_solve"]]
     metrics["realtime_factor_per_env"] = [n * sim_s / float(np.mean(t))
