@@ -215,6 +215,14 @@ void dispatch_sweep_tasks(int L, const SweepArgs &a, const int32_t *body_new, co
     }
 }
 
+// Lanes per vertex of colour c's launch: colours below launch.small_color
+// vertices are latency-bound and get 32 (one slot per lane), the others the
+// tuned count.
+int color_lanes(const vbd_solver *s, int c)
+{
+    return s->col_size[c] < s->launch.small_color ? 32 : s->launch.lanes;
+}
+
 // The staged kernel for colour c when the solver has staging tables for it.
 bool staged_color(const vbd_solver *s, int c) { return s->st_ok && c < (int)s->st_blk0.size() && s->st_blk0[c] >= 0; }
 
@@ -296,10 +304,11 @@ int ensure_batch_buffers(vbd_solver *s)
 int enqueue_body_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
 {
     const SweepArgs a = sweep_args(s, kd_dt);
-    const int L = s->launch.lanes, block = s->launch.block;
+    const int block = s->launch.block;
     for (int c = 0; c < s->ncolors; ++c) {
         const int nv = s->col_size[c];
         if (!nv) continue;
+        const int L = color_lanes(s, c);
         if (staged_color(s, c))
             launch_staged_bodies(s, a, c, nullptr, 0);
         else if (s->material == kSNH)
@@ -318,11 +327,11 @@ int enqueue_body_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
 
 // Task lists for the bodies not yet done (bstate_host must be current): per
 // colour, the bodies' colour ranges with adjacent ranges merged, cut into
-// blocks of block / L vertices.
+// blocks of block / L vertices (L the colour's lanes per vertex).
 int build_tasks(vbd_solver *s)
 {
     const int C = s->ncolors, nb = s->n_bodies;
-    const int vpb = std::max(1, s->launch.block / s->launch.lanes);
+    auto vpb_of = [&](int c) { return std::max(1, s->launch.block / color_lanes(s, c)); };
     s->tasks_host.clear();
     s->task_off.assign(C + 1, 0);
     s->task_vertices = 0;
@@ -345,6 +354,7 @@ int build_tasks(vbd_solver *s)
             }
             continue;
         }
+        const int vpb = vpb_of(c);
         int run0 = -1, run1 = -1;
         auto flush = [&]() {
             for (int v = run0; v < run1; v += vpb) s->tasks_host.push_back(make_int2(v, std::min(vpb, run1 - v)));
@@ -371,7 +381,7 @@ int build_tasks(vbd_solver *s)
     size_t nt = std::max<size_t>(1, s->tasks_host.size());
     if (nt > s->task_cap) {
         size_t cap = 16;
-        for (int c = 0; c < C; ++c) cap += (size_t)s->col_size[c] / vpb + nb + 1;
+        for (int c = 0; c < C; ++c) cap += (size_t)s->col_size[c] / vpb_of(c) + nb + 1;
         nt = std::max(nt, cap);
         if (s->d_tasks) VBD_HIP(hipFree(s->d_tasks));
         s->d_tasks = nullptr;
@@ -388,10 +398,11 @@ int build_tasks(vbd_solver *s)
 int enqueue_task_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
 {
     const SweepArgs a = sweep_args(s, kd_dt);
-    const int L = s->launch.lanes, block = s->launch.block;
+    const int block = s->launch.block;
     for (int c = 0; c < s->ncolors; ++c) {
         const int t0 = s->task_off[c], nt = s->task_off[c + 1] - t0;
         if (!nt) continue;
+        const int L = color_lanes(s, c);
         if (staged_color(s, c))
             launch_staged_bodies(s, a, c, s->d_tasks + t0, nt);
         else if (s->material == kSNH)
