@@ -7,13 +7,18 @@
 // body keeps its own convergence state, and a body that has converged drops
 // out of the remaining sweeps of the step.  Two execution forms:
 //
-//   colour launches  one launch per colour across every body, then a check
-//                    launch (one wave per body); the iterations are captured
-//                    in a hipGraph and replayed in chunks, the host waiting
-//                    once per chunk on a device count of live bodies.  Rows
-//                    of kStageRows vertices of a colour whose bodies have all
-//                    converged are skipped by the launches (a live count per
-//                    row, kept by the check launch).
+//   colour launches  one launch per colour across every body; the first
+//                    launch of an iteration also retires the bodies that
+//                    converged in the iteration before (each vertex takes the
+//                    same decision from the previous iteration's per-body
+//                    maxima; extra waves, one per body, write the new state),
+//                    and a run of iterations ends with one closing launch.
+//                    Runs are captured in hipGraphs and replayed in chunks,
+//                    the host waiting once per chunk on a device count of
+//                    live bodies.  Rows of kStageRows vertices of a colour
+//                    whose bodies have all converged are skipped by the
+//                    launches (a live count per row, kept by whichever
+//                    launch retires a body).
 //   resident         one workgroup per body runs its whole step
 //                    (k_step_resident), one launch per step
 //
@@ -42,11 +47,66 @@ namespace {
 // host and another submission.
 constexpr int kFullChunkMargin = VBD_FULL_CHUNK_MARGIN;
 
-template <int L, int B, int MAT>
-__global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_t *body_new, const BodyState *bs,
-                                                       uint32_t *bmet, const int32_t *rows, int v0, int nv)
+// The per-body bookkeeping of an iteration, done by extra waves (one per
+// body) in the iteration's first colour launch: in an iteration after the
+// first of a run (ph.fold) the body's new state from the previous
+// iteration's state and maxima (the same decision its vertices take in that
+// launch; a body already done is copied), and for a body that retires the
+// live counts of its rows and the live-body count; in every iteration the
+// maxima array of the iteration after next is cleared.  The launch's first
+// nfold blocks do this (dispatched first, they run beside the colour's own
+// blocks instead of after them).
+struct BodyFold {
+    BodyState *next = nullptr;
+    uint32_t *mclear = nullptr;
+    int32_t *n_live = nullptr;
+    const int32_t *row_off = nullptr, *rows = nullptr;
+    int32_t *row_live = nullptr;
+    int nb = 0;
+    int nfold = 0;
+};
+
+// A body's state after an iteration with maxima m (s: before it).
+__device__ __forceinline__ BodyState body_after(const BodyState &s, uint32_t m, float tol, int max_iter, bool &retire)
 {
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    retire = false;
+    if (s.done) return s;
+    retire = body_retires(s, m, tol, max_iter);
+    return BodyState{retire ? 1 : 0, s.iters + 1, m, 0};
+}
+
+__device__ __forceinline__ void retire_rows(int b, int lane, const int32_t *row_off, const int32_t *rows,
+                                            int32_t *row_live)
+{
+    for (int i = row_off[b] + lane; i < row_off[b + 1]; i += 64) atomicSub(&row_live[rows[i]], 1);
+}
+
+__device__ void fold_bodies(const BodyPhase &ph, const BodyFold &f)
+{
+    const int b = (int)(((int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x) >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    if (b >= f.nb) return;
+    if (lane == 0) f.mclear[b] = 0u;
+    if (!ph.fold) return;
+    // every lane reads the same state; lane 0 writes the new one
+    bool retire;
+    const BodyState o = body_after(ph.prev[b], ph.mprev[b], ph.tol, ph.max_iter, retire);
+    if (lane == 0) {
+        f.next[b] = o;
+        if (retire) atomicSub(f.n_live, 1);
+    }
+    if (retire) retire_rows(b, lane, f.row_off, f.rows, f.row_live);
+}
+
+template <int L, int B, int MAT>
+__global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_t *body_new, BodyPhase ph,
+                                                       BodyFold f, const int32_t *rows, int v0, int nv)
+{
+    if ((int)blockIdx.x < f.nfold) {
+        fold_bodies(ph, f);
+        return;
+    }
+    const int b = xcd_remap((int)blockIdx.x - f.nfold, (int)gridDim.x - f.nfold);
     const int t = b * blockDim.x + threadIdx.x;
     const int vi = t / L, lane = t % L;
     const bool in = vi < nv;
@@ -61,20 +121,19 @@ __global__ __launch_bounds__(1024) void k_sweep_bodies(SweepArgs a, const int32_
     const int live = rows[__builtin_amdgcn_readfirstlane(min(vi, nv - 1) / kStageRows)];
     asm volatile("" ::"v"(body), "s"(live) : "memory");
     if (live == 0) return;
-    // read by every lane (v is a swept vertex for all), so no branch around the load
-    const int done = bs[body].done;
-    const bool valid = in && !done;
+    // read by every lane (v is a swept vertex for all), so no branch around the loads
+    const bool valid = in && !body_done(ph, body);
     const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane, pre);
-    body_metric_max(valid ? body : -1, m, bmet);
+    body_metric_max(valid ? body : -1, m, ph.macc);
 }
 
 // Colour sweep over a task list: block b updates the vertices
 // [tasks[b].x, tasks[b].x + tasks[b].y) (at most blockDim.x / L of them) of
 // bodies still live.  Used once most bodies have converged, so a launch
-// covers only their ranges.
+// covers only their ranges (one iteration per run: ph.fold is 0).
 template <int L, int B, int MAT>
-__global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t *body_new, const BodyState *bs,
-                                                      uint32_t *bmet, const int2 *tasks)
+__global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t *body_new, BodyPhase ph,
+                                                      const int2 *tasks)
 {
     const int2 t = tasks[xcd_remap(blockIdx.x, gridDim.x)];
     const int vi = threadIdx.x / L, lane = threadIdx.x % L;
@@ -83,9 +142,9 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t
     // the vertex's own loads go out with the body lookup, not after it
     const VertexPre pre = vertex_pre(a, v, in);
     const int body = body_new[v];
-    const bool valid = in && !bs[body].done;
+    const bool valid = in && !body_done(ph, body);
     const uint32_t m = update_vertex<L, B, MAT>(a, v, valid, lane, pre);
-    body_metric_max(valid ? body : -1, m, bmet);
+    body_metric_max(valid ? body : -1, m, ph.macc);
 }
 
 // A staged colour (staged_sweep in vbd_sweep.h) across every body: 256
@@ -95,10 +154,15 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t
 template <int MAT>
 __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0, int nv, const int32_t *st_nb,
                                                              const uint2 *st_rec, int blk0, int n_dict,
-                                                             const int32_t *body_new, const BodyState *bs,
-                                                             const int32_t *rows, uint32_t *bmet, const int2 *blist)
+                                                             const int32_t *body_new, BodyPhase ph, BodyFold f,
+                                                             const int32_t *rows, const int2 *blist)
 {
-    const int b = blist ? blist[blockIdx.x].x : xcd_remap(blockIdx.x, gridDim.x);
+    if ((int)blockIdx.x < f.nfold) {
+        fold_bodies(ph, f);
+        return;
+    }
+    const int i = (int)blockIdx.x - f.nfold;
+    const int b = blist ? blist[i].x : xcd_remap(i, (int)gridDim.x - f.nfold);
     uint32_t m;
     int body;
     // The block's maxima of its first kFoldBodies bodies gather in LDS and go
@@ -111,41 +175,32 @@ __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0
     if (threadIdx.x < kFoldBodies) s_bm[threadIdx.x] = 0u;
     const int b0 = body_new[v0 + b * kStageRows];
     // (the whole block returns together)
-    if (!staged_sweep<MAT, true>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, body_new, bs, rows, body, m)) return;
-    body_metric_max(body, m, bmet, s_bm, b0, kFoldBodies);
+    if (!staged_sweep<MAT, true>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, body_new, &ph, rows, body, m)) return;
+    body_metric_max(body, m, ph.macc, s_bm, b0, kFoldBodies);
     __syncthreads();
-    if (threadIdx.x < kFoldBodies && s_bm[threadIdx.x]) atomicMax(&bmet[b0 + threadIdx.x], s_bm[threadIdx.x]);
+    if (threadIdx.x < kFoldBodies && s_bm[threadIdx.x]) atomicMax(&ph.macc[b0 + threadIdx.x], s_bm[threadIdx.x]);
 }
 
-// After an iteration's colour launches: per body (one wave each) the
-// iteration's max |dx_x + dx_y + dx_z| (bmet, folded by the launches with
-// body_metric_max), its iteration count, and the body retires when it
-// converged or used its iterations; a body that retires leaves the live
-// counts of its rows (the wave's lanes share them out).  bmet is cleared for
-// the next iteration.
-__global__ __launch_bounds__(256) void k_check_bodies(int nb, uint32_t *bmet, BodyState *bs, int max_iter, float tol,
-                                                      int32_t *n_live, const int32_t *body_row_off,
-                                                      const int32_t *body_rows, int32_t *row_live)
+// After a run of iterations (one wave per body): the bodies' state after its
+// last iteration from the state before it (prev) and its maxima (mprev), into
+// out, the state the next run starts from; the maxima array of the next run's
+// first iteration (mclear) is cleared.  prev and out, mprev and mclear may be
+// the same arrays (each body's entry is read before it is written).
+__global__ __launch_bounds__(256) void k_close_bodies(int nb, const BodyState *prev, const uint32_t *mprev,
+                                                      BodyState *out, uint32_t *mclear, float tol, int max_iter,
+                                                      int32_t *n_live, const int32_t *row_off,
+                                                      const int32_t *rows, int32_t *row_live)
 {
     const int b = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
     if (b >= nb) return;
-    BodyState *st = bs + b;
-    // every lane reads the same state; lane 0 writes it back
-    if (st->done) return;
-    const uint32_t mx = bmet[b];
-    const int it = st->iters + 1;
-    const bool retire = __uint_as_float(mx) < tol || it >= max_iter;
+    bool retire;
+    const BodyState o = body_after(prev[b], mprev[b], tol, max_iter, retire);
     if (lane == 0) {
-        bmet[b] = 0u;
-        st->iters = it;
-        st->metric = mx;
-        if (retire) {
-            st->done = 1;
-            atomicSub(n_live, 1);
-        }
+        out[b] = o;
+        mclear[b] = 0u;
+        if (retire) atomicSub(n_live, 1);
     }
-    if (retire)
-        for (int i = body_row_off[b] + lane; i < body_row_off[b + 1]; i += 64) atomicSub(&row_live[body_rows[i]], 1);
+    if (retire) retire_rows(b, lane, row_off, rows, row_live);
 }
 
 __global__ void k_init_bodies(int n, float4 *u, const double4 *uprev, const float4 *vel, double4 *tgt, const float *mass,
@@ -154,17 +209,19 @@ __global__ void k_init_bodies(int n, float4 *u, const double4 *uprev, const floa
                               int32_t *row_live)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) {
-        bs[i] = BodyState{0, 0, 0u, 0};
-        bmet[i] = 0u;
-    }
+    if (i < nb) bs[i] = BodyState{0, 0, 0u, 0};
+    if (i < 3 * nb) bmet[i] = 0u;
     if (i < nrows) row_live[i] = row_live0[i];
     if (i == 0) *n_live = nb;
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, grav[body_new[i]]);
 }
 
-__global__ void k_finish_all(int n, const float4 *u, double4 *uprev, float4 *vel, float dt)
+// The velocity update once every body is done (else nothing): enqueued behind
+// every chunk of iterations, so it follows the last chunk without a wait on
+// the host.
+__global__ void k_finish_all_if(int n, const float4 *u, double4 *uprev, float4 *vel, float dt, const int32_t *n_live)
 {
+    if (*n_live != 0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 x = unpack_disp4(u[i]), p = uprev[i];
@@ -173,45 +230,46 @@ __global__ void k_finish_all(int n, const float4 *u, double4 *uprev, float4 *vel
 }
 
 template <int L, int MAT>
-void launch_sweep_bodies(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
+void launch_sweep_bodies(const SweepArgs &a, const int32_t *body_new, const BodyPhase &ph, BodyFold f,
                          const int32_t *rows, int v0, int nv, int block, hipStream_t st)
 {
-    k_sweep_bodies<L, Batch<L>::value, MAT><<<nblk((int64_t)nv * L, block), block, 0, st>>>(a, body_new, bs, bmet,
-                                                                                            rows, v0, nv);
+    const int nmain = nblk((int64_t)nv * L, block);
+    f.nfold = f.nb > 0 ? nblk((int64_t)f.nb * 64, block) : 0;
+    k_sweep_bodies<L, Batch<L>::value, MAT><<<nmain + f.nfold, block, 0, st>>>(a, body_new, ph, f, rows, v0, nv);
 }
 
 template <int MAT>
-void dispatch_sweep_bodies(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
-                           const int32_t *rows, int v0, int nv, int block, hipStream_t st)
+void dispatch_sweep_bodies(int L, const SweepArgs &a, const int32_t *body_new, const BodyPhase &ph,
+                           const BodyFold &f, const int32_t *rows, int v0, int nv, int block, hipStream_t st)
 {
     switch (L) {
-    case 1: launch_sweep_bodies<1, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
-    case 2: launch_sweep_bodies<2, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
-    case 8: launch_sweep_bodies<8, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
-    case 16: launch_sweep_bodies<16, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
-    case 32: launch_sweep_bodies<32, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
-    default: launch_sweep_bodies<4, MAT>(a, body_new, bs, bmet, rows, v0, nv, block, st); break;
+    case 1: launch_sweep_bodies<1, MAT>(a, body_new, ph, f, rows, v0, nv, block, st); break;
+    case 2: launch_sweep_bodies<2, MAT>(a, body_new, ph, f, rows, v0, nv, block, st); break;
+    case 8: launch_sweep_bodies<8, MAT>(a, body_new, ph, f, rows, v0, nv, block, st); break;
+    case 16: launch_sweep_bodies<16, MAT>(a, body_new, ph, f, rows, v0, nv, block, st); break;
+    case 32: launch_sweep_bodies<32, MAT>(a, body_new, ph, f, rows, v0, nv, block, st); break;
+    default: launch_sweep_bodies<4, MAT>(a, body_new, ph, f, rows, v0, nv, block, st); break;
     }
 }
 
 template <int L, int MAT>
-void launch_sweep_tasks(const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
-                        const int2 *tasks, int ntasks, int block, hipStream_t st)
+void launch_sweep_tasks(const SweepArgs &a, const int32_t *body_new, const BodyPhase &ph, const int2 *tasks,
+                        int ntasks, int block, hipStream_t st)
 {
-    k_sweep_tasks<L, Batch<L>::value, MAT><<<ntasks, block, 0, st>>>(a, body_new, bs, bmet, tasks);
+    k_sweep_tasks<L, Batch<L>::value, MAT><<<ntasks, block, 0, st>>>(a, body_new, ph, tasks);
 }
 
 template <int MAT>
-void dispatch_sweep_tasks(int L, const SweepArgs &a, const int32_t *body_new, const BodyState *bs, uint32_t *bmet,
+void dispatch_sweep_tasks(int L, const SweepArgs &a, const int32_t *body_new, const BodyPhase &ph,
                           const int2 *tasks, int ntasks, int block, hipStream_t st)
 {
     switch (L) {
-    case 1: launch_sweep_tasks<1, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
-    case 2: launch_sweep_tasks<2, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
-    case 8: launch_sweep_tasks<8, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
-    case 16: launch_sweep_tasks<16, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
-    case 32: launch_sweep_tasks<32, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
-    default: launch_sweep_tasks<4, MAT>(a, body_new, bs, bmet, tasks, ntasks, block, st); break;
+    case 1: launch_sweep_tasks<1, MAT>(a, body_new, ph, tasks, ntasks, block, st); break;
+    case 2: launch_sweep_tasks<2, MAT>(a, body_new, ph, tasks, ntasks, block, st); break;
+    case 8: launch_sweep_tasks<8, MAT>(a, body_new, ph, tasks, ntasks, block, st); break;
+    case 16: launch_sweep_tasks<16, MAT>(a, body_new, ph, tasks, ntasks, block, st); break;
+    case 32: launch_sweep_tasks<32, MAT>(a, body_new, ph, tasks, ntasks, block, st); break;
+    default: launch_sweep_tasks<4, MAT>(a, body_new, ph, tasks, ntasks, block, st); break;
     }
 }
 
@@ -226,19 +284,65 @@ int color_lanes(const vbd_solver *s, int c)
 // The staged kernel for colour c when the solver has staging tables for it.
 bool staged_color(const vbd_solver *s, int c) { return s->st_ok && c < (int)s->st_blk0.size() && s->st_blk0[c] >= 0; }
 
-void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *blist, int nblocks)
+void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *blist, int nblocks,
+                          const BodyPhase &ph, BodyFold f)
 {
     const int v0 = s->col_start[c], nv = s->col_size[c];
-    const dim3 grid(blist ? nblocks : nblk(nv, kStageRows));
-    if (grid.x == 0) return;
+    const int nmain = blist ? nblocks : nblk(nv, kStageRows);
+    f.nfold = f.nb > 0 ? nblk((int64_t)f.nb * 64, 256) : 0;
+    if (nmain + f.nfold == 0) return;
+    const dim3 grid(nmain + f.nfold);
+    const int32_t *rows = s->row_live + s->row_base[c];
     if (s->material == kSNH)
         k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
-                                                                s->n_dict, s->body_new, s->bstate,
-                                                                s->row_live + s->row_base[c], s->dxm, blist);
+                                                                s->n_dict, s->body_new, ph, f, rows, blist);
     else
         k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
-                                                                 s->n_dict, s->body_new, s->bstate,
-                                                                 s->row_live + s->row_base[c], s->dxm, blist);
+                                                                 s->n_dict, s->body_new, ph, f, rows, blist);
+}
+
+// Per-body state and maxima of iteration k of a run: two state arrays
+// (iteration k reads array k % 2) and three maxima arrays (k % 3), so that
+// an iteration's first launch can retire bodies from the previous
+// iteration's arrays while writing the next ones.  Array 0 holds the state a
+// run starts from and the host reads.
+BodyState *body_state(vbd_solver *s, int k) { return s->bstate + (size_t)(k & 1) * s->n_bodies; }
+uint32_t *body_maxima(vbd_solver *s, int k) { return s->dxm + (size_t)(k % 3) * s->n_bodies; }
+
+BodyPhase body_phase(vbd_solver *s, int k, float tol, int max_iter)
+{
+    BodyPhase ph;
+    ph.cur = body_state(s, k);
+    ph.macc = body_maxima(s, k);
+    ph.tol = tol;
+    ph.max_iter = max_iter;
+    if (k > 0) {
+        ph.prev = body_state(s, k - 1);
+        ph.mprev = body_maxima(s, k - 1);
+        ph.fold = 1;
+    }
+    return ph;
+}
+
+BodyFold body_fold(vbd_solver *s, int k)
+{
+    BodyFold f;
+    f.next = body_state(s, k);
+    f.mclear = body_maxima(s, k + 1);
+    f.n_live = s->n_live;
+    f.row_off = s->body_row_off;
+    f.rows = s->body_rows;
+    f.row_live = s->row_live;
+    f.nb = s->n_bodies;
+    return f;
+}
+
+// Close a run of k iterations: the state after its last one into array 0.
+void close_run(vbd_solver *s, int k, float tol, int max_iter)
+{
+    k_close_bodies<<<nblk((int64_t)s->n_bodies * 64, 256), 256, 0, s->stream>>>(
+        s->n_bodies, body_state(s, k - 1), body_maxima(s, k - 1), body_state(s, 0), body_maxima(s, 0), tol, max_iter,
+        s->n_live, s->body_row_off, s->body_rows, s->row_live);
 }
 
 SweepArgs sweep_args(vbd_solver *s, float kd_dt)
@@ -262,9 +366,9 @@ SweepArgs sweep_args(vbd_solver *s, float kd_dt)
 int ensure_batch_buffers(vbd_solver *s)
 {
     if (s->bstate) return 0;
-    VBD_HIP(hipMalloc((void **)&s->bstate, sizeof(BodyState) * s->n_bodies));
-    VBD_HIP(hipMalloc((void **)&s->dxm, sizeof(uint32_t) * s->n_bodies));
-    VBD_HIP(hipMemset(s->dxm, 0, sizeof(uint32_t) * s->n_bodies));
+    VBD_HIP(hipMalloc((void **)&s->bstate, sizeof(BodyState) * 2 * s->n_bodies));
+    VBD_HIP(hipMalloc((void **)&s->dxm, sizeof(uint32_t) * 3 * s->n_bodies));
+    VBD_HIP(hipMemset(s->dxm, 0, sizeof(uint32_t) * 3 * s->n_bodies));
     // rows of kStageRows vertices per colour, and per body the rows that hold
     // its vertices (a body's colour range is contiguous)
     const int C = s->ncolors, nb = s->n_bodies;
@@ -300,27 +404,38 @@ int ensure_batch_buffers(vbd_solver *s)
     return 0;
 }
 
-// C colour launches + one check launch = one iteration of every live body.
-int enqueue_body_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
+// A run of k iterations of every live body: per iteration one launch per
+// colour, the first carrying the per-body bookkeeping of the iteration before
+// (extra waves), then one closing launch for the run.
+int enqueue_body_run(vbd_solver *s, int k, float kd_dt, float tol, int max_iter)
 {
     const SweepArgs a = sweep_args(s, kd_dt);
     const int block = s->launch.block;
-    for (int c = 0; c < s->ncolors; ++c) {
-        const int nv = s->col_size[c];
-        if (!nv) continue;
-        const int L = color_lanes(s, c);
-        if (staged_color(s, c))
-            launch_staged_bodies(s, a, c, nullptr, 0);
-        else if (s->material == kSNH)
-            dispatch_sweep_bodies<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->row_live + s->row_base[c],
-                                        s->col_start[c], nv, block, s->stream);
-        else
-            dispatch_sweep_bodies<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->row_live + s->row_base[c],
-                                         s->col_start[c], nv, block, s->stream);
+    for (int it = 0; it < k; ++it) {
+        // the first launch decides the retirements (fold), the others read
+        // the state its extra waves wrote
+        const BodyPhase ph0 = body_phase(s, it, tol, max_iter);
+        BodyPhase rest = ph0;
+        rest.fold = 0;
+        bool first = true;
+        for (int c = 0; c < s->ncolors; ++c) {
+            const int nv = s->col_size[c];
+            if (!nv) continue;
+            const BodyFold f = first ? body_fold(s, it) : BodyFold{};
+            const BodyPhase &ph = first ? ph0 : rest;
+            first = false;
+            const int L = color_lanes(s, c);
+            const int32_t *rows = s->row_live + s->row_base[c];
+            if (staged_color(s, c))
+                launch_staged_bodies(s, a, c, nullptr, 0, ph, f);
+            else if (s->material == kSNH)
+                dispatch_sweep_bodies<kSNH>(L, a, s->body_new, ph, f, rows, s->col_start[c], nv, block, s->stream);
+            else
+                dispatch_sweep_bodies<kStVK>(L, a, s->body_new, ph, f, rows, s->col_start[c], nv, block,
+                                             s->stream);
+        }
     }
-    k_check_bodies<<<nblk((int64_t)s->n_bodies * 64, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
-                                                                   s->n_live, s->body_row_off, s->body_rows,
-                                                                   s->row_live);
+    close_run(s, k, tol, max_iter);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -394,26 +509,25 @@ int build_tasks(vbd_solver *s)
     return 0;
 }
 
-// One iteration of the live bodies through the current task lists.
+// One iteration of the live bodies through the current task lists (a run
+// of one iteration).
 int enqueue_task_iteration(vbd_solver *s, float kd_dt, float tol, int max_iter)
 {
     const SweepArgs a = sweep_args(s, kd_dt);
     const int block = s->launch.block;
+    const BodyPhase ph = body_phase(s, 0, tol, max_iter);
     for (int c = 0; c < s->ncolors; ++c) {
         const int t0 = s->task_off[c], nt = s->task_off[c + 1] - t0;
         if (!nt) continue;
         const int L = color_lanes(s, c);
         if (staged_color(s, c))
-            launch_staged_bodies(s, a, c, s->d_tasks + t0, nt);
+            launch_staged_bodies(s, a, c, s->d_tasks + t0, nt, ph, BodyFold{});
         else if (s->material == kSNH)
-            dispatch_sweep_tasks<kSNH>(L, a, s->body_new, s->bstate, s->dxm, s->d_tasks + t0, nt, block, s->stream);
+            dispatch_sweep_tasks<kSNH>(L, a, s->body_new, ph, s->d_tasks + t0, nt, block, s->stream);
         else
-            dispatch_sweep_tasks<kStVK>(L, a, s->body_new, s->bstate, s->dxm, s->d_tasks + t0, nt, block,
-                                        s->stream);
+            dispatch_sweep_tasks<kStVK>(L, a, s->body_new, ph, s->d_tasks + t0, nt, block, s->stream);
     }
-    k_check_bodies<<<nblk((int64_t)s->n_bodies * 64, 256), 256, 0, s->stream>>>(s->n_bodies, s->dxm, s->bstate, max_iter, tol,
-                                                                   s->n_live, s->body_row_off, s->body_rows,
-                                                                   s->row_live);
+    close_run(s, 1, tol, max_iter);
     VBD_HIP(hipGetLastError());
     return 0;
 }
@@ -446,8 +560,7 @@ int get_graph(vbd_solver *s, int k, float kd_dt, float tol, int max_iter, hipGra
     g.lanes = s->launch.lanes;
     g.block = s->launch.block;
     VBD_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int j = 0; j < k && rc == 0; ++j) rc = enqueue_body_iteration(s, kd_dt, tol, max_iter);
+    const int rc = enqueue_body_run(s, k, kd_dt, tol, max_iter);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
     if (rc != 0) return rc;
@@ -468,7 +581,7 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
     VBD_TRY(ensure_batch_buffers(s));
     const float kd_dt = (float)(s->damping / dt);
     const float ftol = (float)tol;
-    k_init_bodies<<<nblk(std::max({s->n, s->n_bodies, s->n_rows}), 256), 256, 0, s->stream>>>(
+    k_init_bodies<<<nblk(std::max({s->n, 3 * s->n_bodies, s->n_rows}), 256), 256, 0, s->stream>>>(
         s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt, s->n_bodies,
         s->bstate, s->dxm, s->n_live, s->n_rows, s->row_live0, s->row_live);
     VBD_HIP(hipGetLastError());
@@ -500,13 +613,18 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
                 rest -= p;
             }
         } else {
-            for (int j = 0; j < k; ++j) VBD_TRY(enqueue_body_iteration(s, kd_dt, ftol, max_iter));
+            VBD_TRY(enqueue_body_run(s, k, kd_dt, ftol, max_iter));
         }
         launched += k;
+        // the velocity update if every body is done, then the live count and
+        // every body's state after the chunk, in one wait (after the last
+        // chunk that state is the step's result)
+        k_finish_all_if<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->n, s->u, s->uprev, s->vel, (float)dt,
+                                                                 s->n_live);
+        VBD_HIP(hipGetLastError());
         VBD_HIP(hipMemcpyAsync(s->n_live_host, s->n_live, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
-        if (compact)
-            VBD_HIP(hipMemcpyAsync(s->bstate_host, s->bstate, sizeof(BodyState) * s->n_bodies,
-                                   hipMemcpyDeviceToHost, s->stream));
+        VBD_HIP(hipMemcpyAsync(s->bstate_host, s->bstate, sizeof(BodyState) * s->n_bodies, hipMemcpyDeviceToHost,
+                               s->stream));
         VBD_HIP(hipStreamSynchronize(s->stream));
         if (*s->n_live_host == 0) break;
         if (compact) {
@@ -523,11 +641,6 @@ int step_bodies_launches(vbd_solver *s, double dt, double tol, int max_iter)
         // iterations as the last step's slowest body did
         chunk = s->launch.chunk > 0 ? s->launch.chunk : tasks ? std::max(2, prev_max - launched) : 2;
     }
-    k_finish_all<<<nblk(s->n, 256), 256, 0, s->stream>>>(s->n, s->u, s->uprev, s->vel, (float)dt);
-    VBD_HIP(hipGetLastError());
-    VBD_HIP(hipMemcpyAsync(s->bstate_host, s->bstate, sizeof(BodyState) * s->n_bodies, hipMemcpyDeviceToHost,
-                           s->stream));
-    VBD_HIP(hipStreamSynchronize(s->stream));
     int mx = 0;
     std::vector<int> its(s->n_bodies);
     for (int b = 0; b < s->n_bodies; ++b) {
@@ -556,7 +669,7 @@ int step_bodies_resident(vbd_solver *s, double dt, double tol, int max_iter)
     if (s->warm) {
         // initial guess outside the resident kernel: plain guess, then the network's where it wins
         VBD_TRY(ensure_batch_buffers(s));
-        k_init_bodies<<<nblk(std::max({s->n, s->n_bodies, s->n_rows}), 256), 256, 0, s->stream>>>(
+        k_init_bodies<<<nblk(std::max({s->n, 3 * s->n_bodies, s->n_rows}), 256), 256, 0, s->stream>>>(
             s->n, s->u, s->uprev, s->vel, s->tgt, s->mass, s->active, s->body_new, s->d_grav, (float)dt,
             s->n_bodies, s->bstate, s->dxm, s->n_live, s->n_rows, s->row_live0, s->row_live);
         VBD_HIP(hipGetLastError());

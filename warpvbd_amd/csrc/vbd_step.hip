@@ -12,7 +12,8 @@
 //                  device
 //   k_step_resident (vbd_sweep.h) the whole step in one workgroup when every
 //                  colour is small
-//   k_step_finish  v = (x - x_prev)/dt, x_prev = x (:609-610)
+//   k_step_finish_if  v = (x - x_prev)/dt, x_prev = x (:609-610), behind
+//                  every chunk, applied by the chunk that completes the step
 //
 // The iteration index and the converged flag live in device memory (Ctrl), so
 // the host enqueues chunks of iterations and waits once per chunk; launches
@@ -292,8 +293,14 @@ __global__ __launch_bounds__(1024) void k_dx_max(int n, const float4 *u, const f
     }
 }
 
-__global__ void k_step_finish(int n, const float4 *u, double4 *uprev, float4 *vel, float dt)
+// v = (x - x_prev)/dt, x_prev = x once the step is complete (converged, or
+// its last allowed iteration ran), else nothing: enqueued behind every chunk
+// of iterations, so the velocity update follows the last chunk without a wait
+// on the host.
+__global__ void k_step_finish_if(int n, const float4 *u, double4 *uprev, float4 *vel, float dt, const Ctrl *ctrl,
+                                 int max_iter)
 {
+    if (!ctrl->converged && ctrl->iters < max_iter) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 x = unpack_disp4(u[i]), p = uprev[i];
@@ -694,13 +701,15 @@ int step_device_run(vbd_solver *s, double dt, double tol, int max_iter, int *ite
             VBD_TRY(enqueue(launched, k));
         }
         launched += k;
+        // the velocity update if this chunk completed the step, then one wait
+        k_step_finish_if<<<nblk(n, 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, (float)dt, s->ctrl,
+                                                               max_iter);
+        VBD_HIP(hipGetLastError());
         VBD_HIP(hipMemcpyAsync(s->ctrl_host, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
         VBD_HIP(hipStreamSynchronize(s->stream));
         conv = s->ctrl_host->converged != 0;
         chunk = s->launch.chunk > 0 ? s->launch.chunk : 2;
     }
-    k_step_finish<<<nblk(n, 256), 256, 0, s->stream>>>(n, s->u, s->uprev, s->vel, (float)dt);
-    VBD_HIP(hipGetLastError());
     const int it = s->ctrl_host->iters;
     s->prev_iters = s->last_iters;
     s->last_iters = it;

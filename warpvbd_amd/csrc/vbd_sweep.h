@@ -398,6 +398,41 @@ __device__ __forceinline__ float3 eval_vertex(const SweepArgs &a, int v, bool va
 // Slot records a lane of the staged sweep holds: 32 incident tets / 4 lanes.
 constexpr int kStagedSlots = kMaxElementsPerVertex / 4;
 
+// Per-body convergence state seen by one iteration of a batched solver.  An
+// iteration reads its bodies' state from cur, except the first colour launch
+// of an iteration after the first of a run (fold), which decides each body's
+// retirement itself from the previous iteration's state and per-body maxima
+// (prev, mprev): every vertex of a body recomputes the same decision, and
+// extra waves of that launch write the new state (fold_bodies).  macc
+// receives this iteration's per-body maxima.
+struct BodyPhase {
+    const BodyState *cur = nullptr;
+    const BodyState *prev = nullptr;
+    const uint32_t *mprev = nullptr;
+    uint32_t *macc = nullptr;
+    float tol = 0.f;
+    int max_iter = 0;
+    int fold = 0;
+};
+
+// A live body retires after an iteration whose max |dx_x+dx_y+dx_z| fell
+// below tol, or that used its last allowed iteration.
+__device__ __forceinline__ bool body_retires(const BodyState &s, uint32_t m, float tol, int max_iter)
+{
+    return (__uint_as_float(m) < tol) | (s.iters + 1 >= max_iter);
+}
+
+// Whether body's vertices sit out this iteration (both loads are issued
+// without a branch between them).
+__device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
+{
+    if (!ph.fold) return ph.cur[body].done != 0;
+    const BodyState s = ph.prev[body];
+    const uint32_t m = ph.mprev[body];
+    return (s.done != 0) | body_retires(s, m, ph.tol, ph.max_iter);
+}
+
+
 // One block of the staged colour sweep: 256 threads, 4 lanes per vertex, the
 // vertices [v0 + 64 b, v0 + 64 b + 64) of a colour of nv.  Two memory rounds,
 // then arithmetic only:
@@ -421,7 +456,7 @@ constexpr int kStagedSlots = kMaxElementsPerVertex / 4;
 template <int MAT, bool BODIES>
 __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, int b, const int32_t *st_nb,
                                              const uint2 *st_rec, int blk0, int n_dict, const int32_t *body_new,
-                                             const BodyState *bst, const int32_t *rows, int &body_out, uint32_t &m)
+                                             const BodyPhase *ph, const int32_t *rows, int &body_out, uint32_t &m)
 {
     constexpr int L = 4;
     constexpr int kStageUnroll = kStageMax / 256;
@@ -466,7 +501,7 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     // round 2 (the body's flag is read by every thread, so that no branch
     // makes the neighbour loads wait for it)
     int done = 0;
-    if constexpr (BODIES) done = bst[body].done;
+    if constexpr (BODIES) done = body_done(*ph, body);
     float4 p[kStageUnroll];
 #pragma unroll
     for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
