@@ -232,13 +232,19 @@ int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t
  * per-tet arrays, src/vbd_solver.py:295-343, are what the records hold). */
 void vbd_use_record_dictionary(int32_t enable);
 int vbd_records(vbd_solver_t *h, int32_t *n_dictionary, int32_t *bytes_per_slot);
-/* Neighbour staging of single-body solvers with a shared dictionary: colours
- * of at least 40,960 vertices swept with 4 lanes per vertex and 256
- * threads read each block's distinct neighbours and the dictionary once into
- * LDS and an 8-byte record per slot (the results are the same bit for bit).
- * *n_colors: the colours that use it.  No counterpart in WarpVBD. */
+/* Neighbour staging with a shared dictionary: colours of at least 40,960
+ * vertices swept at the default launch (4 lanes per vertex, 256 threads) read
+ * each block's distinct neighbours and the dictionary once into LDS and an
+ * 8-byte record per slot (the results are the same bit for bit).  In a
+ * single-body solver those colours also give each vertex 4, 2 or 1 lanes by
+ * its slot count (more than 10, 6..10, at most 5), so that a lane walks at
+ * most 5 slots of a voxel mesh; the blocks of 1-lane vertices read their
+ * neighbours from global memory.  *n_colors: the colours that use staging.
+ * No counterpart in WarpVBD. */
 int vbd_staged_colors(vbd_solver_t *h, int32_t *n_colors);
-/* Neighbour staging in solvers created afterwards (process-wide; default on). */
+/* Neighbour staging in solvers created afterwards (process-wide; default on;
+ * off, the big colours keep their lanes per vertex and read every neighbour
+ * from global memory). */
 void vbd_use_neighbour_staging(int32_t enable);
 /* bytes[n_colors]: the HBM bytes one sweep of each colour reads and writes
  * with this layout, sum_v (68 + bytes_per_slot k_v) + 16 |gathered neighbours|:

@@ -19,7 +19,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32.
+# -ffp-contract=on: a multiply and an add are fused only within one source
+# expression, so the same arithmetic rounds the same way wherever it is inlined
+# (with fusion across statements the staged and unstaged sweeps of one vertex
+# differed in the last bit).
 FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-function",
+         "-ffp-contract=on",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(HERE, "..", "include")]
 

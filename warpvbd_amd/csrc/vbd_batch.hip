@@ -152,8 +152,8 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t
 // blist[i].x for the i-th block when a list is given (the blocks that still
 // hold a live body's vertex), else all of them.
 template <int MAT>
-__global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0, int nv, const int32_t *st_nb,
-                                                             const uint2 *st_rec, int blk0, int n_dict,
+__global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0, int nv, LaneLayout lay,
+                                                             const int32_t *st_nb, const uint2 *st_rec, int n_dict,
                                                              const int32_t *body_new, BodyPhase ph, BodyFold f,
                                                              const int32_t *rows, const int2 *blist)
 {
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0
     if (threadIdx.x < kFoldBodies) s_bm[threadIdx.x] = 0u;
     const int b0 = body_new[v0 + b * kStageRows];
     // (the whole block returns together)
-    if (!staged_sweep<MAT, true>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, body_new, &ph, rows, body, m)) return;
+    if (!staged_sweep<MAT, true>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, body_new, &ph, rows, body, m)) return;
     body_metric_max(body, m, ph.macc, s_bm, b0, kFoldBodies);
     __syncthreads();
     if (threadIdx.x < kFoldBodies && s_bm[threadIdx.x]) atomicMax(&ph.macc[b0 + threadIdx.x], s_bm[threadIdx.x]);
@@ -282,7 +282,10 @@ int color_lanes(const vbd_solver *s, int c)
 }
 
 // The staged kernel for colour c when the solver has staging tables for it.
-bool staged_color(const vbd_solver *s, int c) { return s->st_ok && c < (int)s->st_blk0.size() && s->st_blk0[c] >= 0; }
+bool staged_color(const vbd_solver *s, int c)
+{
+    return s->st_ok && c < (int)s->st_col.size() && s->st_col[c].ids >= 0 && s->st_col[c].b1 > 0;
+}
 
 void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *blist, int nblocks,
                           const BodyPhase &ph, BodyFold f)
@@ -294,10 +297,10 @@ void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *
     const dim3 grid(nmain + f.nfold);
     const int32_t *rows = s->row_live + s->row_base[c];
     if (s->material == kSNH)
-        k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
+        k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c], s->st_nb, s->st_rec,
                                                                 s->n_dict, s->body_new, ph, f, rows, blist);
     else
-        k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_nb, s->st_rec, s->st_blk0[c],
+        k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c], s->st_nb, s->st_rec,
                                                                  s->n_dict, s->body_new, ph, f, rows, blist);
 }
 

@@ -95,6 +95,7 @@ int vbd_create(const double *rest_xyz, int32_t n_vertices, const int32_t *tets, 
         }
     r = setup_mesh(s, rest_xyz, tets, density, youngs, poisson, active, nullptr, 1, nullptr);
     if (r == 0) r = build_staging(s);
+    if (r == 0) plan_segments(s);  // the big colours' launches take their lane layouts
     if (r != 0) {
         vbd_destroy(s);
         return r;
@@ -601,13 +602,12 @@ int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
         const double nv = (double)(s->col_start[c + 1] - s->col_start[c]);
         // packed position 16 + FP64 target 32 + slot offset 4 + write-back 16
         bytes[c] = 68.0 * nv + per_slot * s->col_slots[c] + 16.0 * s->col_boundary[c];
-        // staged sweep (4 lanes, 256 threads, the default launch): 8-byte
-        // slot records, plus each block's list of kStageMax neighbour ids
-        if (s->st_ok && c < (int)s->st_blk0.size() && s->st_blk0[c] >= 0 && s->launch.lanes == 4 &&
-            s->launch.block == 256) {
-            const double nblocks = (double)((s->col_size[c] + kStageRows - 1) / kStageRows);
-            bytes[c] = 68.0 * nv + 8.0 * s->col_slots[c] + 16.0 * s->col_boundary[c] + 4.0 * kStageMax * nblocks;
-        }
+        // big-colour sweep with staging lists (the default launch): 8-byte
+        // slot records, plus the blocks' lists of neighbour ids
+        const LaneLayout *l = c < (int)s->st_col.size() ? &s->st_col[c] : nullptr;
+        if (s->st_ok && l && l->ids >= 0 && l->b1 > 0 && s->launch.lanes == 4 && s->launch.block == 256)
+            bytes[c] = 68.0 * nv + 8.0 * s->col_slots[c] + 16.0 * s->col_boundary[c] +
+                       4.0 * (double)stage_list_offset(l->b1, l->b4);
     }
     return 0;
 }
@@ -617,7 +617,7 @@ int vbd_staged_colors(vbd_solver_t *s, int32_t *n_colors)
     CHECK_ARG(s && n_colors, "null argument");
     int k = 0;
     if (s->st_ok)
-        for (int b : s->st_blk0) k += b >= 0;
+        for (const LaneLayout &l : s->st_col) k += l.ids >= 0 && l.b1 > 0;
     *n_colors = k;
     return 0;
 }

@@ -243,6 +243,30 @@ __device__ __forceinline__ void acc_group_sum(Acc &a)
     }
 }
 
+// group_sum for aligned groups of 1 << lg lanes, lg in {0, 1, 2} per lane:
+// group_sum<4>'s two DPP stages, each kept only by the lanes whose group
+// spans it (same additions in the same order as group_sum<2> / <4>).
+__device__ __forceinline__ float group_sum_lanes(float x, int lg)
+{
+    const float y = dpp_f32<kDppXor1>(x);
+    x = lg >= 1 ? x + y : x;
+    const float z = dpp_f32<kDppXor2>(x);
+    return lg >= 2 ? x + z : x;
+}
+
+__device__ __forceinline__ void acc_group_sum_lanes(Acc &a, int lg)
+{
+    a.g0 = group_sum_lanes(a.g0, lg);
+    a.g1 = group_sum_lanes(a.g1, lg);
+    a.g2 = group_sum_lanes(a.g2, lg);
+    a.h00 = group_sum_lanes(a.h00, lg);
+    a.h01 = group_sum_lanes(a.h01, lg);
+    a.h02 = group_sum_lanes(a.h02, lg);
+    a.h11 = group_sum_lanes(a.h11, lg);
+    a.h12 = group_sum_lanes(a.h12, lg);
+    a.h22 = group_sum_lanes(a.h22, lg);
+}
+
 // Solve H dx = -G for a symmetric 3x3 block by the adjugate.  A block whose
 // determinant is finite but tiny relative to its diagonal product is skipped
 // (dx = 0): this is the guard for vertices without incident tets and for

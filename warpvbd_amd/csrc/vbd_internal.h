@@ -14,10 +14,17 @@ namespace vbd {
 
 constexpr int kMaxColors = 256;            // src/coloring.py:50
 constexpr int kMaxElementsPerVertex = 32;  // src/vbd_solver.py:12
-// neighbour staging of the big-colour sweep (4 lanes per vertex, 256 threads)
-constexpr int kStageRows = 64;    // vertices per block
-constexpr int kStageMax = 1024;   // distinct neighbours a block may stage (LDS: 24 bytes each)
+// neighbour staging of the big-colour sweep (256 threads per block)
+constexpr int kStageRows = 64;    // vertices per block of 4-lane vertices (the batched sweep's rows)
+constexpr int kStageMax = 1024;   // distinct neighbours a block of 4-lane vertices may stage (LDS: 24 bytes each)
+constexpr int kStageMaxWide = 1280;  // the same for a block holding 2-lane vertices (128 per block)
 constexpr int kStageDict = 64;    // dictionary entries staged with them (48 bytes each)
+// Lanes per vertex of a single body's big colours, by slot count: at most
+// kLaneSlots slots per lane.  A voxel mesh's interior vertices have 20 slots
+// (4 lanes), its face vertices 10 (2 lanes) and its edge vertices 5 (1 lane),
+// so every lane walks 5 slots and a colour needs fewer waves than at 4 lanes
+// for every vertex.
+constexpr int kLaneSlots = 5;
 // Colours below this many vertices are not staged: the staging pass adds a
 // fixed latency per block, repaid only when a colour gives each of the 256
 // CUs two or more blocks (measured per big-colour launch at 256 threads: 9.3k
@@ -96,6 +103,68 @@ struct Launch {
     bool compact = true;  // batched colour launches: after the first chunk, launch only live bodies' ranges
     int resident_block = 1024;  // threads per body in the one-workgroup-per-body form
 };
+
+// Lane class of a vertex with k slots in a single body's big colour: 0 = 4
+// lanes (k > 2 kLaneSlots), 1 = 2 lanes, 2 = 1 lane (k <= kLaneSlots).  A
+// colour's vertices are ordered by class (stable), so its lanes are
+// [4 per class-0 vertex][2 per class-1 vertex][1 per class-2 vertex].
+__host__ __device__ inline int lane_class(int k) { return k > 2 * kLaneSlots ? 0 : (k > kLaneSlots ? 1 : 2); }
+
+// The launch layout of one big colour: n4, n2, n1 vertices of each class;
+// 256-thread blocks over the colour's lanes; blocks below b4 hold only 4-lane
+// vertices and stage up to kStageMax neighbours, blocks in [b4, b1) up to
+// kStageMaxWide; blocks from b1 on (those holding 1-lane vertices, or every
+// block when the colour has no staging lists) read their neighbours from
+// global memory.  ids: first neighbour id of the colour's lists in st_nb.
+struct LaneLayout {
+    int n4 = 0, n2 = 0, n1 = 0;
+    int b4 = 0, b1 = 0, nblocks = 0;
+    int64_t ids = -1;  // -1: no staging lists
+};
+
+// Layout of a colour of nv vertices (n4, n2 of classes 0 and 1); staged: the
+// lists exist (blocks below b1 use them).
+inline LaneLayout lane_layout(int n4, int n2, int nv, bool staged, int64_t ids)
+{
+    LaneLayout l;
+    l.n4 = n4;
+    l.n2 = n2;
+    l.n1 = nv - n4 - n2;
+    const int64_t lanes = 4 * (int64_t)n4 + 2 * (int64_t)n2 + l.n1;
+    l.nblocks = (int)((lanes + 255) / 256);
+    // the first block holding a lane of a 2- or 1-lane vertex, then of a 1-lane one
+    l.b4 = n2 + l.n1 > 0 ? (int)(4 * (int64_t)n4 / 256) : l.nblocks;
+    l.b1 = !staged ? 0 : (l.n1 > 0 ? (int)((4 * (int64_t)n4 + 2 * (int64_t)n2) / 256) : l.nblocks);
+    l.ids = staged ? ids : -1;
+    return l;
+}
+
+// The colour-local vertex and its lane within the vertex's group for lane t
+// of a layout (lg = log2 of the group's lanes).
+__host__ __device__ inline int lane_vertex(int64_t t, int n4, int n2, int &g, int &lg)
+{
+    const int64_t t4 = 4 * (int64_t)n4, t2 = t4 + 2 * (int64_t)n2;
+    if (t < t4) {
+        g = (int)(t & 3);
+        lg = 2;
+        return (int)(t >> 2);
+    }
+    if (t < t2) {
+        g = (int)((t - t4) & 1);
+        lg = 1;
+        return n4 + (int)((t - t4) >> 1);
+    }
+    g = 0;
+    lg = 0;
+    return n4 + n2 + (int)(t - t2);
+}
+
+// First neighbour id of block b's list (blocks below b4 hold kStageMax ids,
+// the rest kStageMaxWide).
+__host__ __device__ inline int64_t stage_list_offset(int b, int b4)
+{
+    return b < b4 ? (int64_t)b * kStageMax : (int64_t)b4 * kStageMax + (int64_t)(b - b4) * kStageMaxWide;
+}
 
 // One colour's launch.
 struct Segment {
@@ -203,14 +272,18 @@ struct vbd_solver {
     float4 *slots = nullptr;      // [n_slots] compact slot records (ids, entry), or tet records [m*4]
     float2 *slots2 = nullptr;     // full records' qd while they are built; null afterwards
     float4 *dict = nullptr;       // [n_dict*3] record dictionary (vbd_device.h)
-    // Neighbour staging for the 4-lane, 256-thread sweep of colours of at least
-    // kStageMinColor vertices (single-body solvers with a shared dictionary): per
-    // block of kStageRows vertices its distinct neighbours, and per slot an
-    // 8-byte record of local indices into that list plus the dictionary entry.
+    // Per colour: the vertices of lane classes 0 and 1 (lane_class), which the
+    // colour-major order puts first (single-body solvers; a batch keeps one
+    // class, (col_size, 0)).
+    std::vector<int2> col_n42;
+    // Neighbour staging for the 256-thread sweep of colours of at least
+    // kStageMinColor vertices (shared dictionary): per block its distinct
+    // neighbours, and per slot an 8-byte record of local indices into that list
+    // plus the dictionary entry.
     bool st_ok = false;
-    std::vector<int32_t> st_blk0;  // per colour: its first block, or -1 (not staged)
+    std::vector<vbd::LaneLayout> st_col;  // per colour: its layout and lists (nblocks 0: not a big colour)
     std::vector<int64_t> st_ids;   // per colour: neighbour ids its blocks stage (sum over blocks)
-    int32_t *st_nb = nullptr;      // [blocks * kStageMax] neighbour ids (colour-major), -1 past each block's list
+    int32_t *st_nb = nullptr;      // neighbour id lists (stage_list_offset), -1 past each block's list
     uint2 *st_rec = nullptr;       // [n_slots] (l0 | l1 << 16, l2 | entry << 16)
     int32_t n_dict = 0;
     bool dict_shared = false;     // entries shared between slots (else one per slot)
@@ -289,6 +362,10 @@ void plan_segments(vbd_solver *s);
 int build_small_records(vbd_solver *s);  // padded records of the colours plan_segments gives 32 lanes
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);
 int build_staging(vbd_solver *s);  // neighbour staging tables (st_*), when the mesh allows
+// The lane layout of colour c's launch when it runs the big-colour sweep
+// (single-body solver, 4 lanes, 256 threads, at least kStageMinColor
+// vertices), else nullptr.
+const LaneLayout *big_color_layout(const vbd_solver *s, int c);
 extern bool g_record_dictionary;  // share record dictionary entries in later creates (default on)
 extern bool g_staging;            // neighbour staging in later creates (default on)
 hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,

@@ -258,16 +258,20 @@ __global__ void k_tet_setup(const double *X, const int32_t *tets, int m, const d
     lam[e] = (E[e] * nu[e]) / ((1.0 + nu[e]) * (1.0 - 2.0 * nu[e]));
 }
 
-// Sort key of the colour-major order: colour-major, body-minor; vertices that
-// are not swept (pinned or uncoloured) go after every colour.
+// Sort key of the colour-major order: colour-major, body-minor, then (with
+// ncls = 3) the lane class of the vertex's slot count; vertices that are not
+// swept (pinned or uncoloured) go after every colour.  Vertices of one colour
+// are not neighbours, so their order within a colour does not change a sweep.
 __global__ void k_sort_keys(const int32_t *colors, const uint8_t *active, const int32_t *body_of, int nb, int n,
-                            uint32_t *keys, int32_t *vals)
+                            const int32_t *inc_off, int ncls, uint32_t *keys, int32_t *vals)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     int c = colors[v];
-    uint32_t ck = (active[v] && c >= 0) ? (uint32_t)c : (uint32_t)kMaxColors;
-    keys[v] = ck * (uint32_t)nb + (uint32_t)body_of[v];
+    const bool swept = active[v] && c >= 0;
+    uint32_t ck = swept ? (uint32_t)c : (uint32_t)kMaxColors;
+    const int cls = ncls > 1 && swept ? lane_class(inc_off[v + 1] - inc_off[v]) : 0;
+    keys[v] = (ck * (uint32_t)nb + (uint32_t)body_of[v]) * (uint32_t)ncls + (uint32_t)cls;
     vals[v] = v;
 }
 
@@ -890,7 +894,8 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_TRY(build_incidence_and_colors(d_tets, m, n, s->inc_off, s->inc, s->colors, &s->ncolors, &s->max_incident,
                                        &s->max_degree, &s->n_ref, true, st, scr));
 
-    // 4. colour-major renumbering (stable radix sort on colour, then body)
+    // 4. colour-major renumbering (stable radix sort on colour, then body, then
+    // for a single body the lane class)
     std::vector<int32_t> body_of(n, 0);
     if (body_voff)
         for (int b = 0; b < nb; ++b)
@@ -898,7 +903,8 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     int32_t *d_body_of;
     VBD_TRY(scr.get(&d_body_of, n));
     VBD_HIP(hipMemcpyAsync(d_body_of, body_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
-    const int64_t nkeys = (int64_t)(kMaxColors + 1) * nb;
+    const int ncls = s->batched ? 1 : 3;
+    const int64_t nkeys = (int64_t)(kMaxColors + 1) * nb * ncls;
     unsigned end_bit = 1;
     while (((int64_t)1 << end_bit) < nkeys) ++end_bit;
     uint32_t *keys, *keys_sorted;
@@ -909,7 +915,7 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_TRY(dmalloc(&s->perm, n));
     VBD_TRY(dmalloc(&s->iperm, n));
     VBD_TRY(dmalloc(&s->body_new, n));
-    k_sort_keys<<<nblk(n), kTB, 0, st>>>(s->colors, d_act_orig, d_body_of, nb, n, keys, vals);
+    k_sort_keys<<<nblk(n), kTB, 0, st>>>(s->colors, d_act_orig, d_body_of, nb, n, s->inc_off, ncls, keys, vals);
     {
         size_t tb = 0;
         VBD_HIP(rocprim::radix_sort_pairs(nullptr, tb, keys, keys_sorted, vals, s->perm, (size_t)n, 0, end_bit, st));
@@ -929,14 +935,21 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     VBD_HIP(hipMemcpyAsync(hf.data(), kfirst, sizeof(int32_t) * nkeys, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipMemcpyAsync(he.data(), kend, sizeof(int32_t) * nkeys, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
-    for (int64_t k = 0; k < nkeys; ++k) hh[k] = he[k] - hf[k];
+    // per (colour, body): the vertex count over the lane classes
+    std::vector<int32_t> hcb((size_t)(kMaxColors + 1) * nb, 0);
+    for (int64_t k = 0; k < nkeys; ++k) {
+        hh[k] = he[k] - hf[k];
+        hcb[k / ncls] += hh[k];
+    }
     s->col_start.assign(s->ncolors + 1, 0);
     s->col_size.assign(s->ncolors, 0);
+    s->col_n42.assign(s->ncolors, make_int2(0, 0));
     for (int c = 0; c < s->ncolors; ++c) {
         int tot = 0;
-        for (int b = 0; b < nb; ++b) tot += hh[(size_t)c * nb + b];
+        for (int b = 0; b < nb; ++b) tot += hcb[(size_t)c * nb + b];
         s->col_size[c] = tot;
         s->col_start[c + 1] = s->col_start[c] + tot;
+        s->col_n42[c] = ncls == 3 ? make_int2(hh[3 * (size_t)c], hh[3 * (size_t)c + 1]) : make_int2(tot, 0);
     }
     s->n_swept = s->col_start[s->ncolors];
     // per body: (start, count) of each colour, then of its other vertices
@@ -946,7 +959,7 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
         for (int c = 0; c <= kMaxColors; ++c) {
             const int cc = c < s->ncolors ? c : (c == kMaxColors ? s->ncolors : -1);
             for (int b = 0; b < nb; ++b) {
-                const int cnt = hh[(size_t)c * nb + b];
+                const int cnt = hcb[(size_t)c * nb + b];
                 if (cc >= 0) s->ranges_host[(size_t)b * (s->ncolors + 1) + cc] = make_int2((int)pos, cnt);
                 pos += cnt;
             }
@@ -1059,17 +1072,26 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     return 0;
 }
 
-// Neighbour staging tables (the st_* fields).  Colours of at least
-// kStageMinColor vertices are cut into blocks of kStageRows consecutive vertices,
-// as the 4-lane, 256-thread sweep assigns them; each block's distinct
-// neighbour ids are listed in order of first appearance, and each of its slots
-// gets (l0 | l1 << 16, l2 | entry << 16) with l_k the local indices of the
-// slot's other vertices.  Needs a shared dictionary (entries below 2^16); a
-// block with more than kStageMax neighbours turns staging off for the solver.
+// Lane layouts and neighbour staging tables (the st_* fields).  Colours of at
+// least kStageMinColor vertices get their lane layout (lane_layout: a single
+// body's vertices take 4, 2 or 1 lanes by slot count, a batch's 4) and, with
+// a shared dictionary, staging lists: each 256-thread block below the
+// layout's b1 lists its distinct neighbour ids in order of first appearance
+// (kStageMax per block of 4-lane vertices, kStageMaxWide per block holding
+// 2-lane ones), and each of its slots gets (l0 | l1 << 16, l2 | entry << 16)
+// with l_k the local indices of the slot's other vertices.  A block with more
+// neighbours than its list holds turns staging off for the solver (the
+// layouts stay, and every block reads its neighbours from global memory).
 int build_staging(vbd_solver *s)
 {
     s->st_ok = false;
-    s->st_blk0.assign(s->ncolors, -1);
+    s->st_col.assign(s->ncolors, LaneLayout());
+    s->st_ids.assign(s->ncolors, 0);
+    const int min_color = std::max(s->launch.small_color, kStageMinColor);
+    for (int c = 0; c < s->ncolors; ++c) {
+        const int nv = s->col_size[c];
+        if (nv >= min_color) s->st_col[c] = lane_layout(s->col_n42[c].x, s->col_n42[c].y, nv, false, -1);
+    }
     if (!g_staging || !s->dict_shared || s->n_dict > kStageDict || s->n_slots <= 0) return 0;
     hipStream_t st = s->stream;
     const int nsw = s->n_swept;
@@ -1078,41 +1100,47 @@ int build_staging(vbd_solver *s)
     VBD_HIP(hipMemcpyAsync(off.data(), s->slot_off, sizeof(int32_t) * (nsw + 1), hipMemcpyDeviceToHost, st));
     VBD_HIP(hipMemcpyAsync(rec.data(), s->slots, sizeof(float4) * s->n_slots, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
-    std::vector<int32_t> nb;  // kStageMax ids per block, -1 past the block's list
+    std::vector<int32_t> nb;  // the blocks' lists, -1 past each block's neighbours
     std::vector<uint2> srec(s->n_slots, make_uint2(0u, 0u));
     std::vector<int32_t> stamp(s->n, -1), local(s->n, 0);
-    std::vector<int32_t> blk0(s->ncolors, -1);
+    std::vector<LaneLayout> col(s->ncolors);
     std::vector<int64_t> ids(s->ncolors, 0);
-    int nblocks = 0;
+    int nblocks = 0;  // blocks listed so far (the stamp of the current one)
     for (int c = 0; c < s->ncolors; ++c) {
         const int nv = s->col_size[c];
-        if (nv < std::max(s->launch.small_color, kStageMinColor)) continue;
-        blk0[c] = nblocks;
+        if (nv < min_color) continue;
+        const int n4 = s->col_n42[c].x, n2 = s->col_n42[c].y;
+        const LaneLayout l = lane_layout(n4, n2, nv, true, (int64_t)nb.size());
+        const int64_t lanes = 4 * (int64_t)n4 + 2 * (int64_t)n2 + l.n1;
         const int c0 = s->col_start[c];
-        for (int j = 0; j < nv; j += kStageRows, ++nblocks) {
-            const int va = c0 + j, vb = c0 + std::min(nv, j + kStageRows);
-            const size_t base = nb.size();
-            nb.resize(base + kStageMax, -1);
+        nb.resize(nb.size() + stage_list_offset(l.b1, l.b4), -1);
+        for (int b = 0; b < l.b1; ++b, ++nblocks) {
+            int g, lg;
+            const int va = c0 + lane_vertex(256 * (int64_t)b, n4, n2, g, lg);
+            const int vb = c0 + lane_vertex(std::min<int64_t>(256 * (int64_t)b + 256, lanes) - 1, n4, n2, g, lg) + 1;
+            const size_t base = (size_t)(l.ids + stage_list_offset(b, l.b4));
+            const int cap = b < l.b4 ? kStageMax : kStageMaxWide;
             int cnt = 0;
             for (int64_t q = off[va]; q < off[vb]; ++q) {
                 uint32_t w[4];
                 std::memcpy(w, &rec[q], sizeof(w));
-                uint32_t l[3];
+                uint32_t lk[3];
                 for (int k = 0; k < 3; ++k) {
                     const int id = (int)w[k];
                     if (stamp[id] != nblocks) {
-                        if (cnt == kStageMax) return 0;
+                        if (cnt == cap) return 0;
                         stamp[id] = nblocks;
                         local[id] = cnt;
                         nb[base + cnt] = id;
                         ++cnt;
                     }
-                    l[k] = (uint32_t)local[id];
+                    lk[k] = (uint32_t)local[id];
                 }
-                srec[q] = make_uint2(l[0] | l[1] << 16, l[2] | w[3] << 16);
+                srec[q] = make_uint2(lk[0] | lk[1] << 16, lk[2] | w[3] << 16);
             }
             ids[c] += cnt;
         }
+        col[c] = l;
     }
     if (nblocks == 0) return 0;
     // zero records past the last slot: the sweep loads a lane's records
@@ -1123,7 +1151,8 @@ int build_staging(vbd_solver *s)
     VBD_HIP(hipMemcpyAsync(s->st_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipMemcpyAsync(s->st_rec, srec.data(), sizeof(uint2) * srec.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipStreamSynchronize(st));
-    s->st_blk0 = blk0;
+    for (int c = 0; c < s->ncolors; ++c)
+        if (col[c].nblocks > 0) s->st_col[c] = col[c];
     s->st_ids = ids;
     s->st_ok = true;
     return 0;

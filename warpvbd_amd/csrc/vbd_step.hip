@@ -141,18 +141,27 @@ __global__ __launch_bounds__(1024) void k_sweep(
     finish_sweep(a, c, mb, close, part_all, np_all, tol);
 }
 
-// k_sweep<4, 1, MAT> with neighbour staging (staged_sweep in vbd_sweep.h):
-// 256 threads per block of kStageRows vertices; the block's metric maximum
-// goes to a.part, and the iteration's last launch folds them all.
+// A single body's big colour (at least kStageMinColor vertices at the
+// default launch): 256-thread blocks over the colour's lanes as lay lays them
+// out (4, 2 or 1 lanes per vertex by slot count, so that no lane walks more
+// than kLaneSlots slots of a voxel mesh); blocks below lay.b1 stage their
+// neighbours in LDS (staged_sweep in vbd_sweep.h), the others read them from
+// global memory (lanes_sweep).  The block's metric maximum goes to a.part, and
+// the iteration's last launch folds them all.
 template <int MAT>
-__global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int nv, int close,
+__global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int nv, LaneLayout lay, int close,
                                                       const uint32_t *part_all, int np_all, float tol,
-                                                      const int32_t *st_nb, const uint2 *st_rec, int blk0, int n_dict)
+                                                      const int32_t *st_nb, const uint2 *st_rec, int n_dict)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     uint32_t m;
-    int body;
-    if (!staged_sweep<MAT, false>(a, v0, nv, b, st_nb, st_rec, blk0, n_dict, nullptr, nullptr, nullptr, body, m)) return;
+    if (b >= lay.b1) {
+        if (!lanes_sweep<MAT>(a, v0, nv, lay, b, m)) return;
+    } else {
+        int body;
+        if (!staged_sweep<MAT, false>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, nullptr, nullptr, nullptr, body, m))
+            return;
+    }
     const uint32_t mb = block_max_u32(m);
     finish_sweep(a, a.ctrl, mb, close, part_all, np_all, tol);
 }
@@ -360,17 +369,17 @@ void launch_sweep(const SweepArgs &a, int v0, int nv, int block, int close, cons
                           a, v0, nv, close, part_all, np_all, tol);
 }
 
-void launch_staged(int material, const SweepArgs &a, int v0, int nv, int close, const uint32_t *part_all,
-                   int np_all, float tol, const int32_t *st_nb, const uint2 *st_rec, int blk0, int n_dict,
-                   hipEvent_t e0, hipEvent_t e1, hipStream_t st)
+void launch_staged(int material, const SweepArgs &a, int v0, int nv, const LaneLayout &lay, int close,
+                   const uint32_t *part_all, int np_all, float tol, const int32_t *st_nb, const uint2 *st_rec,
+                   int n_dict, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
 {
-    const dim3 grid(nblk(nv, kStageRows));
+    const dim3 grid(lay.nblocks);
     if (material == kSNH)
-        hipExtLaunchKernelGGL(k_sweep_staged<kSNH>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, close, part_all, np_all,
-                              tol, st_nb, st_rec, blk0, n_dict);
+        hipExtLaunchKernelGGL(k_sweep_staged<kSNH>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, lay, close, part_all,
+                              np_all, tol, st_nb, st_rec, n_dict);
     else
-        hipExtLaunchKernelGGL(k_sweep_staged<kStVK>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, close, part_all,
-                              np_all, tol, st_nb, st_rec, blk0, n_dict);
+        hipExtLaunchKernelGGL(k_sweep_staged<kStVK>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, lay, close,
+                              part_all, np_all, tol, st_nb, st_rec, n_dict);
 }
 
 void launch_small(int material, const SweepArgs &a, int v0, int nv, int block, int close, const uint32_t *part_all,
@@ -401,10 +410,19 @@ void dispatch_sweep(int L, const SweepArgs &a, int v0, int nv, int block, int cl
 
 }  // namespace
 
+const LaneLayout *big_color_layout(const vbd_solver *s, int c)
+{
+    if (s->batched || s->launch.lanes != 4 || s->launch.block != 256 || c >= (int)s->st_col.size()) return nullptr;
+    const LaneLayout &l = s->st_col[c];
+    if (l.nblocks == 0 || s->col_size[c] < s->launch.small_color) return nullptr;
+    return &l;
+}
+
 // One launch per non-empty colour.  Colours below small_color vertices are
 // latency bound: they get 32 lanes per vertex (one slot per lane) so a vertex
-// finishes in one load round.  When every colour is small the whole step runs
-// in one workgroup instead (k_step_resident).
+// finishes in one load round.  Big colours of a single body take their lane
+// layout by slot count (k_sweep_staged).  When every colour is small the
+// whole step runs in one workgroup instead (k_step_resident).
 void plan_segments(vbd_solver *s)
 {
     s->segments.clear();
@@ -415,7 +433,9 @@ void plan_segments(vbd_solver *s)
         if (nv == 0) continue;
         const int lanes = nv < s->launch.small_color ? 32 : s->launch.lanes;
         const int64_t threads = (int64_t)nv * lanes;
-        s->segments.push_back({c, c + 1, 0, nblk(threads, s->launch.block), lanes, s->launch.block});
+        const LaneLayout *lay = big_color_layout(s, c);
+        s->segments.push_back({c, c + 1, 0, lay ? lay->nblocks : nblk(threads, s->launch.block), lanes,
+                               s->launch.block});
     }
     // the iteration's last launch: one block of up to 1024 threads when its
     // colour fits (the padded small-colour kernel then folds the other
@@ -519,15 +539,14 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
         const int v0 = s->col_start[g.c0], nv = s->col_size[g.c0];
         hipEvent_t ev[2] = {nullptr, nullptr};
         if (profile && nv >= s->prof_min) VBD_TRY(record_event(s, g.c0, iter, ev));
-        // neighbour staging: built for 4 lanes per vertex and 256 threads
-        const int blk0 = s->st_ok && g.lanes == 4 && block == 256 && g.c0 < (int)s->st_blk0.size()
-                             ? s->st_blk0[g.c0]
-                             : -1;
+        // big colours at the default launch (4 lanes, 256 threads): the lane
+        // layout by slot count, with neighbour staging where it was built
+        const LaneLayout *lay = big_color_layout(s, g.c0);
         const float4 *prec = s->small_rec && g.lanes == 32 && s->small_off[g.c0] >= 0
                                  ? s->small_rec + 32 * s->small_off[g.c0]
                                  : nullptr;
-        if (blk0 >= 0)
-            launch_staged(s->material, a, v0, nv, close, s->part, s->n_parts, tol, s->st_nb, s->st_rec, blk0,
+        if (lay)
+            launch_staged(s->material, a, v0, nv, *lay, close, s->part, s->n_parts, tol, s->st_nb, s->st_rec,
                           s->n_dict, ev[0], ev[1], s->stream);
         else if (prec)
             launch_small(s->material, a, v0, nv, g.block, close, s->part, s->n_parts, tol, prec, ev[0], ev[1],

@@ -194,20 +194,21 @@ __device__ __forceinline__ VertexLocal vertex_local(const VertexPre &pre)
     return q;
 }
 
-// After the slot loop: the group sum of gradient and Hessian, damping and
-// inertia, and the guarded 3x3 solve.  us receives the vertex's current
-// displacement in FP64 (from the parts the loop holds).  Lanes without a
-// vertex get dx = 0.
-template <int L>
-__device__ __forceinline__ float3 vertex_finish(const SweepArgs &a, int v, bool valid, Acc &acc, VertexLocal &q,
-                                                double3 &us)
+// The vertex's current displacement in FP64 from the parts the loop holds
+// (the empty asm keeps the compiler from carrying the FP64 displacement
+// through the loop instead of rebuilding it here).
+__device__ __forceinline__ double3 vertex_disp(VertexLocal &q)
 {
-    // the empty asm keeps the compiler from carrying the FP64 displacement
-    // through the loop instead of rebuilding it from the parts here
     asm volatile("" : "+v"(q.hi.x), "+v"(q.hi.y), "+v"(q.hi.z), "+v"(q.lo.x), "+v"(q.lo.y), "+v"(q.lo.z));
-    us = make_double3((double)q.hi.x + (double)q.lo.x, (double)q.hi.y + (double)q.lo.y,
-                      (double)q.hi.z + (double)q.lo.z);
-    acc_group_sum<L>(acc);
+    return make_double3((double)q.hi.x + (double)q.lo.x, (double)q.hi.y + (double)q.lo.y,
+                        (double)q.hi.z + (double)q.lo.z);
+}
+
+// After the group sum: damping and inertia, and the guarded 3x3 solve (us:
+// the vertex's current displacement).  Lanes without a vertex get dx = 0.
+__device__ __forceinline__ float3 vertex_close(const SweepArgs &a, int v, bool valid, Acc &acc, const VertexLocal &q,
+                                               const double3 &us)
+{
     if (!valid) return make_float3(0.f, 0.f, 0.f);
     if (a.kd_dt != 0.f) {
         // damping (src/vbd_solver.py:222-227): g += kd/dt H_el (x - x_prev), H += kd/dt H_el
@@ -228,6 +229,59 @@ __device__ __forceinline__ float3 vertex_finish(const SweepArgs &a, int v, bool 
     acc.h11 += q.w;
     acc.h22 += q.w;
     return solve3_guarded(acc);
+}
+
+// After the slot loop: the group sum of gradient and Hessian, damping and
+// inertia, and the guarded 3x3 solve.  us receives the vertex's current
+// displacement in FP64.  Lanes without a vertex get dx = 0.
+template <int L>
+__device__ __forceinline__ float3 vertex_finish(const SweepArgs &a, int v, bool valid, Acc &acc, VertexLocal &q,
+                                                double3 &us)
+{
+    us = vertex_disp(q);
+    acc_group_sum<L>(acc);
+    return vertex_close(a, v, valid, acc, q, us);
+}
+
+// vertex_finish for groups of 1 << lg lanes that vary from lane to lane
+// (lg <= 2): a wave whose groups all have 4 lanes takes the fixed form; the
+// per-lane form adds the same values in the same order.
+__device__ __forceinline__ float3 vertex_finish_lanes(const SweepArgs &a, int v, bool valid, Acc &acc,
+                                                      VertexLocal &q, double3 &us, int lg)
+{
+    us = vertex_disp(q);
+    if (__all(lg == 2)) acc_group_sum<4>(acc);
+    else acc_group_sum_lanes(acc, lg);
+    return vertex_close(a, v, valid, acc, q, us);
+}
+
+// A lane's slots s, s + step, ... below s1, one per batch, software-pipelined:
+// the next compact record is requested after the current slot's constants
+// and gathers, so each batch waits for one memory latency, not two.
+template <int MAT>
+__device__ __forceinline__ void slot_loop(const SweepArgs &a, int s, int s1, int step, const float3 &hi,
+                                          const float3 &lo, Acc &acc)
+{
+    float4 c = s < s1 ? a.slots[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (; s < s1; s += step) {
+        // 32-bit unsigned byte offsets from the arrays' (uniform) bases:
+        // address = base + offset, no 64-bit index arithmetic per load
+        // entry * 48 bytes in 32 bits (create admits fewer than 2^26 slots)
+        const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) +
+                                                           __float_as_uint(c.w) * 48u);
+        const float4 d0 = d[0], d1 = d[1], d2 = d[2];
+        const float4 x0 = *at(a.u, __float_as_uint(c.x));
+        const float4 x1 = *at(a.u, __float_as_uint(c.y));
+        const float4 x2 = *at(a.u, __float_as_uint(c.z));
+        // keep the request for the next record behind this slot's loads
+        __builtin_amdgcn_sched_barrier(0);
+        const float4 cn = *at(a.slots, (uint32_t)min(s + step, s1 - 1));  // the last one reloads a record in cache
+        __builtin_amdgcn_sched_barrier(0);
+        const SlotRel sr = slot_rel(x0, x1, x2, hi, lo);
+        slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, acc);
+        c = cn;
+    }
 }
 
 // Gradient and Hessian of one vertex's local energy (all incident tets plus
@@ -285,31 +339,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
         }
 #else
         if constexpr (B == 1) {
-            // One slot per lane per batch, software-pipelined: the next compact
-            // record is requested after the current slot's constants and
-            // gathers, so each batch waits for one memory latency, not two.
-            int s = s0 + lane;
-            float4 c = s < s1 ? a.slots[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-            for (; s < s1; s += L) {
-                // 32-bit unsigned byte offsets from the arrays' (uniform) bases:
-                // address = base + offset, no 64-bit index arithmetic per load
-                // entry * 48 bytes in 32 bits (create admits fewer than 2^26 slots)
-                const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) +
-                                                                   __float_as_uint(c.w) * 48u);
-                const float4 d0 = d[0], d1 = d[1], d2 = d[2];
-                const float4 x0 = *at(a.u, __float_as_uint(c.x));
-                const float4 x1 = *at(a.u, __float_as_uint(c.y));
-                const float4 x2 = *at(a.u, __float_as_uint(c.z));
-                // keep the request for the next record behind this slot's loads
-                __builtin_amdgcn_sched_barrier(0);
-                const float4 cn = *at(a.slots, (uint32_t)min(s + L, s1 - 1));  // the last one reloads a record in cache
-                __builtin_amdgcn_sched_barrier(0);
-                const SlotRel sr = slot_rel(x0, x1, x2, hi, lo);
-                slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                               make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2,
-                               acc);
-                c = cn;
-            }
+            slot_loop<MAT>(a, s0 + lane, s1, L, hi, lo, acc);
         } else
         // B slots per lane per batch: all records, then all gathers, then the
         // arithmetic, so each batch costs two memory latencies, not 2B.
@@ -433,11 +463,14 @@ __device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
 }
 
 
-// One block of the staged colour sweep: 256 threads, 4 lanes per vertex, the
-// vertices [v0 + 64 b, v0 + 64 b + 64) of a colour of nv.  Two memory rounds,
+// One block of the staged colour sweep: 256 threads over lanes
+// [256 b, 256 b + 256) of a colour of nv vertices at v0 laid out by lay
+// (lane_vertex: 4, 2 or 1 lanes per vertex by slot count; a batch's colours
+// have 4 lanes for every vertex, blocks of kStageRows).  Two memory rounds,
 // then arithmetic only:
-//   1. the vertex's own data, the block's neighbour ids (st_nb: kStageMax per
-//      block, -1 past its list) and the record dictionary;
+//   1. the vertex's own data, the block's neighbour ids (st_nb at lay.ids +
+//      stage_list_offset: kStageMax ids per block below lay.b4, kStageMaxWide
+//      from there, -1 past its list) and the record dictionary;
 //   2. the neighbours' positions and every 8-byte slot record of the lane
 //      (st_rec: (l0 | l1 << 16, l2 | entry << 16), local indices into the
 //      block's list and the dictionary entry, with kStagedSlots * 4 zero
@@ -445,7 +478,7 @@ __device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
 // The neighbours are unpacked into LDS (FP32 parts and remainders) with the
 // dictionary, and the slot loop reads LDS only.  Neighbours belong to other
 // colours, which the launch does not write, so staging them first is exact;
-// the arithmetic is k_sweep's, so the results are the same bit for bit.
+// the arithmetic is lanes_sweep's, so the results are the same bit for bit.
 // BODIES: a batched solver's launch -- a vertex of a body that has converged
 // is not updated (its lanes get body -1), instead of the step's convergence
 // flag being tested, and rows[b] counts the live bodies with a vertex in the
@@ -454,27 +487,40 @@ __device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
 // receives the lane's metric bits (0 for lanes without an updated vertex),
 // body the vertex's body.
 template <int MAT, bool BODIES>
-__device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, int b, const int32_t *st_nb,
-                                             const uint2 *st_rec, int blk0, int n_dict, const int32_t *body_new,
-                                             const BodyPhase *ph, const int32_t *rows, int &body_out, uint32_t &m)
+__device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, const LaneLayout &lay, int b,
+                                             const int32_t *st_nb, const uint2 *st_rec, int n_dict,
+                                             const int32_t *body_new, const BodyPhase *ph, const int32_t *rows,
+                                             int &body_out, uint32_t &m)
 {
-    constexpr int L = 4;
-    constexpr int kStageUnroll = kStageMax / 256;
-    static_assert(kStageMax % 256 == 0, "each of the 256 threads stages kStageUnroll neighbours");
+    // a batch's blocks hold 4-lane vertices only; a single body's may stage up
+    // to kStageMaxWide neighbours
+    constexpr int kCap = BODIES ? kStageMax : kStageMaxWide;
+    constexpr int kStageUnroll = kCap / 256;
+    static_assert(kStageMax % 256 == 0 && kStageMaxWide % 256 == 0 && kStageMaxWide >= kStageMax,
+                  "each of the 256 threads stages kStageUnroll neighbours");
     static_assert(3 * kStageDict <= 256, "one dictionary row per thread");
-    __shared__ float4 s_na[kStageMax];
-    __shared__ float2 s_nb[kStageMax];
+    __shared__ float4 s_na[kCap];
+    __shared__ float2 s_nb[kCap];
     __shared__ float4 s_dict[3 * kStageDict];
     const int t = b * 256 + threadIdx.x;
-    const int vi = t / L, lane = t % L;
+    int g, lg, vi;
+    if constexpr (BODIES) {
+        vi = t >> 2;
+        g = t & 3;
+        lg = 2;
+    } else {
+        vi = lane_vertex(t, lay.n4, lay.n2, g, lg);
+    }
     const bool in = vi < nv;
     const int v = v0 + (in ? vi : 0);
     // round 1
     const VertexPre pre = vertex_pre(a, v, in);
-    const int32_t *ids = st_nb + (size_t)(blk0 + b) * kStageMax;
+    const bool wide = b >= lay.b4;  // the same for every thread
+    const int32_t *ids = st_nb + lay.ids + stage_list_offset(b, lay.b4);
     int id[kStageUnroll];
 #pragma unroll
-    for (int q = 0; q < kStageUnroll; ++q) id[q] = ids[threadIdx.x + 256 * q];
+    for (int q = 0; q < kStageUnroll; ++q)
+        id[q] = 256 * q < kStageMax || wide ? ids[threadIdx.x + 256 * q] : -1;
     const bool has_dict = (int)threadIdx.x < 3 * n_dict;
     const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
     int body = 0;
@@ -506,9 +552,9 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
 #pragma unroll
     for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
     uint2 rec[kStagedSlots];
-    const uint2 *rl = st_rec + pre.s0 + lane;
+    const uint2 *rl = st_rec + pre.s0 + g;
 #pragma unroll
-    for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[L * k];
+    for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[k << lg];
     if (has_dict) s_dict[threadIdx.x] = drow;
 #pragma unroll
     for (int q = 0; q < kStageUnroll; ++q) {
@@ -525,8 +571,8 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     acc_zero(acc);
     VertexLocal ql = vertex_local(pre);
     const float3 hi = ql.hi, lo = ql.lo;
-    // this lane's slots: pre.s0 + lane, + L, ... below pre.s1
-    const int nk = valid && pre.s1 > pre.s0 + lane ? (pre.s1 - pre.s0 - lane + L - 1) / L : 0;
+    // this lane's slots: pre.s0 + g, + (1 << lg), ... below pre.s1
+    const int nk = valid && pre.s1 > pre.s0 + g ? (pre.s1 - pre.s0 - g + (1 << lg) - 1) >> lg : 0;
 #pragma unroll
     for (int k = 0; k < kStagedSlots; ++k) {
         if (k < nk) {
@@ -546,11 +592,46 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
         }
     }
     double3 us;
-    const float3 dx = vertex_finish<L>(a, v, valid, acc, ql, us);
+    float3 dx;
+    if constexpr (BODIES) dx = vertex_finish<4>(a, v, valid, acc, ql, us);
+    else dx = vertex_finish_lanes(a, v, valid, acc, ql, us, lg);
     m = 0u;
     body_out = valid ? body : -1;
     if (valid) {
-        if (lane == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
+        if (g == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
+        m = metric_bits(dx);
+    }
+    return true;
+}
+
+// One block of a single body's big-colour sweep without staging lists (the
+// blocks from lay.b1 on): the lanes of lay, each walking its slots by the
+// compact records (slot_loop).  Same arithmetic and lane groups as
+// staged_sweep.  Returns false when the step has converged already.
+template <int MAT>
+__device__ __forceinline__ bool lanes_sweep(const SweepArgs &a, int v0, int nv, const LaneLayout &lay, int b,
+                                            uint32_t &m)
+{
+    const int t = b * 256 + threadIdx.x;
+    int g, lg;
+    const int vi = lane_vertex(t, lay.n4, lay.n2, g, lg);
+    const bool in = vi < nv;
+    const int v = v0 + (in ? vi : 0);
+    const VertexPre pre = vertex_pre(a, v, in);
+    if (a.ctrl->converged) {
+        m = 0u;
+        return false;
+    }
+    Acc acc;
+    acc_zero(acc);
+    VertexLocal ql = vertex_local(pre);
+    const float3 hi = ql.hi, lo = ql.lo;
+    if (in) slot_loop<MAT>(a, pre.s0 + g, pre.s1, 1 << lg, hi, lo, acc);
+    double3 us;
+    const float3 dx = vertex_finish_lanes(a, v, in, acc, ql, us, lg);
+    m = 0u;
+    if (in) {
+        if (g == 0) a.u[v] = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
         m = metric_bits(dx);
     }
     return true;
