@@ -403,15 +403,35 @@ __global__ __launch_bounds__(256) void k_dgrad(int n, const float *__restrict__ 
 }
 
 // dh[i] += dx[i, :64] + sum_{j in N(i)} dx[j, 64:] / deg(j)   (adjoint of the mean)
+// Eight neighbours at a time: their ids, then their degrees and rows are
+// requested together, so a row waits for two memory latencies per eight
+// neighbours instead of two per neighbour (same operations, same order).
 __global__ void k_backprop_mean(int n, const int32_t *nb_off, const int32_t *nb, const float *dx, float *dh)
 {
+    constexpr int U = 8;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * kW) return;
     const int i = (int)(t / kW), f = (int)(t % kW);
     float s = dh[t] + dx[(size_t)i * 2 * kW + f];
-    for (int q = nb_off[i]; q < nb_off[i + 1]; ++q) {
-        const int j = nb[q];
-        s += dx[(size_t)j * 2 * kW + kW + f] / (float)(nb_off[j + 1] - nb_off[j]);
+    const int q0 = nb_off[i], q1 = nb_off[i + 1];
+    for (int q = q0; q < q1; q += U) {
+        int j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) j[u] = q + u < q1 ? nb[q + u] : -1;
+        float v[U];
+        int d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v[u] = 0.f;
+            d[u] = 1;
+            if (j[u] >= 0) {
+                v[u] = dx[(size_t)j[u] * 2 * kW + kW + f];
+                d[u] = nb_off[j[u] + 1] - nb_off[j[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) s += v[u] / (float)d[u];
     }
     dh[t] = s;
 }
@@ -422,48 +442,66 @@ __global__ void k_relu_grad(int64_t n, const float *dh, const float *z, float *d
     if (t < n) dz[t] = z[t] > 0.f ? dh[t] : 0.f;
 }
 
-// Weight gradient of one linear layer: gW[o][k] += sum_i dz[i][o] x[i][k],
-// gb[o] += sum_i dz[i][o], x = [X1 | X2] (BF16) with K columns.  Each block
-// folds a strided set of 32-row chunks (staged in LDS) into private sums and
-// writes one partial row; k_sum_partials adds the partials into G.
+// Weight gradient of one linear layer on the matrix cores:
+//   gW[o][k] += sum_i dz[i][o] x[i][k],  gb[o] += sum_i dz[i][o],
+// x = [X1 | X2] (BF16) with K columns.  Wave w owns the outputs o in
+// [16w, 16w + 16) and every column: v_mfma_f32_16x16x4_f32 takes four rows i
+// per instruction, lane l holding dz[i + l/16][o] (A, 16 o x 4 i) and
+// x[i + l/16][c + l%16] (B, 4 i x 16 columns), both read straight from the
+// row-major arrays (16 consecutive floats or BF16 values per row), so the
+// products are the FP32 products of the same operands with no transposition.
+// A block takes kWgRows rows at a time, strided over the grid (loads for all
+// of them requested before the products), and writes one partial row;
+// k_sum_partials adds the partials into G.
+constexpr int kWgRows = 16;
+
 template <int K>
-__global__ __launch_bounds__(256) void k_weight_grad(int n, const float *dz, const uint16_t *X1, int K1,
-                                                     const uint16_t *X2, float *partial)
+__global__ __launch_bounds__(256) void k_weight_grad(int n, const float *__restrict__ dz,
+                                                     const uint16_t *__restrict__ X1, int K1,
+                                                     const uint16_t *__restrict__ X2, float *__restrict__ partial)
 {
-    constexpr int RB = 32;
-    constexpr int PER = kW * K / 256;  // outputs per thread
-    __shared__ float s_x[RB][K + 1];
-    __shared__ float s_dz[RB][kW];
-    const int o = threadIdx.x / (256 / kW);
-    const int kb = (threadIdx.x % (256 / kW)) * PER;
-    float acc[PER];
+    constexpr int T = K / 16;           // 16-column tiles
+    constexpr int G = kWgRows / 4;      // four-row groups per chunk
+    static_assert(K % 16 == 0 && kWgRows % 4 == 0, "whole tiles");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int o = 16 * wave + (lane & 15), ir = lane >> 4;
+    const int K2 = K - K1;
+    f32x4v_t acc[T];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) acc[q] = 0.f;
+    for (int t = 0; t < T; ++t) acc[t] = f32x4v_t{0.f, 0.f, 0.f, 0.f};
     float accb = 0.f;
-    for (int r0 = blockIdx.x * RB; r0 < n; r0 += gridDim.x * RB) {
-        for (int q = threadIdx.x; q < RB * K; q += 256) {
-            const int rr = q / K, k = q % K, i = r0 + rr;
-            float v = 0.f;
-            if (i < n) v = bf2f(k < K1 ? X1[(size_t)i * K1 + k] : X2[(size_t)i * (K - K1) + (k - K1)]);
-            s_x[rr][k] = v;
-        }
-        for (int q = threadIdx.x; q < RB * kW; q += 256) {
-            const int rr = q / kW, oo = q % kW, i = r0 + rr;
-            s_dz[rr][oo] = i < n ? dz[(size_t)i * kW + oo] : 0.f;
-        }
-        __syncthreads();
-        for (int rr = 0; rr < RB; ++rr) {
-            const float d = s_dz[rr][o];
+    for (int r0 = blockIdx.x * kWgRows; r0 < n; r0 += gridDim.x * kWgRows) {
+        float a[G];
+        uint16_t x[G][T];
 #pragma unroll
-            for (int q = 0; q < PER; ++q) acc[q] += d * s_x[rr][kb + q];
-            accb += d;
+        for (int g = 0; g < G; ++g) {
+            const int i = r0 + 4 * g + ir;
+            const bool ok = i < n;
+            a[g] = ok ? dz[(size_t)i * kW + o] : 0.f;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int c = 16 * t + (lane & 15);
+                // 16 t < K1 is the same for every lane (K1 is a multiple of 16)
+                x[g][t] = !ok ? (uint16_t)0 : 16 * t < K1 ? X1[(size_t)i * K1 + c] : X2[(size_t)i * K2 + (c - K1)];
+            }
         }
-        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            accb += a[g];
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[g], bf2f(x[g][t]), acc[t], 0, 0, 0);
+        }
     }
+    // the lanes holding o for the four row offsets
+    accb += __shfl_xor(accb, 16, 64);
+    accb += __shfl_xor(accb, 32, 64);
     float *p = partial + (size_t)blockIdx.x * (kW * K + kW);
 #pragma unroll
-    for (int q = 0; q < PER; ++q) p[o * K + kb + q] = acc[q];
-    if (kb == 0) p[kW * K + o] = accb;
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[(size_t)(16 * wave + 4 * ir + j) * K + 16 * t + (lane & 15)] = acc[t][j];
+    if (ir == 0) p[kW * K + o] = accb;
 }
 
 __global__ __launch_bounds__(256) void k_sum_partials(int nparts, int len, const float *partial, float *G, int goff_w,
@@ -1377,7 +1415,9 @@ int vbd_mpnn_create(vbd_solver_t *s, int32_t sample_capacity, vbd_mpnn_t **out)
         m->guess_chunks = std::max(1, nblk(mx, kGuessChunk));
     }
     m->wg_blocks = std::max(1, std::min(512, nblk(n, 32)));
-    m->out_blocks = std::max(1, std::min(2048, nblk(n, 16)));
+    // output-layer blocks: each folds its rows into one partial row, and
+    // k_sum_out adds the partials (more blocks than CUs only lengthen that)
+    m->out_blocks = std::max(1, std::min(256, nblk(n, 16)));
     const size_t part = std::max((size_t)m->wg_blocks * (kW * 2 * kW + kW), (size_t)m->out_blocks * kOutPart);
     if ((rc = dalloc(&m->partial, part))) return fail(rc);
     if (hipHostMalloc((void **)&m->use_alt_host, sizeof(int32_t) * s->n_bodies, hipHostMallocDefault) != hipSuccess ||
