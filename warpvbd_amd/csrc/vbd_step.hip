@@ -531,7 +531,6 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
     a.ns = std::max<int64_t>(s->n_slots, 1);
     a.ctrl = s->ctrl;
     a.kd_dt = s->kd_dt;
-    const int block = s->launch.block;
     for (size_t k = 0; k < s->segments.size(); ++k) {
         const Segment &g = s->segments[k];
         a.part = s->part + g.part_off;
