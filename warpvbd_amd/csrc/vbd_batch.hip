@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0
     if (threadIdx.x < kFoldBodies) s_bm[threadIdx.x] = 0u;
     const int b0 = body_new[v0 + b * kStageRows];
     // (the whole block returns together)
-    if (!staged_sweep<MAT, true>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, body_new, &ph, rows, body, m)) return;
+    if (!staged_sweep<MAT, true, false>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, body_new, &ph, rows, body, m)) return;
     body_metric_max(body, m, ph.macc, s_bm, b0, kFoldBodies);
     __syncthreads();
     if (threadIdx.x < kFoldBodies && s_bm[threadIdx.x]) atomicMax(&ph.macc[b0 + threadIdx.x], s_bm[threadIdx.x]);

@@ -25,6 +25,14 @@ constexpr int kStageDict = 64;    // dictionary entries staged with them (48 byt
 // so every lane walks 5 slots and a colour needs fewer waves than at 4 lanes
 // for every vertex.
 constexpr int kLaneSlots = 5;
+// The lane classes are used when at least this share of the big colours'
+// vertices take 4 lanes (interior-dominated bodies: 88% on the 256x32x32
+// beam).  Ordering vertices by class separates a body's surface vertices from
+// its interior ones, which costs L2 locality; with many small bodies in one
+// mesh (57% on 512 concatenated mass-springs) that cost outweighs the waves
+// saved (measured 126 against 117 us per iteration), so such meshes keep 4
+// lanes for every vertex.
+constexpr double kLaneClassMinShare = 0.75;
 // Colours below this many vertices are not staged: the staging pass adds a
 // fixed latency per block, repaid only when a colour gives each of the 256
 // CUs two or more blocks (measured per big-colour launch at 256 threads: 9.3k

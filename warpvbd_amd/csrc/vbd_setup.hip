@@ -903,7 +903,26 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
     int32_t *d_body_of;
     VBD_TRY(scr.get(&d_body_of, n));
     VBD_HIP(hipMemcpyAsync(d_body_of, body_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
-    const int ncls = s->batched ? 1 : 3;
+    int ncls = 1;
+    if (!s->batched) {
+        // lane classes only for interior-dominated big colours (kLaneClassMinShare)
+        std::vector<int32_t> hc(n), ho(n + 1);
+        VBD_HIP(hipMemcpy(hc.data(), s->colors, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        VBD_HIP(hipMemcpy(ho.data(), s->inc_off, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+        std::vector<int64_t> nvc(std::max(s->ncolors, 1), 0), n4c(std::max(s->ncolors, 1), 0);
+        for (int v = 0; v < n; ++v)
+            if (act[v] && hc[v] >= 0 && hc[v] < s->ncolors) {
+                ++nvc[hc[v]];
+                n4c[hc[v]] += lane_class(ho[v + 1] - ho[v]) == 0;
+            }
+        int64_t big = 0, big4 = 0;
+        for (int c = 0; c < s->ncolors; ++c)
+            if (nvc[c] >= kStageMinColor) {
+                big += nvc[c];
+                big4 += n4c[c];
+            }
+        if (big > 0 && (double)big4 >= kLaneClassMinShare * (double)big) ncls = 3;
+    }
     const int64_t nkeys = (int64_t)(kMaxColors + 1) * nb * ncls;
     unsigned end_bit = 1;
     while (((int64_t)1 << end_bit) < nkeys) ++end_bit;

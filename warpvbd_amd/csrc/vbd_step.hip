@@ -148,18 +148,19 @@ __global__ __launch_bounds__(1024) void k_sweep(
 // neighbours in LDS (staged_sweep in vbd_sweep.h), the others read them from
 // global memory (lanes_sweep).  The block's metric maximum goes to a.part, and
 // the iteration's last launch folds them all.
-template <int MAT>
+template <int MAT, bool WIDE>
 __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int nv, LaneLayout lay, int close,
                                                       const uint32_t *part_all, int np_all, float tol,
                                                       const int32_t *st_nb, const uint2 *st_rec, int n_dict)
 {
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = lane_block(blockIdx.x, lay);
     uint32_t m;
     if (b >= lay.b1) {
         if (!lanes_sweep<MAT>(a, v0, nv, lay, b, m)) return;
     } else {
         int body;
-        if (!staged_sweep<MAT, false>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, nullptr, nullptr, nullptr, body, m))
+        if (!staged_sweep<MAT, false, WIDE>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, nullptr, nullptr, nullptr, body,
+                                            m))
             return;
     }
     const uint32_t mb = block_max_u32(m);
@@ -374,12 +375,12 @@ void launch_staged(int material, const SweepArgs &a, int v0, int nv, const LaneL
                    int n_dict, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
 {
     const dim3 grid(lay.nblocks);
-    if (material == kSNH)
-        hipExtLaunchKernelGGL(k_sweep_staged<kSNH>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, lay, close, part_all,
-                              np_all, tol, st_nb, st_rec, n_dict);
-    else
-        hipExtLaunchKernelGGL(k_sweep_staged<kStVK>, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, lay, close,
-                              part_all, np_all, tol, st_nb, st_rec, n_dict);
+    // the larger lists (and LDS) only when a staged block holds 2-lane vertices
+    const bool wide = lay.b4 < lay.b1;
+    auto kern = material == kSNH ? (wide ? k_sweep_staged<kSNH, true> : k_sweep_staged<kSNH, false>)
+                                 : (wide ? k_sweep_staged<kStVK, true> : k_sweep_staged<kStVK, false>);
+    hipExtLaunchKernelGGL(kern, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, lay, close, part_all, np_all, tol, st_nb,
+                          st_rec, n_dict);
 }
 
 void launch_small(int material, const SweepArgs &a, int v0, int nv, int block, int close, const uint32_t *part_all,

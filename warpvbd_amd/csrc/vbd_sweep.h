@@ -62,6 +62,25 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// The block of a lane layout that hardware block i sweeps.  Blocks are
+// dispatched round-robin over the 8 XCDs (hardware block i on XCD i % 8, as
+// its (i / 8)-th).  Each XCD takes one contiguous share of the blocks holding
+// 1- or 2-lane vertices (from lay.b4 on: heavier staging lists, or unstaged),
+// as even as the count allows, and sweeps it first, then one contiguous share
+// of the 4-lane blocks, the rest of its blocks: every XCD gets its part of
+// the heavier blocks, and its L2 sees compact neighbourhoods.
+__device__ __forceinline__ int lane_block(int i, const LaneLayout &lay)
+{
+    const int nb = lay.nblocks, nh = nb - lay.b4;
+    const int x = i & 7, j = i >> 3;
+    const int hq = nh >> 3, hr = nh & 7, cq = nb >> 3, cr = nb & 7;
+    const int hx = hq + (x < hr);               // this XCD's share of the heavier blocks
+    const int h0 = x * hq + min(x, hr);         // ... starting here
+    if (j < hx) return lay.b4 + h0 + j;
+    const int f0 = x * cq + min(x, cr) - h0;    // its 4-lane share starts after the other XCDs' shares
+    return f0 + (j - hx);
+}
+
 // &base[i] for a 32-bit unsigned element index (arrays here hold fewer than
 // 2^26 float4: create rejects larger meshes), formed as base + 16 i in 32 bits
 // so that a load can take the base from scalar registers and the offset from
@@ -486,15 +505,16 @@ __device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
 // no body of the block is live (the whole block then does nothing); m
 // receives the lane's metric bits (0 for lanes without an updated vertex),
 // body the vertex's body.
-template <int MAT, bool BODIES>
+// WIDE: the launch has blocks of 2-lane vertices, whose lists hold up to
+// kStageMaxWide neighbours (more LDS per block); otherwise kStageMax.
+template <int MAT, bool BODIES, bool WIDE>
 __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, const LaneLayout &lay, int b,
                                              const int32_t *st_nb, const uint2 *st_rec, int n_dict,
                                              const int32_t *body_new, const BodyPhase *ph, const int32_t *rows,
                                              int &body_out, uint32_t &m)
 {
-    // a batch's blocks hold 4-lane vertices only; a single body's may stage up
-    // to kStageMaxWide neighbours
-    constexpr int kCap = BODIES ? kStageMax : kStageMaxWide;
+    static_assert(!(BODIES && WIDE), "a batch's blocks hold 4-lane vertices only");
+    constexpr int kCap = WIDE ? kStageMaxWide : kStageMax;
     constexpr int kStageUnroll = kCap / 256;
     static_assert(kStageMax % 256 == 0 && kStageMaxWide % 256 == 0 && kStageMaxWide >= kStageMax,
                   "each of the 256 threads stages kStageUnroll neighbours");
