@@ -234,8 +234,9 @@ void vbd_use_record_dictionary(int32_t enable);
 int vbd_records(vbd_solver_t *h, int32_t *n_dictionary, int32_t *bytes_per_slot);
 /* Neighbour staging with a shared dictionary: colours of at least 40,960
  * vertices swept at the default launch (4 lanes per vertex, 256 threads) read
- * each block's distinct neighbours and the dictionary once into LDS and an
- * 8-byte record per slot (the results are the same bit for bit).  In a
+ * each block's neighbours once, form each (vertex, neighbour) difference once
+ * into LDS with the dictionary, and read a 4-byte record per slot (the
+ * results are the same bit for bit).  In a
  * single-body solver those colours also give each vertex 4, 2 or 1 lanes by
  * its slot count (more than 10, 6..10, at most 5), so that a lane walks at
  * most 5 slots of a voxel mesh; the blocks of 1-lane vertices read their
@@ -250,8 +251,9 @@ void vbd_use_neighbour_staging(int32_t enable);
  * with this layout, sum_v (68 + bytes_per_slot k_v) + 16 |gathered neighbours|:
  * per swept vertex its 16-byte packed position, 32-byte FP64 inertia target,
  * 4-byte slot offset and 16-byte write-back; per gathered neighbour its packed
- * position; with neighbour staging (vbd_staged_colors) 8 bytes per slot plus
- * 4 per staged neighbour id (vbd_color_bytes is the algorithmic model: 40
+ * position; with neighbour staging (vbd_staged_colors) 4 more per vertex,
+ * 4 bytes per slot and 4 per staged (vertex, neighbour) entry
+ * (vbd_color_bytes is the algorithmic model: 40
  * bytes per vertex, the 68 bytes per slot of the reference's per-tet arrays,
  * 12 per neighbour). */
 int vbd_color_layout_bytes(vbd_solver_t *h, double *bytes);

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t
 // hold a live body's vertex), else all of them.
 template <int MAT>
 __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0, int nv, LaneLayout lay,
-                                                             const int32_t *st_nb, const uint2 *st_rec, int n_dict,
+                                                             StageTables st, int n_dict,
                                                              const int32_t *body_new, BodyPhase ph, BodyFold f,
                                                              const int32_t *rows, const int2 *blist)
 {
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0
     if (threadIdx.x < kFoldBodies) s_bm[threadIdx.x] = 0u;
     const int b0 = body_new[v0 + b * kStageRows];
     // (the whole block returns together)
-    if (!staged_sweep<MAT, true, false>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, body_new, &ph, rows, body, m)) return;
+    if (!staged_sweep<MAT, true, false>(a, v0, nv, lay, b, st, n_dict, body_new, &ph, rows, body, m)) return;
     body_metric_max(body, m, ph.macc, s_bm, b0, kFoldBodies);
     __syncthreads();
     if (threadIdx.x < kFoldBodies && s_bm[threadIdx.x]) atomicMax(&ph.macc[b0 + threadIdx.x], s_bm[threadIdx.x]);
@@ -297,10 +297,12 @@ void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *
     const dim3 grid(nmain + f.nfold);
     const int32_t *rows = s->row_live + s->row_base[c];
     if (s->material == kSNH)
-        k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c], s->st_nb, s->st_rec,
+        k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c],
+                                                                StageTables{s->st_nb, s->st_rec, s->st_voff},
                                                                 s->n_dict, s->body_new, ph, f, rows, blist);
     else
-        k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c], s->st_nb, s->st_rec,
+        k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c],
+                                                                 StageTables{s->st_nb, s->st_rec, s->st_voff},
                                                                  s->n_dict, s->body_new, ph, f, rows, blist);
 }
 

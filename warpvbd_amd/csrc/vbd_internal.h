@@ -16,8 +16,11 @@ constexpr int kMaxColors = 256;            // src/coloring.py:50
 constexpr int kMaxElementsPerVertex = 32;  // src/vbd_solver.py:12
 // neighbour staging of the big-colour sweep (256 threads per block)
 constexpr int kStageRows = 64;    // vertices per block of 4-lane vertices (the batched sweep's rows)
-constexpr int kStageMax = 1024;   // distinct neighbours a block of 4-lane vertices may stage (LDS: 24 bytes each)
-constexpr int kStageMaxWide = 1280;  // the same for a block holding 2-lane vertices (128 per block)
+// A staged block lists, vertex by vertex, each vertex's distinct neighbours
+// (an entry per (vertex, neighbour) pair; 16 bytes of LDS each):
+constexpr int kStageMax = 1152;      // entries of a block of 4-lane vertices (64 x degree 18)
+constexpr int kStageMaxWide = 1408;  // entries of a block holding 2-lane vertices (128 per block)
+constexpr int kStageEntryBits = 5;   // a vertex has at most 31 neighbours in its list (slot records' field)
 constexpr int kStageDict = 64;    // dictionary entries staged with them (48 bytes each)
 // Lanes per vertex of a single body's big colours, by slot count: at most
 // kLaneSlots slots per lane.  A voxel mesh's interior vertices have 20 slots
@@ -120,7 +123,7 @@ __host__ __device__ inline int lane_class(int k) { return k > 2 * kLaneSlots ? 0
 
 // The launch layout of one big colour: n4, n2, n1 vertices of each class;
 // 256-thread blocks over the colour's lanes; blocks below b4 hold only 4-lane
-// vertices and stage up to kStageMax neighbours, blocks in [b4, b1) up to
+// vertices and stage up to kStageMax entries, blocks in [b4, b1) up to
 // kStageMaxWide; blocks from b1 on (those holding 1-lane vertices, or every
 // block when the colour has no staging lists) read their neighbours from
 // global memory.  ids: first neighbour id of the colour's lists in st_nb.
@@ -167,8 +170,8 @@ __host__ __device__ inline int lane_vertex(int64_t t, int n4, int n2, int &g, in
     return n4 + n2 + (int)(t - t2);
 }
 
-// First neighbour id of block b's list (blocks below b4 hold kStageMax ids,
-// the rest kStageMaxWide).
+// First entry of block b's list (blocks below b4 hold kStageMax entries, the
+// rest kStageMaxWide).
 __host__ __device__ inline int64_t stage_list_offset(int b, int b4)
 {
     return b < b4 ? (int64_t)b * kStageMax : (int64_t)b4 * kStageMax + (int64_t)(b - b4) * kStageMaxWide;
@@ -290,9 +293,10 @@ struct vbd_solver {
     // plus the dictionary entry.
     bool st_ok = false;
     std::vector<vbd::LaneLayout> st_col;  // per colour: its layout and lists (nblocks 0: not a big colour)
-    std::vector<int64_t> st_ids;   // per colour: neighbour ids its blocks stage (sum over blocks)
-    int32_t *st_nb = nullptr;      // neighbour id lists (stage_list_offset), -1 past each block's list
-    uint2 *st_rec = nullptr;       // [n_slots] (l0 | l1 << 16, l2 | entry << 16)
+    std::vector<int64_t> st_ids;   // per colour: entries its blocks stage (sum over blocks)
+    int32_t *st_nb = nullptr;      // entry lists (stage_list_offset): neighbour id | owner << 25, -1 past a list
+    uint32_t *st_rec = nullptr;    // [n_slots] n0 | n1 << 5 | n2 << 10 | entry << 15 (n_k: the vertex's list)
+    int32_t *st_voff = nullptr;    // [n_swept] a vertex's first entry in its block's list
     int32_t n_dict = 0;
     bool dict_shared = false;     // entries shared between slots (else one per slot)
     // Colours swept with 32 lanes per vertex (below launch.small_color): their

@@ -1095,12 +1095,16 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
 // least kStageMinColor vertices get their lane layout (lane_layout: a single
 // body's vertices take 4, 2 or 1 lanes by slot count, a batch's 4) and, with
 // a shared dictionary, staging lists: each 256-thread block below the
-// layout's b1 lists its distinct neighbour ids in order of first appearance
-// (kStageMax per block of 4-lane vertices, kStageMaxWide per block holding
-// 2-lane ones), and each of its slots gets (l0 | l1 << 16, l2 | entry << 16)
-// with l_k the local indices of the slot's other vertices.  A block with more
-// neighbours than its list holds turns staging off for the solver (the
-// layouts stay, and every block reads its neighbours from global memory).
+// layout's b1 lists, vertex by vertex, each vertex's distinct neighbours in
+// order of first appearance in its slots (entries id | owner << 25, owner the
+// vertex's index in the block; kStageMax entries per block of 4-lane
+// vertices, kStageMaxWide per block holding 2-lane ones), each vertex gets
+// its first entry's index in the list (st_voff), and each of its slots gets
+// n0 | n1 << 5 | n2 << 10 | entry << 15 with n_k the index of the slot's k-th
+// other vertex in the vertex's own list.  A block with more entries than its
+// list holds, a vertex with more than 31 neighbours, or a mesh of 2^25 - 1
+// vertices or more turns staging off for the solver (the layouts stay, and
+// every block reads its neighbours from global memory).
 int build_staging(vbd_solver *s)
 {
     s->st_ok = false;
@@ -1111,7 +1115,8 @@ int build_staging(vbd_solver *s)
         const int nv = s->col_size[c];
         if (nv >= min_color) s->st_col[c] = lane_layout(s->col_n42[c].x, s->col_n42[c].y, nv, false, -1);
     }
-    if (!g_staging || !s->dict_shared || s->n_dict > kStageDict || s->n_slots <= 0) return 0;
+    // (ids below 2^25 - 1, so that no entry reads as the -1 that ends a list)
+    if (!g_staging || !s->dict_shared || s->n_dict > kStageDict || s->n_slots <= 0 || s->n >= (1 << 25) - 1) return 0;
     hipStream_t st = s->stream;
     const int nsw = s->n_swept;
     std::vector<int32_t> off(nsw + 1);
@@ -1119,12 +1124,14 @@ int build_staging(vbd_solver *s)
     VBD_HIP(hipMemcpyAsync(off.data(), s->slot_off, sizeof(int32_t) * (nsw + 1), hipMemcpyDeviceToHost, st));
     VBD_HIP(hipMemcpyAsync(rec.data(), s->slots, sizeof(float4) * s->n_slots, hipMemcpyDeviceToHost, st));
     VBD_HIP(hipStreamSynchronize(st));
-    std::vector<int32_t> nb;  // the blocks' lists, -1 past each block's neighbours
-    std::vector<uint2> srec(s->n_slots, make_uint2(0u, 0u));
+    constexpr uint32_t kNbMask = (1u << kStageEntryBits) - 1u;
+    std::vector<int32_t> nb;  // the blocks' lists, -1 past each block's entries
+    std::vector<uint32_t> srec(s->n_slots, 0u);
+    std::vector<int32_t> voff(std::max(nsw, 1), 0);
     std::vector<int32_t> stamp(s->n, -1), local(s->n, 0);
     std::vector<LaneLayout> col(s->ncolors);
     std::vector<int64_t> ids(s->ncolors, 0);
-    int nblocks = 0;  // blocks listed so far (the stamp of the current one)
+    int nblocks = 0;
     for (int c = 0; c < s->ncolors; ++c) {
         const int nv = s->col_size[c];
         if (nv < min_color) continue;
@@ -1137,25 +1144,30 @@ int build_staging(vbd_solver *s)
             int g, lg;
             const int va = c0 + lane_vertex(256 * (int64_t)b, n4, n2, g, lg);
             const int vb = c0 + lane_vertex(std::min<int64_t>(256 * (int64_t)b + 256, lanes) - 1, n4, n2, g, lg) + 1;
+            if (vb - va > 128) return 0;  // the owner field holds 7 bits
             const size_t base = (size_t)(l.ids + stage_list_offset(b, l.b4));
             const int cap = b < l.b4 ? kStageMax : kStageMaxWide;
             int cnt = 0;
-            for (int64_t q = off[va]; q < off[vb]; ++q) {
-                uint32_t w[4];
-                std::memcpy(w, &rec[q], sizeof(w));
-                uint32_t lk[3];
-                for (int k = 0; k < 3; ++k) {
-                    const int id = (int)w[k];
-                    if (stamp[id] != nblocks) {
-                        if (cnt == cap) return 0;
-                        stamp[id] = nblocks;
-                        local[id] = cnt;
-                        nb[base + cnt] = id;
-                        ++cnt;
+            for (int v = va; v < vb; ++v) {
+                voff[v] = cnt;
+                int nd = 0;  // the vertex's distinct neighbours so far
+                for (int64_t q = off[v]; q < off[v + 1]; ++q) {
+                    uint32_t w[4];
+                    std::memcpy(w, &rec[q], sizeof(w));
+                    uint32_t lk[3];
+                    for (int k = 0; k < 3; ++k) {
+                        const int id = (int)w[k];
+                        if (stamp[id] != v) {
+                            if (cnt == cap || nd == (int)kNbMask + 1) return 0;
+                            stamp[id] = v;
+                            local[id] = nd++;
+                            nb[base + cnt++] = (int32_t)((uint32_t)id | (uint32_t)(v - va) << 25);
+                        }
+                        lk[k] = (uint32_t)local[id];
                     }
-                    lk[k] = (uint32_t)local[id];
+                    if (w[3] >= (1u << 17)) return 0;
+                    srec[q] = lk[0] | lk[1] << 5 | lk[2] << 10 | w[3] << 15;
                 }
-                srec[q] = make_uint2(lk[0] | lk[1] << 16, lk[2] | w[3] << 16);
             }
             ids[c] += cnt;
         }
@@ -1164,11 +1176,13 @@ int build_staging(vbd_solver *s)
     if (nblocks == 0) return 0;
     // zero records past the last slot: the sweep loads a lane's records
     // without a bound (the ones past its vertex's last slot are not used)
-    srec.resize(srec.size() + kMaxElementsPerVertex, make_uint2(0u, 0u));
+    srec.resize(srec.size() + kMaxElementsPerVertex, 0u);
     VBD_TRY(dmalloc(&s->st_nb, nb.size()));
     VBD_TRY(dmalloc(&s->st_rec, srec.size()));
+    VBD_TRY(dmalloc(&s->st_voff, voff.size()));
     VBD_HIP(hipMemcpyAsync(s->st_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice, st));
-    VBD_HIP(hipMemcpyAsync(s->st_rec, srec.data(), sizeof(uint2) * srec.size(), hipMemcpyHostToDevice, st));
+    VBD_HIP(hipMemcpyAsync(s->st_rec, srec.data(), sizeof(uint32_t) * srec.size(), hipMemcpyHostToDevice, st));
+    VBD_HIP(hipMemcpyAsync(s->st_voff, voff.data(), sizeof(int32_t) * voff.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipStreamSynchronize(st));
     for (int c = 0; c < s->ncolors; ++c)
         if (col[c].nblocks > 0) s->st_col[c] = col[c];

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(1024) void k_sweep(
 template <int MAT, bool WIDE>
 __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int nv, LaneLayout lay, int close,
                                                       const uint32_t *part_all, int np_all, float tol,
-                                                      const int32_t *st_nb, const uint2 *st_rec, int n_dict)
+                                                      StageTables st, int n_dict)
 {
     const int b = lane_block(blockIdx.x, lay);
     uint32_t m;
@@ -159,8 +159,7 @@ __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int n
         if (!lanes_sweep<MAT>(a, v0, nv, lay, b, m)) return;
     } else {
         int body;
-        if (!staged_sweep<MAT, false, WIDE>(a, v0, nv, lay, b, st_nb, st_rec, n_dict, nullptr, nullptr, nullptr, body,
-                                            m))
+        if (!staged_sweep<MAT, false, WIDE>(a, v0, nv, lay, b, st, n_dict, nullptr, nullptr, nullptr, body, m))
             return;
     }
     const uint32_t mb = block_max_u32(m);
@@ -371,16 +370,16 @@ void launch_sweep(const SweepArgs &a, int v0, int nv, int block, int close, cons
 }
 
 void launch_staged(int material, const SweepArgs &a, int v0, int nv, const LaneLayout &lay, int close,
-                   const uint32_t *part_all, int np_all, float tol, const int32_t *st_nb, const uint2 *st_rec,
-                   int n_dict, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
+                   const uint32_t *part_all, int np_all, float tol, const StageTables &tables, int n_dict,
+                   hipEvent_t e0, hipEvent_t e1, hipStream_t st)
 {
     const dim3 grid(lay.nblocks);
     // the larger lists (and LDS) only when a staged block holds 2-lane vertices
     const bool wide = lay.b4 < lay.b1;
     auto kern = material == kSNH ? (wide ? k_sweep_staged<kSNH, true> : k_sweep_staged<kSNH, false>)
                                  : (wide ? k_sweep_staged<kStVK, true> : k_sweep_staged<kStVK, false>);
-    hipExtLaunchKernelGGL(kern, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, lay, close, part_all, np_all, tol, st_nb,
-                          st_rec, n_dict);
+    hipExtLaunchKernelGGL(kern, grid, dim3(256), 0, st, e0, e1, 0, a, v0, nv, lay, close, part_all, np_all, tol, tables,
+                          n_dict);
 }
 
 void launch_small(int material, const SweepArgs &a, int v0, int nv, int block, int close, const uint32_t *part_all,
@@ -546,7 +545,8 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
                                  ? s->small_rec + 32 * s->small_off[g.c0]
                                  : nullptr;
         if (lay)
-            launch_staged(s->material, a, v0, nv, *lay, close, s->part, s->n_parts, tol, s->st_nb, s->st_rec,
+            launch_staged(s->material, a, v0, nv, *lay, close, s->part, s->n_parts, tol,
+                          StageTables{s->st_nb, s->st_rec, s->st_voff},
                           s->n_dict, ev[0], ev[1], s->stream);
         else if (prec)
             launch_small(s->material, a, v0, nv, g.block, close, s->part, s->n_parts, tol, prec, ev[0], ev[1],

@@ -142,28 +142,35 @@ __device__ __forceinline__ double4 unpack_disp4(const float4 &p)
 
 // The differences slot_eval takes, from the packed displacements x0, x1, x2
 // of a slot's other vertices and the swept vertex's FP32 parts hi and
-// remainders lo: e0 = u_o0 - u_o2, e1 = u_o1 - u_o2, d2 = u_o2 - u_self.  The
-// FP32 parts and the remainders are differenced separately, so each result
-// carries the rounding of one FP32 operation on the difference, not that of
-// the stored values.
+// remainders lo: with d_k = u_ok - u_self (the FP32 parts and the remainders
+// differenced separately, so d_k carries the rounding of one FP32 operation
+// on the difference, not that of the stored values), e0 = d_0 - d_2,
+// e1 = d_1 - d_2 and d2 = d_2.  A vertex's d_k depends only on the vertex
+// and the neighbour, so the staged sweep forms it once per (vertex,
+// neighbour) pair (neighbour_diff) and every slot reads it.
 struct SlotRel {
     float3 e0, e1, d2;
 };
 
-__device__ __forceinline__ float3 diff_packed(const float4 &x, const float3 &lx, const float4 &y, const float3 &ly)
+__device__ __forceinline__ float3 neighbour_diff(const float4 &x, const float3 &lx, const float3 &hi, const float3 &lo)
 {
-    return make_float3((x.x - y.x) + (lx.x - ly.x), (x.y - y.y) + (lx.y - ly.y), (x.z - y.z) + (lx.z - ly.z));
+    return make_float3((x.x - hi.x) + (lx.x - lo.x), (x.y - hi.y) + (lx.y - lo.y), (x.z - hi.z) + (lx.z - lo.z));
+}
+
+__device__ __forceinline__ SlotRel slot_rel_d(const float3 &d0, const float3 &d1, const float3 &d2)
+{
+    SlotRel r;
+    r.e0 = make_float3(d0.x - d2.x, d0.y - d2.y, d0.z - d2.z);
+    r.e1 = make_float3(d1.x - d2.x, d1.y - d2.y, d1.z - d2.z);
+    r.d2 = d2;
+    return r;
 }
 
 __device__ __forceinline__ SlotRel slot_rel(const float4 &x0, const float4 &x1, const float4 &x2, const float3 &hi,
                                             const float3 &lo)
 {
-    const float3 l0 = disp_lo(x0), l1 = disp_lo(x1), l2 = disp_lo(x2);
-    SlotRel r;
-    r.e0 = diff_packed(x0, l0, x2, l2);
-    r.e1 = diff_packed(x1, l1, x2, l2);
-    r.d2 = make_float3((x2.x - hi.x) + (l2.x - lo.x), (x2.y - hi.y) + (l2.y - lo.y), (x2.z - hi.z) + (l2.z - lo.z));
-    return r;
+    return slot_rel_d(neighbour_diff(x0, disp_lo(x0), hi, lo), neighbour_diff(x1, disp_lo(x1), hi, lo),
+                      neighbour_diff(x2, disp_lo(x2), hi, lo));
 }
 
 // A vertex's own data, loaded before anything that depends on other loads so
@@ -482,22 +489,35 @@ __device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
 }
 
 
+// The staging tables of a solver (build_staging): per block a list of
+// entries, vertex by vertex each vertex's distinct neighbours (neighbour id |
+// owner << 25, the owner the vertex's index in the block; -1 past the list);
+// per vertex its first entry; per slot n0 | n1 << 5 | n2 << 10 | entry << 15,
+// n_k the index of its k-th other vertex in the vertex's entries and entry
+// its record dictionary entry.
+struct StageTables {
+    const int32_t *nb;
+    const uint32_t *rec;
+    const int32_t *voff;
+};
+
 // One block of the staged colour sweep: 256 threads over lanes
 // [256 b, 256 b + 256) of a colour of nv vertices at v0 laid out by lay
 // (lane_vertex: 4, 2 or 1 lanes per vertex by slot count; a batch's colours
 // have 4 lanes for every vertex, blocks of kStageRows).  Two memory rounds,
-// then arithmetic only:
-//   1. the vertex's own data, the block's neighbour ids (st_nb at lay.ids +
-//      stage_list_offset: kStageMax ids per block below lay.b4, kStageMaxWide
-//      from there, -1 past its list) and the record dictionary;
-//   2. the neighbours' positions and every 8-byte slot record of the lane
-//      (st_rec: (l0 | l1 << 16, l2 | entry << 16), local indices into the
-//      block's list and the dictionary entry, with kStagedSlots * 4 zero
-//      records past the last slot, so a lane reads its 8 without a bound).
-// The neighbours are unpacked into LDS (FP32 parts and remainders) with the
-// dictionary, and the slot loop reads LDS only.  Neighbours belong to other
-// colours, which the launch does not write, so staging them first is exact;
-// the arithmetic is lanes_sweep's, so the results are the same bit for bit.
+// then the neighbour differences, then arithmetic only:
+//   1. the vertex's own data and first entry, the block's entries (at lay.ids
+//      + stage_list_offset: kStageMax per block below lay.b4, kStageMaxWide
+//      from there) and the record dictionary;
+//   2. the entries' neighbour positions and every 4-byte slot record of the
+//      lane (kStagedSlots * 4 zero records past the last slot, so a lane
+//      reads its 8 without a bound).
+// Each vertex's own displacement goes to LDS, then each entry's difference
+// d = u_neighbour - u_vertex (neighbour_diff) is formed once and stored in
+// LDS, and each slot reads its three differences and its dictionary entry
+// from LDS (slot_rel_d).  The neighbours belong to other colours, which the
+// launch does not write, so staging them first is exact, and the arithmetic
+// is slot_rel's, so the results are the same bit for bit as lanes_sweep's.
 // BODIES: a batched solver's launch -- a vertex of a body that has converged
 // is not updated (its lanes get body -1), instead of the step's convergence
 // flag being tested, and rows[b] counts the live bodies with a vertex in the
@@ -506,41 +526,48 @@ __device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
 // receives the lane's metric bits (0 for lanes without an updated vertex),
 // body the vertex's body.
 // WIDE: the launch has blocks of 2-lane vertices, whose lists hold up to
-// kStageMaxWide neighbours (more LDS per block); otherwise kStageMax.
+// kStageMaxWide entries (more LDS per block); otherwise kStageMax.
 template <int MAT, bool BODIES, bool WIDE>
 __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, const LaneLayout &lay, int b,
-                                             const int32_t *st_nb, const uint2 *st_rec, int n_dict,
-                                             const int32_t *body_new, const BodyPhase *ph, const int32_t *rows,
-                                             int &body_out, uint32_t &m)
+                                             const StageTables &st, int n_dict, const int32_t *body_new,
+                                             const BodyPhase *ph, const int32_t *rows, int &body_out, uint32_t &m)
 {
     static_assert(!(BODIES && WIDE), "a batch's blocks hold 4-lane vertices only");
     constexpr int kCap = WIDE ? kStageMaxWide : kStageMax;
-    constexpr int kStageUnroll = kCap / 256;
-    static_assert(kStageMax % 256 == 0 && kStageMaxWide % 256 == 0 && kStageMaxWide >= kStageMax,
-                  "each of the 256 threads stages kStageUnroll neighbours");
+    constexpr int kStageUnroll = (kCap + 255) / 256;
+    constexpr int kOwners = WIDE ? 128 : 64;  // vertices of a staged block
+    static_assert(kStageMaxWide >= kStageMax, "a wide list holds a narrow one");
     static_assert(3 * kStageDict <= 256, "one dictionary row per thread");
-    __shared__ float4 s_na[kCap];
-    __shared__ float2 s_nb[kCap];
+    __shared__ float4 s_d[kCap];           // per entry: d.xyz
+    __shared__ float4 s_own[kOwners];      // per vertex: FP32 parts .xyz, remainder x in .w
+    __shared__ float2 s_own2[kOwners];     // remainders y, z
     __shared__ float4 s_dict[3 * kStageDict];
     const int t = b * 256 + threadIdx.x;
-    int g, lg, vi;
+    int g, lg, vi, vfirst;
     if constexpr (BODIES) {
         vi = t >> 2;
         g = t & 3;
         lg = 2;
+        vfirst = b * 64;
     } else {
         vi = lane_vertex(t, lay.n4, lay.n2, g, lg);
+        int g0, lg0;
+        vfirst = lane_vertex(b * 256, lay.n4, lay.n2, g0, lg0);  // the block's first vertex
     }
     const bool in = vi < nv;
     const int v = v0 + (in ? vi : 0);
     // round 1
     const VertexPre pre = vertex_pre(a, v, in);
+    const int ov = st.voff[v];
     const bool wide = b >= lay.b4;  // the same for every thread
-    const int32_t *ids = st_nb + lay.ids + stage_list_offset(b, lay.b4);
+    const int cap = wide ? kStageMaxWide : kStageMax;
+    const int32_t *ids = st.nb + lay.ids + stage_list_offset(b, lay.b4);
     int id[kStageUnroll];
 #pragma unroll
-    for (int q = 0; q < kStageUnroll; ++q)
-        id[q] = 256 * q < kStageMax || wide ? ids[threadIdx.x + 256 * q] : -1;
+    for (int q = 0; q < kStageUnroll; ++q) {
+        const int k = (int)threadIdx.x + 256 * q;
+        id[q] = k < kCap && k < cap ? ids[k] : -1;
+    }
     const bool has_dict = (int)threadIdx.x < 3 * n_dict;
     const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
     int body = 0;
@@ -570,45 +597,50 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     if constexpr (BODIES) done = body_done(*ph, body);
     float4 p[kStageUnroll];
 #pragma unroll
-    for (int q = 0; q < kStageUnroll; ++q) p[q] = id[q] >= 0 ? a.u[id[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
-    uint2 rec[kStagedSlots];
-    const uint2 *rl = st_rec + pre.s0 + g;
+    for (int q = 0; q < kStageUnroll; ++q)
+        p[q] = id[q] != -1 ? a.u[(uint32_t)id[q] & 0x1FFFFFFu] : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t rec[kStagedSlots];
+    const uint32_t *rl = st.rec + pre.s0 + g;
 #pragma unroll
     for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[k << lg];
     if (has_dict) s_dict[threadIdx.x] = drow;
+    VertexLocal ql = vertex_local(pre);
+    const float3 hi = ql.hi, lo = ql.lo;
+    if (in && g == 0) {
+        s_own[vi - vfirst] = make_float4(hi.x, hi.y, hi.z, lo.x);
+        s_own2[vi - vfirst] = make_float2(lo.y, lo.z);
+    }
+    __syncthreads();
+    // each entry's difference to its vertex
 #pragma unroll
     for (int q = 0; q < kStageUnroll; ++q) {
-        if (id[q] >= 0) {
-            const int k = threadIdx.x + 256 * q;
-            const float3 lo = disp_lo(p[q]);
-            s_na[k] = make_float4(p[q].x, p[q].y, p[q].z, lo.x);
-            s_nb[k] = make_float2(lo.y, lo.z);
+        if (id[q] != -1) {
+            const int o = (int)((uint32_t)id[q] >> 25);
+            const float4 oh = s_own[o];
+            const float2 ol = s_own2[o];
+            const float3 d = neighbour_diff(p[q], disp_lo(p[q]), make_float3(oh.x, oh.y, oh.z),
+                                            make_float3(oh.w, ol.x, ol.y));
+            s_d[threadIdx.x + 256 * q] = make_float4(d.x, d.y, d.z, 0.f);
         }
     }
     __syncthreads();
     const bool valid = in && !done;
     Acc acc;
     acc_zero(acc);
-    VertexLocal ql = vertex_local(pre);
-    const float3 hi = ql.hi, lo = ql.lo;
     // this lane's slots: pre.s0 + g, + (1 << lg), ... below pre.s1
     const int nk = valid && pre.s1 > pre.s0 + g ? (pre.s1 - pre.s0 - g + (1 << lg) - 1) >> lg : 0;
+    constexpr uint32_t kM = (1u << kStageEntryBits) - 1u;
 #pragma unroll
     for (int k = 0; k < kStagedSlots; ++k) {
         if (k < nk) {
-            const uint2 r = rec[k];
-            const uint32_t e = r.y >> 16, l0 = r.x & 0xffffu, l1 = r.x >> 16, l2 = r.y & 0xffffu;
+            const uint32_t r = rec[k];
+            const uint32_t e = r >> 15, n0 = r & kM, n1 = (r >> 5) & kM, n2 = (r >> 10) & kM;
             const float4 d0 = s_dict[3 * e], d1 = s_dict[3 * e + 1], d2 = s_dict[3 * e + 2];
-            const float4 p0 = s_na[l0], p1 = s_na[l1], p2 = s_na[l2];
-            const float2 q0 = s_nb[l0], q1 = s_nb[l1], q2 = s_nb[l2];
-            // the differences slot_rel forms, from the unpacked parts
-            const float3 m0 = make_float3(p0.w, q0.x, q0.y), m1 = make_float3(p1.w, q1.x, q1.y),
-                         m2 = make_float3(p2.w, q2.x, q2.y);
-            const float3 e0 = diff_packed(p0, m0, p2, m2), e1 = diff_packed(p1, m1, p2, m2);
-            const float3 f2 = make_float3((p2.x - hi.x) + (m2.x - lo.x), (p2.y - hi.y) + (m2.y - lo.y),
-                                          (p2.z - hi.z) + (m2.z - lo.z));
+            const float4 q0 = s_d[ov + n0], q1 = s_d[ov + n1], q2 = s_d[ov + n2];
+            const SlotRel sr = slot_rel_d(make_float3(q0.x, q0.y, q0.z), make_float3(q1.x, q1.y, q1.z),
+                                          make_float3(q2.x, q2.y, q2.z));
             slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), e0, e1, f2, acc);
+                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, acc);
         }
     }
     double3 us;
