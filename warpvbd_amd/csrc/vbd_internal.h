@@ -288,9 +288,9 @@ struct vbd_solver {
     // class, (col_size, 0)).
     std::vector<int2> col_n42;
     // Neighbour staging for the 256-thread sweep of colours of at least
-    // kStageMinColor vertices (shared dictionary): per block its distinct
-    // neighbours, and per slot an 8-byte record of local indices into that list
-    // plus the dictionary entry.
+    // kStageMinColor vertices (shared dictionary): per block one entry per
+    // (vertex, neighbour), per vertex its first entry, and per slot a 4-byte
+    // record of indices into the vertex's entries plus the dictionary entry.
     bool st_ok = false;
     std::vector<vbd::LaneLayout> st_col;  // per colour: its layout and lists (nblocks 0: not a big colour)
     std::vector<int64_t> st_ids;   // per colour: entries its blocks stage (sum over blocks)
