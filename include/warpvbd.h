@@ -247,6 +247,19 @@ int vbd_staged_colors(vbd_solver_t *h, int32_t *n_colors);
  * off, the big colours keep their lanes per vertex and read every neighbour
  * from global memory). */
 void vbd_use_neighbour_staging(int32_t enable);
+/* The tail sweep: a single body's trailing colours below small_color (the
+ * last 2 to 4 of them; the 256x32x32 beam's colours of 407, 407 and 1
+ * vertices) swept in one launch instead of one per colour, when their
+ * vertices split into clusters joined by no edge (at most 32 vertices of a
+ * colour and 1,024 distinct neighbours each) and the record dictionary is
+ * shared: one 1024-thread block per cluster reads its neighbours once into
+ * LDS and sweeps the colours one after another from there (the results are
+ * the same bit for bit).  *n_colors: the colours it sweeps (0: none),
+ * *n_clusters: its blocks.  No counterpart in WarpVBD. */
+int vbd_tail_sweep(vbd_solver_t *h, int32_t *n_colors, int32_t *n_clusters);
+/* The tail sweep in solvers created, and launch settings changed, afterwards
+ * (process-wide; default on). */
+void vbd_use_tail_sweep(int32_t enable);
 /* bytes[n_colors]: the HBM bytes one sweep of each colour reads and writes
  * with this layout, sum_v (68 + bytes_per_slot k_v) + 16 |gathered neighbours|:
  * per swept vertex its 16-byte packed position, 32-byte FP64 inertia target,

@@ -271,17 +271,70 @@ def test_small_colour_kernel_is_bit_exact():
     small_color = 0 the same colours run through the 32-lane per-colour kernel
     that reads records by slot offset.  The arithmetic is the same, so
     positions, iteration counts and metrics agree bit for bit."""
+    from warpvbd_amd import _lib
     sc = cantilever((96, 12, 12))
     a = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
     b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
     sizes = a.color_sizes()
     assert sizes.max() > 2048 and sizes.min() < 2048
-    a.set_launch(32, 256, 2048, 0)
+    _lib.use_tail_sweep(False)  # a launch per small colour
+    try:
+        a.set_launch(32, 256, 2048, 0)
+    finally:
+        _lib.use_tail_sweep(True)
     b.set_launch(32, 256, 0, 0)
+    assert a.tail_sweep() == (0, 0) and b.tail_sweep() == (0, 0)
     for _ in range(6):
         xa = a.step(a.old_positions, 1e-3).numpy()
         xb = b.step(b.old_positions, 1e-3).numpy()
         assert a.last_iterations == b.last_iterations and a.last_metric == b.last_metric
+        assert np.array_equal(xa, xb)
+
+
+@pytest.mark.parametrize("material", ["snh", "stvk"])
+@pytest.mark.parametrize("dims,clusters", [((128, 16, 16), 23), ((96, 12, 12), None)])
+def test_tail_sweep_is_bit_exact(material, dims, clusters):
+    """The trailing small colours (107, 107 and 1 vertices on the 128x16x16
+    beam) swept in one launch by clusters of vertices joined by no edge, the
+    neighbours held in LDS, against a launch per colour: the same slot
+    arithmetic on the same values, so positions, iteration counts and
+    metrics agree bit for bit over tolerance-terminated steps."""
+    from warpvbd_amd import _lib
+    sc = cantilever(dims)
+    a = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
+    _lib.use_tail_sweep(False)
+    try:
+        b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
+    finally:
+        _lib.use_tail_sweep(True)
+    k, nc = a.tail_sweep()
+    assert k == 3 and nc > 0 and b.tail_sweep() == (0, 0)
+    if clusters is not None:
+        assert nc == clusters
+    for _ in range(6):
+        xa = a.step(a.old_positions, 1e-3).numpy()
+        xb = b.step(b.old_positions, 1e-3).numpy()
+        assert a.last_iterations == b.last_iterations and a.last_metric == b.last_metric
+        assert np.array_equal(xa, xb)
+
+
+def test_tail_sweep_follows_launch_settings():
+    """Launch settings rebuild the plan: with small_color = 0 no colour is
+    small and the tail sweep is off; the default settings bring it back, and
+    the steps agree with a solver that never changed them."""
+    sc = cantilever((64, 8, 8))
+    a = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
+    b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, verbose=False)
+    on = a.tail_sweep()
+    assert on[0] >= 2
+    a.set_launch(4, 256, 0, 0)
+    assert a.tail_sweep() == (0, 0)
+    a.set_launch(4, 256, 2048, 0)
+    assert a.tail_sweep() == on
+    for _ in range(3):
+        xa = a.step(a.old_positions, 1e-3).numpy()
+        xb = b.step(b.old_positions, 1e-3).numpy()
+        assert a.last_iterations == b.last_iterations
         assert np.array_equal(xa, xb)
 
 

@@ -67,6 +67,8 @@ SIGNATURES = {
     "vbd_color_layout_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_d)]),
     "vbd_records": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "vbd_staged_colors": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
+    "vbd_tail_sweep": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "vbd_use_tail_sweep": (None, [_i32]),
     "vbd_mpnn_guess_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "vbd_mpnn_set_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_get_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
@@ -163,6 +165,13 @@ def use_neighbour_staging(enable: bool = True) -> None:
     (default on; same results either way); applies to solvers created
     afterwards."""
     load().vbd_use_neighbour_staging(int(bool(enable)))
+
+
+def use_tail_sweep(enable: bool = True) -> None:
+    """Sweep a single body's trailing small colours in one launch by clusters
+    of vertices (default on; same results either way); applies to solvers
+    created, and launch settings changed, afterwards."""
+    load().vbd_use_tail_sweep(int(bool(enable)))
 
 
 def use_device(device) -> int:

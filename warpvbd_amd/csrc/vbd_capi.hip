@@ -41,7 +41,8 @@ void vbd_destroy(vbd_solver_t *s)
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict,
-                   s->st_nb, s->st_rec, s->st_voff, s->small_rec, s->u_last, s->d_dx_max};
+                   s->st_nb, s->st_rec, s->st_voff, s->small_rec, s->u_last, s->d_dx_max,
+                   s->tail_v, s->tail_rec, s->tail_nb};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (double4 *p : s->snap)
@@ -589,6 +590,16 @@ int vbd_profile_read(vbd_solver_t *s, double *ms, int64_t *launches, double *byt
 
 void vbd_use_record_dictionary(int32_t enable) { g_record_dictionary = enable != 0; }
 void vbd_use_neighbour_staging(int32_t enable) { g_staging = enable != 0; }
+void vbd_use_tail_sweep(int32_t enable) { g_tail = enable != 0; }
+
+int vbd_tail_sweep(vbd_solver_t *s, int32_t *n_colors, int32_t *n_clusters)
+{
+    CHECK_ARG(s && n_colors && n_clusters, "null argument");
+    const bool on = !s->segments.empty() && s->segments.back().tail;
+    *n_colors = on ? s->tail_k : 0;
+    *n_clusters = on ? s->tail_clusters : 0;
+    return 0;
+}
 
 int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
 {

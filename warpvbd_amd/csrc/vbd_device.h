@@ -194,6 +194,22 @@ __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, float bb
     }
 }
 
+// a + b, component by component
+__device__ __forceinline__ Acc acc_add(const Acc &a, const Acc &b)
+{
+    Acc r;
+    r.g0 = a.g0 + b.g0;
+    r.g1 = a.g1 + b.g1;
+    r.g2 = a.g2 + b.g2;
+    r.h00 = a.h00 + b.h00;
+    r.h01 = a.h01 + b.h01;
+    r.h02 = a.h02 + b.h02;
+    r.h11 = a.h11 + b.h11;
+    r.h12 = a.h12 + b.h12;
+    r.h22 = a.h22 + b.h22;
+    return r;
+}
+
 // x from another lane of the same 16-lane row by a DPP lane pattern
 // (quad_perm, row mirror), without a trip through LDS.
 template <int CTRL>

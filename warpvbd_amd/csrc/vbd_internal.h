@@ -42,6 +42,16 @@ constexpr double kLaneClassMinShare = 0.75;
 // vertices 7.35 vs 6.31 us unstaged, 30k 9.14 vs 7.83, 47k 10.35 vs 10.67,
 // 70k 13.05 vs 14.64).
 constexpr int kStageMinColor = 40960;
+// The tail sweep (k_sweep_tail): a single body's trailing colours of 32 lanes
+// per vertex swept in one launch, one block (32 vertices of a colour x a few
+// lanes each) per cluster of tail vertices joined by no edge to another
+// cluster, each cluster's neighbours held in LDS.
+constexpr int kTailNb = 1024;  // neighbours of a cluster (16 bytes of LDS each)
+constexpr int kTailRows = 32;  // vertices of one colour in a cluster
+constexpr int kMaxTail = 4;    // colours swept by one tail launch
+// at most two blocks per CU: a colour of many vertices joined to no tail
+// vertex would give a block to each of them (a 1024-thread block for a vertex)
+constexpr int kTailMaxClusters = 512;
 
 // Scratch of the allocation-free entry points (a file-scope device array in
 // vbd_setup.hip, one copy per GPU for the life of the process).
@@ -74,6 +84,7 @@ struct Ctrl {
     int32_t iters;      // iterations executed (index + 1 of the last one)
     uint32_t metric;    // bits of the last iteration's metric
     uint32_t blocks_done;  // blocks of the iteration's last launch that finished
+    uint32_t launch_max;   // the tail sweep's running metric maximum over its blocks
 };
 
 // Per-body convergence state of the running step (batched solver).
@@ -184,6 +195,7 @@ struct Segment {
     int nparts;    // blocks of this launch
     int lanes;     // lanes per vertex of this launch
     int block;     // threads per block of this launch
+    int tail = 0;  // the tail sweep over colours [c0, c1) (k_sweep_tail), nparts clusters
 };
 
 struct DevBuf {
@@ -304,6 +316,19 @@ struct vbd_solver {
     // slots), so a lane finds its record without the slot offset.
     float4 *small_rec = nullptr;
     std::vector<int64_t> small_off;  // per colour: first row of 32 records, or -1
+    // The tail sweep (single body): the colours tail_col[0..tail_k) swept by
+    // tail_clusters blocks.  Per (cluster, colour k, row) the row's vertex and
+    // that vertex's index in the cluster's neighbour table (-1: none); per row
+    // and lane the lane's slot as three table indices and its dictionary
+    // entry (-1 past the vertex's slots); per cluster its neighbours' ids,
+    // tail_nb_stride per cluster, -1 past its count.
+    bool tail_ok = false;
+    int tail_k = 0, tail_clusters = 0, tail_nb_stride = 0;
+    int tail_col[vbd::kMaxTail] = {};
+    int tail_rows[vbd::kMaxTail] = {};  // rows of colour k per cluster (the most any cluster uses)
+    int2 *tail_v = nullptr;             // [cluster][sum of tail_rows]
+    uint2 *tail_rec = nullptr;          // [cluster][sum of tail_rows][32]: n0 | n1 << 10 | n2 << 20, entry
+    int32_t *tail_nb = nullptr;         // [cluster][tail_nb_stride]
     int32_t *colors = nullptr;    // original order
     int32_t *inc_off = nullptr;   // original order vertex -> incident (tet<<2|local), ascending tet
     int32_t *inc = nullptr;
@@ -371,7 +396,9 @@ int export_positions_dev(vbd_solver *s, const float4 *src, double *dev_out);
 int import_positions_dev(vbd_solver *s, const double *dev_in, double4 *dst);
 int export_velocities(vbd_solver *s, double *host_out);
 void plan_segments(vbd_solver *s);
-int build_small_records(vbd_solver *s);  // padded records of the colours plan_segments gives 32 lanes
+// padded records of the colours plan_segments gives 32 lanes, then the tail
+// sweep's tables (and the plan again, with the tail as one launch)
+int build_small_records(vbd_solver *s);
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1);
 int build_staging(vbd_solver *s);  // neighbour staging tables (st_*), when the mesh allows
 // The lane layout of colour c's launch when it runs the big-colour sweep
@@ -380,6 +407,7 @@ int build_staging(vbd_solver *s);  // neighbour staging tables (st_*), when the 
 const LaneLayout *big_color_layout(const vbd_solver *s, int c);
 extern bool g_record_dictionary;  // share record dictionary entries in later creates (default on)
 extern bool g_staging;            // neighbour staging in later creates (default on)
+extern bool g_tail;               // the tail sweep in later creates and launch changes (default on)
 hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,
                                       hipStream_t st);
 int step_bodies(vbd_solver *s, double dt, double tol, int max_iter);

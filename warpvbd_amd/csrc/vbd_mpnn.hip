@@ -725,6 +725,7 @@ __global__ void k_init_history(int n, float4 *u, const double4 *uprev, const flo
         ctrl->iters = 0;
         ctrl->metric = 0u;
         ctrl->blocks_done = 0u;
+        ctrl->launch_max = 0u;
     }
     if (i >= n) return;
     const double4 p = uprev[i];

@@ -723,6 +723,7 @@ __global__ void k_rescale_dict(int64_t nd, float4 *dict, float fa, float fb)
 
 bool g_record_dictionary = true;  // vbd_use_record_dictionary
 bool g_staging = true;            // vbd_use_neighbour_staging
+bool g_tail = true;               // vbd_use_tail_sweep
 
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1)
 {

@@ -20,6 +20,7 @@
 // after convergence return at their first instruction.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include <hip/hip_ext.h>
 
@@ -250,6 +251,178 @@ __global__ __launch_bounds__(1024) void k_sweep_small(SweepArgs a, int v0, int n
     }
 }
 
+// The tail sweep's tables for one launch (the tail_* fields of the solver).
+// Row j of colour k in cluster c is row c * rows_total + row_off[k] + j of v
+// and of rec (32 lanes per row).
+struct TailArgs {
+    const int2 *v;        // per row: vertex (-1: none), its index in the cluster's table (-1: none)
+    const uint2 *rec;     // per row and lane: n0 | n1 << 10 | n2 << 20, dictionary entry (-1: none)
+    const int32_t *nb;    // per cluster: nb_stride neighbour ids, -1 past its count
+    int nb_stride;
+    int rows_total;
+    int rows[kMaxTail];   // rows of colour k per cluster
+    int row_off[kMaxTail];
+    int n_dict;
+};
+
+// Lanes per vertex of the tail sweep: each lane evaluates 32 / kTailLanes
+// consecutive slot positions of its vertex.
+#ifndef VBD_TAIL_LANES
+#define VBD_TAIL_LANES 16
+#endif
+constexpr int kTailLanes = VBD_TAIL_LANES;
+static_assert(kTailLanes == 8 || kTailLanes == 16 || kTailLanes == 32, "8, 16 or 32 lanes per tail vertex");
+
+// One slot of a tail vertex from zero (record r, entry -1: none).
+template <int MAT>
+__device__ __forceinline__ Acc tail_slot(uint2 r, const float4 *s_x, const float4 *s_dict, const VertexLocal &ql)
+{
+    Acc p;
+    acc_zero(p);
+    const int e = (int)r.y;
+    if (e >= 0) {
+        const float4 x0 = s_x[r.x & 1023u], x1 = s_x[(r.x >> 10) & 1023u], x2 = s_x[(r.x >> 20) & 1023u];
+        const float4 d0 = s_dict[3 * e], d1 = s_dict[3 * e + 1], d2 = s_dict[3 * e + 2];
+        const SlotRel sr = slot_rel(x0, x1, x2, ql.hi, ql.lo);
+        slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, p);
+    }
+    return p;
+}
+
+// One lane's share of a tail vertex's slots: its S = 32 / kTailLanes
+// consecutive slot positions (records q), each evaluated from zero and summed
+// as a balanced tree (s0 + s1, (s0 + s1) + (s2 + s3)).  With
+// group_sum<kTailLanes> over the lanes after it, a vertex's sum is the
+// balanced tree over its 32 slot positions that group_sum<32> forms over
+// k_sweep_small's one slot per lane, so the two give the same result bit for
+// bit.
+template <int MAT, int S>
+__device__ __forceinline__ Acc tail_slots(const uint2 *q, const float4 *s_x, const float4 *s_dict,
+                                          const VertexLocal &ql)
+{
+    if constexpr (S == 1) return tail_slot<MAT>(q[0], s_x, s_dict, ql);
+    else return acc_add(tail_slots<MAT, S / 2>(q, s_x, s_dict, ql), tail_slots<MAT, S / 2>(q + S / 2, s_x, s_dict, ql));
+}
+
+// The tail colours (K trailing colours of 32 lanes per vertex) of a single
+// body in one launch: one block per cluster, 32 rows of a colour x
+// kTailLanes lanes, lane l evaluating the row vertex's slot positions
+// S l .. S l + S - 1 (tail_slots).  No edge joins two clusters, so a
+// cluster's vertices depend only on each other and on the colours before the
+// tail, which this launch does not write.  Two memory rounds: (1) every row's
+// vertex and every lane's records of the K colours, the cluster's neighbour
+// ids, the dictionary, the iteration index, and this block's share of the
+// other launches' block maxima (final: those launches have finished); (2) the
+// rows' own data and the neighbours' packed positions, into LDS.  Then colour
+// by colour from LDS, a block barrier between colours: a row's update goes
+// to memory and to its entry in the table.  The results are k_sweep_small's
+// bit for bit.  Each block adds its metric maximum into Ctrl.launch_max and
+// counts itself done (release); the last one reads the maximum (acquire) and
+// closes the iteration (src/vbd_solver.py:578).
+template <int MAT, int K>
+__global__ __launch_bounds__(kTailRows * kTailLanes) void k_sweep_tail(SweepArgs a, TailArgs t,
+                                                                       const uint32_t *part_all, int np_before,
+                                                                       float tol)
+{
+    constexpr int L = kTailLanes, S = 32 / L, kThreads = kTailRows * L, kNbPer = kTailNb / kThreads;
+    static_assert(kTailNb % kThreads == 0, "the neighbour table splits evenly over the block");
+    __shared__ float4 s_x[kTailNb];
+    __shared__ float4 s_dict[3 * kStageDict];
+    static_assert(3 * kStageDict <= kThreads, "a dictionary row per thread");
+    Ctrl *c = a.ctrl;
+    const int row = (int)threadIdx.x / L, lane = (int)threadIdx.x % L;
+    const int64_t cl = blockIdx.x;
+    // round 1
+    int2 vo[K];
+    uint2 rc[K][S];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        vo[k] = make_int2(-1, -1);
+#pragma unroll
+        for (int j = 0; j < S; ++j) rc[k][j] = make_uint2(0u, 0xFFFFFFFFu);
+        if (row < t.rows[k]) {
+            const int64_t r = cl * t.rows_total + t.row_off[k] + row;
+            vo[k] = t.v[r];
+#pragma unroll
+            for (int j = 0; j < S; ++j) rc[k][j] = t.rec[r * 32 + S * lane + j];
+        }
+    }
+    int id[kNbPer];
+#pragma unroll
+    for (int j = 0; j < kNbPer; ++j) {
+        const int i = (int)threadIdx.x + kThreads * j;
+        id[j] = i < t.nb_stride ? t.nb[cl * t.nb_stride + i] : -1;
+    }
+    const bool has_dict = (int)threadIdx.x < 3 * t.n_dict;
+    const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t m = 0u;
+    for (int64_t i = cl * blockDim.x + threadIdx.x; i < np_before; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t y = part_all[i];
+        m = m > y ? m : y;
+    }
+    const int it = c->it;
+    // the same for every block: a launch after convergence sweeps nothing
+    if (c->converged) return;
+    // round 2
+    float4 px[kNbPer];
+#pragma unroll
+    for (int j = 0; j < kNbPer; ++j)
+        px[j] = id[j] >= 0 ? *at(a.u, (uint32_t)id[j]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 uv[K];
+    double4 tg[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        uv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        tg[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (vo[k].x >= 0) {
+            uv[k] = a.u[vo[k].x];
+            tg[k] = a.tgt[vo[k].x];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kNbPer; ++j) s_x[threadIdx.x + kThreads * j] = px[j];
+    if (has_dict) s_dict[threadIdx.x] = drow;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int v = vo[k].x;
+        const bool valid = v >= 0;
+        VertexPre pre;
+        pre.uv = uv[k];
+        pre.tg = tg[k];
+        pre.s0 = pre.s1 = 0;
+        Acc acc;
+        acc_zero(acc);
+        VertexLocal ql = vertex_local(pre);
+        if (valid) acc = tail_slots<MAT, S>(rc[k], s_x, s_dict, ql);
+        double3 us;
+        const float3 dx = vertex_finish<L>(a, valid ? v : 0, valid, acc, ql, us);
+        if (valid) {
+            if (lane == 0) {
+                const float4 nu = pack_disp(us.x + dx.x, us.y + dx.y, us.z + dx.z);
+                a.u[v] = nu;
+                if (vo[k].y >= 0) s_x[vo[k].y] = nu;
+            }
+            const uint32_t mk = metric_bits(dx);
+            m = m > mk ? m : mk;
+        }
+        if (k + 1 < K) __syncthreads();  // the next colour reads this one's updates
+    }
+    const uint32_t mb = block_max_u32(m);
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_max(&c->launch_max, mb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t prev = __hip_atomic_fetch_add(&c->blocks_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gridDim.x - 1) {
+            // every block's maximum is in launch_max (their increments came first)
+            const uint32_t x = __hip_atomic_load(&c->launch_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            c->blocks_done = 0u;
+            c->launch_max = 0u;
+            close_iteration(c, it, x, tol);
+        }
+    }
+}
+
 // The padded record table of one colour: row (v - v0) holds the vertex's
 // compact records in its 32 lanes, entry -1 past its last slot.
 __global__ void k_pad_records(const float4 *slots, const int32_t *slot_off, int v0, int nv, float4 *out)
@@ -278,6 +451,7 @@ __global__ void k_step_init(int n, float4 *u, const double4 *uprev, const float4
         ctrl->iters = 0;
         ctrl->metric = 0u;
         ctrl->blocks_done = 0u;
+        ctrl->launch_max = 0u;
     }
     if (i < n) init_vertex(i, u, uprev, vel, tgt, mass, active, dt, g);
 }
@@ -395,6 +569,35 @@ void launch_small(int material, const SweepArgs &a, int v0, int nv, int block, i
 }
 
 template <int MAT>
+void launch_tail_k(int K, const SweepArgs &a, const TailArgs &t, int clusters, const uint32_t *part_all,
+                   int np_before, float tol, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
+{
+    auto kern = K == 2 ? k_sweep_tail<MAT, 2> : K == 3 ? k_sweep_tail<MAT, 3> : k_sweep_tail<MAT, 4>;
+    hipExtLaunchKernelGGL(kern, dim3(clusters), dim3(kTailRows * kTailLanes), 0, st, e0, e1, 0, a, t, part_all,
+                          np_before, tol);
+}
+
+void launch_tail(const vbd_solver *s, const SweepArgs &a, int np_before, float tol, hipEvent_t e0, hipEvent_t e1)
+{
+    TailArgs t;
+    t.v = s->tail_v;
+    t.rec = s->tail_rec;
+    t.nb = s->tail_nb;
+    t.nb_stride = s->tail_nb_stride;
+    t.rows_total = 0;
+    for (int k = 0; k < kMaxTail; ++k) {
+        t.rows[k] = k < s->tail_k ? s->tail_rows[k] : 0;
+        t.row_off[k] = t.rows_total;
+        t.rows_total += t.rows[k];
+    }
+    t.n_dict = s->n_dict;
+    if (s->material == kSNH)
+        launch_tail_k<kSNH>(s->tail_k, a, t, s->tail_clusters, s->part, np_before, tol, e0, e1, s->stream);
+    else
+        launch_tail_k<kStVK>(s->tail_k, a, t, s->tail_clusters, s->part, np_before, tol, e0, e1, s->stream);
+}
+
+template <int MAT>
 void dispatch_sweep(int L, const SweepArgs &a, int v0, int nv, int block, int close, const uint32_t *part_all,
                     int np_all, float tol, hipEvent_t e0, hipEvent_t e1, hipStream_t st)
 {
@@ -437,10 +640,25 @@ void plan_segments(vbd_solver *s)
         s->segments.push_back({c, c + 1, 0, lay ? lay->nblocks : nblk(threads, s->launch.block), lanes,
                                s->launch.block});
     }
+    // the tail sweep's colours, when its tables were built for this plan's
+    // trailing small colours, in one launch (k_sweep_tail)
+    if (s->tail_ok && (int)s->segments.size() >= s->tail_k) {
+        const size_t first = s->segments.size() - s->tail_k;
+        bool match = true;
+        for (int k = 0; k < s->tail_k; ++k) {
+            const Segment &g = s->segments[first + k];
+            match = match && g.c0 == s->tail_col[k] && g.lanes == 32 && s->col_size[g.c0] < s->launch.small_color;
+        }
+        if (match) {
+            s->segments.resize(first);
+            s->segments.push_back({s->tail_col[0], s->tail_col[s->tail_k - 1] + 1, 0, s->tail_clusters, kTailLanes,
+                                   kTailRows * kTailLanes, 1});
+        }
+    }
     // the iteration's last launch: one block of up to 1024 threads when its
     // colour fits (the padded small-colour kernel then folds the other
     // launches' maxima alongside its own vertices)
-    if (!s->segments.empty()) {
+    if (!s->segments.empty() && !s->segments.back().tail) {
         Segment &g = s->segments.back();
         const int64_t threads = (int64_t)s->col_size[g.c0] * g.lanes;
         if (g.lanes == 32 && threads <= 1024) {
@@ -473,8 +691,188 @@ static int ensure_partials(vbd_solver *s)
     return 0;
 }
 
+// The tail sweep's tables (tail_*): the trailing colours that the plan sweeps
+// with 32 lanes per vertex (below launch.small_color; the last kMaxTail of
+// them, at least 2), when their vertices split into at most
+// kTailMaxClusters clusters joined by no edge with at most kTailRows vertices
+// of a colour and kTailNb distinct neighbours each (fewer of the trailing
+// colours are tried when they do not);
+// else the solver keeps a launch per colour.  A shared dictionary of at most
+// kStageDict entries is needed (it is staged in LDS).  Built on the host from
+// the tail vertices' slot records (the tail is the end of the swept range).
+static int build_tail(vbd_solver *s)
+{
+    s->tail_ok = false;
+    for (void **p : {(void **)&s->tail_v, (void **)&s->tail_rec, (void **)&s->tail_nb}) {
+        if (*p) VBD_HIP(hipFree(*p));
+        *p = nullptr;
+    }
+#if VBD_TET_RECORDS
+    return 0;
+#else
+    if (!g_tail || s->n_bodies > 1 || s->batched || !s->slots || !s->dict_shared || s->n_dict > kStageDict ||
+        s->n_dict <= 0)
+        return 0;
+    std::vector<int> cand;  // the trailing small colours, in order
+    for (int k = (int)s->segments.size() - 1; k >= 0; --k) {
+        const Segment &g = s->segments[k];
+        if (g.tail || g.c1 != g.c0 + 1 || g.lanes != 32 || s->col_size[g.c0] >= s->launch.small_color) break;
+        cand.insert(cand.begin(), g.c0);
+    }
+    if (cand.size() < 2) return 0;
+    if ((int)cand.size() > kMaxTail) cand.erase(cand.begin(), cand.end() - kMaxTail);
+    const int va = s->col_start[cand[0]], vb = s->col_start[cand.back()] + s->col_size[cand.back()];
+    std::vector<int32_t> off(vb - va + 1);
+    VBD_HIP(hipMemcpyAsync(off.data(), s->slot_off + va, sizeof(int32_t) * off.size(), hipMemcpyDeviceToHost,
+                           s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    std::vector<float4> rec(off.back() - off[0]);
+    if (!rec.empty())
+        VBD_HIP(hipMemcpyAsync(rec.data(), s->slots + off[0], sizeof(float4) * rec.size(), hipMemcpyDeviceToHost,
+                               s->stream));
+    VBD_HIP(hipStreamSynchronize(s->stream));
+    auto ids = [&](int v, int64_t q, uint32_t w[4]) { std::memcpy(w, &rec[off[v - va] - off[0] + q], 16); };
+    std::vector<int32_t> stamp(s->n, -1), local(s->n, 0);
+    for (int K = (int)cand.size(); K >= 2; --K) {
+        const int c0 = cand[cand.size() - K];
+        const int v0 = s->col_start[c0], nt = vb - v0;
+        // clusters: components of the tail vertices joined by a slot
+        std::vector<int> par(nt);
+        for (int i = 0; i < nt; ++i) par[i] = i;
+        auto find = [&](int i) {
+            while (par[i] != i) i = par[i] = par[par[i]];
+            return i;
+        };
+        for (int v = v0; v < vb; ++v)
+            for (int64_t q = 0; q < off[v + 1 - va] - off[v - va]; ++q) {
+                uint32_t w[4];
+                ids(v, q, w);
+                for (int j = 0; j < 3; ++j)
+                    if ((int)w[j] >= v0 && (int)w[j] < vb) par[find(v - v0)] = find((int)w[j] - v0);
+            }
+        std::vector<int> cl_of(nt, -1), root_cl(nt, -1);
+        int ncl = 0;
+        for (int i = 0; i < nt; ++i) {
+            const int r = find(i);
+            if (root_cl[r] < 0) root_cl[r] = ncl++;
+            cl_of[i] = root_cl[r];
+        }
+        if (ncl > kTailMaxClusters) continue;
+        // rows: per (cluster, colour) its vertices in order
+        std::vector<int> cnt((size_t)ncl * K, 0), rowk(nt);
+        int rows[kMaxTail] = {};
+        bool ok = true;
+        for (int k = 0; k < K && ok; ++k) {
+            const int c = cand[cand.size() - K + k];
+            for (int v = s->col_start[c]; v < s->col_start[c] + s->col_size[c]; ++v) {
+                int &n = cnt[(size_t)cl_of[v - v0] * K + k];
+                rowk[v - v0] = n++;
+                rows[k] = std::max(rows[k], n);
+                ok = ok && n <= kTailRows && off[v + 1 - va] - off[v - va] <= 32;
+            }
+        }
+        if (!ok) continue;
+        int rt = 0, roff[kMaxTail] = {};
+        for (int k = 0; k < K; ++k) {
+            roff[k] = rt;
+            rt += rows[k];
+        }
+        // neighbour tables: per cluster its distinct neighbours in order of
+        // first appearance over its vertices' slots
+        std::vector<std::vector<int>> members(ncl);
+        for (int i = 0; i < nt; ++i) members[cl_of[i]].push_back(v0 + i);
+        std::vector<int32_t> nb;
+        int stride = 0;
+        std::vector<int> ncount(ncl, 0);
+        for (int cl = 0; cl < ncl && ok; ++cl)
+            for (int v : members[cl])
+                for (int64_t q = 0; q < off[v + 1 - va] - off[v - va] && ok; ++q) {
+                    uint32_t w[4];
+                    ids(v, q, w);
+                    for (int j = 0; j < 3; ++j)
+                        if (stamp[w[j]] != cl + K * s->n) {  // stamps differ between attempts
+                            stamp[w[j]] = cl + K * s->n;
+                            ++ncount[cl];
+                        }
+                    ok = ncount[cl] <= kTailNb;
+                }
+        if (!ok) continue;
+        for (int cl = 0; cl < ncl; ++cl) stride = std::max(stride, ncount[cl]);
+        stride = (stride + 63) & ~63;
+        // at most 64 MB of tables
+        if ((double)ncl * rt * 32 * sizeof(uint2) + (double)ncl * stride * 4 > 64.0 * (1 << 20)) continue;
+        nb.assign((size_t)ncl * stride, -1);
+        std::vector<int2> tv((size_t)ncl * rt, make_int2(-1, -1));
+        std::vector<uint2> tr((size_t)ncl * rt * 32, make_uint2(0u, 0xFFFFFFFFu));
+        for (int cl = 0; cl < ncl; ++cl) {
+            int n = 0;
+            const int tag = -2 - cl;  // this pass's stamps (the counting pass left nonnegative ones)
+            for (int v : members[cl])
+                for (int64_t q = 0; q < off[v + 1 - va] - off[v - va]; ++q) {
+                    uint32_t w[4];
+                    ids(v, q, w);
+                    for (int j = 0; j < 3; ++j)
+                        if (stamp[w[j]] != tag) {
+                            stamp[w[j]] = tag;
+                            local[w[j]] = n;
+                            nb[(size_t)cl * stride + n++] = (int32_t)w[j];
+                        }
+                }
+            for (int v : members[cl]) {
+                int k = K - 1;
+                while (v < s->col_start[cand[cand.size() - K + k]]) --k;
+                const size_t r = (size_t)cl * rt + roff[k] + rowk[v - v0];
+                tv[r] = make_int2(v, stamp[v] == tag ? local[v] : -1);
+                for (int64_t q = 0; q < off[v + 1 - va] - off[v - va]; ++q) {
+                    uint32_t w[4];
+                    ids(v, q, w);
+                    tr[r * 32 + q] = make_uint2((uint32_t)local[w[0]] | (uint32_t)local[w[1]] << 10 |
+                                                    (uint32_t)local[w[2]] << 20,
+                                                w[3]);
+                }
+            }
+        }
+        // every index the kernel takes from the tables stays in its buffer
+        for (int cl = 0; cl < ncl && ok; ++cl)
+            for (int r = 0; r < rt && ok; ++r) {
+                const int2 q = tv[(size_t)cl * rt + r];
+                ok = q.x < vb && q.y < ncount[cl];
+                for (int l = 0; l < 32 && ok; ++l) {
+                    const uint2 e = tr[((size_t)cl * rt + r) * 32 + l];
+                    if ((int)e.y < 0) continue;
+                    ok = q.x >= 0 && (int)e.y < s->n_dict && (int)(e.x & 1023u) < ncount[cl] &&
+                         (int)((e.x >> 10) & 1023u) < ncount[cl] && (int)(e.x >> 20) < ncount[cl];
+                }
+            }
+        if (!ok) return 0;
+        VBD_HIP(hipMalloc((void **)&s->tail_v, sizeof(int2) * tv.size()));
+        VBD_HIP(hipMalloc((void **)&s->tail_rec, sizeof(uint2) * tr.size()));
+        VBD_HIP(hipMalloc((void **)&s->tail_nb, sizeof(int32_t) * nb.size()));
+        VBD_HIP(hipMemcpyAsync(s->tail_v, tv.data(), sizeof(int2) * tv.size(), hipMemcpyHostToDevice, s->stream));
+        VBD_HIP(hipMemcpyAsync(s->tail_rec, tr.data(), sizeof(uint2) * tr.size(), hipMemcpyHostToDevice,
+                               s->stream));
+        VBD_HIP(hipMemcpyAsync(s->tail_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice,
+                               s->stream));
+        VBD_HIP(hipStreamSynchronize(s->stream));
+        s->tail_k = K;
+        s->tail_clusters = ncl;
+        s->tail_nb_stride = stride;
+        for (int k = 0; k < kMaxTail; ++k) {
+            s->tail_col[k] = k < K ? cand[cand.size() - K + k] : 0;
+            s->tail_rows[k] = k < K ? rows[k] : 0;
+        }
+        s->tail_ok = true;
+        return 0;
+    }
+    return 0;
+#endif
+}
+
 int build_small_records(vbd_solver *s)
 {
+    // a launch per colour while the tables are built
+    s->tail_ok = false;
+    plan_segments(s);
     if (s->small_rec) VBD_HIP(hipFree(s->small_rec));
     s->small_rec = nullptr;
     s->small_off.assign(s->ncolors, -1);
@@ -498,6 +896,8 @@ int build_small_records(vbd_solver *s)
     }
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(s->stream));
+    VBD_TRY(build_tail(s));
+    plan_segments(s);
     return 0;
 #endif
 }
@@ -544,7 +944,9 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
         const float4 *prec = s->small_rec && g.lanes == 32 && s->small_off[g.c0] >= 0
                                  ? s->small_rec + 32 * s->small_off[g.c0]
                                  : nullptr;
-        if (lay)
+        if (g.tail)
+            launch_tail(s, a, g.part_off, tol, ev[0], ev[1]);
+        else if (lay)
             launch_staged(s->material, a, v0, nv, *lay, close, s->part, s->n_parts, tol,
                           StageTables{s->st_nb, s->st_rec, s->st_voff},
                           s->n_dict, ev[0], ev[1], s->stream);

@@ -365,6 +365,14 @@ class VBDSolver:
         check(_lib.load().vbd_staged_colors(self._h, ctypes.byref(n)), "vbd_staged_colors")
         return n.value
 
+    def tail_sweep(self) -> tuple:
+        """(colours, clusters) of the tail sweep: the trailing small colours
+        swept in one launch, one block per cluster of their vertices ((0, 0):
+        a launch per colour; same results either way)."""
+        k, c = ctypes.c_int32(), ctypes.c_int32()
+        check(_lib.load().vbd_tail_sweep(self._h, ctypes.byref(k), ctypes.byref(c)), "vbd_tail_sweep")
+        return k.value, c.value
+
     def record_layout(self) -> dict:
         """Slot records of this solver: record dictionary entries (as many as
         slots when no constants are shared) and HBM bytes per slot read by a
