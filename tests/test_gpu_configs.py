@@ -242,6 +242,28 @@ def test_beam_128x16x16_history_guess_matches_oracle():
         c = (xr - y) * act
 
 
+def test_mixed_1024_per_block_dictionaries_are_bit_exact(mixed1024):
+    """1,024 materials give the batch about 20k dictionary entries, more than
+    a block's LDS table holds, so each staged block lists the entries its
+    slots use (a block spans at most two bodies here: at most 40 entries).
+    Against the same batch without staging the same values reach the same
+    arithmetic: per-body iteration counts and positions agree bit for bit."""
+    from warpvbd_amd import _lib
+    a = BatchedVBDSolver(mixed1024, dx_tol=1e-7, max_iter=1000, verbose=False)
+    _lib.use_neighbour_staging(False)
+    try:
+        b = BatchedVBDSolver(mixed1024, dx_tol=1e-7, max_iter=1000, verbose=False)
+    finally:
+        _lib.use_neighbour_staging(True)
+    assert a.record_layout()["dictionary_entries"] > 64
+    assert a.staged_colors() == 4 and b.staged_colors() == 0
+    for _ in range(4):
+        xa = a.step(dt=1e-3).numpy()
+        xb = b.step(dt=1e-3).numpy()
+        assert np.array_equal(a.body_iterations, b.body_iterations)
+        assert np.array_equal(xa, xb)
+
+
 def test_mass_spring_512_live_block_lists_equal_full_launches(springs512):
     """The batch's big colours run the staged sweep; once most bodies have
     converged, only the staging blocks that hold a live body's vertex are

@@ -150,12 +150,13 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(SweepArgs a, const int32_t
 // A staged colour (staged_sweep in vbd_sweep.h) across every body: 256
 // threads per block of kStageRows consecutive vertices of the colour; block
 // blist[i].x for the i-th block when a list is given (the blocks that still
-// hold a live body's vertex), else all of them.
-template <int MAT>
+// hold a live body's vertex), else all of them.  DL: per-block dictionary
+// lists (staged_sweep).
+template <int MAT, bool DL>
 __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0, int nv, LaneLayout lay,
                                                              StageTables st, int n_dict,
                                                              const int32_t *body_new, BodyPhase ph, BodyFold f,
-                                                             const int32_t *rows, const int2 *blist)
+                                                             const int32_t *rows, const int2 *blist, const int32_t *dl)
 {
     if ((int)blockIdx.x < f.nfold) {
         fold_bodies(ph, f);
@@ -175,7 +176,8 @@ __global__ __launch_bounds__(256) void k_sweep_staged_bodies(SweepArgs a, int v0
     if (threadIdx.x < kFoldBodies) s_bm[threadIdx.x] = 0u;
     const int b0 = body_new[v0 + b * kStageRows];
     // (the whole block returns together)
-    if (!staged_sweep<MAT, true, false>(a, v0, nv, lay, b, st, n_dict, body_new, &ph, rows, body, m)) return;
+    if (!staged_sweep<MAT, true, false, DL>(a, v0, nv, lay, b, st, n_dict, body_new, &ph, rows, body, m, dl))
+        return;
     body_metric_max(body, m, ph.macc, s_bm, b0, kFoldBodies);
     __syncthreads();
     if (threadIdx.x < kFoldBodies && s_bm[threadIdx.x]) atomicMax(&ph.macc[b0 + threadIdx.x], s_bm[threadIdx.x]);
@@ -296,14 +298,12 @@ void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *
     if (nmain + f.nfold == 0) return;
     const dim3 grid(nmain + f.nfold);
     const int32_t *rows = s->row_live + s->row_base[c];
-    if (s->material == kSNH)
-        k_sweep_staged_bodies<kSNH><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c],
-                                                                StageTables{s->st_nb, s->st_rec, s->st_voff},
-                                                                s->n_dict, s->body_new, ph, f, rows, blist);
-    else
-        k_sweep_staged_bodies<kStVK><<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c],
-                                                                 StageTables{s->st_nb, s->st_rec, s->st_voff},
-                                                                 s->n_dict, s->body_new, ph, f, rows, blist);
+    const StageTables st{s->st_nb, s->st_rec, s->st_voff};
+    const bool dl = s->st_dl != nullptr;
+    auto kern = s->material == kSNH ? (dl ? k_sweep_staged_bodies<kSNH, true> : k_sweep_staged_bodies<kSNH, false>)
+                                    : (dl ? k_sweep_staged_bodies<kStVK, true> : k_sweep_staged_bodies<kStVK, false>);
+    kern<<<grid, 256, 0, s->stream>>>(a, v0, nv, s->st_col[c], st, s->n_dict, s->body_new, ph, f, rows, blist,
+                                      s->st_dl);
 }
 
 // Per-body state and maxima of iteration k of a run: two state arrays

@@ -42,7 +42,7 @@ void vbd_destroy(vbd_solver_t *s)
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict,
                    s->st_nb, s->st_rec, s->st_voff, s->small_rec, s->u_last, s->d_dx_max,
-                   s->tail_v, s->tail_rec, s->tail_nb};
+                   s->tail_v, s->tail_rec, s->tail_nb, s->st_dl};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (double4 *p : s->snap)

@@ -22,6 +22,9 @@ constexpr int kStageMax = 1152;      // entries of a block of 4-lane vertices (6
 constexpr int kStageMaxWide = 1408;  // entries of a block holding 2-lane vertices (128 per block)
 constexpr int kStageEntryBits = 5;   // a vertex has at most 31 neighbours in its list (slot records' field)
 constexpr int kStageDict = 64;    // dictionary entries staged with them (48 bytes each)
+// A solver whose dictionary holds more entries (a batch of bodies of many
+// materials: 20 entries per body) gives each staged block its own list of the
+// at most kStageDict entries its slots use.
 // Lanes per vertex of a single body's big colours, by slot count: at most
 // kLaneSlots slots per lane.  A voxel mesh's interior vertices have 20 slots
 // (4 lanes), its face vertices 10 (2 lanes) and its edge vertices 5 (1 lane),
@@ -309,6 +312,7 @@ struct vbd_solver {
     int32_t *st_nb = nullptr;      // entry lists (stage_list_offset): neighbour id | owner << 25, -1 past a list
     uint32_t *st_rec = nullptr;    // [n_slots] n0 | n1 << 5 | n2 << 10 | entry << 15 (n_k: the vertex's list)
     int32_t *st_voff = nullptr;    // [n_swept] a vertex's first entry in its block's list
+    int32_t *st_dl = nullptr;      // per staged block kStageDict dictionary entries, -1 padded (null: shared)
     int32_t n_dict = 0;
     bool dict_shared = false;     // entries shared between slots (else one per slot)
     // Colours swept with 32 lanes per vertex (below launch.small_color): their

@@ -1102,10 +1102,15 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
 // vertices, kStageMaxWide per block holding 2-lane ones), each vertex gets
 // its first entry's index in the list (st_voff), and each of its slots gets
 // n0 | n1 << 5 | n2 << 10 | entry << 15 with n_k the index of the slot's k-th
-// other vertex in the vertex's own list.  A block with more entries than its
-// list holds, a vertex with more than 31 neighbours, or a mesh of 2^25 - 1
-// vertices or more turns staging off for the solver (the layouts stay, and
-// every block reads its neighbours from global memory).
+// other vertex in the vertex's own list.  A batch whose dictionary holds more
+// than kStageDict entries (bodies of many materials) also lists per block the
+// dictionary entries its slots use, in order of first use (st_dl, kStageDict
+// per block), and a slot's entry field is its index in that list (a single
+// body's solver stages only a shared dictionary).  A block
+// with more entries than its lists hold, a vertex with more than 31
+// neighbours, or a mesh of 2^25 - 1 vertices or more turns staging off for
+// the solver (the layouts stay, and every block reads its neighbours from
+// global memory).
 int build_staging(vbd_solver *s)
 {
     s->st_ok = false;
@@ -1117,7 +1122,9 @@ int build_staging(vbd_solver *s)
         if (nv >= min_color) s->st_col[c] = lane_layout(s->col_n42[c].x, s->col_n42[c].y, nv, false, -1);
     }
     // (ids below 2^25 - 1, so that no entry reads as the -1 that ends a list)
-    if (!g_staging || !s->dict_shared || s->n_dict > kStageDict || s->n_slots <= 0 || s->n >= (1 << 25) - 1) return 0;
+    const bool per_block = s->n_dict > kStageDict;
+    if (!g_staging || !s->dict_shared || (per_block && !s->batched) || s->n_slots <= 0 || s->n >= (1 << 25) - 1)
+        return 0;
     hipStream_t st = s->stream;
     const int nsw = s->n_swept;
     std::vector<int32_t> off(nsw + 1);
@@ -1130,6 +1137,8 @@ int build_staging(vbd_solver *s)
     std::vector<uint32_t> srec(s->n_slots, 0u);
     std::vector<int32_t> voff(std::max(nsw, 1), 0);
     std::vector<int32_t> stamp(s->n, -1), local(s->n, 0);
+    std::vector<int32_t> dl;  // per-block dictionary lists
+    std::vector<int32_t> estamp(per_block ? s->n_dict : 0, -1), elocal(per_block ? s->n_dict : 0, 0);
     std::vector<LaneLayout> col(s->ncolors);
     std::vector<int64_t> ids(s->ncolors, 0);
     int nblocks = 0;
@@ -1138,6 +1147,8 @@ int build_staging(vbd_solver *s)
         if (nv < min_color) continue;
         const int n4 = s->col_n42[c].x, n2 = s->col_n42[c].y;
         const LaneLayout l = lane_layout(n4, n2, nv, true, (int64_t)nb.size());
+        // a batch's per-block lists are found at lay.ids / kStageMax
+        if (per_block && (l.b4 != l.nblocks || l.ids != (int64_t)nblocks * kStageMax)) return 0;
         const int64_t lanes = 4 * (int64_t)n4 + 2 * (int64_t)n2 + l.n1;
         const int c0 = s->col_start[c];
         nb.resize(nb.size() + stage_list_offset(l.b1, l.b4), -1);
@@ -1148,7 +1159,8 @@ int build_staging(vbd_solver *s)
             if (vb - va > 128) return 0;  // the owner field holds 7 bits
             const size_t base = (size_t)(l.ids + stage_list_offset(b, l.b4));
             const int cap = b < l.b4 ? kStageMax : kStageMaxWide;
-            int cnt = 0;
+            int cnt = 0, ne = 0;
+            if (per_block) dl.resize((size_t)(nblocks + 1) * kStageDict, -1);
             for (int v = va; v < vb; ++v) {
                 voff[v] = cnt;
                 int nd = 0;  // the vertex's distinct neighbours so far
@@ -1166,8 +1178,19 @@ int build_staging(vbd_solver *s)
                         }
                         lk[k] = (uint32_t)local[id];
                     }
-                    if (w[3] >= (1u << 17)) return 0;
-                    srec[q] = lk[0] | lk[1] << 5 | lk[2] << 10 | w[3] << 15;
+                    uint32_t e = w[3];
+                    if (per_block) {
+                        if (e >= (uint32_t)s->n_dict) return 0;
+                        if (estamp[e] != nblocks) {
+                            if (ne == kStageDict) return 0;
+                            estamp[e] = nblocks;
+                            elocal[e] = ne;
+                            dl[(size_t)nblocks * kStageDict + ne++] = (int32_t)e;
+                        }
+                        e = (uint32_t)elocal[e];
+                    }
+                    if (e >= (1u << 17)) return 0;
+                    srec[q] = lk[0] | lk[1] << 5 | lk[2] << 10 | e << 15;
                 }
             }
             ids[c] += cnt;
@@ -1184,6 +1207,10 @@ int build_staging(vbd_solver *s)
     VBD_HIP(hipMemcpyAsync(s->st_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipMemcpyAsync(s->st_rec, srec.data(), sizeof(uint32_t) * srec.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipMemcpyAsync(s->st_voff, voff.data(), sizeof(int32_t) * voff.size(), hipMemcpyHostToDevice, st));
+    if (per_block) {
+        VBD_TRY(dmalloc(&s->st_dl, dl.size()));
+        VBD_HIP(hipMemcpyAsync(s->st_dl, dl.data(), sizeof(int32_t) * dl.size(), hipMemcpyHostToDevice, st));
+    }
     VBD_HIP(hipStreamSynchronize(st));
     for (int c = 0; c < s->ncolors; ++c)
         if (col[c].nblocks > 0) s->st_col[c] = col[c];

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_sweep_staged(SweepArgs a, int v0, int n
         if (!lanes_sweep<MAT>(a, v0, nv, lay, b, m)) return;
     } else {
         int body;
-        if (!staged_sweep<MAT, false, WIDE>(a, v0, nv, lay, b, st, n_dict, nullptr, nullptr, nullptr, body, m))
+        if (!staged_sweep<MAT, false, WIDE, false>(a, v0, nv, lay, b, st, n_dict, nullptr, nullptr, nullptr, body, m))
             return;
     }
     const uint32_t mb = block_max_u32(m);

@@ -527,10 +527,15 @@ struct StageTables {
 // body the vertex's body.
 // WIDE: the launch has blocks of 2-lane vertices, whose lists hold up to
 // kStageMaxWide entries (more LDS per block); otherwise kStageMax.
-template <int MAT, bool BODIES, bool WIDE>
+// DL: each block of a batch has its own dictionary list (dl, kStageDict
+// entries per block, -1 padded, the colour's first block's at lay.ids /
+// kStageMax: a batch's blocks all take that stride; read in round 1, the
+// rows in round 2); otherwise every block stages the shared dictionary.
+template <int MAT, bool BODIES, bool WIDE, bool DL>
 __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv, const LaneLayout &lay, int b,
                                              const StageTables &st, int n_dict, const int32_t *body_new,
-                                             const BodyPhase *ph, const int32_t *rows, int &body_out, uint32_t &m)
+                                             const BodyPhase *ph, const int32_t *rows, int &body_out, uint32_t &m,
+                                             const int32_t *dl = nullptr)
 {
     static_assert(!(BODIES && WIDE), "a batch's blocks hold 4-lane vertices only");
     constexpr int kCap = WIDE ? kStageMaxWide : kStageMax;
@@ -568,8 +573,14 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
         const int k = (int)threadIdx.x + 256 * q;
         id[q] = k < kCap && k < cap ? ids[k] : -1;
     }
-    const bool has_dict = (int)threadIdx.x < 3 * n_dict;
-    const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // the dictionary rows: the shared dictionary's, or (DL) the entry of the
+    // block's list this thread's row belongs to, its row read in round 2
+    const bool has_dict = (int)threadIdx.x < 3 * (DL ? kStageDict : n_dict);
+    float4 drow = !DL && has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    int de = -1;
+    static_assert(!DL || BODIES, "per-block dictionaries are a batch's");
+    if constexpr (DL)
+        if (has_dict) de = dl[(lay.ids / kStageMax + b) * kStageDict + (int)threadIdx.x / 3];
     int body = 0;
     if constexpr (BODIES) {
         body = body_new[v];
@@ -603,6 +614,8 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     const uint32_t *rl = st.rec + pre.s0 + g;
 #pragma unroll
     for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[k << lg];
+    if constexpr (DL)
+        if (de >= 0) drow = a.dict[3 * de + (int)threadIdx.x % 3];
     if (has_dict) s_dict[threadIdx.x] = drow;
     VertexLocal ql = vertex_local(pre);
     const float3 hi = ql.hi, lo = ql.lo;
