@@ -232,11 +232,12 @@ int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t
  * per-tet arrays, src/vbd_solver.py:295-343, are what the records hold). */
 void vbd_use_record_dictionary(int32_t enable);
 int vbd_records(vbd_solver_t *h, int32_t *n_dictionary, int32_t *bytes_per_slot);
-/* Neighbour staging with a shared dictionary: colours of at least 40,960
- * vertices swept at the default launch (4 lanes per vertex, 256 threads) read
- * each block's neighbours once, form each (vertex, neighbour) difference once
- * into LDS with the dictionary, and read a 4-byte record per slot (the
- * results are the same bit for bit).  In a
+/* Neighbour staging: colours of at least 40,960 vertices swept at the default
+ * launch (4 lanes per vertex, 256 threads) read each block's neighbours once,
+ * form each (vertex, neighbour) difference once into LDS with the dictionary,
+ * and read a 4-byte record per slot (the results are the same bit for bit).
+ * It needs a dictionary of at most 64 entries, or, in a batch, at most 64 per
+ * 64-vertex block (a batch of many materials lists each block's own).  In a
  * single-body solver those colours also give each vertex 4, 2 or 1 lanes by
  * its slot count (more than 10, 6..10, at most 5), so that a lane walks at
  * most 5 slots of a voxel mesh; the blocks of 1-lane vertices read their
