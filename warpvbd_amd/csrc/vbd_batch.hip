@@ -298,7 +298,7 @@ void launch_staged_bodies(vbd_solver *s, const SweepArgs &a, int c, const int2 *
     if (nmain + f.nfold == 0) return;
     const dim3 grid(nmain + f.nfold);
     const int32_t *rows = s->row_live + s->row_base[c];
-    const StageTables st{s->st_nb, s->st_rec, s->st_voff};
+    const StageTables st{s->st_nb, s->st_rec, s->st_vrec};
     const bool dl = s->st_dl != nullptr;
     auto kern = s->material == kSNH ? (dl ? k_sweep_staged_bodies<kSNH, true> : k_sweep_staged_bodies<kSNH, false>)
                                     : (dl ? k_sweep_staged_bodies<kStVK, true> : k_sweep_staged_bodies<kStVK, false>);

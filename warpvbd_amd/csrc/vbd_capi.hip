@@ -41,7 +41,7 @@ void vbd_destroy(vbd_solver_t *s)
     void *dev[] = {s->perm, s->iperm, s->X,      s->u,       s->uprev, s->vel,  s->tgt,    s->mass,
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict,
-                   s->st_nb, s->st_rec, s->st_voff, s->small_rec, s->u_last, s->d_dx_max,
+                   s->st_nb, s->st_rec, s->st_vrec, s->small_rec, s->u_last, s->d_dx_max,
                    s->tail_v, s->tail_rec, s->tail_nb, s->st_dl};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -613,12 +613,14 @@ int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
         const double nv = (double)(s->col_start[c + 1] - s->col_start[c]);
         // packed position 16 + FP64 target 32 + slot offset 4 + write-back 16
         bytes[c] = 68.0 * nv + per_slot * s->col_slots[c] + 16.0 * s->col_boundary[c];
-        // big-colour sweep with staging lists (the default launch): a 4-byte
-        // first entry per vertex, 4-byte slot records, and the blocks' lists
-        // of 4-byte entries (one per vertex and neighbour)
+        // big-colour sweep with staging lists (the default launch): per vertex
+        // an 8-byte word (first entry and slot count, record sequence) in place
+        // of the slot offsets, the 4-byte records of the distinct sequences the
+        // colour uses, and the blocks' lists of 4-byte entries (one per vertex
+        // and neighbour)
         const LaneLayout *l = c < (int)s->st_col.size() ? &s->st_col[c] : nullptr;
         if (s->st_ok && l && l->ids >= 0 && l->b1 > 0 && s->launch.lanes == 4 && s->launch.block == 256)
-            bytes[c] = 72.0 * nv + 4.0 * s->col_slots[c] + 16.0 * s->col_boundary[c] +
+            bytes[c] = 72.0 * nv + 4.0 * s->st_prec[c] + 16.0 * s->col_boundary[c] +
                        4.0 * (double)stage_list_offset(l->b1, l->b4);
     }
     return 0;

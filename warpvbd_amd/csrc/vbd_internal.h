@@ -310,8 +310,14 @@ struct vbd_solver {
     std::vector<vbd::LaneLayout> st_col;  // per colour: its layout and lists (nblocks 0: not a big colour)
     std::vector<int64_t> st_ids;   // per colour: entries its blocks stage (sum over blocks)
     int32_t *st_nb = nullptr;      // entry lists (stage_list_offset): neighbour id | owner << 25, -1 past a list
-    uint32_t *st_rec = nullptr;    // [n_slots] n0 | n1 << 5 | n2 << 10 | entry << 15 (n_k: the vertex's list)
-    int32_t *st_voff = nullptr;    // [n_swept] a vertex's first entry in its block's list
+    // the distinct slot-record sequences of the staged vertices, one after
+    // another, 32 zero records past the last: per slot n0 | n1 << 5 | n2 << 10
+    // | entry << 15 (n_k: indices into the vertex's list)
+    uint32_t *st_rec = nullptr;
+    // [n_swept] per staged vertex: its first entry in its block's list | its
+    // slot count << 16, and its record sequence's first record in st_rec
+    int2 *st_vrec = nullptr;
+    std::vector<double> st_prec;   // per colour: records of the distinct sequences its vertices use
     int32_t *st_dl = nullptr;      // per staged block kStageDict dictionary entries, -1 padded (null: shared)
     int32_t n_dict = 0;
     bool dict_shared = false;     // entries shared between slots (else one per slot)

@@ -18,6 +18,9 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <string>
+#include <unordered_map>
+#include <utility>
 
 #include <rocprim/rocprim.hpp>
 
@@ -1099,10 +1102,13 @@ int setup_mesh(vbd_solver *s, const double *rest_xyz, const int32_t *tets, const
 // layout's b1 lists, vertex by vertex, each vertex's distinct neighbours in
 // order of first appearance in its slots (entries id | owner << 25, owner the
 // vertex's index in the block; kStageMax entries per block of 4-lane
-// vertices, kStageMaxWide per block holding 2-lane ones), each vertex gets
-// its first entry's index in the list (st_voff), and each of its slots gets
-// n0 | n1 << 5 | n2 << 10 | entry << 15 with n_k the index of the slot's k-th
-// other vertex in the vertex's own list.  A batch whose dictionary holds more
+// vertices, kStageMaxWide per block holding 2-lane ones), and each of a
+// vertex's slots gets the record n0 | n1 << 5 | n2 << 10 | entry << 15 with
+// n_k the index of the slot's k-th other vertex in the vertex's own list.
+// Vertices with the same incidence pattern (a voxel mesh's interior vertices,
+// each face's, edge's ...) have the same record sequence, so st_rec holds
+// each distinct sequence once and each vertex gets one 8-byte word (st_vrec):
+// its first entry | its slot count << 16, and its sequence's first record.  A batch whose dictionary holds more
 // than kStageDict entries (bodies of many materials) also lists per block the
 // dictionary entries its slots use, in order of first use (st_dl, kStageDict
 // per block), and a slot's entry field is its index in that list (a single
@@ -1134,8 +1140,11 @@ int build_staging(vbd_solver *s)
     VBD_HIP(hipStreamSynchronize(st));
     constexpr uint32_t kNbMask = (1u << kStageEntryBits) - 1u;
     std::vector<int32_t> nb;  // the blocks' lists, -1 past each block's entries
-    std::vector<uint32_t> srec(s->n_slots, 0u);
-    std::vector<int32_t> voff(std::max(nsw, 1), 0);
+    std::vector<uint32_t> srec;  // the distinct record sequences
+    std::unordered_map<std::string, std::pair<uint32_t, int>> seqs;  // sequence -> (first record, last colour)
+    std::vector<int2> vrec(std::max(nsw, 1), make_int2(0, 0));
+    std::vector<double> prec(s->ncolors, 0.0);
+    std::vector<uint32_t> seq;
     std::vector<int32_t> stamp(s->n, -1), local(s->n, 0);
     std::vector<int32_t> dl;  // per-block dictionary lists
     std::vector<int32_t> estamp(per_block ? s->n_dict : 0, -1), elocal(per_block ? s->n_dict : 0, 0);
@@ -1162,7 +1171,9 @@ int build_staging(vbd_solver *s)
             int cnt = 0, ne = 0;
             if (per_block) dl.resize((size_t)(nblocks + 1) * kStageDict, -1);
             for (int v = va; v < vb; ++v) {
-                voff[v] = cnt;
+                if (cnt > 0xFFFF || off[v + 1] - off[v] > kMaxElementsPerVertex) return 0;
+                const int first = cnt;
+                seq.clear();
                 int nd = 0;  // the vertex's distinct neighbours so far
                 for (int64_t q = off[v]; q < off[v + 1]; ++q) {
                     uint32_t w[4];
@@ -1190,8 +1201,20 @@ int build_staging(vbd_solver *s)
                         e = (uint32_t)elocal[e];
                     }
                     if (e >= (1u << 17)) return 0;
-                    srec[q] = lk[0] | lk[1] << 5 | lk[2] << 10 | e << 15;
+                    seq.push_back(lk[0] | lk[1] << 5 | lk[2] << 10 | e << 15);
                 }
+                const std::string key(reinterpret_cast<const char *>(seq.data()), seq.size() * sizeof(uint32_t));
+                auto it = seqs.find(key);
+                if (it == seqs.end()) {
+                    if (srec.size() + seq.size() >= (size_t)INT32_MAX) return 0;
+                    it = seqs.emplace(key, std::make_pair((uint32_t)srec.size(), -1)).first;
+                    srec.insert(srec.end(), seq.begin(), seq.end());
+                }
+                if (it->second.second != c) {  // records the colour reads, once per sequence
+                    it->second.second = c;
+                    prec[c] += (double)seq.size();
+                }
+                vrec[v] = make_int2(first | (int)seq.size() << 16, (int)it->second.first);
             }
             ids[c] += cnt;
         }
@@ -1203,10 +1226,10 @@ int build_staging(vbd_solver *s)
     srec.resize(srec.size() + kMaxElementsPerVertex, 0u);
     VBD_TRY(dmalloc(&s->st_nb, nb.size()));
     VBD_TRY(dmalloc(&s->st_rec, srec.size()));
-    VBD_TRY(dmalloc(&s->st_voff, voff.size()));
+    VBD_TRY(dmalloc(&s->st_vrec, vrec.size()));
     VBD_HIP(hipMemcpyAsync(s->st_nb, nb.data(), sizeof(int32_t) * nb.size(), hipMemcpyHostToDevice, st));
     VBD_HIP(hipMemcpyAsync(s->st_rec, srec.data(), sizeof(uint32_t) * srec.size(), hipMemcpyHostToDevice, st));
-    VBD_HIP(hipMemcpyAsync(s->st_voff, voff.data(), sizeof(int32_t) * voff.size(), hipMemcpyHostToDevice, st));
+    VBD_HIP(hipMemcpyAsync(s->st_vrec, vrec.data(), sizeof(int2) * vrec.size(), hipMemcpyHostToDevice, st));
     if (per_block) {
         VBD_TRY(dmalloc(&s->st_dl, dl.size()));
         VBD_HIP(hipMemcpyAsync(s->st_dl, dl.data(), sizeof(int32_t) * dl.size(), hipMemcpyHostToDevice, st));
@@ -1215,6 +1238,7 @@ int build_staging(vbd_solver *s)
     for (int c = 0; c < s->ncolors; ++c)
         if (col[c].nblocks > 0) s->st_col[c] = col[c];
     s->st_ids = ids;
+    s->st_prec = prec;
     s->st_ok = true;
     return 0;
 }

@@ -948,7 +948,7 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
             launch_tail(s, a, g.part_off, tol, ev[0], ev[1]);
         else if (lay)
             launch_staged(s->material, a, v0, nv, *lay, close, s->part, s->n_parts, tol,
-                          StageTables{s->st_nb, s->st_rec, s->st_voff},
+                          StageTables{s->st_nb, s->st_rec, s->st_vrec},
                           s->n_dict, ev[0], ev[1], s->stream);
         else if (prec)
             launch_small(s->material, a, v0, nv, g.block, close, s->part, s->n_parts, tol, prec, ev[0], ev[1],

@@ -498,7 +498,7 @@ __device__ __forceinline__ bool body_done(const BodyPhase &ph, int body)
 struct StageTables {
     const int32_t *nb;
     const uint32_t *rec;
-    const int32_t *voff;
+    const int2 *vrec;
 };
 
 // One block of the staged colour sweep: 256 threads over lanes
@@ -561,9 +561,15 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     }
     const bool in = vi < nv;
     const int v = v0 + (in ? vi : 0);
-    // round 1
-    const VertexPre pre = vertex_pre(a, v, in);
-    const int ov = st.voff[v];
+    // round 1: the vertex's own data, its first entry, slot count and record
+    // sequence (st.vrec), the block's entries and the dictionary
+    VertexPre pre;
+    pre.uv = a.u[v];
+    pre.tg = a.tgt[v];
+    const int2 vr = st.vrec[v];
+    const int ov = vr.x & 0xFFFF;
+    pre.s0 = vr.y;
+    pre.s1 = in ? vr.y + (vr.x >> 16) : vr.y;
     const bool wide = b >= lay.b4;  // the same for every thread
     const int cap = wide ? kStageMaxWide : kStageMax;
     const int32_t *ids = st.nb + lay.ids + stage_list_offset(b, lay.b4);
