@@ -242,18 +242,20 @@ def test_free_fall_exact_and_one_iteration():
         assert np.allclose(x[:, 2], sc.vertices[:, 2] - 9.81e-6 * n * (n + 1) / 2, rtol=0, atol=2e-7)
 
 
-@pytest.mark.parametrize("material", ["snh", "stvk"])
-def test_neighbour_staging_is_bit_exact(material):
+@pytest.mark.parametrize("material,damping", [("snh", 0.0), ("stvk", 0.0), ("snh", 1.0)])
+def test_neighbour_staging_is_bit_exact(material, damping):
     """The big-colour sweep with neighbour staging against the same launch
     without it on the 224x28x28 beam (colours of 47k vertices): the per-slot
     arithmetic is the same and only where neighbours and constants are read
-    from differs, so positions and iteration counts agree bit for bit."""
+    from differs, so positions and iteration counts agree bit for bit (with
+    the damping term too, which reads the step's start in the vertex update)."""
     from warpvbd_amd import _lib
     sc = cantilever((224, 28, 28))
-    a = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
+    kw = dict(sc.solver_kwargs(), damping_coefficient=damping)
+    a = VBDSolver(**kw, dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
     _lib.use_neighbour_staging(False)
     try:
-        b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
+        b = VBDSolver(**kw, dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
     finally:
         _lib.use_neighbour_staging(True)
     assert a.staged_colors() == 4 and b.staged_colors() == 0
@@ -291,20 +293,22 @@ def test_small_colour_kernel_is_bit_exact():
         assert np.array_equal(xa, xb)
 
 
-@pytest.mark.parametrize("material", ["snh", "stvk"])
+@pytest.mark.parametrize("material,damping", [("snh", 0.0), ("stvk", 0.0), ("snh", 1.0)])
 @pytest.mark.parametrize("dims,clusters", [((128, 16, 16), 23), ((96, 12, 12), None)])
-def test_tail_sweep_is_bit_exact(material, dims, clusters):
+def test_tail_sweep_is_bit_exact(material, damping, dims, clusters):
     """The trailing small colours (107, 107 and 1 vertices on the 128x16x16
     beam) swept in one launch by clusters of vertices joined by no edge, the
     neighbours held in LDS, against a launch per colour: the same slot
     arithmetic on the same values, so positions, iteration counts and
-    metrics agree bit for bit over tolerance-terminated steps."""
+    metrics agree bit for bit over tolerance-terminated steps (with the
+    damping term too)."""
     from warpvbd_amd import _lib
     sc = cantilever(dims)
-    a = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
+    kw = dict(sc.solver_kwargs(), damping_coefficient=damping)
+    a = VBDSolver(**kw, dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
     _lib.use_tail_sweep(False)
     try:
-        b = VBDSolver(**sc.solver_kwargs(), dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
+        b = VBDSolver(**kw, dx_tol=1e-7, max_iter=1000, material=material, verbose=False)
     finally:
         _lib.use_tail_sweep(True)
     k, nc = a.tail_sweep()
