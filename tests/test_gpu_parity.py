@@ -322,6 +322,40 @@ def test_tail_sweep_is_bit_exact(material, damping, dims, clusters):
         assert np.array_equal(xa, xb)
 
 
+@pytest.mark.parametrize("dims,seed,tail", [((256, 32, 32), 11, False), ((96, 12, 12), 12, True)])
+def test_shuffled_mesh_launch_forms_are_bit_exact(dims, seed, tail):
+    """Beams with their vertex ids shuffled: first-fit colouring in that order
+    gives other colour sizes and tail clusters (the 256x32x32 beam: 13
+    colours below the staging size, two small trailing colours in more than
+    512 clusters, so neither form applies; the 96x12x12 beam: its last four
+    colours in about 460 clusters of one to a few vertices).  Whatever the
+    tables hold, the solver with staging and the tail sweep must match the one
+    with both off bit for bit."""
+    from warpvbd_amd import _lib
+    sc = cantilever(dims)
+    n = len(sc.vertices)
+    perm = np.random.default_rng(seed).permutation(n)  # new id of each vertex
+    inv = np.argsort(perm)
+    kw = dict(sc.solver_kwargs(), initial_positions=sc.vertices[inv], elements=perm[sc.elements],
+              active_mask=sc.active_mask[inv])
+    a = VBDSolver(**kw, dx_tol=1e-7, max_iter=1000, verbose=False)
+    _lib.use_neighbour_staging(False)
+    _lib.use_tail_sweep(False)
+    try:
+        b = VBDSolver(**kw, dx_tol=1e-7, max_iter=1000, verbose=False)
+    finally:
+        _lib.use_neighbour_staging(True)
+        _lib.use_tail_sweep(True)
+    print("colour sizes", a.color_sizes().tolist(), "staged", a.staged_colors(), "tail", a.tail_sweep(), flush=True)
+    assert b.staged_colors() == 0 and b.tail_sweep() == (0, 0)
+    assert (a.tail_sweep()[0] >= 2) == tail
+    for _ in range(3):
+        xa = a.step(a.old_positions, 1e-3).numpy()
+        xb = b.step(b.old_positions, 1e-3).numpy()
+        assert a.last_iterations == b.last_iterations and a.last_metric == b.last_metric
+        assert np.array_equal(xa, xb)
+
+
 def test_tail_sweep_follows_launch_settings():
     """Launch settings rebuild the plan: with small_color = 0 no colour is
     small and the tail sweep is off; the default settings bring it back, and
