@@ -981,6 +981,9 @@ struct vbd_mpnn {
     bool decided = false, dec_skipping = false, dec_plain_out = false, dec_direct_done = false;
     int32_t *use_alt = nullptr, *use_alt_host = nullptr;  // per body: 0 plain, 1 network, 2 history guess
     hipEvent_t use_alt_ready = nullptr;  // the copy of use_alt queued by the last guess is complete
+    // the guess every body took in the last warm step when the host chose it
+    // (0 plain, 2 history: no comparison ran), else -1 (read use_alt)
+    int host_choice = -1;
     uint32_t *bmax = nullptr;                             // per body: max residual bits of the three guesses
     int counts[kGuesses] = {};                            // bodies per guess in the last warm step
     int guess_chunks = 1;                                 // chunks of the largest body's swept vertices
@@ -1260,10 +1263,9 @@ static int apply_guess_kernels(vbd_mpnn *m, double dt)
     const bool skipping = m->dec_skipping;
     const bool history = m->policy == VBD_WARM_HISTORY || skipping;
     const bool have_hist = m->hs > 0.f;
+    m->host_choice = -1;
     if (m->policy == VBD_WARM_PLAIN || (history && !have_hist)) {
-        k_fill_u32<<<nblk(s->n_bodies, 256), 256, 0, st>>>(s->n_bodies, reinterpret_cast<uint32_t *>(m->use_alt),
-                                                           0u);
-        VBD_HIP(hipGetLastError());
+        m->host_choice = 0;  // every body keeps the plain guess
         return 0;
     }
     const int force = m->policy == VBD_WARM_NETWORK ? 1 : 0;
@@ -1274,9 +1276,8 @@ static int apply_guess_kernels(vbd_mpnn *m, double dt)
         if (!m->dec_direct_done)
             k_guess<<<nblk(m->n, 256), 256, 0, st>>>(m->n, s->perm, s->tgt, s->active, nullptr, 0.f, m->cprev,
                                                      m->hs, m->cprev2, m->hs2, s->u);
-        k_fill_u32<<<nblk(s->n_bodies, 256), 256, 0, st>>>(s->n_bodies, reinterpret_cast<uint32_t *>(m->use_alt),
-                                                           2u);
         VBD_HIP(hipGetLastError());
+        m->host_choice = 2;  // every body takes the history guess
         return 0;
     }
     if (history) {
@@ -1308,12 +1309,13 @@ static int apply_guess_kernels(vbd_mpnn *m, double dt)
     return 0;
 }
 
-// The guess kernels, then the per-body choice queued for the host: the step's
-// own wait for convergence completes the copy, so reading it needs no extra
-// round trip.
+// The guess kernels, then the per-body choice queued for the host when the
+// device made it: the step's own wait for convergence completes the copy, so
+// reading it needs no extra round trip.
 int mpnn_apply_guess(vbd_mpnn *m, double dt)
 {
     VBD_TRY(apply_guess_kernels(m, dt));
+    if (m->host_choice >= 0) return 0;
     hipStream_t st = m->s->stream;
     VBD_HIP(hipMemcpyAsync(m->use_alt_host, m->use_alt, sizeof(int32_t) * m->s->n_bodies, hipMemcpyDeviceToHost,
                            st));
@@ -1324,8 +1326,13 @@ int mpnn_apply_guess(vbd_mpnn *m, double dt)
 int mpnn_used_guess(vbd_mpnn *m, int *used)
 {
     const int nb = m->s->n_bodies;
-    VBD_HIP(hipEventSynchronize(m->use_alt_ready));
     for (int g = 0; g < kGuesses; ++g) m->counts[g] = 0;
+    if (m->host_choice >= 0) {
+        m->counts[m->host_choice] = nb;
+        *used = m->counts[1];
+        return 0;
+    }
+    VBD_HIP(hipEventSynchronize(m->use_alt_ready));
     for (int b = 0; b < nb; ++b) {
         const int g = m->use_alt_host[b];
         if (g >= 0 && g < kGuesses) ++m->counts[g];
