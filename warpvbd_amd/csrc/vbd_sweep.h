@@ -506,12 +506,14 @@ struct StageTables {
 // (lane_vertex: 4, 2 or 1 lanes per vertex by slot count; a batch's colours
 // have 4 lanes for every vertex, blocks of kStageRows).  Two memory rounds,
 // then the neighbour differences, then arithmetic only:
-//   1. the vertex's own data and first entry, the block's entries (at lay.ids
-//      + stage_list_offset: kStageMax per block below lay.b4, kStageMaxWide
+//   1. the vertex's own data and its word in st.vrec (first entry, slot
+//      count, record sequence), the block's entries (at lay.ids +
+//      stage_list_offset: kStageMax per block below lay.b4, kStageMaxWide
 //      from there) and the record dictionary;
 //   2. the entries' neighbour positions and every 4-byte slot record of the
-//      lane (kStagedSlots * 4 zero records past the last slot, so a lane
-//      reads its 8 without a bound).
+//      lane from the vertex's sequence in st.rec (the table of distinct
+//      sequences ends with 32 zero records, so a lane reads its 8 without a
+//      bound).
 // Each vertex's own displacement goes to LDS, then each entry's difference
 // d = u_neighbour - u_vertex (neighbour_diff) is formed once and stored in
 // LDS, and each slot reads its three differences and its dictionary entry
@@ -646,7 +648,8 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     const bool valid = in && !done;
     Acc acc;
     acc_zero(acc);
-    // this lane's slots: pre.s0 + g, + (1 << lg), ... below pre.s1
+    // this lane's slots: records pre.s0 + g, + (1 << lg), ... below pre.s1 of
+    // the vertex's sequence
     const int nk = valid && pre.s1 > pre.s0 + g ? (pre.s1 - pre.s0 - g + (1 << lg) - 1) >> lg : 0;
     constexpr uint32_t kM = (1u << kStageEntryBits) - 1u;
 #pragma unroll
