@@ -222,10 +222,11 @@ int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t
 
 /* Slot record layout.  Each slot is a 16-byte compact record (the tet's other
  * three vertices and a dictionary entry); the entry holds its 44 bytes of
- * constants.  Meshes whose slots share constants (voxel meshes: 5 rest
- * shapes x 4 local indices per material) share entries, so a sweep reads 16
- * bytes per slot from HBM; otherwise each slot has its own entry (64 bytes).
- * The values, and so the results, are the same either way.
+ * constants and, for SNH, 40 bytes of rest-edge constants.  Meshes whose
+ * slots share constants (voxel meshes: 5 rest shapes x 4 local indices per
+ * material) share entries, so a sweep reads 16 bytes per slot from HBM;
+ * otherwise each slot has its own entry (104 bytes per slot for SNH, 64 for
+ * StVK).  The values, and so the results, are the same either way.
  * vbd_use_record_dictionary(0) gives solvers created afterwards one entry per
  * slot (process-wide; default on).  vbd_records reports the number of entries
  * and the HBM bytes per slot a sweep reads.  No counterpart in WarpVBD (its

@@ -42,7 +42,7 @@ void vbd_destroy(vbd_solver_t *s)
                    s->active, s->slot_off, s->slot_ent, s->slots, s->slots2, s->colors, s->inc_off, s->inc, s->part, s->ctrl,
                    s->xio,  s->d_col_start, s->d_ranges, s->d_grav, s->d_body_out, s->body_new, s->tets_dev, s->dict,
                    s->st_nb, s->st_rec, s->st_vrec, s->small_rec, s->u_last, s->d_dx_max,
-                   s->tail_v, s->tail_rec, s->tail_nb, s->st_dl};
+                   s->tail_v, s->tail_rec, s->tail_nb, s->st_dl, s->dict_ext};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (double4 *p : s->snap)
@@ -607,7 +607,8 @@ int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
 #if VBD_TET_RECORDS
     const double per_slot = 20.0;
 #else
-    const double per_slot = s->dict_shared ? 16.0 : 64.0;
+    // one entry per slot: 48 bytes of constants, and for SNH 40 of rest edges
+    const double per_slot = s->dict_shared ? 16.0 : s->material == VBD_MATERIAL_SNH ? 104.0 : 64.0;
 #endif
     for (int c = 0; c < s->ncolors; ++c) {
         const double nv = (double)(s->col_start[c + 1] - s->col_start[c]);
@@ -644,7 +645,7 @@ int vbd_records(vbd_solver_t *s, int32_t *n_dictionary, int32_t *bytes_per_slot)
     if (bytes_per_slot) *bytes_per_slot = 20;  // 64-byte tet record shared by 4 slots + 4-byte entry
 #else
     if (n_dictionary) *n_dictionary = s->n_dict;
-    if (bytes_per_slot) *bytes_per_slot = s->dict_shared ? 16 : 64;
+    if (bytes_per_slot) *bytes_per_slot = s->dict_shared ? 16 : s->material == VBD_MATERIAL_SNH ? 104 : 64;
 #endif
     return 0;
 }

@@ -298,6 +298,7 @@ struct vbd_solver {
     float4 *slots = nullptr;      // [n_slots] compact slot records (ids, entry), or tet records [m*4]
     float2 *slots2 = nullptr;     // full records' qd while they are built; null afterwards
     float4 *dict = nullptr;       // [n_dict*3] record dictionary (vbd_device.h)
+    float4 *dict_ext = nullptr;   // [n_dict*3] rest-edge constants of each entry (vbd_device.h)
     // Per colour: the vertices of lane classes 0 and 1 (lane_class), which the
     // colour-major order puts first (single-body solvers; a batch keeps one
     // class, (col_size, 0)).

@@ -1097,7 +1097,7 @@ SweepArgs sweep_args_at(vbd_solver *s, float4 *u, const double4 *tgt, double dt)
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
-    a.slots2 = s->slots2;
+    a.dict_ext = s->dict_ext;
     a.dict = s->dict;
     a.ent = s->slot_ent;
     a.ns = std::max<int64_t>(s->n_slots, 1);

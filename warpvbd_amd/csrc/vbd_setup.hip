@@ -509,6 +509,22 @@ __global__ void k_record_identity(int64_t ns, const float4 *slots, const float2 
     dict[3 * s + 2] = make_float4(qd.x, qd.y, qa.w, slot_bb(make_float3(qb.x, qb.y, qb.z)));
 }
 
+// Rest-edge constants of each dictionary entry (layout and derivation in
+// vbd_device.h), from the entry's b vectors.
+__global__ void k_dict_ext(int64_t nd, const float4 *dict, float4 *ext)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nd) return;
+    const float4 d0 = dict[3 * k], d1 = dict[3 * k + 1], d2 = dict[3 * k + 2];
+    float4 x0, x1;
+    float2 x2;
+    edge_consts(make_float3(d0.x, d0.y, d0.z), make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), x0, x1,
+                x2);
+    ext[3 * k] = x0;
+    ext[3 * k + 1] = x1;
+    ext[3 * k + 2] = make_float4(x2.x, x2.y, 0.f, 0.f);
+}
+
 // Distinct neighbour vertices gathered by one colour's sweep (for the byte model).
 __global__ void k_mark_boundary(const float4 *slots, const int32_t *ent, const int32_t *slot_off, int v0, int v1,
                                 int tag, int32_t *mark, int32_t *count)
@@ -547,7 +563,7 @@ int build_record_dictionary(vbd_solver *s, bool share, hipStream_t st, Scratch &
 {
     const int64_t ns = s->n_slots;
     float4 *compact = nullptr, *dict = nullptr;
-    auto install = [&](int64_t nd) {
+    auto install = [&](int64_t nd) -> int {
         (void)hipFree(s->slots);
         (void)hipFree(s->slots2);
         s->slots = compact;
@@ -555,6 +571,13 @@ int build_record_dictionary(vbd_solver *s, bool share, hipStream_t st, Scratch &
         s->dict = dict;
         s->n_dict = (int32_t)nd;
         s->dict_shared = nd < ns;
+        (void)hipFree(s->dict_ext);
+        s->dict_ext = nullptr;
+        VBD_TRY(dmalloc(&s->dict_ext, 3 * (size_t)std::max<int64_t>(nd, 1)));
+        if (nd > 0) k_dict_ext<<<nblk(nd), kTB, 0, st>>>(nd, s->dict, s->dict_ext);
+        VBD_HIP(hipGetLastError());
+        VBD_HIP(hipStreamSynchronize(st));
+        return 0;
     };
     auto identity = [&]() -> int {
         VBD_TRY(dmalloc(&compact, (size_t)std::max<int64_t>(ns, 1)));
@@ -565,8 +588,7 @@ int build_record_dictionary(vbd_solver *s, bool share, hipStream_t st, Scratch &
         if (ns > 0) k_record_identity<<<nblk(ns), kTB, 0, st>>>(ns, s->slots, s->slots2, compact, dict);
         VBD_HIP(hipGetLastError());
         VBD_HIP(hipStreamSynchronize(st));
-        install(ns);
-        return 0;
+        return install(ns);
     };
     if (!share || ns < 4) return identity();
     uint64_t *h, *hs;
@@ -604,8 +626,7 @@ int build_record_dictionary(vbd_solver *s, bool share, hipStream_t st, Scratch &
     k_record_dict<<<nblk(ns), kTB, 0, st>>>(ns, is, head, pos, s->slots, s->slots2, compact, dict);
     VBD_HIP(hipGetLastError());
     VBD_HIP(hipStreamSynchronize(st));
-    install(nd);
-    return 0;
+    return install(nd);
 }
 
 int exclusive_scan_i32(const int32_t *in, int32_t *out, int n, hipStream_t st, Scratch &scr)

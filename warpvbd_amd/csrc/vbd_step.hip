@@ -214,12 +214,13 @@ __global__ __launch_bounds__(1024) void k_sweep_small(SweepArgs a, int v0, int n
     if (valid && live && e >= 0) {
         const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) + (uint32_t)e * 48u);
         const float4 d0 = d[0], d1 = d[1], d2 = d[2];
+        const EdgeK ek = edge_k<MAT>(a.dict_ext, (uint32_t)e);
         const float4 x0 = *at(a.u, __float_as_uint(r.x));
         const float4 x1 = *at(a.u, __float_as_uint(r.y));
         const float4 x2 = *at(a.u, __float_as_uint(r.z));
         const SlotRel sr = slot_rel(x0, x1, x2, ql.hi, ql.lo);
         slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, acc);
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2, acc);
     }
     double3 us;
     const float3 dx = vertex_finish<L>(a, v, valid && live, acc, ql, us);
@@ -275,7 +276,8 @@ static_assert(kTailLanes == 8 || kTailLanes == 16 || kTailLanes == 32, "8, 16 or
 
 // One slot of a tail vertex from zero (record r, entry -1: none).
 template <int MAT>
-__device__ __forceinline__ Acc tail_slot(uint2 r, const float4 *s_x, const float4 *s_dict, const VertexLocal &ql)
+__device__ __forceinline__ Acc tail_slot(uint2 r, const float4 *s_x, const float4 *s_dict, const float4 *s_ext,
+                                         const VertexLocal &ql)
 {
     Acc p;
     acc_zero(p);
@@ -283,9 +285,10 @@ __device__ __forceinline__ Acc tail_slot(uint2 r, const float4 *s_x, const float
     if (e >= 0) {
         const float4 x0 = s_x[r.x & 1023u], x1 = s_x[(r.x >> 10) & 1023u], x2 = s_x[(r.x >> 20) & 1023u];
         const float4 d0 = s_dict[3 * e], d1 = s_dict[3 * e + 1], d2 = s_dict[3 * e + 2];
+        const EdgeK ek = edge_k<MAT>(s_ext, (uint32_t)e);
         const SlotRel sr = slot_rel(x0, x1, x2, ql.hi, ql.lo);
         slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, p);
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2, p);
     }
     return p;
 }
@@ -299,10 +302,12 @@ __device__ __forceinline__ Acc tail_slot(uint2 r, const float4 *s_x, const float
 // bit.
 template <int MAT, int S>
 __device__ __forceinline__ Acc tail_slots(const uint2 *q, const float4 *s_x, const float4 *s_dict,
-                                          const VertexLocal &ql)
+                                          const float4 *s_ext, const VertexLocal &ql)
 {
-    if constexpr (S == 1) return tail_slot<MAT>(q[0], s_x, s_dict, ql);
-    else return acc_add(tail_slots<MAT, S / 2>(q, s_x, s_dict, ql), tail_slots<MAT, S / 2>(q + S / 2, s_x, s_dict, ql));
+    if constexpr (S == 1) return tail_slot<MAT>(q[0], s_x, s_dict, s_ext, ql);
+    else
+        return acc_add(tail_slots<MAT, S / 2>(q, s_x, s_dict, s_ext, ql),
+                       tail_slots<MAT, S / 2>(q + S / 2, s_x, s_dict, s_ext, ql));
 }
 
 // The tail colours (K trailing colours of 32 lanes per vertex) of a single
@@ -329,6 +334,7 @@ __global__ __launch_bounds__(kTailRows * kTailLanes) void k_sweep_tail(SweepArgs
     static_assert(kTailNb % kThreads == 0, "the neighbour table splits evenly over the block");
     __shared__ float4 s_x[kTailNb];
     __shared__ float4 s_dict[3 * kStageDict];
+    __shared__ float4 s_ext[MAT == kSNH ? 3 * kStageDict : 1];  // the entries' rest-edge constants
     static_assert(3 * kStageDict <= kThreads, "a dictionary row per thread");
     Ctrl *c = a.ctrl;
     const int row = (int)threadIdx.x / L, lane = (int)threadIdx.x % L;
@@ -356,6 +362,7 @@ __global__ __launch_bounds__(kTailRows * kTailLanes) void k_sweep_tail(SweepArgs
     }
     const bool has_dict = (int)threadIdx.x < 3 * t.n_dict;
     const float4 drow = has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 xrow = MAT == kSNH && has_dict ? a.dict_ext[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t m = 0u;
     for (int64_t i = cl * blockDim.x + threadIdx.x; i < np_before; i += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t y = part_all[i];
@@ -382,7 +389,10 @@ __global__ __launch_bounds__(kTailRows * kTailLanes) void k_sweep_tail(SweepArgs
     }
 #pragma unroll
     for (int j = 0; j < kNbPer; ++j) s_x[threadIdx.x + kThreads * j] = px[j];
-    if (has_dict) s_dict[threadIdx.x] = drow;
+    if (has_dict) {
+        s_dict[threadIdx.x] = drow;
+        if constexpr (MAT == kSNH) s_ext[threadIdx.x] = xrow;
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -395,7 +405,7 @@ __global__ __launch_bounds__(kTailRows * kTailLanes) void k_sweep_tail(SweepArgs
         Acc acc;
         acc_zero(acc);
         VertexLocal ql = vertex_local(pre);
-        if (valid) acc = tail_slots<MAT, S>(rc[k], s_x, s_dict, ql);
+        if (valid) acc = tail_slots<MAT, S>(rc[k], s_x, s_dict, s_ext, ql);
         double3 us;
         const float3 dx = vertex_finish<L>(a, valid ? v : 0, valid, acc, ql, us);
         if (valid) {
@@ -925,7 +935,7 @@ static int enqueue_iteration(vbd_solver *s, int iter, float tol, bool profile)
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
-    a.slots2 = s->slots2;
+    a.dict_ext = s->dict_ext;
     a.dict = s->dict;
     a.ent = s->slot_ent;
     a.ns = std::max<int64_t>(s->n_slots, 1);
@@ -1033,7 +1043,7 @@ static int step_resident(vbd_solver *s, double dt, double tol, int max_iter, int
     a.uprev = s->uprev;
     a.slot_off = s->slot_off;
     a.slots = s->slots;
-    a.slots2 = s->slots2;
+    a.dict_ext = s->dict_ext;
     a.dict = s->dict;
     a.ent = s->slot_ent;
     a.ns = std::max<int64_t>(s->n_slots, 1);

@@ -22,7 +22,7 @@ struct SweepArgs {
     const double4 *__restrict__ uprev;
     const int32_t *__restrict__ slot_off;
     const float4 *__restrict__ slots;   // compact slot records (ids, entry), or tet records (VBD_TET_RECORDS)
-    const float2 *__restrict__ slots2;  // unused (null)
+    const float4 *__restrict__ dict_ext;  // rest-edge constants of the dictionary's entries
     const float4 *__restrict__ dict;    // record dictionary
     const int32_t *__restrict__ ent;    // per-slot tet << 2 | local index (VBD_TET_RECORDS)
     int64_t ns;
@@ -296,6 +296,7 @@ __device__ __forceinline__ void slot_loop(const SweepArgs &a, int s, int s1, int
         const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) +
                                                            __float_as_uint(c.w) * 48u);
         const float4 d0 = d[0], d1 = d[1], d2 = d[2];
+        const EdgeK ek = edge_k<MAT>(a.dict_ext, __float_as_uint(c.w));
         const float4 x0 = *at(a.u, __float_as_uint(c.x));
         const float4 x1 = *at(a.u, __float_as_uint(c.y));
         const float4 x2 = *at(a.u, __float_as_uint(c.z));
@@ -305,7 +306,7 @@ __device__ __forceinline__ void slot_loop(const SweepArgs &a, int s, int s1, int
         __builtin_amdgcn_sched_barrier(0);
         const SlotRel sr = slot_rel(x0, x1, x2, hi, lo);
         slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, acc);
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2, acc);
         c = cn;
     }
 }
@@ -359,7 +360,10 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
                     const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
-                    slot_eval<MAT>(ka[j], kb[j], kc[j], slot_bb(bs[j]), bs[j], b0[j], b1[j], r.e0, r.e1, r.d2, acc);
+                    EdgeK ek;
+                    edge_consts(bs[j], b0[j], b1[j], ek.x0, ek.x1, ek.x2);
+                    slot_eval<MAT>(ka[j], kb[j], kc[j], slot_bb(bs[j]), bs[j], b0[j], b1[j], ek, r.e0, r.e1, r.d2,
+                                   acc);
                 }
             }
         }
@@ -373,6 +377,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             float4 q[B][3];
             float2 qd[B];
             float qbb[B];
+            EdgeK qk[B];
             float4 x[B][3];
             // compact record (ids, entry), then the entry's constants
 #pragma unroll
@@ -387,6 +392,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                     q[j][2] = d[1];
                     qd[j] = make_float2(d2.x, d2.y);
                     qbb[j] = d2.w;
+                    qk[j] = edge_k<MAT>(a.dict_ext, (uint32_t)__float_as_int(c.w));
                 }
             }
 #pragma unroll
@@ -404,7 +410,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                     const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
                     slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), qbb[j], make_float3(qq[1].x, qq[1].y, qq[1].z),
                                    make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[2].w, qd[j].x, qd[j].y),
-                                   r.e0, r.e1, r.d2, acc);
+                                   qk[j], r.e0, r.e1, r.d2, acc);
                 }
             }
         }
@@ -549,6 +555,7 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     __shared__ float4 s_own[kOwners];      // per vertex: FP32 parts .xyz, remainder x in .w
     __shared__ float2 s_own2[kOwners];     // remainders y, z
     __shared__ float4 s_dict[3 * kStageDict];
+    __shared__ float4 s_ext[MAT == kSNH ? 3 * kStageDict : 1];  // the entries' rest-edge constants
     const int t = b * 256 + threadIdx.x;
     int g, lg, vi, vfirst;
     if constexpr (BODIES) {
@@ -585,6 +592,7 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     // block's list this thread's row belongs to, its row read in round 2
     const bool has_dict = (int)threadIdx.x < 3 * (DL ? kStageDict : n_dict);
     float4 drow = !DL && has_dict ? a.dict[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 xrow = MAT == kSNH && !DL && has_dict ? a.dict_ext[threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
     int de = -1;
     static_assert(!DL || BODIES, "per-block dictionaries are a batch's");
     if constexpr (DL)
@@ -622,9 +630,16 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
     const uint32_t *rl = st.rec + pre.s0 + g;
 #pragma unroll
     for (int k = 0; k < kStagedSlots; ++k) rec[k] = rl[k << lg];
-    if constexpr (DL)
-        if (de >= 0) drow = a.dict[3 * de + (int)threadIdx.x % 3];
-    if (has_dict) s_dict[threadIdx.x] = drow;
+    if constexpr (DL) {
+        if (de >= 0) {
+            drow = a.dict[3 * de + (int)threadIdx.x % 3];
+            if constexpr (MAT == kSNH) xrow = a.dict_ext[3 * de + (int)threadIdx.x % 3];
+        }
+    }
+    if (has_dict) {
+        s_dict[threadIdx.x] = drow;
+        if constexpr (MAT == kSNH) s_ext[threadIdx.x] = xrow;
+    }
     VertexLocal ql = vertex_local(pre);
     const float3 hi = ql.hi, lo = ql.lo;
     if (in && g == 0) {
@@ -658,11 +673,13 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
             const uint32_t r = rec[k];
             const uint32_t e = r >> 15, n0 = r & kM, n1 = (r >> 5) & kM, n2 = (r >> 10) & kM;
             const float4 d0 = s_dict[3 * e], d1 = s_dict[3 * e + 1], d2 = s_dict[3 * e + 2];
+            const EdgeK ek = edge_k<MAT>(s_ext, e);
             const float4 q0 = s_d[ov + n0], q1 = s_d[ov + n1], q2 = s_d[ov + n2];
             const SlotRel sr = slot_rel_d(make_float3(q0.x, q0.y, q0.z), make_float3(q1.x, q1.y, q1.z),
                                           make_float3(q2.x, q2.y, q2.z));
             slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), sr.e0, sr.e1, sr.d2, acc);
+                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2,
+                           acc);
         }
     }
     double3 us;
