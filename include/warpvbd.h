@@ -222,11 +222,13 @@ int vbd_step_warm(vbd_solver_t *h, vbd_mpnn_t *m, double dt, double tol, int32_t
 
 /* Slot record layout.  Each slot is a 16-byte compact record (the tet's other
  * three vertices and a dictionary entry); the entry holds its 44 bytes of
- * constants and, for SNH, 40 bytes of rest-edge constants.  Meshes whose
- * slots share constants (voxel meshes: 5 rest shapes x 4 local indices per
- * material) share entries, so a sweep reads 16 bytes per slot from HBM;
- * otherwise each slot has its own entry (104 bytes per slot for SNH, 64 for
- * StVK).  The values, and so the results, are the same either way.
+ * constants.  Meshes whose slots share constants (voxel meshes: 5 rest
+ * shapes x 4 local indices per material) share entries, so a sweep reads 16
+ * bytes per slot from HBM; otherwise each slot has its own entry (64 bytes).
+ * The values, and so the results, are the same either way.  A shared
+ * dictionary of at most 64 entries (or a batch's) also holds each entry's
+ * rest edges, from which SNH sweeps form cof(F) b and J - 1 with fewer
+ * operations (a solver's sweeps all use the same form).
  * vbd_use_record_dictionary(0) gives solvers created afterwards one entry per
  * slot (process-wide; default on).  vbd_records reports the number of entries
  * and the HBM bytes per slot a sweep reads.  No counterpart in WarpVBD (its
@@ -262,6 +264,20 @@ int vbd_tail_sweep(vbd_solver_t *h, int32_t *n_colors, int32_t *n_clusters);
 /* The tail sweep in solvers created, and launch settings changed, afterwards
  * (process-wide; default on). */
 void vbd_use_tail_sweep(int32_t enable);
+/* The rest-edge form of the SNH slot arithmetic: with a shared record
+ * dictionary of at most 64 entries, or a batch's, each entry also holds its
+ * rest edges E0 = X_o0 - X_o2, E1 = X_o1 - X_o2, R = X_self - X_o2 (scaled by
+ * k = det[b_o0 b_o1 b_self]), and every sweep of the solver forms cof(F) b and
+ * J - 1 from them and the displacement differences (15 VALU instructions per
+ * slot fewer than from F's rows; another rounding of the same quantities).
+ * Larger dictionaries are read from global memory by every slot, where three
+ * more loads per slot cost more than they save, so those solvers keep F's
+ * rows.  *on: whether the solver uses the form.  No counterpart in WarpVBD
+ * (its src/vbd_solver.py:101-124 forms the SNH terms from F). */
+int vbd_edge_form(vbd_solver_t *h, int32_t *on);
+/* The rest-edge form in solvers created afterwards (process-wide; default on;
+ * off, every sweep forms cof(F) b and J - 1 from F's rows). */
+void vbd_use_edge_form(int32_t enable);
 /* bytes[n_colors]: the HBM bytes one sweep of each colour reads and writes
  * with this layout, sum_v (68 + bytes_per_slot k_v) + 16 |gathered neighbours|:
  * per swept vertex its 16-byte packed position, 32-byte FP64 inertia target,

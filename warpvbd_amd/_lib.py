@@ -69,6 +69,8 @@ SIGNATURES = {
     "vbd_staged_colors": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "vbd_tail_sweep": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "vbd_use_tail_sweep": (None, [_i32]),
+    "vbd_edge_form": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
+    "vbd_use_edge_form": (None, [_i32]),
     "vbd_mpnn_guess_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "vbd_mpnn_set_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
     "vbd_mpnn_get_params": (ctypes.c_int, [_vp, _P(ctypes.c_float)]),
@@ -172,6 +174,14 @@ def use_tail_sweep(enable: bool = True) -> None:
     of vertices (default on; same results either way); applies to solvers
     created, and launch settings changed, afterwards."""
     load().vbd_use_tail_sweep(int(bool(enable)))
+
+
+def use_edge_form(enable: bool = True) -> None:
+    """Form the SNH terms cof(F) b and J - 1 from each dictionary entry's rest
+    edges where the dictionary has at most 64 entries or belongs to a batch
+    (default on; off: from F's rows); applies to solvers created
+    afterwards."""
+    load().vbd_use_edge_form(int(bool(enable)))
 
 
 def use_device(device) -> int:

@@ -591,6 +591,14 @@ int vbd_profile_read(vbd_solver_t *s, double *ms, int64_t *launches, double *byt
 void vbd_use_record_dictionary(int32_t enable) { g_record_dictionary = enable != 0; }
 void vbd_use_neighbour_staging(int32_t enable) { g_staging = enable != 0; }
 void vbd_use_tail_sweep(int32_t enable) { g_tail = enable != 0; }
+void vbd_use_edge_form(int32_t enable) { g_edge_form = enable != 0; }
+
+int vbd_edge_form(vbd_solver_t *s, int32_t *on)
+{
+    CHECK_ARG(s && on, "null argument");
+    *on = s->dict_ext != nullptr;
+    return 0;
+}
 
 int vbd_tail_sweep(vbd_solver_t *s, int32_t *n_colors, int32_t *n_clusters)
 {
@@ -607,8 +615,9 @@ int vbd_color_layout_bytes(vbd_solver_t *s, double *bytes)
 #if VBD_TET_RECORDS
     const double per_slot = 20.0;
 #else
-    // one entry per slot: 48 bytes of constants, and for SNH 40 of rest edges
-    const double per_slot = s->dict_shared ? 16.0 : s->material == VBD_MATERIAL_SNH ? 104.0 : 64.0;
+    // one entry per slot: 48 bytes of constants (the rest-edge constants go
+    // with shared entries only)
+    const double per_slot = s->dict_shared ? 16.0 : 64.0;
 #endif
     for (int c = 0; c < s->ncolors; ++c) {
         const double nv = (double)(s->col_start[c + 1] - s->col_start[c]);
@@ -645,7 +654,7 @@ int vbd_records(vbd_solver_t *s, int32_t *n_dictionary, int32_t *bytes_per_slot)
     if (bytes_per_slot) *bytes_per_slot = 20;  // 64-byte tet record shared by 4 slots + 4-byte entry
 #else
     if (n_dictionary) *n_dictionary = s->n_dict;
-    if (bytes_per_slot) *bytes_per_slot = s->dict_shared ? 16 : s->material == VBD_MATERIAL_SNH ? 104 : 64;
+    if (bytes_per_slot) *bytes_per_slot = s->dict_shared ? 16 : 64;
 #endif
     return 0;
 }

@@ -23,7 +23,10 @@
 //   StVK: ka = V mu,  kb = V lambda,  kc = 0
 // kc follows from ka and the material, so it is not stored.
 //
-// Each entry also has rest-edge constants (3 float4 in dict_ext, SNH only):
+// Solvers whose dictionary is read from LDS (at most 64 entries, or a batch's
+// per-block lists) also give each entry rest-edge constants (3 float4 in
+// dict_ext, read by SNH only; null otherwise, where three more scattered loads
+// per slot would cost more than the arithmetic they save):
 //     x0 = (P.xyz, k)   x1 = (Q.xyz, R.x)   x2 = (R.y, R.z)
 // with E0 = X_o0 - X_o2, E1 = X_o1 - X_o2, R = X_self - X_o2 the rest edges
 // (the rows of [b0 b1 bs]^-1), k = det[b0 b1 bs], P = k E0 and Q = k E1; so
@@ -132,17 +135,18 @@ __device__ __forceinline__ void edge_consts(const float3 bs, const float3 b0, co
 }
 
 // The rest-edge constants of entry e of a table laid out as dict_ext (global
-// memory or LDS); zero, and not read, for StVK, which does not use them.
+// memory or LDS); zero, and not read, for StVK, which does not use them, and
+// when edge is false (a solver without the table).
 struct EdgeK {
     float4 x0, x1;
     float2 x2;
 };
 
 template <int MAT>
-__device__ __forceinline__ EdgeK edge_k(const float4 *ext, uint32_t e)
+__device__ __forceinline__ EdgeK edge_k(const float4 *ext, uint32_t e, bool edge = true)
 {
     EdgeK k;
-    if constexpr (MAT == kSNH) {
+    if (MAT == kSNH && edge) {
         const float4 *p = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(ext) + e * 48u);
         k.x0 = p[0];
         k.x1 = p[1];
@@ -156,8 +160,8 @@ __device__ __forceinline__ EdgeK edge_k(const float4 *ext, uint32_t e)
 
 template <int MAT>
 __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, float bb, const float3 bs, const float3 b0,
-                                          const float3 b1, const EdgeK &ek, const float3 e0, const float3 e1,
-                                          const float3 d2, Acc &a)
+                                          const float3 b1, const EdgeK &ek, bool edge, const float3 e0,
+                                          const float3 e1, const float3 d2, Acc &a)
 {
     const float4 x0 = ek.x0, x1 = ek.x1;
     const float2 x2 = ek.x2;
@@ -179,19 +183,39 @@ __device__ __forceinline__ void slot_eval(float ka, float kb, float kc, float bb
     if constexpr (MAT == kSNH) {
         const float Ic = F00 * F00 + F01 * F01 + F02 * F02 + F10 * F10 + F11 * F11 + F12 * F12 + F20 * F20 +
                          F21 * F21 + F22 * F22;
-        // cof(F) b = dJ/dx_self = k A x C with A = E0 + e0 and C = E1 + e1 the
-        // deformed edges, so cb = bs + c with the change
-        //   c = k (A x C - E0 x E1) = P x e1 + e0 x (Q + k e1),
-        // formed from the small differences alone.  J is affine in x_self and
-        // zero on the face (o0, o1, o2): J = cb . (R - d2), and with bs . R = 1
-        //   J - 1 = c . R - cb . d2,
-        // accurate for small D.
-        const float w0 = fmaf(x0.w, e1.x, x1.x), w1 = fmaf(x0.w, e1.y, x1.y), w2 = fmaf(x0.w, e1.z, x1.z);
-        const float c0 = x0.y * e1.z - x0.z * e1.y + e0.y * w2 - e0.z * w1;
-        const float c1 = x0.z * e1.x - x0.x * e1.z + e0.z * w0 - e0.x * w2;
-        const float c2 = x0.x * e1.y - x0.y * e1.x + e0.x * w1 - e0.y * w0;
-        const float cb0 = bs.x + c0, cb1 = bs.y + c1, cb2 = bs.z + c2;
-        const float Jm1 = fmaf(-cb2, d2.z, fmaf(-cb1, d2.y, fmaf(-cb0, d2.x, c0 * x1.w + c1 * x2.x + c2 * x2.y)));
+        float cb0, cb1, cb2, Jm1;
+        if (edge) {
+            // cof(F) b = dJ/dx_self = k A x C with A = E0 + e0 and C = E1 + e1
+            // the deformed edges, so cb = bs + c with the change
+            //   c = k (A x C - E0 x E1) = P x e1 + e0 x (Q + k e1),
+            // formed from the small differences alone.  J is affine in x_self
+            // and zero on the face (o0, o1, o2): J = cb . (R - d2), and with
+            // bs . R = 1
+            //   J - 1 = c . R - cb . d2,
+            // accurate for small D.  15 VALU instructions fewer than below.
+            const float w0 = fmaf(x0.w, e1.x, x1.x), w1 = fmaf(x0.w, e1.y, x1.y), w2 = fmaf(x0.w, e1.z, x1.z);
+            const float c0 = x0.y * e1.z - x0.z * e1.y + e0.y * w2 - e0.z * w1;
+            const float c1 = x0.z * e1.x - x0.x * e1.z + e0.z * w0 - e0.x * w2;
+            const float c2 = x0.x * e1.y - x0.y * e1.x + e0.x * w1 - e0.y * w0;
+            cb0 = bs.x + c0;
+            cb1 = bs.y + c1;
+            cb2 = bs.z + c2;
+            Jm1 = fmaf(-cb2, d2.z, fmaf(-cb1, d2.y, fmaf(-cb0, d2.x, c0 * x1.w + c1 * x2.x + c2 * x2.y)));
+        } else {
+            // J - 1 = tr D + (principal 2x2 minors of D) + det D, accurate for small D
+            const float m2 = (D00 * D11 - D01 * D10) + (D00 * D22 - D02 * D20) + (D11 * D22 - D12 * D21);
+            const float detD = D00 * (D11 * D22 - D12 * D21) - D01 * (D10 * D22 - D12 * D20) +
+                               D02 * (D10 * D21 - D11 * D20);
+            Jm1 = (D00 + D11 + D22) + m2 + detD;
+            // cof(F) b.  Row i of cof(F) = J F^-T is F_j x F_k (rows of F,
+            // (i,j,k) cyclic).  With p = F_1 x b the scalar triple products give
+            //   cb0 = (F_1 x F_2) . b = -F_2 . p,   cb2 = (F_0 x F_1) . b = F_0 . p,
+            // and cb1 = (F_2 x F_0) . b directly.
+            const float p0 = F11 * bs.z - F12 * bs.y, p1 = F12 * bs.x - F10 * bs.z, p2 = F10 * bs.y - F11 * bs.x;
+            cb0 = -(F20 * p0 + F21 * p1 + F22 * p2);
+            cb2 = F00 * p0 + F01 * p1 + F02 * p2;
+            cb1 = (F21 * F02 - F22 * F01) * bs.x + (F22 * F00 - F20 * F02) * bs.y + (F20 * F01 - F21 * F00) * bs.z;
+        }
         // hardware reciprocal (1 ulp): the SNH weights need no correctly rounded divide
         const float r = __builtin_amdgcn_rcpf(Ic + 1.f);
         const float s1 = ka * (Ic * r);        // V mu'(1 - 1/(Ic+1))

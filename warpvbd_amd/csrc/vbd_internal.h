@@ -419,6 +419,7 @@ const LaneLayout *big_color_layout(const vbd_solver *s, int c);
 extern bool g_record_dictionary;  // share record dictionary entries in later creates (default on)
 extern bool g_staging;            // neighbour staging in later creates (default on)
 extern bool g_tail;               // the tail sweep in later creates and launch changes (default on)
+extern bool g_edge_form;          // rest-edge constants for staged dictionaries in later creates (default on)
 hipError_t rocprim_exclusive_scan_i32(void *tmp, size_t &bytes, const int32_t *in, int32_t *out, int n,
                                       hipStream_t st);
 int step_bodies(vbd_solver *s, double dt, double tol, int max_iter);

@@ -571,10 +571,13 @@ int build_record_dictionary(vbd_solver *s, bool share, hipStream_t st, Scratch &
         s->dict = dict;
         s->n_dict = (int32_t)nd;
         s->dict_shared = nd < ns;
+        // rest-edge constants where the sweeps read the dictionary from LDS
+        // (the staged and tail sweeps need them there)
         (void)hipFree(s->dict_ext);
         s->dict_ext = nullptr;
-        VBD_TRY(dmalloc(&s->dict_ext, 3 * (size_t)std::max<int64_t>(nd, 1)));
-        if (nd > 0) k_dict_ext<<<nblk(nd), kTB, 0, st>>>(nd, s->dict, s->dict_ext);
+        if (!g_edge_form || !s->dict_shared || (nd > kStageDict && !s->batched)) return 0;
+        VBD_TRY(dmalloc(&s->dict_ext, 3 * (size_t)nd));
+        k_dict_ext<<<nblk(nd), kTB, 0, st>>>(nd, s->dict, s->dict_ext);
         VBD_HIP(hipGetLastError());
         VBD_HIP(hipStreamSynchronize(st));
         return 0;
@@ -748,6 +751,7 @@ __global__ void k_rescale_dict(int64_t nd, float4 *dict, float fa, float fb)
 bool g_record_dictionary = true;  // vbd_use_record_dictionary
 bool g_staging = true;            // vbd_use_neighbour_staging
 bool g_tail = true;               // vbd_use_tail_sweep
+bool g_edge_form = true;          // vbd_use_edge_form
 
 int rescale_material(vbd_solver *s, double E0, double nu0, double E1, double nu1)
 {
@@ -1150,7 +1154,8 @@ int build_staging(vbd_solver *s)
     }
     // (ids below 2^25 - 1, so that no entry reads as the -1 that ends a list)
     const bool per_block = s->n_dict > kStageDict;
-    if (!g_staging || !s->dict_shared || (per_block && !s->batched) || s->n_slots <= 0 || s->n >= (1 << 25) - 1)
+    if (!g_staging || !s->dict_shared || !s->dict_ext || (per_block && !s->batched) || s->n_slots <= 0 ||
+        s->n >= (1 << 25) - 1)
         return 0;
     hipStream_t st = s->stream;
     const int nsw = s->n_swept;

@@ -214,13 +214,15 @@ __global__ __launch_bounds__(1024) void k_sweep_small(SweepArgs a, int v0, int n
     if (valid && live && e >= 0) {
         const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) + (uint32_t)e * 48u);
         const float4 d0 = d[0], d1 = d[1], d2 = d[2];
-        const EdgeK ek = edge_k<MAT>(a.dict_ext, (uint32_t)e);
+        const bool edge = a.dict_ext != nullptr;
+        const EdgeK ek = edge_k<MAT>(a.dict_ext, (uint32_t)e, edge);
         const float4 x0 = *at(a.u, __float_as_uint(r.x));
         const float4 x1 = *at(a.u, __float_as_uint(r.y));
         const float4 x2 = *at(a.u, __float_as_uint(r.z));
         const SlotRel sr = slot_rel(x0, x1, x2, ql.hi, ql.lo);
         slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2, acc);
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, edge, sr.e0, sr.e1, sr.d2,
+                       acc);
     }
     double3 us;
     const float3 dx = vertex_finish<L>(a, v, valid && live, acc, ql, us);
@@ -288,7 +290,8 @@ __device__ __forceinline__ Acc tail_slot(uint2 r, const float4 *s_x, const float
         const EdgeK ek = edge_k<MAT>(s_ext, (uint32_t)e);
         const SlotRel sr = slot_rel(x0, x1, x2, ql.hi, ql.lo);
         slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2, p);
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, true, sr.e0, sr.e1, sr.d2,
+                       p);
     }
     return p;
 }
@@ -720,7 +723,8 @@ static int build_tail(vbd_solver *s)
 #if VBD_TET_RECORDS
     return 0;
 #else
-    if (!g_tail || s->n_bodies > 1 || s->batched || !s->slots || !s->dict_shared || s->n_dict > kStageDict ||
+    if (!g_tail || s->n_bodies > 1 || s->batched || !s->slots || !s->dict_shared || !s->dict_ext ||
+        s->n_dict > kStageDict ||
         s->n_dict <= 0)
         return 0;
     std::vector<int> cand;  // the trailing small colours, in order

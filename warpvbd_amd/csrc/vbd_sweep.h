@@ -289,6 +289,7 @@ __device__ __forceinline__ void slot_loop(const SweepArgs &a, int s, int s1, int
                                           const float3 &lo, Acc &acc)
 {
     float4 c = s < s1 ? a.slots[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool edge = a.dict_ext != nullptr;  // the same for every thread
     for (; s < s1; s += step) {
         // 32-bit unsigned byte offsets from the arrays' (uniform) bases:
         // address = base + offset, no 64-bit index arithmetic per load
@@ -296,7 +297,7 @@ __device__ __forceinline__ void slot_loop(const SweepArgs &a, int s, int s1, int
         const float4 *d = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a.dict) +
                                                            __float_as_uint(c.w) * 48u);
         const float4 d0 = d[0], d1 = d[1], d2 = d[2];
-        const EdgeK ek = edge_k<MAT>(a.dict_ext, __float_as_uint(c.w));
+        const EdgeK ek = edge_k<MAT>(a.dict_ext, __float_as_uint(c.w), edge);
         const float4 x0 = *at(a.u, __float_as_uint(c.x));
         const float4 x1 = *at(a.u, __float_as_uint(c.y));
         const float4 x2 = *at(a.u, __float_as_uint(c.z));
@@ -306,7 +307,8 @@ __device__ __forceinline__ void slot_loop(const SweepArgs &a, int s, int s1, int
         __builtin_amdgcn_sched_barrier(0);
         const SlotRel sr = slot_rel(x0, x1, x2, hi, lo);
         slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2, acc);
+                       make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, edge, sr.e0, sr.e1, sr.d2,
+                       acc);
         c = cn;
     }
 }
@@ -360,10 +362,9 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
             for (int j = 0; j < B; ++j) {
                 if (s + j * L < s1) {
                     const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
-                    EdgeK ek;
-                    edge_consts(bs[j], b0[j], b1[j], ek.x0, ek.x1, ek.x2);
-                    slot_eval<MAT>(ka[j], kb[j], kc[j], slot_bb(bs[j]), bs[j], b0[j], b1[j], ek, r.e0, r.e1, r.d2,
-                                   acc);
+                    const EdgeK ek = edge_k<MAT>(nullptr, 0u, false);
+                    slot_eval<MAT>(ka[j], kb[j], kc[j], slot_bb(bs[j]), bs[j], b0[j], b1[j], ek, false, r.e0, r.e1,
+                                   r.d2, acc);
                 }
             }
         }
@@ -392,7 +393,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                     q[j][2] = d[1];
                     qd[j] = make_float2(d2.x, d2.y);
                     qbb[j] = d2.w;
-                    qk[j] = edge_k<MAT>(a.dict_ext, (uint32_t)__float_as_int(c.w));
+                    qk[j] = edge_k<MAT>(a.dict_ext, (uint32_t)__float_as_int(c.w), a.dict_ext != nullptr);
                 }
             }
 #pragma unroll
@@ -410,7 +411,7 @@ __device__ __forceinline__ float3 vertex_dx(const SweepArgs &a, int v, bool vali
                     const SlotRel r = slot_rel(x[j][0], x[j][1], x[j][2], hi, lo);
                     slot_eval<MAT>(qq[0].w, qq[1].w, slot_kc<MAT>(qq[0].w), qbb[j], make_float3(qq[1].x, qq[1].y, qq[1].z),
                                    make_float3(qq[2].x, qq[2].y, qq[2].z), make_float3(qq[2].w, qd[j].x, qd[j].y),
-                                   qk[j], r.e0, r.e1, r.d2, acc);
+                                   qk[j], a.dict_ext != nullptr, r.e0, r.e1, r.d2, acc);
                 }
             }
         }
@@ -678,8 +679,8 @@ __device__ __forceinline__ bool staged_sweep(const SweepArgs &a, int v0, int nv,
             const SlotRel sr = slot_rel_d(make_float3(q0.x, q0.y, q0.z), make_float3(q1.x, q1.y, q1.z),
                                           make_float3(q2.x, q2.y, q2.z));
             slot_eval<MAT>(d2.z, d0.w, slot_kc<MAT>(d2.z), d2.w, make_float3(d0.x, d0.y, d0.z),
-                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, sr.e0, sr.e1, sr.d2,
-                           acc);
+                           make_float3(d1.x, d1.y, d1.z), make_float3(d1.w, d2.x, d2.y), ek, true, sr.e0, sr.e1,
+                           sr.d2, acc);
         }
     }
     double3 us;

@@ -376,10 +376,13 @@ class VBDSolver:
     def record_layout(self) -> dict:
         """Slot records of this solver: record dictionary entries (as many as
         slots when no constants are shared) and HBM bytes per slot read by a
-        sweep (16 with shared entries, else 64)."""
-        n, b = ctypes.c_int32(), ctypes.c_int32()
+        sweep (16 with shared entries, else 64).  edge_form: whether the
+        entries also hold their rest edges, from which the SNH sweeps form
+        cof(F) b and J - 1."""
+        n, b, e = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(_lib.load().vbd_records(self._h, ctypes.byref(n), ctypes.byref(b)), "vbd_records")
-        return {"dictionary_entries": n.value, "bytes_per_slot": b.value}
+        check(_lib.load().vbd_edge_form(self._h, ctypes.byref(e)), "vbd_edge_form")
+        return {"dictionary_entries": n.value, "bytes_per_slot": b.value, "edge_form": bool(e.value)}
 
     def color_layout_bytes(self) -> np.ndarray:
         """HBM bytes of one sweep of each colour with this solver's layout:
